@@ -1,0 +1,219 @@
+"""ctypes view of ``include/drv.h`` (the pipeline's C ABI) and of the input
+producer ``csrc/inputs/drv_inputs.h``.
+
+The structures mirror the C records field for field; ``load_drv()`` raises if
+the HIP library has not been built - there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIBDRV = PKG / "libdrv.so"
+LIBINPUTS = PKG / "libdrv_inputs.so"
+
+DRV_OK = 0
+DRV_ERR_INVALID_ARG = -1
+DRV_ERR_HIP = -2
+DRV_ERR_NO_MEMORY = -3
+DRV_ERR_STATE = -4
+DRV_MAX_CASCADES = 4
+
+f3 = C.c_float * 3
+f16 = C.c_float * 16
+
+
+class DrvConfig(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("max_caches", C.c_int32),
+        ("cav_resolution", C.c_int32),
+        ("cav_cascades", C.c_int32),
+        ("voxel_resolution", C.c_int32),
+        ("sh_bands", C.c_int32),
+        ("indirect_shadow", C.c_int32),
+        ("cav_transition_size", C.c_float),
+        ("cav_cascade0_world_size", C.c_float),
+    ]
+
+
+class DrvCascade(C.Structure):
+    _fields_ = [
+        ("min", f3),
+        ("world_voxel_size", C.c_float),
+        ("max", f3),
+        ("pad0", C.c_float),
+        ("decision_min", f3),
+        ("pad1", C.c_float),
+        ("decision_max", f3),
+        ("pad2", C.c_float),
+    ]
+
+
+class DrvVolume(C.Structure):
+    _fields_ = [
+        ("volume_world_min", f3),
+        ("voxel_size_in_world", C.c_float),
+        ("volume_world_max", f3),
+        ("cav_transition_zone_size", C.c_float),
+        ("cascades", DrvCascade * DRV_MAX_CASCADES),
+    ]
+
+
+class DrvFrame(C.Structure):
+    _fields_ = [("inverse_view_projection", f16), ("camera_position", f3)]
+
+
+class DrvShConstants(C.Structure):
+    _fields_ = [
+        (n, C.c_float)
+        for n in (
+            "cos_lobe0", "cos_lobe1", "cos_lobe2n2_p1_n1", "cos_lobe20", "cos_lobe2p2",
+            "eva0", "eva1", "eva2n2_p1_n1", "eva20", "eva2p2",
+        )
+    ]
+
+
+class DrvSpotParams(C.Structure):
+    _fields_ = [
+        ("intensity", f3),
+        ("position", f3),
+        ("direction", f3),
+        ("half_angle", C.c_float),
+        ("rsm_resolution", C.c_uint32),
+        ("rsm_read_lod", C.c_uint32),
+        ("indirect_shadow_lod", C.c_uint32),
+        ("near_plane", C.c_float),
+        ("far_plane", C.c_float),
+    ]
+
+
+class DrvSpotLight(C.Structure):
+    _fields_ = [
+        ("light_intensity", f3),
+        ("light_position", f3),
+        ("light_direction", f3),
+        ("light_cos_half_angle", C.c_float),
+        ("light_view_projection", f16),
+        ("inverse_light_view_projection", f16),
+        ("rsm_render_resolution", C.c_int32),
+        ("rsm_read_resolution", C.c_int32),
+        ("val_area_factor", C.c_float),
+        ("indirect_shadow_lod", C.c_float),
+        ("indirect_shadow_block_size", C.c_float),
+        ("indirect_shadow_sample_interval", C.c_int32),
+        ("indirect_shadow_super_val_width", C.c_float),
+        ("indirect_shadow_sampling_offset", C.c_float),
+    ]
+
+
+class DrvGBuffer(C.Structure):
+    _fields_ = [
+        ("depth", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("albedo", C.c_void_p),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+    ]
+
+
+class DrvRsm(C.Structure):
+    _fields_ = [
+        ("flux", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("depth_lin_sq", C.c_void_p),
+        ("depth_lin_sq_shadow", C.c_void_p),
+        ("read_resolution", C.c_int32),
+    ]
+
+
+class DrvStageTimes(C.Structure):
+    _fields_ = [
+        (n, C.c_float)
+        for n in ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
+                  "gather_ms", "reduce_ms", "apply_ms")
+    ] + [("cone_traces", C.c_uint64), ("cone_steps", C.c_uint64)]
+
+
+P = C.POINTER
+vp = C.c_void_p
+
+# name -> (restype, argtypes); every entry point declared in include/drv.h.
+DRV_FUNCTIONS = {
+    "drv_create": (C.c_int, [P(DrvConfig), P(vp)]),
+    "drv_destroy": (C.c_int, [vp]),
+    "drv_last_error": (C.c_char_p, [vp]),
+    "drv_set_stream": (C.c_int, [vp, vp]),
+    "drv_synchronize": (C.c_int, [vp]),
+    "drv_get_sh_constants": (C.c_int, [vp, P(DrvShConstants)]),
+    "drv_set_frame": (C.c_int, [vp, P(DrvFrame)]),
+    "drv_update_volume": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_float)]),
+    "drv_set_volume": (C.c_int, [vp, P(DrvVolume)]),
+    "drv_get_volume": (C.c_int, [vp, P(DrvVolume)]),
+    "drv_prepare_spot_light": (C.c_int, [P(DrvSpotParams), P(DrvSpotLight)]),
+    "drv_voxelize_begin": (C.c_int, [vp]),
+    "drv_voxelize_mesh": (C.c_int, [vp, vp, vp, C.c_uint32, P(C.c_float)]),
+    "drv_voxelize_end": (C.c_int, [vp, C.c_float]),
+    "drv_allocate_caches": (C.c_int, [vp, P(DrvGBuffer)]),
+    "drv_light_caches": (C.c_int, [vp, P(DrvRsm), P(DrvSpotLight)]),
+    "drv_apply_caches": (C.c_int, [vp, P(DrvGBuffer), vp, C.c_int32]),
+    "drv_get_cache_count": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32)]),
+    "drv_read_caches": (C.c_int, [vp, vp, C.c_uint32]),
+    "drv_read_cav": (C.c_int, [vp, vp]),
+    "drv_read_voxels": (C.c_int, [vp, C.c_int32, vp]),
+    "drv_read_visibility": (C.c_int, [vp, vp, C.c_uint64]),
+    "drv_set_profiling": (C.c_int, [vp, C.c_int32]),
+    "drv_get_stage_times": (C.c_int, [vp, P(DrvStageTimes)]),
+}
+
+INPUT_FUNCTIONS = {
+    "drvin_scene_build": (C.c_int64, [C.c_int32, C.c_uint64, C.c_int32, C.c_int32, vp, vp, vp, C.c_int64, vp]),
+    "drvin_camera": (C.c_int, [P(C.c_float), P(C.c_float), C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]),
+    "drvin_raster_gbuffer": (C.c_int, [vp, vp, vp, C.c_int64, vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int32]),
+    "drvin_raster_rsm": (C.c_int, [vp, vp, vp, C.c_int64, P(DrvSpotLight), vp, vp, vp, C.c_int32]),
+    "drvin_rsm_downsample": (C.c_int, [C.c_int32, vp, vp, vp, vp, vp, vp]),
+    "drvin_f32_to_f16": (C.c_uint16, [C.c_float]),
+    "drvin_f16_to_f32": (C.c_float, [C.c_uint16]),
+    "drvin_pack_normal16i": (None, [P(C.c_float), P(C.c_int16)]),
+}
+
+
+def _bind(lib: C.CDLL, table: dict) -> C.CDLL:
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_drv = None
+_inputs = None
+
+
+def load_drv() -> C.CDLL:
+    """Loads the HIP pipeline library; raises if it is missing."""
+    global _drv
+    if _drv is None:
+        if not LIBDRV.exists():
+            raise RuntimeError(
+                f"{LIBDRV} is not built: run `python -m dynamicradiancevolume_amd.build` "
+                "(there is no non-HIP fallback)"
+            )
+        _drv = _bind(C.CDLL(str(LIBDRV)), DRV_FUNCTIONS)
+    return _drv
+
+
+def load_inputs() -> C.CDLL:
+    global _inputs
+    if _inputs is None:
+        if not LIBINPUTS.exists():
+            raise RuntimeError(f"{LIBINPUTS} is not built: run `python -m dynamicradiancevolume_amd.build`")
+        _inputs = _bind(C.CDLL(str(LIBINPUTS)), INPUT_FUNCTIONS)
+    return _inputs
+
+
+def fvec(values, n: int):
+    arr = (C.c_float * n)(*[float(v) for v in values])
+    return arr

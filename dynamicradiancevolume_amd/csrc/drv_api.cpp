@@ -1,0 +1,594 @@
+// C ABI of the pipeline: the host side of Renderer (rendering/renderer.cpp)
+// and Voxelization (rendering/voxelization.cpp) re-expressed over HIP device
+// buffers and one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/drv.h"
+#include "drv_internal.hpp"
+#include "mat4.hpp"
+
+using drvk::Counter;
+
+struct drv_ctx {
+  drv_config cfg{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int cu_count = 256;
+  std::string err;
+
+  drv_sh_constants shc{};
+  drv_frame frame{};
+  drv_volume vol{};
+  bool have_frame = false;
+  bool have_volume = false;
+  bool allocated = false;  // drv_allocate_caches ran since creation
+
+  float* caches = nullptr;
+  Counter* counter = nullptr;
+  uint32_t* cav = nullptr;
+  size_t cav_cells = 0;
+  uint8_t* vox_target = nullptr;
+  uint8_t* vox_mips = nullptr;
+  uint64_t* vox_corners = nullptr;
+  drvk::VoxelLayout vox;
+  float* srgb_lut = nullptr;
+
+  drvk::Vpl* vpls = nullptr;
+  size_t vpl_cap = 0;
+  drvk::ShadowBlock* blocks = nullptr;
+  size_t block_cap = 0;
+  float* visibility = nullptr;
+  size_t vis_blocks_cap = 0;  // blocks the visibility buffer holds (pitch = max_caches)
+  float* partial = nullptr;
+  uint32_t last_n_blocks = 0;
+  bool counted_steps = false;
+
+  bool profiling = false;
+  int profiling_level = 0;
+  uint64_t* step_counter = nullptr;  // [traces, steps] when counting
+  hipEvent_t ev[16] = {};
+  drv_stage_times times{};
+};
+
+namespace {
+
+constexpr uint32_t kMaxSplits = 32;
+
+int fail(drv_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int check_hip(drv_ctx* ctx, hipError_t e, const char* what) {
+  if (e == hipSuccess) return DRV_OK;
+  return fail(ctx, DRV_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define DRV_HIP(ctx, expr)                                   \
+  do {                                                       \
+    int _rc = check_hip((ctx), (expr), #expr);               \
+    if (_rc != DRV_OK) return _rc;                           \
+  } while (0)
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int log2i(int v) {
+  int l = 0;
+  while ((1 << (l + 1)) <= v) ++l;
+  return l;
+}
+
+template <typename T>
+int dev_alloc(drv_ctx* ctx, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return DRV_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) return fail(ctx, DRV_ERR_NO_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return DRV_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Host matrices of PrepareLights (renderer.cpp:687-692): camera at the light,
+// reverse-Z perspective with fov = 2 * halfAngle and aspect 1.
+void light_matrices(const drv_spot_params& p, float* vp_out, float* ivp_out) {
+  double eye[3] = {p.position[0], p.position[1], p.position[2]};
+  double tgt[3] = {eye[0] + p.direction[0], eye[1] + p.direction[1], eye[2] + p.direction[2]};
+  double up[3] = {0.0, 1.0, 0.0};
+  double view[16], proj[16], vp[16], ivp[16];
+  drvm::look_at(eye, tgt, up, view);
+  drvm::perspective_reverse_z(2.0 * p.half_angle, 1.0, p.near_plane, p.far_plane, proj);
+  drvm::mat4_mul(proj, view, vp);
+  drvm::mat4_invert(vp, ivp);
+  drvm::to_float(vp, vp_out);
+  drvm::to_float(ivp, ivp_out);
+}
+
+void record(drv_ctx* ctx, int i) {
+  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
+float elapsed(drv_ctx* ctx, int a, int b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.0f;
+  return ms;
+}
+
+drvk::VoxelView voxel_view(const drv_ctx* ctx) {
+  drvk::VoxelView v{};
+  v.corners = ctx->vox_corners;
+  v.res = ctx->vox.res;
+  v.levels = ctx->vox.levels;
+  for (int l = 0; l < ctx->vox.levels && l < 16; ++l) v.corner_offset[l] = static_cast<uint32_t>(ctx->vox.corner_offset[l]);
+  return v;
+}
+
+int persistent_grid(const drv_ctx* ctx) { return ctx->cu_count * 8; }
+
+}  // namespace
+
+extern "C" {
+
+int drv_create(const drv_config* config, drv_ctx** out_ctx) {
+  if (!config || !out_ctx) return DRV_ERR_INVALID_ARG;
+  *out_ctx = nullptr;
+  const drv_config& c = *config;
+  if (c.width <= 0 || c.height <= 0 || c.max_caches <= 0 || c.cav_resolution <= 0 || c.cav_cascades <= 0 ||
+      c.cav_cascades > DRV_MAX_CASCADES || !is_pow2(c.voxel_resolution) || c.voxel_resolution < 2 ||
+      c.voxel_resolution > 1024 || (c.sh_bands != 1 && c.sh_bands != 2) || c.cav_cascade0_world_size <= 0.0f)
+    return DRV_ERR_INVALID_ARG;
+  // Address arithmetic of the CAV is 32-bit (cacheGather.comp:25-28).
+  if (static_cast<double>(c.cav_cascades) * c.cav_resolution * c.cav_resolution * c.cav_resolution > 2147483647.0)
+    return DRV_ERR_INVALID_ARG;
+  drv_ctx* ctx = new (std::nothrow) drv_ctx();
+  if (!ctx) return DRV_ERR_NO_MEMORY;
+  ctx->cfg = c;
+  int rc = DRV_OK;
+  auto bail = [&](int code) {
+    drv_destroy(ctx);
+    return code;
+  };
+  if (hipGetDevice(&ctx->device) != hipSuccess) return bail(DRV_ERR_HIP);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->cu_count = prop.multiProcessorCount;
+
+  // UpdateConstantUBO SH factors (renderer.cpp:296-306).
+  const float pi = 3.14159265358979f;
+  drv_sh_constants& s = ctx->shc;
+  s.cos_lobe0 = std::sqrt(pi) / 2.0f;
+  s.cos_lobe1 = std::sqrt(pi / 3.0f);
+  s.cos_lobe2n2_p1_n1 = -std::sqrt(15.0f * pi) / 8.0f;
+  s.cos_lobe20 = std::sqrt(5.0f * pi) / 16.0f;
+  s.cos_lobe2p2 = std::sqrt(15.0f * pi) / 16.0f;
+  s.eva0 = 1.0f / (2.0f * std::sqrt(pi));
+  s.eva1 = std::sqrt(3.0f) / (2.0f * std::sqrt(pi));
+  s.eva2n2_p1_n1 = std::sqrt(15.0f / (4.0f * pi));
+  s.eva20 = std::sqrt(5.0f / (16.0f * pi));
+  s.eva2p2 = std::sqrt(15.0f / (16.0f * pi));
+
+  // AllocateCacheData (renderer.cpp:262-276): the buffer holds max_caches
+  // entries of the active layout.
+  const size_t stride = static_cast<size_t>(drvk::entry_stride(c.sh_bands));
+  if ((rc = dev_alloc(ctx, &ctx->caches, stride * c.max_caches)) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->counter, 1)) != DRV_OK) return bail(rc);
+  // SetCAVCascades: R32UI atlas of (C * R) x R x R (renderer.cpp:1179).
+  ctx->cav_cells = static_cast<size_t>(c.cav_cascades) * c.cav_resolution * c.cav_resolution * c.cav_resolution;
+  if ((rc = dev_alloc(ctx, &ctx->cav, ctx->cav_cells)) != DRV_OK) return bail(rc);
+
+  // Voxelization::SetResolution (voxelization.cpp:56-60): R8 target plus an
+  // R8 volume with its full mip chain, plus the corner-expanded copy.
+  drvk::VoxelLayout& L = ctx->vox;
+  L.res = c.voxel_resolution;
+  L.levels = log2i(L.res) + 1;
+  L.mip_offset.resize(L.levels);
+  L.corner_offset.resize(L.levels);
+  for (int l = 0; l < L.levels; ++l) {
+    const size_t w = static_cast<size_t>(L.res >> l);
+    L.mip_offset[l] = L.mip_bytes;
+    L.mip_bytes += w * w * w;
+    L.corner_offset[l] = L.corner_count;
+    L.corner_count += (w + 1) * (w + 1) * (w + 1);
+  }
+  const size_t res3 = static_cast<size_t>(L.res) * L.res * L.res;
+  if ((rc = dev_alloc(ctx, &ctx->vox_target, res3)) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->vox_mips, L.mip_bytes)) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->vox_corners, L.corner_count)) != DRV_OK) return bail(rc);
+
+  // SRGB8 decode table.
+  float lut[256];
+  for (int i = 0; i < 256; ++i) {
+    const double v = i / 255.0;
+    lut[i] = static_cast<float>(v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4));
+  }
+  if ((rc = dev_alloc(ctx, &ctx->srgb_lut, 256)) != DRV_OK) return bail(rc);
+  if (hipMemcpy(ctx->srgb_lut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) return bail(DRV_ERR_HIP);
+
+  // Zero-initialize the state a frame may read before writing it.
+  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->counter), sizeof(Counter) / 4, 0u, nullptr);
+  drvk::launch_fill_u32(ctx->cav, ctx->cav_cells, 0u, nullptr);
+  drvk::launch_fill_u8(ctx->vox_target, res3, 0, nullptr);
+  drvk::launch_fill_u8(ctx->vox_mips, L.mip_bytes, 0, nullptr);
+  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->vox_corners), L.corner_count * 2, 0u, nullptr);
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(DRV_ERR_HIP);
+  if (hipDeviceSynchronize() != hipSuccess) return bail(DRV_ERR_HIP);
+  *out_ctx = ctx;
+  return DRV_OK;
+}
+
+int drv_destroy(drv_ctx* ctx) {
+  if (!ctx) return DRV_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  dev_free(ctx->caches);
+  dev_free(ctx->counter);
+  dev_free(ctx->cav);
+  dev_free(ctx->vox_target);
+  dev_free(ctx->vox_mips);
+  dev_free(ctx->vox_corners);
+  dev_free(ctx->srgb_lut);
+  dev_free(ctx->vpls);
+  dev_free(ctx->blocks);
+  dev_free(ctx->visibility);
+  dev_free(ctx->partial);
+  dev_free(ctx->step_counter);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete ctx;
+  return DRV_OK;
+}
+
+const char* drv_last_error(const drv_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int drv_set_stream(drv_ctx* ctx, void* hip_stream) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  ctx->stream = static_cast<hipStream_t>(hip_stream);
+  return DRV_OK;
+}
+
+int drv_synchronize(drv_ctx* ctx) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_get_sh_constants(const drv_ctx* ctx, drv_sh_constants* out) {
+  if (!ctx || !out) return DRV_ERR_INVALID_ARG;
+  *out = ctx->shc;
+  return DRV_OK;
+}
+
+int drv_set_frame(drv_ctx* ctx, const drv_frame* frame) {
+  if (!ctx || !frame) return DRV_ERR_INVALID_ARG;
+  ctx->frame = *frame;
+  ctx->have_frame = true;
+  return DRV_OK;
+}
+
+int drv_update_volume(drv_ctx* ctx, const float cam[3], const float smin[3], const float smax[3]) {
+  if (!ctx || !cam || !smin || !smax) return DRV_ERR_INVALID_ARG;
+  const drv_config& cfg = ctx->cfg;
+  drv_volume& out = ctx->vol;
+  std::memset(&out, 0, sizeof(out));
+  // renderer.cpp:350-360: cubic voxel volume around the scene bounds.
+  float vmin[3], vmax[3], ext[3];
+  for (int i = 0; i < 3; ++i) {
+    vmin[i] = smin[i] - 0.001f;
+    vmax[i] = smax[i] + 0.001f;
+    ext[i] = vmax[i] - vmin[i];
+  }
+  const float largest = std::max(std::max(ext[0], ext[1]), ext[2]);
+  for (int i = 0; i < 3; ++i) {
+    vmax[i] = vmax[i] + (largest - ext[i]);
+    out.volume_world_min[i] = vmin[i];
+    out.volume_world_max[i] = vmax[i];
+  }
+  out.voxel_size_in_world = (vmax[0] - vmin[0]) / static_cast<float>(cfg.voxel_resolution);
+  out.cav_transition_zone_size = cfg.cav_transition_size;
+  // renderer.cpp:366-428: camera-centred cascades, snapped to their cell
+  // size; the decision box is unsnapped and shrunk by 1.5 cells.
+  float size = cfg.cav_cascade0_world_size;
+  for (int c = 0; c < cfg.cav_cascades; ++c) {
+    const float cell = size / static_cast<float>(cfg.cav_resolution);
+    drv_cascade& k = out.cascades[c];
+    for (int i = 0; i < 3; ++i) {
+      const float snapped = std::round(cam[i] / cell) * cell;
+      k.min[i] = snapped - size * 0.5f;
+      k.max[i] = snapped + size * 0.5f;
+      k.decision_min[i] = cam[i] - size * 0.5f + cell * 1.5f;
+      k.decision_max[i] = cam[i] + size * 0.5f - cell * 1.5f;
+    }
+    k.world_voxel_size = cell;
+    size = size * 2.0f;
+  }
+  ctx->have_volume = true;
+  return DRV_OK;
+}
+
+int drv_set_volume(drv_ctx* ctx, const drv_volume* volume) {
+  if (!ctx || !volume) return DRV_ERR_INVALID_ARG;
+  ctx->vol = *volume;
+  ctx->have_volume = true;
+  return DRV_OK;
+}
+
+int drv_get_volume(const drv_ctx* ctx, drv_volume* out) {
+  if (!ctx || !out) return DRV_ERR_INVALID_ARG;
+  *out = ctx->vol;
+  return DRV_OK;
+}
+
+int drv_prepare_spot_light(const drv_spot_params* p, drv_spot_light* o) {
+  if (!p || !o || p->rsm_resolution == 0 || !(p->half_angle > 0.0f) || !(p->near_plane > 0.0f) ||
+      !(p->far_plane > p->near_plane))
+    return DRV_ERR_INVALID_ARG;
+  const float dl = std::sqrt(p->direction[0] * p->direction[0] + p->direction[1] * p->direction[1] +
+                             p->direction[2] * p->direction[2]);
+  if (!(dl > 0.0f)) return DRV_ERR_INVALID_ARG;
+  std::memset(o, 0, sizeof(*o));
+  for (int i = 0; i < 3; ++i) {
+    o->light_intensity[i] = p->intensity[i];
+    o->light_position[i] = p->position[i];
+    o->light_direction[i] = p->direction[i] / dl;
+  }
+  o->light_cos_half_angle = std::cos(p->half_angle);
+  light_matrices(*p, o->light_view_projection, o->inverse_light_view_projection);
+  // renderer.cpp:694-719
+  const int res_pow2 = 1 << static_cast<int>(std::ceil(std::log2(static_cast<double>(p->rsm_resolution))));
+  o->rsm_render_resolution = res_pow2;
+  const int read_res = static_cast<int>(res_pow2 / std::pow(2.0, static_cast<double>(p->rsm_read_lod)));
+  if (read_res < 1) return DRV_ERR_INVALID_ARG;
+  o->rsm_read_resolution = read_res;
+  const float clip_w = std::sin(p->half_angle) * p->near_plane * 2.0f;
+  const float vaf = clip_w * clip_w / (p->near_plane * p->near_plane * read_res * read_res);
+  o->val_area_factor = vaf;
+  o->indirect_shadow_lod = static_cast<float>(p->indirect_shadow_lod);
+  const float block = static_cast<float>(1 << p->indirect_shadow_lod);
+  if (block > static_cast<float>(read_res)) return DRV_ERR_INVALID_ARG;  // renderer.cpp:712
+  o->indirect_shadow_block_size = block;
+  o->indirect_shadow_sample_interval = static_cast<int>(block * block);
+  o->indirect_shadow_super_val_width = std::sqrt(vaf) * block;
+  o->indirect_shadow_sampling_offset = 0.5f + std::sqrt(2.0f) * block / 2.0f;
+  return DRV_OK;
+}
+
+// ---- stage B ----------------------------------------------------------
+
+int drv_voxelize_begin(drv_ctx* ctx) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  record(ctx, 0);
+  const size_t res3 = static_cast<size_t>(ctx->vox.res) * ctx->vox.res * ctx->vox.res;
+  drvk::launch_fill_u8(ctx->vox_target, res3, 0, ctx->stream);  // voxelization.cpp:105
+  return check_hip(ctx, hipGetLastError(), "voxelize_begin");
+}
+
+int drv_voxelize_mesh(drv_ctx* ctx, const float* positions, const uint32_t* indices, uint32_t n_tri,
+                      const float world[16]) {
+  if (!ctx || (!positions && n_tri) || !world) return DRV_ERR_INVALID_ARG;
+  if (!ctx->have_volume) return fail(ctx, DRV_ERR_STATE, "voxelize: volume not set");
+  drvk::launch_voxelize(ctx->vol, ctx->vox.res, positions, indices, n_tri, world, ctx->vox_target, ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "voxelize_mesh");
+}
+
+int drv_voxelize_end(drv_ctx* ctx, float adaption_factor) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  record(ctx, 1);
+  // voxelization.cpp:100: nothing is blended or mipmapped when the floored
+  // adaption is zero.
+  if (adaption_factor > 0.0f) {
+    drvk::launch_voxel_blend(ctx->vox.res, ctx->vox_target, ctx->vox_mips, adaption_factor, ctx->stream);
+    drvk::launch_voxel_mips(ctx->vox, ctx->vox_mips, ctx->stream);
+    drvk::launch_voxel_corners(ctx->vox, ctx->vox_mips, ctx->vox_corners, ctx->stream);
+  }
+  record(ctx, 2);
+  return check_hip(ctx, hipGetLastError(), "voxelize_end");
+}
+
+// ---- stage A ----------------------------------------------------------
+
+int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
+  if (!ctx || !gb || !gb->depth) return DRV_ERR_INVALID_ARG;
+  if (gb->width != ctx->cfg.width || gb->height != ctx->cfg.height)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "allocate: G-buffer size differs from the configured backbuffer");
+  if (!ctx->have_frame || !ctx->have_volume) return fail(ctx, DRV_ERR_STATE, "allocate: frame or volume not set");
+  record(ctx, 3);
+  // renderer.cpp:969-970: clear the counter and the address volume.
+  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->counter), sizeof(Counter) / 4, 0u, ctx->stream);
+  drvk::launch_fill_u32(ctx->cav, ctx->cav_cells, 0u, ctx->stream);
+  drvk::launch_allocate(ctx->cfg, ctx->vol, ctx->frame, gb->depth, ctx->cav, ctx->caches, ctx->counter, ctx->stream);
+  drvk::launch_prepare_dispatch(ctx->counter, static_cast<uint32_t>(ctx->cfg.max_caches), ctx->stream);
+  record(ctx, 4);
+  ctx->allocated = true;
+  return check_hip(ctx, hipGetLastError(), "allocate");
+}
+
+// ---- stage C ----------------------------------------------------------
+
+int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light) {
+  if (!ctx || !rsm || !light || !rsm->flux || !rsm->normal || !rsm->depth_lin_sq) return DRV_ERR_INVALID_ARG;
+  if (!ctx->allocated) return fail(ctx, DRV_ERR_STATE, "light_caches: no allocation yet");
+  const int R = rsm->read_resolution;
+  // The Morton decode of cacheLightingRSM.comp:142 needs a square
+  // power-of-two read resolution; the 64-VPL chunking needs R >= 8.
+  if (!is_pow2(R) || R < 8 || R > 65536 || R != light->rsm_read_resolution)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: read resolution must be a power of two >= 8 matching the light");
+  const uint32_t n_vpl = static_cast<uint32_t>(R) * static_cast<uint32_t>(R);
+  const bool shadow = ctx->cfg.indirect_shadow != 0;
+  const uint32_t interval = static_cast<uint32_t>(light->indirect_shadow_sample_interval);
+  const int lod = static_cast<int>(light->indirect_shadow_lod);
+  if (shadow) {
+    if (!rsm->depth_lin_sq_shadow) return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: shadow LOD level missing");
+    if (interval == 0 || interval != (1u << (2 * lod)) || (R >> lod) < 1)
+      return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: inconsistent indirect shadow LOD");
+  }
+  const uint32_t n_blocks = shadow ? n_vpl / interval : 0u;
+  int rc;
+  if (n_vpl > ctx->vpl_cap) {
+    dev_free(ctx->vpls);
+    if ((rc = dev_alloc(ctx, &ctx->vpls, n_vpl)) != DRV_OK) return rc;
+    ctx->vpl_cap = n_vpl;
+  }
+  if (shadow && n_blocks > ctx->block_cap) {
+    dev_free(ctx->blocks);
+    if ((rc = dev_alloc(ctx, &ctx->blocks, n_blocks)) != DRV_OK) return rc;
+    ctx->block_cap = n_blocks;
+  }
+  const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
+  if (shadow && n_blocks > ctx->vis_blocks_cap) {
+    dev_free(ctx->visibility);
+    if ((rc = dev_alloc(ctx, &ctx->visibility, cap * n_blocks)) != DRV_OK) return rc;
+    ctx->vis_blocks_cap = n_blocks;
+  }
+  if (!ctx->partial && (rc = dev_alloc(ctx, &ctx->partial, cap * 27 * kMaxSplits)) != DRV_OK) return rc;
+
+  const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
+  const int grid = persistent_grid(ctx);
+  uint64_t* steps = nullptr;
+  if (ctx->profiling && ctx->profiling_level >= 2) {
+    if (!ctx->step_counter && (rc = dev_alloc(ctx, &ctx->step_counter, 2)) != DRV_OK) return rc;
+    drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->step_counter), 4, 0u, ctx->stream);
+    steps = ctx->step_counter;
+  }
+  record(ctx, 5);
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, ctx->stream);
+  record(ctx, 6);
+  if (shadow)
+    drvk::launch_visibility(ctx->vol, voxel_view(ctx), ctx->caches, stride, ctx->counter, ctx->blocks, n_blocks,
+                            ctx->visibility, static_cast<uint32_t>(cap), steps, grid, ctx->stream);
+  record(ctx, 7);
+  drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
+                      ctx->visibility, static_cast<uint32_t>(cap), shadow ? interval : 64u, shadow ? 1 : 0,
+                      ctx->partial, static_cast<uint32_t>(cap), grid, ctx->stream);
+  record(ctx, 8);
+  drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->partial,
+                         static_cast<uint32_t>(cap), ctx->cu_count * 4, ctx->stream);
+  record(ctx, 9);
+  ctx->counted_steps = steps != nullptr;
+  ctx->last_n_blocks = n_blocks;
+  return check_hip(ctx, hipGetLastError(), "light_caches");
+}
+
+// ---- stage D ----------------------------------------------------------
+
+int drv_apply_caches(drv_ctx* ctx, const drv_gbuffer* gb, float* out_rgb, int32_t accumulate) {
+  if (!ctx || !gb || !gb->depth || !gb->normal || !gb->albedo || !out_rgb) return DRV_ERR_INVALID_ARG;
+  if (gb->width != ctx->cfg.width || gb->height != ctx->cfg.height)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "apply: G-buffer size differs from the configured backbuffer");
+  if (!ctx->allocated) return fail(ctx, DRV_ERR_STATE, "apply: no allocation yet");
+  record(ctx, 10);
+  drvk::launch_apply(ctx->cfg, ctx->vol, ctx->frame, ctx->shc, *gb, ctx->cav, ctx->caches, ctx->counter,
+                     ctx->srgb_lut, out_rgb, accumulate, ctx->stream);
+  record(ctx, 11);
+  return check_hip(ctx, hipGetLastError(), "apply");
+}
+
+// ---- introspection ------------------------------------------------------
+
+int drv_get_cache_count(drv_ctx* ctx, uint32_t* demanded, uint32_t* live) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  Counter c{};
+  DRV_HIP(ctx, hipMemcpyAsync(&c, ctx->counter, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (demanded) *demanded = c.total;
+  if (live) *live = std::min<uint32_t>(c.total, static_cast<uint32_t>(ctx->cfg.max_caches));
+  return DRV_OK;
+}
+
+int drv_read_caches(drv_ctx* ctx, float* host_out, uint32_t count) {
+  if (!ctx || (!host_out && count)) return DRV_ERR_INVALID_ARG;
+  if (count > static_cast<uint32_t>(ctx->cfg.max_caches)) return fail(ctx, DRV_ERR_INVALID_ARG, "read_caches: count > capacity");
+  const size_t bytes = sizeof(float) * drvk::entry_stride(ctx->cfg.sh_bands) * count;
+  if (bytes) DRV_HIP(ctx, hipMemcpyAsync(host_out, ctx->caches, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_read_cav(drv_ctx* ctx, uint32_t* host_out) {
+  if (!ctx || !host_out) return DRV_ERR_INVALID_ARG;
+  DRV_HIP(ctx, hipMemcpyAsync(host_out, ctx->cav, ctx->cav_cells * 4, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_read_voxels(drv_ctx* ctx, int32_t level, uint8_t* host_out) {
+  if (!ctx || !host_out) return DRV_ERR_INVALID_ARG;
+  if (level < -1 || level >= ctx->vox.levels) return fail(ctx, DRV_ERR_INVALID_ARG, "read_voxels: bad level");
+  // level -1 reads the voxelization target (before the blend).
+  const uint8_t* src = level < 0 ? ctx->vox_target : ctx->vox_mips + ctx->vox.mip_offset[level];
+  const size_t w = static_cast<size_t>(ctx->vox.res >> (level < 0 ? 0 : level));
+  DRV_HIP(ctx, hipMemcpyAsync(host_out, src, w * w * w, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_read_visibility(drv_ctx* ctx, float* host_out, uint64_t count) {
+  if (!ctx || (!host_out && count)) return DRV_ERR_INVALID_ARG;
+  uint32_t total = 0, live = 0;
+  int rc = drv_get_cache_count(ctx, &total, &live);
+  if (rc != DRV_OK) return rc;
+  const uint64_t nb = ctx->last_n_blocks;
+  if (count > static_cast<uint64_t>(live) * nb) return fail(ctx, DRV_ERR_INVALID_ARG, "read_visibility: count too large");
+  if (nb == 0 || count == 0) return DRV_OK;
+  // Device layout is block-major with pitch max_caches; the host view is
+  // cache-major [live][n_blocks].
+  const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
+  std::vector<float> buf(static_cast<size_t>(nb) * live);
+  DRV_HIP(ctx, hipMemcpy2DAsync(buf.data(), sizeof(float) * live, ctx->visibility, sizeof(float) * cap,
+                                sizeof(float) * live, nb, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (uint64_t i = 0; i < count; ++i) {
+    const uint64_t c = i / nb, b = i % nb;
+    host_out[i] = buf[b * live + c];
+  }
+  return DRV_OK;
+}
+
+// ---- profiling ------------------------------------------------------------
+
+int drv_set_profiling(drv_ctx* ctx, int32_t enabled) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  ctx->profiling = enabled != 0;
+  ctx->profiling_level = enabled;
+  return DRV_OK;
+}
+
+int drv_get_stage_times(drv_ctx* ctx, drv_stage_times* out) {
+  if (!ctx || !out) return DRV_ERR_INVALID_ARG;
+  if (ctx->profiling) {
+    // Events of the last call of each stage (elapsed() yields 0 for a pair
+    // that was never recorded).
+    DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->times.voxelize_ms = elapsed(ctx, 0, 1);
+    ctx->times.voxel_mips_ms = elapsed(ctx, 1, 2);
+    ctx->times.allocate_ms = elapsed(ctx, 3, 4);
+    ctx->times.light_caches_ms = elapsed(ctx, 5, 9);
+    ctx->times.prepare_ms = elapsed(ctx, 5, 6);
+    ctx->times.visibility_ms = elapsed(ctx, 6, 7);
+    ctx->times.gather_ms = elapsed(ctx, 7, 8);
+    ctx->times.reduce_ms = elapsed(ctx, 8, 9);
+    ctx->times.apply_ms = elapsed(ctx, 10, 11);
+    if (ctx->counted_steps && ctx->step_counter) {
+      uint64_t st[2] = {0, 0};
+      DRV_HIP(ctx, hipMemcpy(st, ctx->step_counter, sizeof(st), hipMemcpyDeviceToHost));
+      ctx->times.cone_traces = st[0];
+      ctx->times.cone_steps = st[1];
+    }
+  }
+  *out = ctx->times;
+  return DRV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+// Stage D: cache-to-pixel interpolation.
+//
+// Restates shader/cacheApply.frag:28-195 with SampleCacheIrradiance
+// (shader/lightcache.glsl:137-183), drawn by Renderer::ApplyCaches
+// (rendering/renderer.cpp:1047-1079) as a full-screen pass with GL_ONE/GL_ONE
+// blending into the HDR target.
+//
+// MI355X design: one lane per pixel over 16x16 pixel tiles (a 256-thread
+// workgroup), so the 8 corner cells of neighbouring pixels share CAV lines and
+// cache entries in L2.  Addresses that are unclaimed (CAV 0 -> 0xFFFFFFFF) or
+// past the live count read as zero, as robust buffer access does in GL.
+#include "drv_internal.hpp"
+#include "k_common.hpp"
+
+namespace drvk {
+namespace {
+
+struct ApplyArgs {
+  drv_volume vol;
+  float inv_vp[16];
+  drv_sh_constants shc;
+  const float* depth;
+  const int16_t* normal;
+  const uint8_t* albedo;
+  const uint32_t* cav;
+  const float* caches;
+  const Counter* counter;
+  const float* srgb_lut;
+  float* out;
+  int width, height, R, C, stride, sh2, transitions, accumulate;
+};
+
+// SampleCacheIrradiance, lightcache.glsl:137-183.
+__device__ __forceinline__ void sample_cache(const ApplyArgs& a, uint32_t addr, uint32_t live, f3 n, float* irr) {
+  if (addr >= live) {
+    irr[0] = irr[1] = irr[2] = 0.0f;
+    return;
+  }
+  const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(addr) * a.stride);
+  const float4 q1 = e4[1], q2 = e4[2], q3 = e4[3];  // {SH1neg1, SH00_r}, {SH10, SH00_g}, {SH1pos1, SH00_b}
+  const float sh00[3] = {q1.w, q2.w, q3.w};
+  const float s1n1[3] = {q1.x, q1.y, q1.z};
+  const float s10[3] = {q2.x, q2.y, q2.z};
+  const float s1p1[3] = {q3.x, q3.y, q3.z};
+  const float CL0 = a.shc.cos_lobe0, CL1 = a.shc.cos_lobe1;
+  float v[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    v[ch] = sh00[ch] * CL0;
+    v[ch] = v[ch] - s1n1[ch] * (CL1 * n.y);
+    v[ch] = v[ch] + s10[ch] * (CL1 * n.z);
+    v[ch] = v[ch] - s1p1[ch] * (CL1 * n.x);
+  }
+  if (a.sh2) {
+    const float4 q4 = e4[4], q5 = e4[5], q6 = e4[6], q7 = e4[7];
+    const float s2n2[3] = {q4.x, q4.y, q4.z};
+    const float s2n1[3] = {q5.x, q5.y, q5.z};
+    const float s20[3] = {q4.w, q5.w, q6.w};
+    const float s2p1[3] = {q6.x, q6.y, q6.z};
+    const float s2p2[3] = {q7.x, q7.y, q7.z};
+    const float CL2 = a.shc.cos_lobe2n2_p1_n1, CL20 = a.shc.cos_lobe20, CL22 = a.shc.cos_lobe2p2;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      v[ch] = v[ch] - s2n2[ch] * (CL2 * n.x * n.y);
+      v[ch] = v[ch] + s2n1[ch] * (CL2 * n.y * n.z);
+      v[ch] = v[ch] + s20[ch] * (CL20 * (n.z * n.z * 3.0f - 1.0f));
+      v[ch] = v[ch] + s2p1[ch] * (CL2 * n.x * n.z);
+      v[ch] = v[ch] + s2p2[ch] * (CL22 * (n.x * n.x - n.y * n.y));
+    }
+  }
+  irr[0] = fmaxf(v[0], 0.0f);
+  irr[1] = fmaxf(v[1], 0.0f);
+  irr[2] = fmaxf(v[2], 0.0f);
+}
+
+// ComputeLightingFromCaches, cacheApply.frag:28-118.
+__device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c, const float* diffuse, uint32_t live,
+                                         float* out) {
+  const drv_cascade& k = a.vol.cascades[c];
+  const float acx = (wp.x - k.min[0]) / k.world_voxel_size;
+  const float acy = (wp.y - k.min[1]) / k.world_voxel_size;
+  const float acz = (wp.z - k.min[2]) / k.world_voxel_size;
+  const int ax = to_int(acx), ay = to_int(acy), az = to_int(acz);
+  const float ix = acx - static_cast<float>(ax), iy = acy - static_cast<float>(ay), iz = acz - static_cast<float>(az);
+  const float jx = 1.0f - ix, jy = 1.0f - iy, jz = 1.0f - iz;
+  const float w[8] = {jx * jy * jz, ix * jy * jz, jx * iy * jz, ix * iy * jz,
+                      jx * jy * iz, ix * jy * iz, jx * iy * iz, ix * iy * iz};
+  const int R = a.R, CR = a.C * a.R;
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    // offsets x fastest: (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),(1,0,1),(0,1,1),(1,1,1)
+    const int x = ax + (i & 1) + R * c, y = ay + ((i >> 1) & 1), z = az + (i >> 2);
+    uint32_t addr = 0u;
+    if (x >= 0 && y >= 0 && z >= 0 && x < CR && y < R && z < R) addr = a.cav[(static_cast<size_t>(z) * R + y) * CR + x];
+    addr -= 1u;
+    float irr[3];
+    sample_cache(a, addr, live, n, irr);
+    acc[0] = acc[0] + irr[0] * w[i];
+    acc[1] = acc[1] + irr[1] * w[i];
+    acc[2] = acc[2] + irr[2] * w[i];
+  }
+  out[0] = acc[0] * diffuse[0] / kPi;
+  out[1] = acc[1] * diffuse[1] / kPi;
+  out[2] = acc[2] * diffuse[2] / kPi;
+}
+
+__global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= a.width || y >= a.height) return;
+  const size_t p = static_cast<size_t>(y) * a.width + x;
+  float* o = a.out + 3 * p;
+  const float d = a.depth[p];
+  if (d < 0.00001f) {  // cacheApply.frag:127-131: discard
+    if (!a.accumulate) o[0] = o[1] = o[2] = 0.0f;
+    return;
+  }
+  const uint32_t live = a.counter->live;
+  const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d);
+  const int c = cascade_of(a.vol, a.C, wp);
+  const f3 n = unpack_normal16i(a.normal[2 * p], a.normal[2 * p + 1]);
+  const float diffuse[3] = {a.srgb_lut[a.albedo[3 * p]], a.srgb_lut[a.albedo[3 * p + 1]],
+                            a.srgb_lut[a.albedo[3 * p + 2]]};
+  float col[3];
+  lighting(a, wp, n, c, diffuse, live, col);
+  if (a.transitions) {  // cacheApply.frag:172-185
+    const float t = cascade_transition(a.vol, c, wp);
+    if (t > 0.0f && c < a.C - 1) {
+      float col2[3];
+      lighting(a, wp, n, c + 1, diffuse, live, col2);
+      col[0] = col[0] * (1.0f - t) + col2[0] * t;
+      col[1] = col[1] * (1.0f - t) + col2[1] * t;
+      col[2] = col[2] * (1.0f - t) + col2[2] * t;
+    }
+  }
+  if (a.accumulate) {
+    o[0] += col[0];
+    o[1] += col[1];
+    o[2] += col[2];
+  } else {
+    o[0] = col[0];
+    o[1] = col[1];
+    o[2] = col[2];
+  }
+}
+
+}  // namespace
+
+void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const drv_sh_constants& shc,
+                  const drv_gbuffer& gb, const uint32_t* cav, const float* caches, const Counter* counter,
+                  const float* srgb_lut, float* out_rgb, int accumulate, hipStream_t s) {
+  ApplyArgs a;
+  a.vol = vol;
+  for (int i = 0; i < 16; ++i) a.inv_vp[i] = frame.inverse_view_projection[i];
+  a.shc = shc;
+  a.depth = gb.depth;
+  a.normal = gb.normal;
+  a.albedo = gb.albedo;
+  a.cav = cav;
+  a.caches = caches;
+  a.counter = counter;
+  a.srgb_lut = srgb_lut;
+  a.out = out_rgb;
+  a.width = cfg.width;
+  a.height = cfg.height;
+  a.R = cfg.cav_resolution;
+  a.C = cfg.cav_cascades;
+  a.stride = entry_stride(cfg.sh_bands);
+  a.sh2 = cfg.sh_bands >= 2 ? 1 : 0;
+  a.transitions = cfg.cav_transition_size > 0.0f ? 1 : 0;
+  a.accumulate = accumulate;
+  const dim3 grid((cfg.width + 15) / 16, (cfg.height + 15) / 16);
+  hipLaunchKernelGGL(k_apply, grid, dim3(256), 0, s, a);
+}
+
+}  // namespace drvk

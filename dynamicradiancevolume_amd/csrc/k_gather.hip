@@ -1,0 +1,447 @@
+// Stage C: per-cache VPL gather with cone-traced indirect shadows projected
+// onto spherical harmonics.
+//
+// Restates shader/cacheLightingRSM.comp:80-391 (SH1/SH2 with
+// INDIRECT_SHADOW) driven as in Renderer::LightCachesRSM
+// (rendering/renderer.cpp:899-933).
+//
+// MI355X design.  The reference runs one thread per cache and rebuilds the
+// 64-VPL LDS tile in every workgroup, then cone-traces inline once per block
+// of B^2 Morton-consecutive VPLs.  Here the pass is split into four kernels:
+//   1. k_prepare_vpls   - every VPL record (LightInfo) and every shadow block
+//                         record (averaged VAL position, cone aperture) is
+//                         built once per light into HBM;
+//   2. k_visibility     - one lane per (cache, shadow block) cone trace over
+//                         the corner-expanded voxel mips (one 8-byte load per
+//                         level per step), written block-major so both writes
+//                         and the gather's reads coalesce;
+//   3. k_gather         - persistent waves pull (64-cache tile, VPL split)
+//                         items; VPL records are wave-uniform, so they stream
+//                         through the scalar cache into SGPRs and every lane
+//                         accumulates 27 SH sums in VGPRs; SH constants are
+//                         folded in at write-out;
+//   4. k_reduce         - sums the per-split partials in a fixed order and
+//                         adds them into the cache entries (the reference's
+//                         `Entry.SH += accumulated`).
+// Splitting the VPL range fills 256 CUs even when there are only a few
+// thousand caches; the split count is derived on the device from the live
+// cache count, so no host round trip is needed.
+#include <algorithm>
+
+#include "drv_internal.hpp"
+#include "k_common.hpp"
+
+namespace drvk {
+namespace {
+
+struct PrepArgs {
+  float inv_lvp[16];
+  float light_pos[3];
+  float val_area_factor;
+  float sampling_offset;
+  float super_val_width;
+  const uint16_t* flux;
+  const int16_t* normal;
+  const uint16_t* dls;
+  const uint16_t* dls_shadow;
+  int R;
+  int R_shadow;
+  uint32_t interval;
+  Vpl* vpls;
+  ShadowBlock* blocks;
+};
+
+// World position of an RSM sample at distance d from the light
+// (cacheLightingRSM.comp:154-155, 175-176).
+__device__ __forceinline__ f3 rsm_position(const PrepArgs& a, float s, float t, float d) {
+  float clip[4];
+  mat_vec(a.inv_lvp, s * 2.0f - 1.0f, t * 2.0f - 1.0f, 0.0f, 1.0f, clip);
+  const float px = clip[0] / clip[3] - a.light_pos[0];
+  const float py = clip[1] / clip[3] - a.light_pos[1];
+  const float pz = clip[2] / clip[3] - a.light_pos[2];
+  const float l = sqrtf(px * px + py * py + pz * pz);
+  return make3(a.light_pos[0] + px / l * d, a.light_pos[1] + py / l * d, a.light_pos[2] + pz / l * d);
+}
+
+__device__ __forceinline__ float rg16f_texel(const uint16_t* img, int w, int i, int j, int comp) {
+  i = min(w - 1, max(0, i));
+  j = min(w - 1, max(0, j));
+  return half_to_float(img[2 * (static_cast<size_t>(j) * w + i) + comp]);
+}
+
+__global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
+  const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  uint32_t x, y;
+  morton_decode(m, x, y);
+  // cacheLightingRSM.comp:141-158
+  const float fR = static_cast<float>(a.R);
+  const float s = (static_cast<float>(x) + 0.5f) / fR;
+  const float t = (static_cast<float>(y) + 0.5f) / fR;
+  const size_t i = static_cast<size_t>(y) * a.R + x;
+  const float d = half_to_float(a.dls[2 * i]);
+  const f3 p = rsm_position(a, s, t, d);
+  const f3 nrm = unpack_normal16i(a.normal[2 * i], a.normal[2 * i + 1]);
+  Vpl v;
+  v.px = p.x;
+  v.py = p.y;
+  v.pz = p.z;
+  v.disc_area = d * d * a.val_area_factor;
+  v.fr = half_to_float(a.flux[3 * i]);
+  v.fg = half_to_float(a.flux[3 * i + 1]);
+  v.fb = half_to_float(a.flux[3 * i + 2]);
+  v.pad0 = 0.0f;
+  v.nx = nrm.x;
+  v.ny = nrm.y;
+  v.nz = nrm.z;
+  v.pad1 = 0.0f;
+  a.vpls[m] = v;
+  if (a.blocks && (m % a.interval) == 0u) {
+    // cacheLightingRSM.comp:171-192: bilinear (d, d^2) at the shadow LOD
+    const float ss = (static_cast<float>(x) + a.sampling_offset) / fR;
+    const float st = (static_cast<float>(y) + a.sampling_offset) / fR;
+    const int w = a.R_shadow;
+    const float u = ss * static_cast<float>(w) - 0.5f, vv = st * static_cast<float>(w) - 0.5f;
+    const float fu = floorf(u), fv = floorf(vv);
+    const int i0 = to_int(fu), j0 = to_int(fv);
+    const float al = u - fu, be = vv - fv;
+    float dd[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      dd[c] = (1.0f - al) * (1.0f - be) * rg16f_texel(a.dls_shadow, w, i0, j0, c) +
+              al * (1.0f - be) * rg16f_texel(a.dls_shadow, w, i0 + 1, j0, c) +
+              (1.0f - al) * be * rg16f_texel(a.dls_shadow, w, i0, j0 + 1, c) +
+              al * be * rg16f_texel(a.dls_shadow, w, i0 + 1, j0 + 1, c);
+    const f3 ap = rsm_position(a, ss, st, dd[0]);
+    const float variance = dd[1] - dd[0] * dd[0];
+    const float cand = sqrtf(variance) * 2.0f / dd[0];
+    // GLSL max(x, y) = (x < y) ? y : x, so a NaN candidate keeps SuperValWidth.
+    const float rho = (a.super_val_width < cand) ? cand : a.super_val_width;
+    a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho};
+  }
+}
+
+// ---- cone-traced visibility ---------------------------------------------
+
+struct VisArgs {
+  VoxelView vox;
+  float vmin[3];
+  float voxel_size;
+  const float* caches;
+  int stride;
+  const Counter* counter;
+  const ShadowBlock* blocks;
+  uint32_t n_blocks;
+  float* vis;
+  uint32_t pitch;
+  uint64_t* steps;  // optional {traces, steps}
+};
+
+constexpr uint32_t kVisBlocksPerItem = 8;
+
+// Trilinear sample of one mip level from the corner-expanded layout
+// (GL 4.5 section 8.14.2, clamp to edge).
+__device__ __forceinline__ float sample_level(const uint64_t* corners, int w, float px, float py, float pz) {
+  const float fw = static_cast<float>(w);
+  const float u = px * fw - 0.5f, v = py * fw - 0.5f, q = pz * fw - 0.5f;
+  const float fu = floorf(u), fv = floorf(v), fq = floorf(q);
+  const float ax = u - fu, ay = v - fv, az = q - fq;
+  const int i = min(w - 1, max(-1, to_int(fu)));
+  const int j = min(w - 1, max(-1, to_int(fv)));
+  const int k = min(w - 1, max(-1, to_int(fq)));
+  const int wp = w + 1;
+  const uint64_t c = corners[(static_cast<size_t>(k + 1) * wp + (j + 1)) * wp + (i + 1)];
+  const uint32_t lo = static_cast<uint32_t>(c), hi = static_cast<uint32_t>(c >> 32);
+  const float t000 = static_cast<float>(lo & 0xffu) / 255.0f;
+  const float t100 = static_cast<float>((lo >> 8) & 0xffu) / 255.0f;
+  const float t010 = static_cast<float>((lo >> 16) & 0xffu) / 255.0f;
+  const float t110 = static_cast<float>(lo >> 24) / 255.0f;
+  const float t001 = static_cast<float>(hi & 0xffu) / 255.0f;
+  const float t101 = static_cast<float>((hi >> 8) & 0xffu) / 255.0f;
+  const float t011 = static_cast<float>((hi >> 16) & 0xffu) / 255.0f;
+  const float t111 = static_cast<float>(hi >> 24) / 255.0f;
+  const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+  return bx * by * bz * t000 + ax * by * bz * t100 + bx * ay * bz * t010 + ax * ay * bz * t110 +
+         bx * by * az * t001 + ax * by * az * t101 + bx * ay * az * t011 + ax * ay * az * t111;
+}
+
+// textureLod(VoxelVolume, p, lod): LINEAR_MIPMAP_LINEAR, base level 0.
+__device__ __forceinline__ float sample_voxels(const VoxelView& vx, float px, float py, float pz, float lod) {
+  const int q = vx.levels - 1;
+  if (!(lod > 0.0f)) return sample_level(vx.corners, vx.res, px, py, pz);
+  if (lod >= static_cast<float>(q)) return sample_level(vx.corners + vx.corner_offset[q], vx.res >> q, px, py, pz);
+  const float fl = floorf(lod);
+  const int d1 = to_int(fl);
+  const float f = lod - fl;
+  const float t1 = sample_level(vx.corners + vx.corner_offset[d1], vx.res >> d1, px, py, pz);
+  const float t2 = sample_level(vx.corners + vx.corner_offset[d1 + 1], vx.res >> (d1 + 1), px, py, pz);
+  return (1.0f - f) * t1 + f * t2;
+}
+
+__device__ __forceinline__ float cone_trace(const VisArgs& a, f3 wp, f3 vpos, const ShadowBlock& b, int& n_steps) {
+  // cacheLightingRSM.comp:195-230
+  const float fres = static_cast<float>(a.vox.res);
+  float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
+  const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
+  tx = tx / ld;
+  ty = ty / ld;
+  tz = tz / ld;
+  const float dx = tx / fres, dy = ty / fres, dz = tz / fres;
+  float cx = vpos.x + dx * 2.0f, cy = vpos.y + dy * 2.0f, cz = vpos.z + dz * 2.0f;
+  float occ = 0.0f, step = 1.0f, dist = 0.0f;
+  const float goal = ld / a.voxel_size - 2.0f;
+  const float k = 2.0f / (1.0f - b.rho);
+  int s = 0;
+  for (; s < 32; ++s) {
+    cx = cx + dx * step;
+    cy = cy + dy * step;
+    cz = cz + dz * step;
+    dist = dist + step;
+    const float r = dist * b.rho;
+    const float o = sample_voxels(a.vox, cx, cy, cz, log2f(r));
+    occ = occ + (1.0f - occ) * o;
+    if (dist >= goal) break;
+    step = fmaxf(1.0f, r * k);
+  }
+  n_steps = min(s + 1, 32);
+  return fminf(1.0f, fmaxf(0.0f, 1.0f - occ));
+}
+
+__global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
+  const uint32_t live = a.counter->live;
+  const uint32_t tiles = (live + 63u) / 64u;
+  const uint32_t groups = (a.n_blocks + kVisBlocksPerItem - 1) / kVisBlocksPerItem;
+  const uint32_t items = tiles * groups;
+  const uint32_t lane = threadIdx.x & 63u;
+  // Wave-uniform work index (readfirstlane lets the compiler keep the item
+  // loop and the VPL / block addresses in SGPRs).
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const float vsz = a.voxel_size * static_cast<float>(a.vox.res);
+  uint32_t my_traces = 0, my_steps = 0;
+  for (uint32_t item = wave; item < items; item += n_waves) {
+    const uint32_t tile = item / groups, group = item % groups;
+    const uint32_t cache = tile * 64u + lane;
+    if (cache >= live) continue;
+    const float* e = a.caches + static_cast<size_t>(cache) * a.stride;
+    const f3 wp = make3(e[0], e[1], e[2]);
+    // cacheLightingRSM.comp:103-104
+    const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
+    const uint32_t b0 = group * kVisBlocksPerItem;
+    const uint32_t b1 = min(a.n_blocks, b0 + kVisBlocksPerItem);
+    for (uint32_t b = b0; b < b1; ++b) {
+      const ShadowBlock blk = a.blocks[b];
+      int st = 0;
+      a.vis[static_cast<size_t>(b) * a.pitch + cache] = cone_trace(a, wp, vpos, blk, st);
+      my_traces += 1u;
+      my_steps += static_cast<uint32_t>(st);
+    }
+  }
+  if (a.steps) {
+    // One atomic pair per wave after a butterfly sum.
+    uint64_t t = my_traces, sv = my_steps;
+    for (int off = 32; off > 0; off >>= 1) {
+      t += __shfl_xor(t, off);
+      sv += __shfl_xor(sv, off);
+    }
+    if (lane == 0 && (t | sv)) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[0]), static_cast<unsigned long long>(t));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[1]), static_cast<unsigned long long>(sv));
+    }
+  }
+}
+
+// ---- SH gather ----------------------------------------------------------
+
+__global__ void k_plan_gather(Counter* ctr, uint32_t n_vpl, uint32_t quantum, uint32_t max_splits,
+                              uint32_t target_items) {
+  const uint32_t tiles64 = (ctr->live + 63u) / 64u;
+  uint32_t s = 1u;
+  while (s < max_splits && tiles64 * s < target_items && (n_vpl / (2u * s)) >= quantum) s *= 2u;
+  ctr->splits = s;
+}
+
+template <bool kSh2, bool kShadow>
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches, int stride, const Counter* counter,
+                                                const Vpl* __restrict__ vpls, uint32_t n_vpl,
+                                                const float* __restrict__ vis, uint32_t vis_pitch,
+                                                uint32_t interval, float* __restrict__ partial,
+                                                uint32_t partial_pitch) {
+  const uint32_t live = counter->live;
+  const uint32_t splits = counter->splits;
+  const uint32_t tiles = (live + 63u) / 64u;
+  const uint32_t items = tiles * splits;
+  const uint32_t chunk = n_vpl / splits;
+  const uint32_t lane = threadIdx.x & 63u;
+  // Wave-uniform work index (readfirstlane lets the compiler keep the item
+  // loop and the VPL / block addresses in SGPRs).
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t item = wave; item < items; item += n_waves) {
+    const uint32_t tile = item / splits, split = item % splits;
+    const uint32_t cache = tile * 64u + lane;
+    const bool valid = cache < live;
+    const uint32_t c = valid ? cache : 0u;
+    const float* e = caches + static_cast<size_t>(c) * stride;
+    const float wx = e[0], wy = e[1], wz = e[2];
+    // Raw sums; the SH evaluation factors are applied at write-out.
+    float s0r = 0, s0g = 0, s0b = 0;     // sum rad              (SH00 / c0)
+    float s1r = 0, s1g = 0, s1b = 0;     // sum t.y rad          (-SH1neg1 / c1)
+    float s2r = 0, s2g = 0, s2b = 0;     // sum t.z rad          (SH10 / c1)
+    float s3r = 0, s3g = 0, s3b = 0;     // sum t.x rad          (-SH1pos1 / c1)
+    float s4r = 0, s4g = 0, s4b = 0;     // sum t.x t.y rad      (-SH2neg2 / c2)
+    float s5r = 0, s5g = 0, s5b = 0;     // sum t.y t.z rad      (SH2neg1 / c2)
+    float s6r = 0, s6g = 0, s6b = 0;     // sum t.z^2 rad        (SH20 = c20 (3 s6 - s0))
+    float s7r = 0, s7g = 0, s7b = 0;     // sum t.x t.z rad      (SH2pos1 / c2)
+    float s8r = 0, s8g = 0, s8b = 0;     // sum (t.x^2 - t.y^2) rad (SH2pos2 / c22)
+    const uint32_t m0 = split * chunk, m1 = m0 + chunk;
+    for (uint32_t mb = m0; mb < m1; mb += interval) {
+      float shadowing = 1.0f;
+      if (kShadow) shadowing = vis[static_cast<size_t>(mb / interval) * vis_pitch + c];
+#pragma unroll 4
+      for (uint32_t m = mb; m < mb + interval; ++m) {
+        const Vpl v = vpls[m];
+        const float tx0 = v.px - wx, ty0 = v.py - wy, tz0 = v.pz - wz;
+        const float d2 = __builtin_fmaf(tz0, tz0, __builtin_fmaf(ty0, ty0, tx0 * tx0));
+        const float inv = __builtin_amdgcn_rsqf(d2);
+        const float tx = tx0 * inv, ty = ty0 * inv, tz = tz0 * inv;
+        const float nd = -__builtin_fmaf(v.nz, tz, __builtin_fmaf(v.ny, ty, v.nx * tx));
+        float k = fminf(1.0f, fmaxf(0.0f, nd));
+        k = k * shadowing;
+        const float f = k * __builtin_amdgcn_rcpf(d2 + v.disc_area);
+        const float rr = v.fr * f, rg = v.fg * f, rb = v.fb * f;
+        s0r += rr; s0g += rg; s0b += rb;
+        s1r = __builtin_fmaf(ty, rr, s1r); s1g = __builtin_fmaf(ty, rg, s1g); s1b = __builtin_fmaf(ty, rb, s1b);
+        s2r = __builtin_fmaf(tz, rr, s2r); s2g = __builtin_fmaf(tz, rg, s2g); s2b = __builtin_fmaf(tz, rb, s2b);
+        s3r = __builtin_fmaf(tx, rr, s3r); s3g = __builtin_fmaf(tx, rg, s3g); s3b = __builtin_fmaf(tx, rb, s3b);
+        if (kSh2) {
+          const float bxy = tx * ty, byz = ty * tz, bzz = tz * tz, bxz = tx * tz;
+          const float bdd = __builtin_fmaf(tx, tx, -(ty * ty));
+          s4r = __builtin_fmaf(bxy, rr, s4r); s4g = __builtin_fmaf(bxy, rg, s4g); s4b = __builtin_fmaf(bxy, rb, s4b);
+          s5r = __builtin_fmaf(byz, rr, s5r); s5g = __builtin_fmaf(byz, rg, s5g); s5b = __builtin_fmaf(byz, rb, s5b);
+          s6r = __builtin_fmaf(bzz, rr, s6r); s6g = __builtin_fmaf(bzz, rg, s6g); s6b = __builtin_fmaf(bzz, rb, s6b);
+          s7r = __builtin_fmaf(bxz, rr, s7r); s7g = __builtin_fmaf(bxz, rg, s7g); s7b = __builtin_fmaf(bxz, rb, s7b);
+          s8r = __builtin_fmaf(bdd, rr, s8r); s8g = __builtin_fmaf(bdd, rg, s8g); s8b = __builtin_fmaf(bdd, rb, s8b);
+        }
+      }
+    }
+    if (!valid) continue;
+    // Raw sums, coefficient-major: partial[(split * 27 + k) * pitch + cache].
+    float* p = partial + static_cast<size_t>(split) * 27u * partial_pitch + cache;
+    const float out[27] = {s0r, s0g, s0b, s1r, s1g, s1b, s2r, s2g, s2b, s3r, s3g, s3b, s4r, s4g,
+                           s4b, s5r, s5g, s5b, s6r, s6g, s6b, s7r, s7g, s7b, s8r, s8g, s8b};
+    const int n_out = kSh2 ? 27 : 12;
+#pragma unroll
+    for (int k = 0; k < 27; ++k)
+      if (k < n_out) p[static_cast<size_t>(k) * partial_pitch] = out[k];
+  }
+}
+
+// Sums the split partials in split order, applies the SH evaluation factors
+// (cacheLightingRSM.comp:267-277) and adds into the entry
+// (cacheLightingRSM.comp:357-373).
+__global__ void k_reduce(float* caches, int stride, const Counter* counter, const float* partial,
+                         uint32_t partial_pitch, drv_sh_constants shc, int sh2) {
+  const uint32_t live = counter->live;
+  const uint32_t splits = counter->splits;
+  const int n_raw = sh2 ? 27 : 12;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < live; c += gridDim.x * blockDim.x) {
+    float s[27];
+    for (int k = 0; k < 27; ++k) s[k] = 0.0f;
+    for (uint32_t sp = 0; sp < splits; ++sp) {
+      const float* p = partial + static_cast<size_t>(sp) * 27u * partial_pitch + c;
+      for (int k = 0; k < n_raw; ++k) s[k] += p[static_cast<size_t>(k) * partial_pitch];
+    }
+    float* e = caches + static_cast<size_t>(c) * stride;
+    for (int ch = 0; ch < 3; ++ch) {
+      const float r0 = s[0 + ch], r1 = s[3 + ch], r2 = s[6 + ch], r3 = s[9 + ch];
+      e[7 + 4 * ch] += shc.eva0 * r0;           // SH00_r/g/b at 7, 11, 15
+      e[4 + ch] += -(shc.eva1 * r1);            // SH1neg1
+      e[8 + ch] += shc.eva1 * r2;               // SH10
+      e[12 + ch] += -(shc.eva1 * r3);           // SH1pos1
+      if (sh2) {
+        const float r4 = s[12 + ch], r5 = s[15 + ch], r6 = s[18 + ch], r7 = s[21 + ch], r8 = s[24 + ch];
+        e[16 + ch] += -(shc.eva2n2_p1_n1 * r4);             // SH2neg2
+        e[20 + ch] += shc.eva2n2_p1_n1 * r5;                // SH2neg1
+        e[19 + 4 * ch] += shc.eva20 * (3.0f * r6 - r0);     // SH20_r/g/b at 19, 23, 27
+        e[24 + ch] += shc.eva2n2_p1_n1 * r7;                // SH2pos1
+        e[28 + ch] += shc.eva2p2 * r8;                      // SH2pos2
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
+                         hipStream_t s) {
+  PrepArgs a;
+  for (int i = 0; i < 16; ++i) a.inv_lvp[i] = light.inverse_light_view_projection[i];
+  for (int i = 0; i < 3; ++i) a.light_pos[i] = light.light_position[i];
+  a.val_area_factor = light.val_area_factor;
+  a.sampling_offset = light.indirect_shadow_sampling_offset;
+  a.super_val_width = light.indirect_shadow_super_val_width;
+  a.flux = rsm.flux;
+  a.normal = rsm.normal;
+  a.dls = rsm.depth_lin_sq;
+  a.dls_shadow = rsm.depth_lin_sq_shadow;
+  a.R = rsm.read_resolution;
+  a.R_shadow = rsm.read_resolution >> static_cast<int>(light.indirect_shadow_lod);
+  a.interval = static_cast<uint32_t>(light.indirect_shadow_sample_interval);
+  a.vpls = vpls;
+  a.blocks = blocks;
+  const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
+  hipLaunchKernelGGL(k_prepare_vpls, dim3((n + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
+                       const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
+                       uint32_t vis_pitch, uint64_t* step_counter, int grid, hipStream_t s) {
+  VisArgs a;
+  a.vox = vox;
+  for (int i = 0; i < 3; ++i) a.vmin[i] = vol.volume_world_min[i];
+  a.voxel_size = vol.voxel_size_in_world;
+  a.caches = caches;
+  a.stride = stride;
+  a.counter = counter;
+  a.blocks = blocks;
+  a.n_blocks = n_blocks;
+  a.vis = visibility;
+  a.pitch = vis_pitch;
+  a.steps = step_counter;
+  hipLaunchKernelGGL(k_visibility, dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,
+                   const Counter* counter, const Vpl* vpls, uint32_t n_vpl, const float* visibility,
+                   uint32_t vis_pitch, uint32_t interval, int indirect_shadow, float* partial,
+                   uint32_t partial_pitch, int grid, hipStream_t s) {
+  (void)shc;
+  // Split planning: enough (tile, split) items for ~16 waves per CU, each
+  // split a whole number of shadow blocks.
+  const uint32_t quantum = std::max<uint32_t>(interval, 64u);
+  hipLaunchKernelGGL(k_plan_gather, dim3(1), dim3(1), 0, s, const_cast<Counter*>(counter), n_vpl, quantum, 32u,
+                     static_cast<uint32_t>(grid) * 4u);
+  const bool sh2 = sh_bands >= 2;
+  if (sh2 && indirect_shadow)
+    hipLaunchKernelGGL((k_gather<true, true>), dim3(grid), dim3(256), 0, s, caches, stride, counter, vpls, n_vpl,
+                       visibility, vis_pitch, interval, partial, partial_pitch);
+  else if (sh2)
+    hipLaunchKernelGGL((k_gather<true, false>), dim3(grid), dim3(256), 0, s, caches, stride, counter, vpls, n_vpl,
+                       visibility, vis_pitch, interval, partial, partial_pitch);
+  else if (indirect_shadow)
+    hipLaunchKernelGGL((k_gather<false, true>), dim3(grid), dim3(256), 0, s, caches, stride, counter, vpls, n_vpl,
+                       visibility, vis_pitch, interval, partial, partial_pitch);
+  else
+    hipLaunchKernelGGL((k_gather<false, false>), dim3(grid), dim3(256), 0, s, caches, stride, counter, vpls, n_vpl,
+                       visibility, vis_pitch, interval, partial, partial_pitch);
+}
+
+void launch_reduce_sh(const drv_sh_constants& shc, int sh_bands, float* caches, int stride, const Counter* counter,
+                      const float* partial, uint32_t partial_pitch, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(256), 0, s, caches, stride, counter, partial, partial_pitch, shc,
+                     sh_bands >= 2 ? 1 : 0);
+}
+
+}  // namespace drvk

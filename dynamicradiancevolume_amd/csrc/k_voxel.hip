@@ -1,0 +1,273 @@
+// Stage B: scene voxelization, temporal blend and the voxel mip chain.
+//
+// Restates Voxelization::VoxelizeScene (rendering/voxelization.cpp:90-176):
+// the conservative raster pass shader/voxelize.{vert,geom,frag}, the blend
+// shader/voxelblend.comp:8-18 and the mip loop shader/voxelmipmap.comp:8-12.
+//
+// MI355X design: the geometry-shader raster becomes one thread per
+// triangle that evaluates the geometry shader's pushed-out edge planes and
+// depth plane directly and walks the covered pixels of its dominant-axis
+// projection; the blend is a flat byte stream; each mip level is an exact
+// integer 2x2x2 mean.  After the mips a "corner-expanded" copy of every level
+// packs the 8 texels of each trilinear footprint into one uint64 so the cone
+// tracer issues one 8-byte load per level instead of eight byte gathers.
+#include <algorithm>
+
+#include "drv_internal.hpp"
+#include "k_common.hpp"
+
+namespace drvk {
+namespace {
+
+struct VoxelizeArgs {
+  float vmin[3], vmax[3];
+  float world[16];
+  const float* positions;
+  const uint32_t* indices;
+  uint32_t n_tri;
+  int res;
+  uint8_t* target;
+};
+
+__device__ __forceinline__ void store_voxel(uint8_t* target, int res, int side, int sx, int sy, int sz) {
+  // UnswizzlePos, voxelize.frag:16-19
+  int x, y, z;
+  if (side == 0) {
+    x = sz; y = sy; z = sx;
+  } else if (side == 1) {
+    x = sx; y = sz; z = sy;
+  } else {
+    x = sx; y = sy; z = sz;
+  }
+  // imageStore outside the image has no effect.
+  if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) return;
+  target[(static_cast<size_t>(z) * res + y) * res + x] = 255;
+}
+
+__global__ __launch_bounds__(256) void k_voxelize(VoxelizeArgs a) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_tri) return;
+  const int res = a.res;
+  const float fres = static_cast<float>(res);
+  const float h = 1.0f / fres;
+  // voxelize.vert:15-23
+  float P[3][3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const uint32_t vi = a.indices ? a.indices[3 * t + v] : 3 * t + v;
+    const float* p = a.positions + 3 * static_cast<size_t>(vi);
+    float w4[4];
+    mat_vec(a.world, p[0], p[1], p[2], 1.0f, w4);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) P[v][i] = (w4[i] - a.vmin[i]) / (a.vmax[i] - a.vmin[i]) * 2.0f - 1.0f;
+  }
+  // voxelize.geom:43-48: dominant axis of the (normalized) face normal
+  const float v0x = P[1][0] - P[0][0], v0y = P[1][1] - P[0][1], v0z = P[1][2] - P[0][2];
+  const float v1x = P[2][0] - P[0][0], v1y = P[2][1] - P[0][1], v1z = P[2][2] - P[0][2];
+  const float c0 = v0y * v1z - v0z * v1y, c1 = v0z * v1x - v0x * v1z, c2 = v0x * v1y - v0y * v1x;
+  const float cl = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+  const float an0 = fabsf(c0 / cl), an1 = fabsf(c1 / cl), an2 = fabsf(c2 / cl);
+  int side = an0 > an1 ? 0 : 1;
+  side = (side == 0 ? an0 : an1) > an2 ? side : 2;
+  // voxelize.geom:53-75: swizzle into the raster plane
+  float r[3][3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    if (side == 0) {
+      r[v][0] = P[v][2]; r[v][1] = P[v][1]; r[v][2] = P[v][0];
+    } else if (side == 1) {
+      r[v][0] = P[v][0]; r[v][1] = P[v][2]; r[v][2] = P[v][1];
+    } else {
+      r[v][0] = P[v][0]; r[v][1] = P[v][1]; r[v][2] = P[v][2];
+    }
+  }
+  // voxelize.geom:81-83: raster AABB grown by half a voxel, in pixels
+  float bb0 = fminf(fminf(r[0][0], r[1][0]), r[2][0]) - h;
+  float bb1 = fminf(fminf(r[0][1], r[1][1]), r[2][1]) - h;
+  float bb2 = fmaxf(fmaxf(r[0][0], r[1][0]), r[2][0]) + h;
+  float bb3 = fmaxf(fmaxf(r[0][1], r[1][1]), r[2][1]) + h;
+  bb0 = (bb0 * 0.5f + 0.5f) * fres;
+  bb1 = (bb1 * 0.5f + 0.5f) * fres;
+  bb2 = (bb2 * 0.5f + 0.5f) * fres;
+  bb3 = (bb3 * 0.5f + 0.5f) * fres;
+  // voxelize.geom:86-102: edge lines cross((d,0),(p,1)) = (d.y, -d.x, d.x p.y - d.y p.x),
+  // oriented by the winding and pushed out by h * (|a| + |b|).
+  const float ax = r[0][0] - r[2][0], ay = r[0][1] - r[2][1];
+  const float bx = r[1][0] - r[0][0], by = r[1][1] - r[0][1];
+  const float ex = r[2][0] - r[1][0], ey = r[2][1] - r[1][1];
+  float pl[3][3] = {{ay, -ax, ax * r[2][1] - ay * r[2][0]},
+                    {by, -bx, bx * r[0][1] - by * r[0][0]},
+                    {ey, -ex, ex * r[1][1] - ey * r[1][0]}};
+  const float wind_raw = ax * by - bx * ay;
+  const float wind = wind_raw > 0.0f ? 1.0f : (wind_raw < 0.0f ? -1.0f : 0.0f);
+  if (wind == 0.0f) return;  // zero-area primitive
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    pl[i][0] = pl[i][0] * wind;
+    pl[i][1] = pl[i][1] * wind;
+    pl[i][2] = pl[i][2] * wind;
+    pl[i][2] = pl[i][2] - (h * fabsf(pl[i][0]) + h * fabsf(pl[i][1]));
+  }
+  // voxelize.geom:118-128: expanded vertices = pairwise line intersections,
+  // each keeping its original depth.
+  float V[3][3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const float* p = pl[v == 0 ? 0 : (v == 1 ? 1 : 2)];
+    const float* q = pl[v == 0 ? 1 : (v == 1 ? 2 : 0)];
+    const float cx = p[1] * q[2] - p[2] * q[1];
+    const float cy = p[2] * q[0] - p[0] * q[2];
+    const float cw = p[0] * q[1] - p[1] * q[0];
+    V[v][0] = cx / cw;
+    V[v][1] = cy / cw;
+    V[v][2] = r[v][2];
+  }
+  const float ex1 = V[1][0] - V[0][0], ey1 = V[1][1] - V[0][1];
+  const float ex2 = V[2][0] - V[0][0], ey2 = V[2][1] - V[0][1];
+  const float area = ex1 * ey2 - ex2 * ey1;
+  if (!(area != 0.0f) || !isfinite(area)) return;
+  const float dz1 = V[1][2] - V[0][2], dz2 = V[2][2] - V[0][2];
+  // dFdxCoarse / dFdyCoarse of gl_FragCoord.z * res == NDC depth gradient
+  const float gx = (dz1 * ey2 - dz2 * ey1) / area;
+  const float gy = (dz2 * ex1 - dz1 * ex2) / area;
+  const float max_change = sqrtf(gx * gx + gy * gy) * 1.414f;
+  const int px0 = max(0, to_int(floorf(bb0 - 0.5f)));
+  const int py0 = max(0, to_int(floorf(bb1 - 0.5f)));
+  const int px1 = min(res - 1, to_int(ceilf(bb2 - 0.5f)));
+  const int py1 = min(res - 1, to_int(ceilf(bb3 - 0.5f)));
+  for (int py = py0; py <= py1; ++py) {
+    const float fy = static_cast<float>(py) + 0.5f;
+    if (fy < bb1 || fy > bb3) continue;
+    const float ny = fy / fres * 2.0f - 1.0f;
+    for (int px = px0; px <= px1; ++px) {
+      const float fx = static_cast<float>(px) + 0.5f;
+      // voxelize.frag:26-30
+      if (fx < bb0 || fx > bb2) continue;
+      const float nx = fx / fres * 2.0f - 1.0f;
+      const bool in = (pl[0][0] * nx + pl[0][1] * ny + pl[0][2] <= 0.0f) &&
+                      (pl[1][0] * nx + pl[1][1] * ny + pl[1][2] <= 0.0f) &&
+                      (pl[2][0] * nx + pl[2][1] * ny + pl[2][2] <= 0.0f);
+      if (!in) continue;
+      const float zn = V[0][2] + gx * (nx - V[0][0]) + gy * (ny - V[0][1]);
+      const float zw = zn * 0.5f + 0.5f;
+      // voxelize.frag:33-56
+      const float vz = zw * fres;
+      const int iz = to_int(vz);
+      store_voxel(a.target, res, side, px, py, iz);
+      if (iz != to_int(vz - max_change)) store_voxel(a.target, res, side, px, py, iz - 1);
+      if (iz != to_int(vz + max_change)) store_voxel(a.target, res, side, px, py, iz + 1);
+    }
+  }
+}
+
+// voxelblend.comp:8-18: old + sign(target - old) * AdaptionFactor, stored as unorm8.
+__global__ void k_voxel_blend(const uint8_t* target, uint8_t* volume, size_t n, float f) {
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) {
+    const float t = static_cast<float>(target[i]) / 255.0f;
+    const float o = static_cast<float>(volume[i]) / 255.0f;
+    const float d = t - o;
+    const float s = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+    float v = o + s * f;
+    v = fminf(1.0f, fmaxf(0.0f, v));
+    volume[i] = static_cast<uint8_t>(floorf(v * 255.0f + 0.5f));
+  }
+}
+
+// voxelmipmap.comp:8-12: the linear sample at the centre of each 2x2x2 child
+// block is the 8-texel mean; it is computed exactly and rounded half up.
+__global__ void k_voxel_mip(const uint8_t* src, uint8_t* dst, int w2) {
+  const int n = w2 * w2 * w2;
+  const int w = 2 * w2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int x = i % w2, y = (i / w2) % w2, z = i / (w2 * w2);
+    const uint8_t* s0 = src + (static_cast<size_t>(2 * z) * w + 2 * y) * w + 2 * x;
+    const uint8_t* s1 = s0 + static_cast<size_t>(w) * w;
+    const int sum = s0[0] + s0[1] + s0[w] + s0[w + 1] + s1[0] + s1[1] + s1[w] + s1[w + 1];
+    dst[i] = static_cast<uint8_t>((sum + 4) >> 3);
+  }
+}
+
+struct CornerArgs {
+  const uint8_t* mips;
+  uint64_t* corners;
+  uint32_t mip_offset[16];
+  uint32_t corner_offset[17];
+  int res;
+  int levels;
+};
+
+__global__ void k_voxel_corners(CornerArgs a) {
+  const uint32_t total = a.corner_offset[a.levels];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    int l = 0;
+    while (l + 1 < a.levels && i >= a.corner_offset[l + 1]) ++l;
+    const int w = a.res >> l, wp = w + 1;
+    const uint32_t j = i - a.corner_offset[l];
+    const int cx = static_cast<int>(j % wp) - 1, cy = static_cast<int>((j / wp) % wp) - 1,
+              cz = static_cast<int>(j / (wp * wp)) - 1;
+    const uint8_t* lvl = a.mips + a.mip_offset[l];
+    uint64_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int x = min(w - 1, max(0, cx + (k & 1)));
+      const int y = min(w - 1, max(0, cy + ((k >> 1) & 1)));
+      const int z = min(w - 1, max(0, cz + (k >> 2)));
+      packed |= static_cast<uint64_t>(lvl[(static_cast<size_t>(z) * w + y) * w + x]) << (8 * k);
+    }
+    a.corners[i] = packed;
+  }
+}
+
+}  // namespace
+
+void launch_voxelize(const drv_volume& vol, int res, const float* positions, const uint32_t* indices, uint32_t n_tri,
+                     const float* world, uint8_t* target, hipStream_t s) {
+  if (n_tri == 0) return;
+  VoxelizeArgs a;
+  for (int i = 0; i < 3; ++i) {
+    a.vmin[i] = vol.volume_world_min[i];
+    a.vmax[i] = vol.volume_world_max[i];
+  }
+  for (int i = 0; i < 16; ++i) a.world[i] = world[i];
+  a.positions = positions;
+  a.indices = indices;
+  a.n_tri = n_tri;
+  a.res = res;
+  a.target = target;
+  hipLaunchKernelGGL(k_voxelize, dim3((n_tri + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_voxel_blend(int res, const uint8_t* target, uint8_t* volume, float adaption, hipStream_t s) {
+  const size_t n = static_cast<size_t>(res) * res * res;
+  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_voxel_blend, dim3(blocks), dim3(256), 0, s, target, volume, n, adaption);
+}
+
+void launch_voxel_mips(const VoxelLayout& L, uint8_t* mips, hipStream_t s) {
+  for (int l = 0; l + 1 < L.levels; ++l) {
+    const int w2 = L.res >> (l + 1);
+    const int n = w2 * w2 * w2;
+    const unsigned blocks = static_cast<unsigned>(std::min(std::max(1, (n + 255) / 256), 4096));
+    hipLaunchKernelGGL(k_voxel_mip, dim3(blocks), dim3(256), 0, s, mips + L.mip_offset[l], mips + L.mip_offset[l + 1],
+                       w2);
+  }
+}
+
+void launch_voxel_corners(const VoxelLayout& L, const uint8_t* mips, uint64_t* corners, hipStream_t s) {
+  CornerArgs a;
+  a.mips = mips;
+  a.corners = corners;
+  a.res = L.res;
+  a.levels = L.levels;
+  for (int l = 0; l < L.levels; ++l) {
+    a.mip_offset[l] = static_cast<uint32_t>(L.mip_offset[l]);
+    a.corner_offset[l] = static_cast<uint32_t>(L.corner_offset[l]);
+  }
+  a.corner_offset[L.levels] = static_cast<uint32_t>(L.corner_count);
+  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((L.corner_count + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_voxel_corners, dim3(blocks), dim3(256), 0, s, a);
+}
+
+}  // namespace drvk

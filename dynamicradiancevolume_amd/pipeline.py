@@ -1,0 +1,197 @@
+"""Python host-side mirror of the reference renderer's indirect-lighting passes.
+
+``Renderer`` exposes the pass interface of ``Renderer`` /
+``Voxelization`` (rendering/renderer.hpp:46-199, rendering/voxelization.hpp:37)
+over the C ABI in ``include/drv.h``: ``voxelize_scene``, ``allocate_caches``,
+``light_caches_rsm`` and ``apply_caches``, plus the host UBO updates
+(``update_volume``, ``set_frame``) and read-back helpers.  Device memory is
+held in torch tensors on ``cuda:0``; every computation runs in the HIP
+library, which must be built (there is no fallback path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _abi
+from .workloads import FrameInputs, Workload
+
+IDENTITY = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+
+# Canonical SH coefficient order and their std430 offsets
+# (shader/lightcache.glsl:33-58): SH00, SH1-1, SH10, SH1+1, SH2-2, SH2-1, SH20, SH2+1, SH2+2.
+SH_OFFSETS = np.array(
+    [[7, 11, 15], [4, 5, 6], [8, 9, 10], [12, 13, 14], [16, 17, 18], [20, 21, 22], [19, 23, 27], [24, 25, 26],
+     [28, 29, 30]], dtype=np.int64)
+
+
+class DrvError(RuntimeError):
+    pass
+
+
+def entries_sh(entries: np.ndarray, sh_bands: int) -> np.ndarray:
+    """[n, stride] std430 entries -> [n, 27] coefficients (zeros for absent bands)."""
+    n = entries.shape[0]
+    out = np.zeros((n, 27), entries.dtype)
+    k = 9 if sh_bands >= 2 else 4
+    out[:, : 3 * k] = entries[:, SH_OFFSETS[:k].reshape(-1)]
+    return out
+
+
+class DeviceFrame:
+    """FrameInputs uploaded to device memory."""
+
+    def __init__(self, fi: FrameInputs, device: str = "cuda:0"):
+        self.inputs = fi
+        dev = torch.device(device)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        self.positions = t(fi.positions)
+        self.depth = t(fi.depth)
+        self.gnormal = t(fi.gnormal)
+        self.galbedo = t(fi.galbedo)
+        flux, rn, dls = fi.read_level
+        self.flux = t(flux)
+        self.rnormal = t(rn)
+        self.dls = t(dls)
+        self.dls_shadow = t(fi.shadow_dls)
+        self.out = torch.zeros((fi.workload.height, fi.workload.width, 3), dtype=torch.float32, device=dev)
+
+    def gbuffer(self) -> _abi.DrvGBuffer:
+        w = self.inputs.workload
+        return _abi.DrvGBuffer(self.depth.data_ptr(), self.gnormal.data_ptr(), self.galbedo.data_ptr(), w.width,
+                               w.height)
+
+    def rsm(self) -> _abi.DrvRsm:
+        return _abi.DrvRsm(self.flux.data_ptr(), self.rnormal.data_ptr(), self.dls.data_ptr(),
+                           self.dls_shadow.data_ptr(), self.inputs.read_resolution)
+
+
+class Renderer:
+    def __init__(self, cfg: _abi.DrvConfig, stream: torch.cuda.Stream | None = None):
+        self.lib = _abi.load_drv()
+        self.cfg = cfg
+        self.ctx = C.c_void_p()
+        self._check(self.lib.drv_create(C.byref(cfg), C.byref(self.ctx)), "drv_create")
+        self.stream = stream
+        if stream is not None:
+            self._check(self.lib.drv_set_stream(self.ctx, C.c_void_p(stream.cuda_stream)), "drv_set_stream")
+
+    @classmethod
+    def for_workload(cls, w: Workload, stream=None, **overrides) -> "Renderer":
+        cfg = w.with_(**overrides).config() if overrides else w.config()
+        return cls(cfg, stream)
+
+    def close(self):
+        if self.ctx:
+            self.lib.drv_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != _abi.DRV_OK:
+            msg = self.lib.drv_last_error(self.ctx).decode() if self.ctx else ""
+            raise DrvError(f"{what} failed ({rc}): {msg}")
+
+    @property
+    def stride(self) -> int:
+        return 32 if self.cfg.sh_bands >= 2 else 16
+
+    # --- UBO updates -----------------------------------------------------
+    def set_frame(self, frame: _abi.DrvFrame):
+        self._check(self.lib.drv_set_frame(self.ctx, C.byref(frame)), "drv_set_frame")
+
+    def update_volume(self, camera_position, scene_min, scene_max):
+        self._check(self.lib.drv_update_volume(self.ctx, _abi.f3(*camera_position), _abi.f3(*scene_min),
+                                               _abi.f3(*scene_max)), "drv_update_volume")
+
+    def volume(self) -> _abi.DrvVolume:
+        v = _abi.DrvVolume()
+        self._check(self.lib.drv_get_volume(self.ctx, C.byref(v)), "drv_get_volume")
+        return v
+
+    def set_volume(self, v: _abi.DrvVolume):
+        self._check(self.lib.drv_set_volume(self.ctx, C.byref(v)), "drv_set_volume")
+
+    def sh_constants(self) -> _abi.DrvShConstants:
+        s = _abi.DrvShConstants()
+        self._check(self.lib.drv_get_sh_constants(self.ctx, C.byref(s)), "drv_get_sh_constants")
+        return s
+
+    # --- passes ------------------------------------------------------------
+    def voxelize_scene(self, positions: torch.Tensor, n_tri: int, world=IDENTITY, adaption_factor: float = 1.0,
+                       indices: torch.Tensor | None = None):
+        self._check(self.lib.drv_voxelize_begin(self.ctx), "drv_voxelize_begin")
+        self._check(self.lib.drv_voxelize_mesh(self.ctx, positions.data_ptr(),
+                                               indices.data_ptr() if indices is not None else None, n_tri,
+                                               _abi.fvec(world, 16)), "drv_voxelize_mesh")
+        self._check(self.lib.drv_voxelize_end(self.ctx, adaption_factor), "drv_voxelize_end")
+
+    def allocate_caches(self, gbuffer: _abi.DrvGBuffer):
+        self._check(self.lib.drv_allocate_caches(self.ctx, C.byref(gbuffer)), "drv_allocate_caches")
+
+    def light_caches_rsm(self, rsm: _abi.DrvRsm, light: _abi.DrvSpotLight):
+        self._check(self.lib.drv_light_caches(self.ctx, C.byref(rsm), C.byref(light)), "drv_light_caches")
+
+    def apply_caches(self, gbuffer: _abi.DrvGBuffer, out: torch.Tensor, accumulate: bool = False):
+        self._check(self.lib.drv_apply_caches(self.ctx, C.byref(gbuffer), out.data_ptr(), int(accumulate)),
+                    "drv_apply_caches")
+
+    def render_frame(self, df: DeviceFrame, voxelize: bool = True, accumulate: bool = False):
+        """Renderer::Draw's DYN_RADIANCE_VOLUME branch (renderer.cpp:539-594) for one view."""
+        fi = df.inputs
+        w = fi.workload
+        self.set_frame(fi.frame())
+        self.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+        if voxelize and self.cfg.indirect_shadow:
+            self.voxelize_scene(df.positions, len(fi.positions), adaption_factor=w.adaption_factor)
+        self.allocate_caches(df.gbuffer())
+        self.light_caches_rsm(df.rsm(), fi.light)
+        self.apply_caches(df.gbuffer(), df.out, accumulate)
+
+    # --- read-back -----------------------------------------------------------
+    def synchronize(self):
+        self._check(self.lib.drv_synchronize(self.ctx), "drv_synchronize")
+
+    def cache_count(self) -> tuple[int, int]:
+        d, l_ = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.drv_get_cache_count(self.ctx, C.byref(d), C.byref(l_)), "drv_get_cache_count")
+        return d.value, l_.value
+
+    def read_caches(self, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.cache_count()[1]
+        out = np.empty((count, self.stride), np.float32)
+        self._check(self.lib.drv_read_caches(self.ctx, out.ctypes.data, count), "drv_read_caches")
+        return out
+
+    def read_cav(self) -> np.ndarray:
+        R, Cn = self.cfg.cav_resolution, self.cfg.cav_cascades
+        out = np.empty((R, R, Cn * R), np.uint32)
+        self._check(self.lib.drv_read_cav(self.ctx, out.ctypes.data), "drv_read_cav")
+        return out
+
+    def read_voxels(self, level: int) -> np.ndarray:
+        w = self.cfg.voxel_resolution >> max(level, 0)
+        out = np.empty((w, w, w), np.uint8)
+        self._check(self.lib.drv_read_voxels(self.ctx, level, out.ctypes.data), "drv_read_voxels")
+        return out
+
+    def read_visibility(self, n_caches: int, n_blocks: int) -> np.ndarray:
+        out = np.empty((n_caches, n_blocks), np.float32)
+        self._check(self.lib.drv_read_visibility(self.ctx, out.ctypes.data, out.size), "drv_read_visibility")
+        return out
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.drv_set_profiling(self.ctx, int(on)), "drv_set_profiling")
+
+    def stage_times(self) -> dict:
+        t = _abi.DrvStageTimes()
+        self._check(self.lib.drv_get_stage_times(self.ctx, C.byref(t)), "drv_get_stage_times")
+        return {n: getattr(t, n) for n, _ in t._fields_}

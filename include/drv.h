@@ -1,0 +1,252 @@
+/*
+ * drv.h - C ABI of the MI355X indirect-lighting pipeline.
+ *
+ * The four per-frame compute passes of DynamicRadianceVolume's renderer
+ * (Renderer::Draw, DynamicRadianceVolume/rendering/renderer.cpp:539-594):
+ *
+ *   VoxelizeScene -> AllocateCaches -> LightCachesRSM -> ApplyCaches
+ *
+ * are exported here as plain C entry points over device pointers.  The
+ * reference keeps these passes private to `Renderer` and binds its data as GL
+ * state (UBOs 0-4, SSBOs 0-1, image unit 0, texture units 0-4); each struct
+ * below restates one of those GL bindings as a plain C record.
+ *
+ * Conventions (shared by every entry point):
+ *  - Every function returns an int status: DRV_OK (0) or a negative DRV_ERR_*
+ *    code.  Nothing throws across the ABI.  drv_last_error() returns a text
+ *    description of the last failure on a context.
+ *  - Image/buffer pointers marked "device" are HIP device pointers that stay
+ *    owned by the caller.  The library owns the cache buffer, the cache address
+ *    volume, the voxel volume and all scratch buffers.
+ *  - Matrices are 16 floats, row-major, applied to column vectors:
+ *    out[r] = sum_c m[4*r + c] * v[c].  This is what the reference's GLSL
+ *    expression `vec4 * mat4` computes on a row-major ei::Mat4x4 upload.
+ *  - 2D images are row-major with row 0 at normalized texture coordinate
+ *    v = 0 (the GL texture origin, NDC y = -1).
+ *  - A context is single-threaded; all work is enqueued on one HIP stream
+ *    (drv_set_stream) and only the drv_read_* / drv_get_cache_count calls
+ *    synchronize with the host.
+ */
+#ifndef DRV_H
+#define DRV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRV_OK 0
+#define DRV_ERR_INVALID_ARG (-1)
+#define DRV_ERR_HIP (-2)
+#define DRV_ERR_NO_MEMORY (-3)
+#define DRV_ERR_STATE (-4)
+
+#define DRV_MAX_CASCADES 4 /* MAX_NUM_ADDRESS_VOLUME_CASCADES, shader/globalubos.glsl:46 */
+
+typedef struct drv_ctx drv_ctx;
+
+/* Static configuration.  Replaces the Renderer constructor defaults
+ * (rendering/renderer.cpp:36-51, 86-90), SetMaxCacheCount
+ * (renderer.cpp:248-288), SetCAVCascades (renderer.cpp:1174-1193) and the
+ * shader #define prefix of ReloadLightingSettingDependentCacheShader
+ * (renderer.cpp:197-246). */
+typedef struct drv_config {
+  int32_t width;               /* backbuffer width in pixels */
+  int32_t height;              /* backbuffer height in pixels */
+  int32_t max_caches;          /* capacity of the light cache buffer */
+  int32_t cav_resolution;      /* cache address volume cells per axis and cascade */
+  int32_t cav_cascades;        /* 1..DRV_MAX_CASCADES */
+  int32_t voxel_resolution;    /* voxel volume cells per axis, power of two >= 2 */
+  int32_t sh_bands;            /* 1 = INDDIFFUSE_VIA_SH1 (64 B/cache), 2 = INDDIFFUSE_VIA_SH2 (128 B/cache) */
+  int32_t indirect_shadow;     /* INDIRECT_SHADOW: cone-traced visibility on/off */
+  float cav_transition_size;   /* CAVTransitionZoneSize; > 0 defines ADDRESSVOL_CASCADE_TRANSITIONS */
+  float cav_cascade0_world_size; /* world size of cascade 0 (renderer.cpp:1185: 4.0); each further cascade doubles */
+} drv_config;
+
+/* One CAV cascade, struct CAVCascade (shader/globalubos.glsl:48-60). */
+typedef struct drv_cascade {
+  float min[3];
+  float world_voxel_size;
+  float max[3];
+  float pad0;
+  float decision_min[3];
+  float pad1;
+  float decision_max[3];
+  float pad2;
+} drv_cascade;
+
+/* UBO VolumeInfo (shader/globalubos.glsl:65-79). */
+typedef struct drv_volume {
+  float volume_world_min[3];
+  float voxel_size_in_world;
+  float volume_world_max[3];
+  float cav_transition_zone_size;
+  drv_cascade cascades[DRV_MAX_CASCADES];
+} drv_volume;
+
+/* The fields of UBO PerFrame (shader/globalubos.glsl:33-43) the path reads. */
+typedef struct drv_frame {
+  float inverse_view_projection[16];
+  float camera_position[3];
+} drv_frame;
+
+/* UBO Constant SH factors (shader/globalubos.glsl:4-16), filled as in
+ * Renderer::UpdateConstantUBO (renderer.cpp:290-306). */
+typedef struct drv_sh_constants {
+  float cos_lobe0, cos_lobe1, cos_lobe2n2_p1_n1, cos_lobe20, cos_lobe2p2;
+  float eva0, eva1, eva2n2_p1_n1, eva20, eva2p2;
+} drv_sh_constants;
+
+/* Spot light description, struct Light (scene/light.hpp:8-54). */
+typedef struct drv_spot_params {
+  float intensity[3];
+  float position[3];
+  float direction[3];          /* need not be normalized */
+  float half_angle;            /* radians */
+  uint32_t rsm_resolution;     /* rounded up to a power of two */
+  uint32_t rsm_read_lod;
+  uint32_t indirect_shadow_lod;
+  float near_plane;            /* Light::nearPlane = 0.1 (scene/scene.cpp:6) */
+  float far_plane;             /* Light::farPlane = 10000 (scene/scene.cpp:7) */
+} drv_spot_params;
+
+/* UBO SpotLight (shader/globalubos.glsl:88-112), as filled by
+ * Renderer::PrepareLights (renderer.cpp:664-725). */
+typedef struct drv_spot_light {
+  float light_intensity[3];
+  float light_position[3];
+  float light_direction[3];
+  float light_cos_half_angle;
+  float light_view_projection[16];
+  float inverse_light_view_projection[16];
+  int32_t rsm_render_resolution;
+  int32_t rsm_read_resolution;
+  float val_area_factor;
+  float indirect_shadow_lod;
+  float indirect_shadow_block_size;
+  int32_t indirect_shadow_sample_interval;
+  float indirect_shadow_super_val_width;
+  float indirect_shadow_sampling_offset;
+} drv_spot_light;
+
+/* G-buffer (renderer.cpp:466-471, texture units 0-3 of shader/gbuffer.glsl). */
+typedef struct drv_gbuffer {
+  const float* depth;     /* device, DEPTH_COMPONENT32F, reverse-Z (clear 0) */
+  const int16_t* normal;  /* device, RG16I packed by PackNormal16I (shader/utils.glsl:83-89) */
+  const uint8_t* albedo;  /* device, SRGB8, 3 bytes per pixel, sRGB-encoded */
+  int32_t width;
+  int32_t height;
+} drv_gbuffer;
+
+/* Reflective shadow map of one light at its read level (texture units 0-2 of
+ * shader/cacheLightingRSM.comp:11-13; the base level is set to rsmReadLod at
+ * renderer.cpp:807-809). */
+typedef struct drv_rsm {
+  const uint16_t* flux;              /* device, RGB16F, read_resolution^2 texels (flux / pi) */
+  const int16_t* normal;             /* device, RG16I */
+  const uint16_t* depth_lin_sq;      /* device, RG16F (d, d^2) at the read level */
+  const uint16_t* depth_lin_sq_shadow; /* device, RG16F (d, d^2) at read level + indirect_shadow_lod */
+  int32_t read_resolution;           /* power of two; RSMReadResolution */
+} drv_rsm;
+
+/* Per-stage timings of the last frame when profiling is on (the scopes of
+ * the reference's FrameProfiler: VoxelizeScene, VoxelBlendMipMap,
+ * AllocateCaches, LightCaches, ApplyCaches). */
+typedef struct drv_stage_times {
+  float voxelize_ms;     /* target clear + raster of all meshes */
+  float voxel_mips_ms;   /* blend + mip chain */
+  float allocate_ms;     /* counter/CAV clear + allocation + dispatch args */
+  float light_caches_ms; /* whole LightCachesRSM of the last light */
+  float prepare_ms;      /* VPL and shadow-block records */
+  float visibility_ms;   /* cone-traced visibility kernel */
+  float gather_ms;       /* SH gather kernel */
+  float reduce_ms;       /* split reduction into the cache entries */
+  float apply_ms;
+  uint64_t cone_traces;  /* (cache, shadow block) traces of the last light, when counting */
+  uint64_t cone_steps;   /* march steps of those traces, when counting */
+} drv_stage_times;
+
+/* ---- lifetime -------------------------------------------------------- */
+
+/* Renderer::Renderer + AllocateCacheData + SetCAVCascades
+ * (renderer.cpp:26-120, 248-288, 1174-1193). */
+int drv_create(const drv_config* config, drv_ctx** out_ctx);
+int drv_destroy(drv_ctx* ctx);
+const char* drv_last_error(const drv_ctx* ctx);
+/* Sets the HIP stream (hipStream_t) all work is enqueued on; NULL = default stream. */
+int drv_set_stream(drv_ctx* ctx, void* hip_stream);
+int drv_synchronize(drv_ctx* ctx);
+
+/* ---- per-frame state (the UBOs) -------------------------------------- */
+
+/* UpdateConstantUBO (renderer.cpp:290-306): the SH factors in use. */
+int drv_get_sh_constants(const drv_ctx* ctx, drv_sh_constants* out);
+/* UpdatePerFrameUBO (renderer.cpp:324-344). */
+int drv_set_frame(drv_ctx* ctx, const drv_frame* frame);
+/* UpdateVolumeUBO (renderer.cpp:346-431): voxel volume from the scene bounding
+ * box, CAV cascades centred on the camera. */
+int drv_update_volume(drv_ctx* ctx, const float camera_position[3],
+                      const float scene_min[3], const float scene_max[3]);
+int drv_set_volume(drv_ctx* ctx, const drv_volume* volume);
+int drv_get_volume(const drv_ctx* ctx, drv_volume* out);
+/* PrepareLights (renderer.cpp:664-725): SpotLight UBO of one light.  Pure host
+ * function, no context needed. */
+int drv_prepare_spot_light(const drv_spot_params* params, drv_spot_light* out);
+
+/* ---- the four stages -------------------------------------------------- */
+
+/* Voxelization::VoxelizeScene (rendering/voxelization.cpp:90-176).
+ * begin clears the voxel target; mesh rasterizes one entity's triangles
+ * (positions: device float[3 * n_vertices]; indices: device uint32[3 * n_tri],
+ * or NULL for non-indexed float[9 * n_tri] positions) through `world`;
+ * end blends the target into the volume with `adaption_factor`
+ * (voxelblend.comp) and rebuilds the mip chain (voxelmipmap.comp). */
+int drv_voxelize_begin(drv_ctx* ctx);
+int drv_voxelize_mesh(drv_ctx* ctx, const float* positions, const uint32_t* indices,
+                      uint32_t n_tri, const float world[16]);
+int drv_voxelize_end(drv_ctx* ctx, float adaption_factor);
+
+/* Renderer::AllocateCaches (renderer.cpp:951-992) = clear of counter and CAV,
+ * shader/cacheGather.comp and shader/cachePrepareLighting.comp. */
+int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gbuffer);
+
+/* Renderer::LightCachesRSM (renderer.cpp:899-933) for one light:
+ * shader/cacheLightingRSM.comp.  Results accumulate (+=) over calls. */
+int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light);
+
+/* Renderer::ApplyCaches (renderer.cpp:1047-1079): shader/cacheApply.frag.
+ * out_rgb: device float[3 * width * height]; accumulate != 0 adds to it
+ * (the reference's GL_ONE/GL_ONE blend), 0 overwrites covered pixels and
+ * writes 0 to discarded ones. */
+int drv_apply_caches(drv_ctx* ctx, const drv_gbuffer* gbuffer, float* out_rgb, int32_t accumulate);
+
+/* ---- introspection (synchronous) --------------------------------------- */
+
+/* TotalLightCacheCount (shader/lightcache.glsl:89; read back at
+ * renderer.cpp:960-966).  demanded = appends attempted; live = min(demanded,
+ * max_caches).  demanded > max_caches means the buffer overflowed. */
+int drv_get_cache_count(drv_ctx* ctx, uint32_t* demanded, uint32_t* live);
+/* Copies the first `count` LightCacheEntry records (std430,
+ * shader/lightcache.glsl:33-76: 16 floats per cache for SH1, 32 for SH2). */
+int drv_read_caches(drv_ctx* ctx, float* host_out, uint32_t count);
+/* Copies the CAV atlas, uint32[(cascades * R) * R * R], x fastest. */
+int drv_read_cav(drv_ctx* ctx, uint32_t* host_out);
+/* Copies one mip level of the voxel volume, uint8[res_l^3], x fastest. */
+int drv_read_voxels(drv_ctx* ctx, int32_t level, uint8_t* host_out);
+/* Copies the per-(cache, shadow block) visibility of the last
+ * drv_light_caches call, float[live * n_blocks], cache-major. */
+int drv_read_visibility(drv_ctx* ctx, float* host_out, uint64_t count);
+
+/* ---- profiling -------------------------------------------------------- */
+
+/* When on, each stage is bracketed by HIP events on the context stream;
+ * level 2 also counts cone-march steps (one atomic per wave). */
+int drv_set_profiling(drv_ctx* ctx, int32_t enabled);
+int drv_get_stage_times(drv_ctx* ctx, drv_stage_times* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRV_H */

@@ -1,0 +1,127 @@
+/*
+ * drv_oracle.h - CPU restatement of DynamicRadianceVolume's indirect-lighting
+ * hot path.  TEST INFRASTRUCTURE ONLY: it is the checker the HIP path is
+ * compared against and the CPU baseline the benchmark times.  Nothing in the
+ * product links, loads or calls it.
+ *
+ * Every function restates one reference shader or host routine in its own
+ * statement order, in IEEE single precision with no FMA contraction
+ * (built with -ffp-contract=off), plus an IEEE double version of the gather
+ * used to state the SH error of every FP32 path.  Each function cites the
+ * reference file:line it follows (paths relative to
+ * DynamicRadianceVolume/).
+ *
+ * GL semantics that the shaders rely on and that are fixed here:
+ *  - image atomics / stores outside the image are no-ops, and an atomic
+ *    outside the image returns 0 (GL 4.5 core section 8.26): an allocation
+ *    corner outside the address volume therefore "wins" its lock and appends
+ *    a cache that no CAV cell points to;
+ *  - SSBO reads/writes beyond the buffer read 0 / are dropped (robust buffer
+ *    access): caches past max_caches are counted but not stored, and apply
+ *    treats such addresses (and unclaimed cells, address 0xFFFFFFFF) as zero;
+ *  - texture filtering is evaluated in FP32 with exact weights (the GL 4.5
+ *    formulas of section 8.14), LINEAR_MIPMAP_LINEAR level selection, clamp
+ *    to edge;
+ *  - unorm8 conversion rounds to nearest (ties up); the 2x2x2 voxel mip mean
+ *    is computed exactly in integers.
+ */
+#ifndef DRV_ORACLE_H
+#define DRV_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/drv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- small pure functions -------------------------------------------- */
+void orc_sh_constants(drv_sh_constants* out);                          /* renderer.cpp:290-306 */
+void orc_morton_decode(uint32_t morton, uint32_t* x, uint32_t* y);     /* cacheLightingRSM.comp:46-62 */
+void orc_unpack_normal16i(int16_t a, int16_t b, float out[3]);          /* utils.glsl:43-60 */
+void orc_pack_normal16i(const float n[3], int16_t out[2]);              /* utils.glsl:65-89 */
+float orc_srgb_to_linear(uint8_t c);                                    /* SRGB8 texture decode */
+int orc_cascade(const drv_volume* vol, int32_t n_cascades, const float wp[3]);            /* lightcache.glsl:109-122 */
+float orc_cascade_transition(const drv_volume* vol, int32_t cascade, const float wp[3]);  /* lightcache.glsl:125-134 */
+/* cacheGather.comp:113-119 / cacheApply.frag:134-135: pixel centre -> world. */
+void orc_unproject_pixel(const float inv_vp[16], int32_t width, int32_t height, int32_t x, int32_t y,
+                         float depth, float wp_out[3]);
+/* renderer.cpp:346-431 */
+void orc_update_volume(const drv_config* cfg, const float camera_position[3], const float scene_min[3],
+                       const float scene_max[3], drv_volume* out);
+/* Scalar fields of renderer.cpp:694-719 (matrices are left untouched). */
+void orc_prepare_spot_scalars(const drv_spot_params* p, drv_spot_light* out);
+
+/* ---- stage A: cache allocation ---------------------------------------- */
+/* renderer.cpp:951-992 + cacheGather.comp:93-164 + cacheGather.comp:20-91.
+ * Clears cav (uint32[(C*R) * R * R]) and writes entries (std430 stride:
+ * 16 floats SH1 / 32 floats SH2, capacity cfg->max_caches).  Tiles and
+ * threads run in dispatch order.  Returns the demanded cache count
+ * (TotalLightCacheCount). */
+uint32_t orc_allocate(const drv_config* cfg, const drv_volume* vol, const drv_frame* frame, const float* depth,
+                      uint32_t* cav, float* entries);
+/* The per-thread leader decision of cacheGather.comp:141-162 for one 16x16
+ * tile of cell ids (-1 = none); writes 0/1 flags for both rules. */
+void orc_tile_leaders(const int32_t own[256], const int32_t own2[256], uint8_t lead[256], uint8_t lead2[256]);
+
+/* ---- stage B: voxelization -------------------------------------------- */
+/* voxelize.vert/.geom/.frag (voxelize.vert:15-23, voxelize.geom:19-112,
+ * voxelize.frag:21-57): marks voxels of `target` (uint8[res^3], x fastest)
+ * with 255.  positions: float[9 * n_tri] (non-indexed). */
+void orc_voxelize(int32_t res, const drv_volume* vol, const float* positions, int64_t n_tri,
+                  const float world[16], uint8_t* target);
+/* voxelblend.comp:8-18 */
+void orc_voxel_blend(int32_t res, const uint8_t* target, uint8_t* volume, float adaption_factor);
+/* voxelmipmap.comp:8-12 + voxelization.cpp:154-172: levels packed one after
+ * another in `mips` (level 0 first, res^3 bytes, then (res/2)^3 ...);
+ * computes levels 1..log2(res) from level 0. */
+void orc_voxel_mips(int32_t res, uint8_t* mips);
+/* textureLod(VoxelVolume, p, lod) with the linear-clamp sampler bound at
+ * renderer.cpp:920-921; single and double precision. */
+float orc_sample_voxels_f32(int32_t res, const uint8_t* mips, const float p[3], float lod);
+double orc_sample_voxels_f64(int32_t res, const uint8_t* mips, const double p[3], double lod);
+
+/* ---- stage C: gather --------------------------------------------------- */
+typedef struct orc_rsm {
+  const uint16_t* flux;                /* RGB16F, read_resolution^2 */
+  const int16_t* normal;               /* RG16I */
+  const uint16_t* depth_lin_sq;        /* RG16F at the read level */
+  const uint16_t* depth_lin_sq_shadow; /* RG16F at read level + lod */
+  int32_t read_resolution;
+} orc_rsm;
+
+/* cacheLightingRSM.comp:80-391 (SH1/SH2, INDIRECT_SHADOW) for the caches
+ * listed in `cache_ids` (NULL = 0..n-1): adds the light's contribution into
+ * `entries` (std430 stride as orc_allocate).  Optional outputs:
+ * visibility float[n * n_blocks] (the shadowing term per shadow block) and
+ * steps (total cone march steps).  n_threads <= 0 uses all cores. */
+void orc_gather_f32(const drv_config* cfg, const drv_volume* vol, const drv_sh_constants* shc,
+                    const drv_spot_light* light, const orc_rsm* rsm, const uint8_t* voxel_mips, float* entries,
+                    const uint32_t* cache_ids, int64_t n, float* visibility, int64_t* steps, int32_t n_threads);
+/* FP64 evaluation of the same sum for cache positions pos[3*n]; writes the
+ * 27 SH coefficients per cache (order: SH00, SH1-1, SH10, SH1+1, SH2-2,
+ * SH2-1, SH20, SH2+1, SH2+2; each RGB) into sh_out[27*n] (overwrites). */
+void orc_gather_f64(const drv_config* cfg, const drv_volume* vol, const drv_sh_constants* shc,
+                    const drv_spot_light* light, const orc_rsm* rsm, const uint8_t* voxel_mips, const float* pos,
+                    int64_t n, double* sh_out, double* visibility, int32_t n_threads);
+/* Cone-traced shadowing of one (cache position, shadow block), FP32
+ * (cacheLightingRSM.comp:167-231); returns shadowing, writes the step count. */
+float orc_cone_trace_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t* voxel_mips,
+                         const drv_spot_light* light, const orc_rsm* rsm, const float wp[3], uint32_t block_morton,
+                         int32_t* steps_out);
+/* Extracts the 27 SH coefficients (canonical order above) from an entry. */
+void orc_entry_sh(const float* entry, int32_t sh_bands, float out27[27]);
+
+/* ---- stage D: apply ----------------------------------------------------- */
+/* cacheApply.frag:28-195 + lightcache.glsl:137-183: out_rgb float[3*W*H];
+ * discarded pixels (depth < 1e-5) get 0. */
+void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* frame, const drv_sh_constants* shc,
+               const float* depth, const int16_t* normal, const uint8_t* albedo, const uint32_t* cav,
+               const float* entries, uint32_t live_count, float* out_rgb, int32_t n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

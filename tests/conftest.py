@@ -1,0 +1,42 @@
+import sys
+from functools import lru_cache
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+@lru_cache(maxsize=None)
+def _frame_inputs(name: str, overrides: tuple):
+    from dynamicradiancevolume_amd import workloads as W
+
+    w = W.WORKLOADS[name]
+    if overrides:
+        w = w.with_(**dict(overrides))
+    return W.make_inputs(w)
+
+
+def frame_inputs(name: str, **overrides):
+    return _frame_inputs(name, tuple(sorted(overrides.items())))
+
+
+def announce(msg: str):
+    """Names the case before each GPU step so a failing run says which one."""
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="session")
+def cornell():
+    return frame_inputs("cornell_256")
+
+
+@pytest.fixture(scope="session")
+def boxes():
+    return frame_inputs("boxes_1080p")

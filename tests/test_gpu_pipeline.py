@@ -1,0 +1,312 @@
+"""Parity of the HIP pipeline (through the C ABI) against the CPU oracle.
+
+Integer work (allocation: claimed cells and positions; voxels and mips) must
+be bit-exact.  Floating-point results (SH coefficients, visibility, the apply
+output) must be within relative L2 1e-4 of the FP32 oracle on the same inputs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from canon import canonical, rel_l2
+from conftest import announce, frame_inputs
+
+pytestmark = pytest.mark.gpu
+
+SH_TOL = 1e-4      # relative L2 of SH coefficients vs the FP32 oracle
+RGB_TOL = 1e-4     # relative L2 of the apply output
+VIS_TOL = 1e-4     # max |visibility difference|
+
+
+def _orc():
+    from oracle import orc
+
+    return orc
+
+
+def _renderer(fi, **overrides):
+    from dynamicradiancevolume_amd.pipeline import Renderer
+
+    cfg = fi.workload.with_(**overrides).config() if overrides else fi.workload.config()
+    return Renderer(cfg)
+
+
+def _device(fi):
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame
+
+    return DeviceFrame(fi)
+
+
+def _assert_alloc_equal(r, fi, cfg, orc, capacity=None):
+    vol = orc.update_volume(cfg, fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    count, cav, entries = orc.allocate(cfg, vol, fi.frame(), fi.depth)
+    demanded, live = r.cache_count()
+    assert demanded == count
+    assert live == min(count, cfg.max_caches)
+    g_entries = r.read_caches(live)
+    g_cav = r.read_cav()
+    cap = cfg.max_caches if capacity is None else capacity
+    gc, gp, gph, gok = canonical(demanded, g_cav, g_entries, cap)
+    oc, op, oph, ook = canonical(count, cav, entries, cap)
+    assert gok and ook
+    np.testing.assert_array_equal(gc, oc)
+    if cap >= count:
+        np.testing.assert_array_equal(gp, op)
+        assert gph == oph
+    return vol, count, cav, entries, g_entries, g_cav
+
+
+@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p"])
+def test_volume_ubo_matches_oracle(name):
+    orc = _orc()
+    fi = frame_inputs(name)
+    announce(f"volume ubo {name}")
+    r = _renderer(fi)
+    cfg = fi.workload.config()
+    r.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    v = r.volume()
+    o = orc.update_volume(cfg, fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    assert bytes(v) == bytes(o)
+
+
+@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p"])
+def test_allocation_bit_exact(name):
+    orc = _orc()
+    fi = frame_inputs(name)
+    df = _device(fi)
+    announce(f"allocate {name}")
+    r = _renderer(fi)
+    r.set_frame(fi.frame())
+    r.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    r.allocate_caches(df.gbuffer())
+    _assert_alloc_equal(r, fi, fi.workload.config(), orc)
+
+
+@pytest.mark.parametrize(
+    "overrides",
+    [
+        dict(width=250, height=131),               # ragged tiles
+        dict(transition_size=0.0),                 # no transition list
+        dict(cav_cascades=1),                      # single cascade: clamped cells + phantoms
+        dict(cav_cascades=2, cav_resolution=16),
+    ],
+)
+def test_allocation_variants(overrides):
+    orc = _orc()
+    fi = frame_inputs("cornell_256", **{k: v for k, v in overrides.items() if k in ("width", "height")})
+    cfg = fi.workload.with_(**overrides).config()
+    df = _device(fi)
+    announce(f"allocate cornell {overrides}")
+    r = _renderer(fi, **overrides)
+    r.set_frame(fi.frame())
+    r.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    r.allocate_caches(df.gbuffer())
+    _assert_alloc_equal(r, fi, cfg, orc)
+
+
+def test_allocation_overflow_counts_and_clamps():
+    """Capacity below the live count: demanded count is still exact, only
+    `capacity` caches are stored, the claimed cell set is unchanged and apply
+    reads caches past the capacity as zero."""
+    orc = _orc()
+    fi = frame_inputs("cornell_256")
+    full = fi.workload.config()
+    vol = orc.update_volume(full, fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    count, _, _ = orc.allocate(full, vol, fi.frame(), fi.depth)
+    cap = count // 3
+    cfg = fi.workload.with_(max_caches=cap).config()
+    df = _device(fi)
+    announce(f"overflow cap={cap} count={count}")
+    r = _renderer(fi, max_caches=cap)
+    r.set_frame(fi.frame())
+    r.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    r.allocate_caches(df.gbuffer())
+    demanded, live = r.cache_count()
+    assert demanded == count and live == cap
+    g_entries = r.read_caches(live)
+    g_cav = r.read_cav()
+    gc, gp, _, gok = canonical(demanded, g_cav, g_entries, cap)
+    _, ocav, oentries = orc.allocate(full, vol, fi.frame(), fi.depth)
+    oc, op, _, _ = canonical(count, ocav, oentries, count)
+    assert gok
+    np.testing.assert_array_equal(gc, oc)
+    stored = gp[:, 0] != 0xFFFFFFFF
+    assert stored.sum() <= cap
+    np.testing.assert_array_equal(gp[stored], op[stored])
+    # gather + apply still run; apply must match the oracle on the same state
+    r.light_caches_rsm(df.rsm(), fi.light)
+    r.apply_caches(df.gbuffer(), df.out)
+    torch.cuda.synchronize()
+    g_entries = r.read_caches(live)
+    shc = orc.sh_constants()
+    ref = orc.apply(cfg, vol, fi.frame(), shc, fi, g_cav, g_entries, live)
+    assert rel_l2(df.out.cpu().numpy(), ref) < RGB_TOL
+
+
+def test_allocation_empty_gbuffer():
+    fi = frame_inputs("cornell_256")
+    df = _device(fi)
+    df.depth.zero_()
+    announce("allocate empty gbuffer")
+    r = _renderer(fi)
+    r.set_frame(fi.frame())
+    r.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    r.voxelize_scene(df.positions, len(fi.positions))
+    r.allocate_caches(df.gbuffer())
+    assert r.cache_count() == (0, 0)
+    r.light_caches_rsm(df.rsm(), fi.light)
+    df.out.fill_(7.0)
+    r.apply_caches(df.gbuffer(), df.out)
+    torch.cuda.synchronize()
+    assert float(df.out.abs().max()) == 0.0
+    assert not r.read_cav().any()
+
+
+@pytest.mark.parametrize("name,adapt", [("cornell_256", 1.0), ("boxes_1080p", 1.0)])
+def test_voxelization_bit_exact(name, adapt):
+    orc = _orc()
+    fi = frame_inputs(name)
+    w = fi.workload
+    df = _device(fi)
+    announce(f"voxelize {name}")
+    r = _renderer(fi)
+    r.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+    r.voxelize_scene(df.positions, len(fi.positions), adaption_factor=adapt)
+    vol = orc.update_volume(w.config(), w.eye, fi.bbox[:3], fi.bbox[3:])
+    target, mips = orc.voxelize(w.voxel_resolution, vol, fi.positions, adaption_factor=adapt)
+    np.testing.assert_array_equal(r.read_voxels(-1), target)
+    level = 0
+    while (w.voxel_resolution >> level) >= 1:
+        np.testing.assert_array_equal(r.read_voxels(level), orc.mip_level(mips, w.voxel_resolution, level),
+                                      err_msg=f"mip level {level}")
+        level += 1
+
+
+def test_voxel_blend_partial_adaption():
+    """Two frames: the second voxelizes a moved scene with AdaptionFactor 0.25,
+    so texels move a quarter step towards the new target."""
+    orc = _orc()
+    fi = frame_inputs("cornell_256")
+    w = fi.workload
+    df = _device(fi)
+    moved = fi.positions.copy()
+    moved[:, 0::3] += 0.7
+    moved_dev = torch.from_numpy(moved).cuda()
+    world = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1)
+    announce("voxel blend 0.25")
+    r = _renderer(fi)
+    r.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+    r.voxelize_scene(df.positions, len(fi.positions), adaption_factor=1.0)
+    r.voxelize_scene(moved_dev, len(moved), world=world, adaption_factor=0.25)
+    vol = orc.update_volume(w.config(), w.eye, fi.bbox[:3], fi.bbox[3:])
+    _, mips = orc.voxelize(w.voxel_resolution, vol, fi.positions, adaption_factor=1.0)
+    _, mips2 = orc.voxelize(w.voxel_resolution, vol, moved, adaption_factor=0.25, previous=mips)
+    for level in range(3):
+        np.testing.assert_array_equal(r.read_voxels(level), orc.mip_level(mips2, w.voxel_resolution, level))
+
+
+def _gather_case(name, n_sub=None, **overrides):
+    orc = _orc()
+    fi = frame_inputs(name)
+    w = fi.workload.with_(**overrides) if overrides else fi.workload
+    cfg = w.config()
+    df = _device(fi)
+    r = _renderer(fi, **overrides)
+    r.set_frame(fi.frame())
+    r.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+    r.voxelize_scene(df.positions, len(fi.positions), adaption_factor=w.adaption_factor)
+    r.allocate_caches(df.gbuffer())
+    _, live = r.cache_count()
+    before = r.read_caches(live)
+    r.light_caches_rsm(df.rsm(), fi.light)
+    after = r.read_caches(live)
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    _, mips = orc.voxelize(w.voxel_resolution, vol, fi.positions, adaption_factor=w.adaption_factor)
+    ids = np.arange(live, dtype=np.uint32) if n_sub is None else np.linspace(0, live - 1, n_sub).astype(np.uint32)
+    ref = before.copy()
+    vis, _ = orc.gather_f32(cfg, vol, orc.sh_constants(), fi.light, fi, mips, ref, ids=ids,
+                            want_visibility=bool(cfg.indirect_shadow))
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    g = entries_sh(after[ids], cfg.sh_bands)
+    o = entries_sh(ref[ids], cfg.sh_bands)
+    np.testing.assert_array_equal(after[ids, :3], before[ids, :3])
+    err = rel_l2(g, o)
+    assert err < SH_TOL, f"SH relative L2 {err:.3e}"
+    if cfg.indirect_shadow:
+        nb = vis.shape[1]
+        gvis = r.read_visibility(live, nb)[ids]
+        assert np.abs(gvis - vis).max() < VIS_TOL
+    return r, df, fi, cfg, vol, live
+
+
+def test_gather_cornell_sh1_shadowed():
+    announce("gather cornell sh1 shadow")
+    _gather_case("cornell_256")
+
+
+def test_gather_cornell_sh2_shadowed():
+    announce("gather cornell sh2 shadow")
+    _gather_case("cornell_256", sh_bands=2)
+
+
+def test_gather_cornell_unshadowed():
+    announce("gather cornell no shadow")
+    _gather_case("cornell_256", indirect_shadow=0, sh_bands=2)
+
+
+def test_gather_boxes_subset():
+    announce("gather boxes_1080p subset")
+    _gather_case("boxes_1080p", n_sub=384)
+
+
+def test_gather_two_lights_accumulate():
+    """LightCachesRSM runs once per light and accumulates (renderer.cpp:924-932)."""
+    orc = _orc()
+    announce("gather two lights")
+    r, df, fi, cfg, vol, live = _gather_case("cornell_256")
+    once = r.read_caches(live)
+    r.light_caches_rsm(df.rsm(), fi.light)
+    twice = r.read_caches(live)
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    assert rel_l2(entries_sh(twice, 1), 2.0 * entries_sh(once, 1)) < SH_TOL
+
+
+@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p"])
+def test_full_frame_matches_oracle(name):
+    """Voxelize -> allocate -> gather -> apply on the GPU against the whole
+    oracle frame (cells matched canonically, SH by cell, RGB per pixel)."""
+    orc = _orc()
+    fi = frame_inputs(name)
+    w = fi.workload
+    cfg = w.config()
+    df = _device(fi)
+    announce(f"full frame {name}")
+    r = _renderer(fi)
+    r.render_frame(df)
+    torch.cuda.synchronize()
+    demanded, live = r.cache_count()
+    g_entries = r.read_caches(live)
+    g_cav = r.read_cav()
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    count, o_cav, o_entries = orc.allocate(cfg, vol, fi.frame(), fi.depth)
+    assert demanded == count
+    gc, gp, _, _ = canonical(demanded, g_cav, g_entries, cfg.max_caches)
+    oc, op, _, _ = canonical(count, o_cav, o_entries, cfg.max_caches)
+    np.testing.assert_array_equal(gc, oc)
+    np.testing.assert_array_equal(gp, op)
+    # Oracle apply over the GPU's caches: checks apply exactly
+    shc = orc.sh_constants()
+    ref = orc.apply(cfg, vol, fi.frame(), shc, fi, g_cav, g_entries, live)
+    got = df.out.cpu().numpy()
+    assert rel_l2(got, ref) < RGB_TOL
+    if name == "cornell_256":
+        # and the independent oracle frame end to end
+        ro = orc.run_frame(fi)
+        from dynamicradiancevolume_amd.pipeline import entries_sh
+
+        gsh = entries_sh(g_entries[(g_cav.reshape(-1)[gc] - 1)], cfg.sh_bands)
+        osh = entries_sh(ro["entries"][(ro["cav"].reshape(-1)[oc] - 1)], cfg.sh_bands)
+        assert rel_l2(gsh, osh) < SH_TOL
+        assert rel_l2(got, ro["rgb"]) < RGB_TOL
