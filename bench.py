@@ -123,13 +123,13 @@ def main() -> int:
             "gather_ms", "reduce_ms", "apply_ms")
     acc = {k: 0.0 for k in keys}
     n_prof = max(3, min(10, args.steps))
-    traces = steps_ = 0
+    traces = steps_ = steps_exec = 0
     for _ in range(n_prof):
         r.render_frame(df)
         t = r.stage_times()
         for k in keys:
             acc[k] += t[k]
-        traces, steps_ = t["cone_traces"], t["cone_steps"]
+        traces, steps_, steps_exec = t["cone_traces"], t["cone_steps"], t["cone_steps_executed"]
     stages = {k: round(v / n_prof, 4) for k, v in acc.items()}
     r.set_profiling(0)
 
@@ -185,6 +185,7 @@ def main() -> int:
             "triangles": int(len(fi.positions)),
             "cone_traces": traces,
             "cone_steps": steps_,
+            "cone_steps_executed": steps_exec,
             "stage_ms": stages,
             "path": "plain cone-traced",
         },

@@ -134,7 +134,7 @@ class DrvStageTimes(C.Structure):
         (n, C.c_float)
         for n in ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
                   "gather_ms", "reduce_ms", "apply_ms")
-    ] + [("cone_traces", C.c_uint64), ("cone_steps", C.c_uint64)]
+    ] + [("cone_traces", C.c_uint64), ("cone_steps", C.c_uint64), ("cone_steps_executed", C.c_uint64)]
 
 
 P = C.POINTER

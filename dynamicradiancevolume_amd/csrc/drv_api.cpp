@@ -53,7 +53,7 @@ struct drv_ctx {
 
   bool profiling = false;
   int profiling_level = 0;
-  uint64_t* step_counter = nullptr;  // [traces, steps] when counting
+  uint64_t* step_counter = nullptr;  // [traces, reference steps, executed steps] when counting
   hipEvent_t ev[16] = {};
   drv_stage_times times{};
 };
@@ -458,8 +458,8 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   const int grid = persistent_grid(ctx);
   uint64_t* steps = nullptr;
   if (ctx->profiling && ctx->profiling_level >= 2) {
-    if (!ctx->step_counter && (rc = dev_alloc(ctx, &ctx->step_counter, 2)) != DRV_OK) return rc;
-    drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->step_counter), 4, 0u, ctx->stream);
+    if (!ctx->step_counter && (rc = dev_alloc(ctx, &ctx->step_counter, 3)) != DRV_OK) return rc;
+    drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->step_counter), 6, 0u, ctx->stream);
     steps = ctx->step_counter;
   }
   record(ctx, 5);
@@ -581,10 +581,11 @@ int drv_get_stage_times(drv_ctx* ctx, drv_stage_times* out) {
     ctx->times.reduce_ms = elapsed(ctx, 8, 9);
     ctx->times.apply_ms = elapsed(ctx, 10, 11);
     if (ctx->counted_steps && ctx->step_counter) {
-      uint64_t st[2] = {0, 0};
+      uint64_t st[3] = {0, 0, 0};
       DRV_HIP(ctx, hipMemcpy(st, ctx->step_counter, sizeof(st), hipMemcpyDeviceToHost));
       ctx->times.cone_traces = st[0];
       ctx->times.cone_steps = st[1];
+      ctx->times.cone_steps_executed = st[2];
     }
   }
   *out = ctx->times;

@@ -135,91 +135,195 @@ struct VisArgs {
   uint32_t n_blocks;
   float* vis;
   uint32_t pitch;
-  uint64_t* steps;  // optional {traces, steps}
+  uint64_t* steps;  // optional {traces, reference steps, executed steps}
 };
 
 constexpr uint32_t kVisBlocksPerItem = 8;
 
-// Trilinear sample of one mip level from the corner-expanded layout
-// (GL 4.5 section 8.14.2, clamp to edge).
-__device__ __forceinline__ float sample_level(const uint64_t* corners, int w, float px, float py, float pz) {
-  const float fw = static_cast<float>(w);
-  const float u = px * fw - 0.5f, v = py * fw - 0.5f, q = pz * fw - 0.5f;
+// Corner-expanded fetch address of point p at mip level l and its trilinear
+// weights (GL 4.5 section 8.14.2, clamp to edge).  p is in level-0 texel
+// units; the lower filter corner floor(p * 2^-l - 0.5) is clamped to
+// [-1, w - 1], which selects the same clamped texels.
+struct Footprint {
+  uint32_t index;
+  float ax, ay, az;
+};
+
+__device__ __forceinline__ Footprint footprint(const VoxelView& vx, int l, float px, float py, float pz) {
+  const int w = vx.res >> l;
+  const float scale = __builtin_amdgcn_ldexpf(1.0f, -l);
+  const float u = __builtin_fmaf(px, scale, -0.5f);
+  const float v = __builtin_fmaf(py, scale, -0.5f);
+  const float q = __builtin_fmaf(pz, scale, -0.5f);
   const float fu = floorf(u), fv = floorf(v), fq = floorf(q);
-  const float ax = u - fu, ay = v - fv, az = q - fq;
-  const int i = min(w - 1, max(-1, to_int(fu)));
-  const int j = min(w - 1, max(-1, to_int(fv)));
-  const int k = min(w - 1, max(-1, to_int(fq)));
-  const int wp = w + 1;
-  const uint64_t c = corners[(static_cast<size_t>(k + 1) * wp + (j + 1)) * wp + (i + 1)];
+  const float hi = static_cast<float>(w - 1);
+  // Clamped in float first so the conversion is always in range (NaN -> -1).
+  const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fu, -1.0f, hi) + 1.0f);
+  const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fv, -1.0f, hi) + 1.0f);
+  const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fq, -1.0f, hi) + 1.0f);
+  const uint32_t wp = static_cast<uint32_t>(w + 1);
+  Footprint f;
+  f.index = vx.corner_offset[l] + __umul24(__umul24(k, wp) + j, wp) + i;
+  f.ax = u - fu;
+  f.ay = v - fv;
+  f.az = q - fq;
+  return f;
+}
+
+// Trilinear blend of the 8 packed unorm8 texels (byte dx + 2 dy + 4 dz),
+// in units of 1/255.
+__device__ __forceinline__ float blend8(uint64_t c, float ax, float ay, float az) {
   const uint32_t lo = static_cast<uint32_t>(c), hi = static_cast<uint32_t>(c >> 32);
-  const float t000 = static_cast<float>(lo & 0xffu) / 255.0f;
-  const float t100 = static_cast<float>((lo >> 8) & 0xffu) / 255.0f;
-  const float t010 = static_cast<float>((lo >> 16) & 0xffu) / 255.0f;
-  const float t110 = static_cast<float>(lo >> 24) / 255.0f;
-  const float t001 = static_cast<float>(hi & 0xffu) / 255.0f;
-  const float t101 = static_cast<float>((hi >> 8) & 0xffu) / 255.0f;
-  const float t011 = static_cast<float>((hi >> 16) & 0xffu) / 255.0f;
-  const float t111 = static_cast<float>(hi >> 24) / 255.0f;
-  const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
-  return bx * by * bz * t000 + ax * by * bz * t100 + bx * ay * bz * t010 + ax * ay * bz * t110 +
-         bx * by * az * t001 + ax * by * az * t101 + bx * ay * az * t011 + ax * ay * az * t111;
+  const float t000 = static_cast<float>(lo & 0xffu), t100 = static_cast<float>((lo >> 8) & 0xffu);
+  const float t010 = static_cast<float>((lo >> 16) & 0xffu), t110 = static_cast<float>(lo >> 24);
+  const float t001 = static_cast<float>(hi & 0xffu), t101 = static_cast<float>((hi >> 8) & 0xffu);
+  const float t011 = static_cast<float>((hi >> 16) & 0xffu), t111 = static_cast<float>(hi >> 24);
+  const float x00 = __builtin_fmaf(ax, t100 - t000, t000);
+  const float x10 = __builtin_fmaf(ax, t110 - t010, t010);
+  const float x01 = __builtin_fmaf(ax, t101 - t001, t001);
+  const float x11 = __builtin_fmaf(ax, t111 - t011, t011);
+  const float y0 = __builtin_fmaf(ay, x10 - x00, x00);
+  const float y1 = __builtin_fmaf(ay, x11 - x01, x01);
+  return __builtin_fmaf(az, y1 - y0, y0);
 }
 
-// textureLod(VoxelVolume, p, lod): LINEAR_MIPMAP_LINEAR, base level 0.
-__device__ __forceinline__ float sample_voxels(const VoxelView& vx, float px, float py, float pz, float lod) {
-  const int q = vx.levels - 1;
-  if (!(lod > 0.0f)) return sample_level(vx.corners, vx.res, px, py, pz);
-  if (lod >= static_cast<float>(q)) return sample_level(vx.corners + vx.corner_offset[q], vx.res >> q, px, py, pz);
-  const float fl = floorf(lod);
-  const int d1 = to_int(fl);
-  const float f = lod - fl;
-  const float t1 = sample_level(vx.corners + vx.corner_offset[d1], vx.res >> d1, px, py, pz);
-  const float t2 = sample_level(vx.corners + vx.corner_offset[d1 + 1], vx.res >> (d1 + 1), px, py, pz);
-  return (1.0f - f) * t1 + f * t2;
-}
+// Below this transmittance a ray counts as fully occluded: the remaining
+// samples can lower the returned visibility by at most this much.
+constexpr float kOpaque = 1.0f / 1048576.0f;
+constexpr int kStepBatch = 4;
 
-__device__ __forceinline__ float cone_trace(const VisArgs& a, f3 wp, f3 vpos, const ShadowBlock& b, int& n_steps) {
-  // cacheLightingRSM.comp:195-230
+// Per-lane state of one cone trace (cacheLightingRSM.comp:195-210).
+struct Trace {
+  float px, py, pz;  // current sample position, level-0 texel units
+  float dx, dy, dz;  // unit direction, texel units per step unit
+  float dist, step, goal, rho, k;
+  float occ;
+  int s;             // samples taken
+  bool done;         // dist reached goal (the reference's break)
+};
+
+__device__ __forceinline__ Trace trace_setup(const VisArgs& a, f3 wp, f3 vpos, const ShadowBlock& b) {
   const float fres = static_cast<float>(a.vox.res);
   float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
   const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
   tx = tx / ld;
   ty = ty / ld;
   tz = tz / ld;
-  const float dx = tx / fres, dy = ty / fres, dz = tz / fres;
-  float cx = vpos.x + dx * 2.0f, cy = vpos.y + dy * 2.0f, cz = vpos.z + dz * 2.0f;
-  float occ = 0.0f, step = 1.0f, dist = 0.0f;
-  const float goal = ld / a.voxel_size - 2.0f;
-  const float k = 2.0f / (1.0f - b.rho);
-  int s = 0;
-  for (; s < 32; ++s) {
-    cx = cx + dx * step;
-    cy = cy + dy * step;
-    cz = cz + dz * step;
-    dist = dist + step;
-    const float r = dist * b.rho;
-    const float o = sample_voxels(a.vox, cx, cy, cz, log2f(r));
-    occ = occ + (1.0f - occ) * o;
-    if (dist >= goal) break;
-    step = fmaxf(1.0f, r * k);
-  }
-  n_steps = min(s + 1, 32);
-  return fminf(1.0f, fmaxf(0.0f, 1.0f - occ));
+  Trace t;
+  // dirInVoxel = toAverageVal / volumeSize; positions are kept scaled by res.
+  t.dx = tx;
+  t.dy = ty;
+  t.dz = tz;
+  t.px = vpos.x * fres + tx * 2.0f;
+  t.py = vpos.y * fres + ty * 2.0f;
+  t.pz = vpos.z * fres + tz * 2.0f;
+  t.dist = 0.0f;
+  t.step = 1.0f;
+  t.goal = ld / a.voxel_size - 2.0f;
+  t.rho = b.rho;
+  t.k = 2.0f / (1.0f - b.rho);
+  t.occ = 0.0f;
+  t.s = 0;
+  t.done = false;
+  return t;
 }
 
+// Advances one trace by up to kStepBatch samples (cacheLightingRSM.comp:211-228).
+// The sample positions depend only on the geometry, so all samples of the
+// batch are planned first and their loads issued together; occlusion is then
+// accumulated in step order.  Returns true when the trace has finished.
+__device__ __forceinline__ bool trace_batch(const VoxelView& vx, Trace& t) {
+  const int q = vx.levels - 1;
+  const float qf = static_cast<float>(q);
+  float sx[kStepBatch], sy[kStepBatch], sz[kStepBatch], frac[kStepBatch];
+  int lv[kStepBatch];
+  Footprint f1[kStepBatch];
+  bool take[kStepBatch];
+  bool need2 = false;
+#pragma unroll
+  for (int i = 0; i < kStepBatch; ++i) {
+    take[i] = !t.done && (t.s + i) < 32;
+    t.px = __builtin_fmaf(t.dx, t.step, t.px);
+    t.py = __builtin_fmaf(t.dy, t.step, t.py);
+    t.pz = __builtin_fmaf(t.dz, t.step, t.pz);
+    t.dist = t.dist + t.step;
+    const float r = t.dist * t.rho;
+    // LINEAR_MIPMAP_LINEAR level pair for lod = log2(r), clamped to [0, q]
+    // (GL 4.5 section 8.14.3); frac = 0 means a single level.
+    const float lod = __builtin_amdgcn_fmed3f(__builtin_amdgcn_logf(r), 0.0f, qf);
+    const float fl = floorf(lod);
+    lv[i] = static_cast<int>(fl);
+    frac[i] = take[i] ? lod - fl : 0.0f;
+    need2 = need2 || frac[i] > 0.0f;
+    sx[i] = t.px;
+    sy[i] = t.py;
+    sz[i] = t.pz;
+    f1[i] = footprint(vx, lv[i], t.px, t.py, t.pz);
+    if (take[i]) {
+      if (t.dist >= t.goal) t.done = true;
+      else t.step = fmaxf(1.0f, r * t.k);
+    }
+  }
+  uint64_t c1[kStepBatch], c2[kStepBatch];
+#pragma unroll
+  for (int i = 0; i < kStepBatch; ++i) c1[i] = take[i] ? vx.corners[f1[i].index] : 0ull;
+  float o[kStepBatch];
+  // The second level is only planned when some lane of the wave blends two.
+  if (__any(need2)) {
+    Footprint f2[kStepBatch];
+#pragma unroll
+    for (int i = 0; i < kStepBatch; ++i) {
+      f2[i] = footprint(vx, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
+      c2[i] = frac[i] > 0.0f ? vx.corners[f2[i].index] : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < kStepBatch; ++i) {
+      const float o1 = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+      const float o2 = blend8(c2[i], f2[i].ax, f2[i].ay, f2[i].az);
+      o[i] = __builtin_fmaf(frac[i], o2 - o1, o1);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kStepBatch; ++i) o[i] = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+  }
+#pragma unroll
+  for (int i = 0; i < kStepBatch; ++i) {
+    if (take[i]) {
+      const float v = o[i] * (1.0f / 255.0f);
+      t.occ = t.occ + (1.0f - t.occ) * v;
+      ++t.s;
+    }
+  }
+  return t.done || t.s >= 32 || (1.0f - t.occ) <= kOpaque;
+}
+
+// Steps the reference still takes after a trace stopped early (geometry only).
+__device__ __forceinline__ int reference_steps(Trace t) {
+  int s = t.s;
+  if (t.done) return s;
+  while (s < 32) {
+    t.dist = t.dist + t.step;
+    ++s;
+    if (t.dist >= t.goal) break;
+    t.step = fmaxf(1.0f, t.dist * t.rho * t.k);
+  }
+  return s;
+}
+
+// A wave owns 64 caches x kVisBlocksPerItem shadow blocks and traces one
+// block at a time for all its lanes (one uniform block record, coherent
+// rays from neighbouring caches towards the same VPL block).
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t groups = (a.n_blocks + kVisBlocksPerItem - 1) / kVisBlocksPerItem;
   const uint32_t items = tiles * groups;
   const uint32_t lane = threadIdx.x & 63u;
-  // Wave-uniform work index (readfirstlane lets the compiler keep the item
-  // loop and the VPL / block addresses in SGPRs).
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  const float vsz = a.voxel_size * static_cast<float>(a.vox.res);
-  uint32_t my_traces = 0, my_steps = 0;
+  const float fres = static_cast<float>(a.vox.res);
+  const float vsz = a.voxel_size * fres;
+  uint32_t my_traces = 0, my_steps = 0, my_exec = 0;
   for (uint32_t item = wave; item < items; item += n_waves) {
     const uint32_t tile = item / groups, group = item % groups;
     const uint32_t cache = tile * 64u + lane;
@@ -231,23 +335,27 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     const uint32_t b0 = group * kVisBlocksPerItem;
     const uint32_t b1 = min(a.n_blocks, b0 + kVisBlocksPerItem);
     for (uint32_t b = b0; b < b1; ++b) {
-      const ShadowBlock blk = a.blocks[b];
-      int st = 0;
-      a.vis[static_cast<size_t>(b) * a.pitch + cache] = cone_trace(a, wp, vpos, blk, st);
+      Trace t = trace_setup(a, wp, vpos, a.blocks[b]);
+      bool finished = false;
+      while (!finished) finished = trace_batch(a.vox, t);
+      a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
       my_traces += 1u;
-      my_steps += static_cast<uint32_t>(st);
+      my_exec += static_cast<uint32_t>(t.s);
+      if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
     }
   }
   if (a.steps) {
-    // One atomic pair per wave after a butterfly sum.
-    uint64_t t = my_traces, sv = my_steps;
+    // One atomic triple per wave after a butterfly sum.
+    uint64_t tr = my_traces, sv = my_steps, ex = my_exec;
     for (int off = 32; off > 0; off >>= 1) {
-      t += __shfl_xor(t, off);
+      tr += __shfl_xor(tr, off);
       sv += __shfl_xor(sv, off);
+      ex += __shfl_xor(ex, off);
     }
-    if (lane == 0 && (t | sv)) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[0]), static_cast<unsigned long long>(t));
+    if (lane == 0 && (tr | sv)) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[0]), static_cast<unsigned long long>(tr));
       atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[1]), static_cast<unsigned long long>(sv));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[2]), static_cast<unsigned long long>(ex));
     }
   }
 }
@@ -340,34 +448,38 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
 
 // Sums the split partials in split order, applies the SH evaluation factors
 // (cacheLightingRSM.comp:267-277) and adds into the entry
-// (cacheLightingRSM.comp:357-373).
-__global__ void k_reduce(float* caches, int stride, const Counter* counter, const float* partial,
-                         uint32_t partial_pitch, drv_sh_constants shc, int sh2) {
+// (cacheLightingRSM.comp:357-373).  One lane per (cache, coefficient,
+// channel); every lane owns one entry field.
+__global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const Counter* counter,
+                                                const float* partial, uint32_t partial_pitch, drv_sh_constants shc,
+                                                int sh2) {
   const uint32_t live = counter->live;
   const uint32_t splits = counter->splits;
-  const int n_raw = sh2 ? 27 : 12;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < live; c += gridDim.x * blockDim.x) {
-    float s[27];
-    for (int k = 0; k < 27; ++k) s[k] = 0.0f;
+  const uint32_t n_coef = sh2 ? 9u : 4u;
+  const uint32_t total = live * n_coef * 3u;
+  const size_t split_stride = static_cast<size_t>(27u) * partial_pitch;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t c = t % live;
+    const uint32_t jc = t / live;  // coefficient * 3 + channel
+    const uint32_t j = jc / 3u, ch = jc % 3u;
+    const float* p = partial + static_cast<size_t>(jc) * partial_pitch + c;
+    const float* p0 = partial + static_cast<size_t>(ch) * partial_pitch + c;  // raw sum of rad
+    float v = 0.0f, v0 = 0.0f;
     for (uint32_t sp = 0; sp < splits; ++sp) {
-      const float* p = partial + static_cast<size_t>(sp) * 27u * partial_pitch + c;
-      for (int k = 0; k < n_raw; ++k) s[k] += p[static_cast<size_t>(k) * partial_pitch];
+      v += p[sp * split_stride];
+      if (j == 6u) v0 += p0[sp * split_stride];
     }
     float* e = caches + static_cast<size_t>(c) * stride;
-    for (int ch = 0; ch < 3; ++ch) {
-      const float r0 = s[0 + ch], r1 = s[3 + ch], r2 = s[6 + ch], r3 = s[9 + ch];
-      e[7 + 4 * ch] += shc.eva0 * r0;           // SH00_r/g/b at 7, 11, 15
-      e[4 + ch] += -(shc.eva1 * r1);            // SH1neg1
-      e[8 + ch] += shc.eva1 * r2;               // SH10
-      e[12 + ch] += -(shc.eva1 * r3);           // SH1pos1
-      if (sh2) {
-        const float r4 = s[12 + ch], r5 = s[15 + ch], r6 = s[18 + ch], r7 = s[21 + ch], r8 = s[24 + ch];
-        e[16 + ch] += -(shc.eva2n2_p1_n1 * r4);             // SH2neg2
-        e[20 + ch] += shc.eva2n2_p1_n1 * r5;                // SH2neg1
-        e[19 + 4 * ch] += shc.eva20 * (3.0f * r6 - r0);     // SH20_r/g/b at 19, 23, 27
-        e[24 + ch] += shc.eva2n2_p1_n1 * r7;                // SH2pos1
-        e[28 + ch] += shc.eva2p2 * r8;                      // SH2pos2
-      }
+    switch (j) {
+      case 0: e[7 + 4 * ch] += shc.eva0 * v; break;                       // SH00_r/g/b at 7, 11, 15
+      case 1: e[4 + ch] += -(shc.eva1 * v); break;                        // SH1neg1
+      case 2: e[8 + ch] += shc.eva1 * v; break;                           // SH10
+      case 3: e[12 + ch] += -(shc.eva1 * v); break;                       // SH1pos1
+      case 4: e[16 + ch] += -(shc.eva2n2_p1_n1 * v); break;               // SH2neg2
+      case 5: e[20 + ch] += shc.eva2n2_p1_n1 * v; break;                  // SH2neg1
+      case 6: e[19 + 4 * ch] += shc.eva20 * (3.0f * v - v0); break;       // SH20_r/g/b at 19, 23, 27
+      case 7: e[24 + ch] += shc.eva2n2_p1_n1 * v; break;                  // SH2pos1
+      default: e[28 + ch] += shc.eva2p2 * v; break;                       // SH2pos2
     }
   }
 }

@@ -164,7 +164,8 @@ typedef struct drv_stage_times {
   float reduce_ms;       /* split reduction into the cache entries */
   float apply_ms;
   uint64_t cone_traces;  /* (cache, shadow block) traces of the last light, when counting */
-  uint64_t cone_steps;   /* march steps of those traces, when counting */
+  uint64_t cone_steps;   /* march steps the reference takes for those traces, when counting */
+  uint64_t cone_steps_executed; /* steps actually sampled (rays stop once opaque) */
 } drv_stage_times;
 
 /* ---- lifetime -------------------------------------------------------- */

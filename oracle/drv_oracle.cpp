@@ -802,6 +802,21 @@ float orc_cone_trace_f32(const drv_volume* vol, int32_t res, const uint8_t* mips
   return s;
 }
 
+float orc_cone_march_f32(const drv_volume* vol, int32_t res, const uint8_t* mips, const float wp[3],
+                         const float avg_pos[3], float rho, int32_t* steps_out) {
+  const V3<float> w = mk<float>(wp[0], wp[1], wp[2]);
+  const float vsz = vol->voxel_size_in_world * static_cast<float>(res);
+  const V3<float> vp = mk<float>((w.x - vol->volume_world_min[0]) / vsz, (w.y - vol->volume_world_min[1]) / vsz,
+                                 (w.z - vol->volume_world_min[2]) / vsz);
+  Block<float> b;
+  b.avg_pos = mk<float>(avg_pos[0], avg_pos[1], avg_pos[2]);
+  b.rho = rho;
+  int st = 0;
+  const float s = cone_trace<float>(vol, res, mips, w, vp, b, &st);
+  if (steps_out) *steps_out = st;
+  return s;
+}
+
 void orc_entry_sh(const float* e, int32_t sh_bands, float out[27]) {
   const int n = sh_bands >= 2 ? 9 : 4;
   for (int k = 0; k < 9; ++k)

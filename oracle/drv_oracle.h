@@ -110,6 +110,10 @@ void orc_gather_f64(const drv_config* cfg, const drv_volume* vol, const drv_sh_c
 float orc_cone_trace_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t* voxel_mips,
                          const drv_spot_light* light, const orc_rsm* rsm, const float wp[3], uint32_t block_morton,
                          int32_t* steps_out);
+/* The cone march alone for a given averaged VAL position and aperture
+ * (cacheLightingRSM.comp:195-230), FP32. */
+float orc_cone_march_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t* voxel_mips, const float wp[3],
+                         const float avg_pos[3], float rho, int32_t* steps_out);
 /* Extracts the 27 SH coefficients (canonical order above) from an entry. */
 void orc_entry_sh(const float* entry, int32_t sh_bands, float out27[27]);
 

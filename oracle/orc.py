@@ -55,6 +55,8 @@ _FUNCS = {
                               P(OrcRsm), vp, vp, C.c_int64, vp, vp, C.c_int32]),
     "orc_cone_trace_f32": (C.c_float, [P(_abi.DrvVolume), C.c_int32, vp, P(_abi.DrvSpotLight), P(OrcRsm),
                                        P(C.c_float), C.c_uint32, P(C.c_int32)]),
+    "orc_cone_march_f32": (C.c_float, [P(_abi.DrvVolume), C.c_int32, vp, P(C.c_float), P(C.c_float), C.c_float,
+                                       P(C.c_int32)]),
     "orc_entry_sh": (None, [P(C.c_float), C.c_int32, P(C.c_float)]),
     "orc_apply": (None, [P(_abi.DrvConfig), P(_abi.DrvVolume), P(_abi.DrvFrame), P(_abi.DrvShConstants), vp, vp, vp,
                          vp, vp, C.c_uint32, vp, C.c_int32]),
@@ -156,7 +158,7 @@ def gather_f32(cfg, vol, shc, light, fi, mips: np.ndarray, entries: np.ndarray, 
     vis = np.zeros((n, nb), np.float32) if want_visibility and nb else None
     steps = C.c_int64(0)
     lib().orc_gather_f32(C.byref(cfg), C.byref(vol), C.byref(shc), C.byref(light), C.byref(rsm),
-                         _ptr(np.ascontiguousarray(mips)), _ptr(entries), _ptr(ids) if ids is not None else None,
+                         _ptr(np.ascontiguousarray(mips)) if mips is not None else None, _ptr(entries), _ptr(ids) if ids is not None else None,
                          n, _ptr(vis) if vis is not None else None, C.byref(steps), n_threads)
     del keep
     return vis, steps.value
@@ -171,7 +173,7 @@ def gather_f64(cfg, vol, shc, light, fi, mips: np.ndarray, positions: np.ndarray
     nb = (fi.read_resolution ** 2) // int(light.indirect_shadow_sample_interval) if cfg.indirect_shadow else 0
     vis = np.zeros((n, nb), np.float64) if want_visibility and nb else None
     lib().orc_gather_f64(C.byref(cfg), C.byref(vol), C.byref(shc), C.byref(light), C.byref(rsm),
-                         _ptr(np.ascontiguousarray(mips)), _ptr(pos), n, _ptr(sh),
+                         _ptr(np.ascontiguousarray(mips)) if mips is not None else None, _ptr(pos), n, _ptr(sh),
                          _ptr(vis) if vis is not None else None, n_threads)
     del keep
     return sh, vis

@@ -1,0 +1,635 @@
+"""Pins of the CPU oracle against closed forms, textbook routines, invariants
+and brute force on small inputs (no GPU).  The reference ships no test
+vectors, so each check is chosen so that a plausible mistake in the oracle
+(a dropped term, a wrong sign or index, a transposed operand) fails it."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+from canon import canonical, rel_l2
+from conftest import frame_inputs
+
+orc = pytest.importorskip("oracle.orc")
+from dynamicradiancevolume_amd import _abi  # noqa: E402
+
+L = orc.lib
+
+
+# ---- Morton decode (cacheLightingRSM.comp:46-62) ---------------------------
+
+def test_morton_decode_exhaustive_16bit():
+    m = np.arange(1 << 16, dtype=np.uint64)
+    bits = (m[:, None] >> np.arange(16, dtype=np.uint64)) & 1
+    ex = (bits[:, 0::2] << np.arange(8, dtype=np.uint64)).sum(1)
+    ey = (bits[:, 1::2] << np.arange(8, dtype=np.uint64)).sum(1)
+    x, y = C.c_uint32(), C.c_uint32()
+    for i in list(range(0, 1 << 16, 7)) + [0xFFFF, 0xAAAA, 0x5555]:
+        L().orc_morton_decode(i, C.byref(x), C.byref(y))
+        assert (x.value, y.value) == (int(ex[i]), int(ey[i]))
+
+
+def test_morton_decode_is_inverse_of_interleave():
+    x, y = C.c_uint32(), C.c_uint32()
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        a, b = (int(v) for v in rng.integers(0, 1 << 16, 2))
+        m = 0
+        for k in range(16):
+            m |= ((a >> k) & 1) << (2 * k) | ((b >> k) & 1) << (2 * k + 1)
+        L().orc_morton_decode(m, C.byref(x), C.byref(y))
+        assert (x.value, y.value) == (a, b)
+
+
+# ---- SH factors (renderer.cpp:296-306) ------------------------------------
+
+def test_sh_constants_match_real_sh_and_irradiance_convolution():
+    s = orc.sh_constants()
+    # Evaluation factors are the real SH normalizations (scipy, CS phase).
+    th, ph = 0.7, 1.1
+    d = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    y00 = sps.sph_harm_y(0, 0, th, ph).real
+    y10 = sps.sph_harm_y(1, 0, th, ph).real
+    y20 = sps.sph_harm_y(2, 0, th, ph).real
+    y22 = math.sqrt(2) * sps.sph_harm_y(2, 2, th, ph).real
+    y21 = -math.sqrt(2) * sps.sph_harm_y(2, 1, th, ph).real
+    assert s.eva0 == pytest.approx(y00, rel=1e-6)
+    assert s.eva1 * d[2] == pytest.approx(y10, rel=1e-6)
+    assert s.eva20 * (3 * d[2] ** 2 - 1) == pytest.approx(y20, rel=1e-6)
+    assert s.eva2p2 * (d[0] ** 2 - d[1] ** 2) == pytest.approx(y22, rel=1e-6)
+    assert s.eva2n2_p1_n1 * d[0] * d[2] == pytest.approx(y21, rel=1e-6)
+    # Cosine-lobe factors = A_l * normalization, A = (pi, 2pi/3, pi/4)
+    # (Ramamoorthi & Hanrahan); the host sets the band-2 cross factor negative.
+    assert s.cos_lobe0 == pytest.approx(math.pi * s.eva0, rel=1e-6)
+    assert s.cos_lobe1 == pytest.approx(2 * math.pi / 3 * s.eva1, rel=1e-6)
+    assert s.cos_lobe2n2_p1_n1 == pytest.approx(-math.pi / 4 * s.eva2n2_p1_n1, rel=1e-6)
+    assert s.cos_lobe20 == pytest.approx(math.pi / 4 * s.eva20, rel=1e-6)
+    assert s.cos_lobe2p2 == pytest.approx(math.pi / 4 * s.eva2p2, rel=1e-6)
+
+
+# ---- normals (utils.glsl:43-89) -------------------------------------------
+
+def test_normal_pack_unpack_roundtrip():
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=(4000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    v = np.vstack([v, np.eye(3), -np.eye(3)])
+    out = (C.c_int16 * 2)()
+    n = (C.c_float * 3)()
+    worst = 0.0
+    for row in v:
+        L().orc_pack_normal16i(_abi.fvec(row, 3), out)
+        L().orc_unpack_normal16i(out[0], out[1], n)
+        u = np.array(n[:])
+        assert abs(np.linalg.norm(u) - 1) < 1e-5
+        worst = max(worst, float(np.linalg.norm(u - row)))
+    # 16-bit angle + z quantization; z = +-1 saturates at 32767 (sin phi ~ 8e-3).
+    assert worst < 1e-2
+
+
+# ---- cascade selection / unprojection (lightcache.glsl:109-134) -----------
+
+def _volume():
+    cfg = frame_inputs("cornell_256").workload.config()
+    return cfg, orc.update_volume(cfg, (0.3, -0.2, 0.1), (-4, -4, -4), (4, 4, 4))
+
+
+def test_update_volume_geometry():
+    cfg, v = _volume()
+    size = cfg.cav_cascade0_world_size
+    for c in range(cfg.cav_cascades):
+        k = v.cascades[c]
+        cell = size / cfg.cav_resolution
+        assert k.world_voxel_size == pytest.approx(cell)
+        for i, cam in enumerate((0.3, -0.2, 0.1)):
+            assert k.max[i] - k.min[i] == pytest.approx(size, rel=1e-5)
+            # snapped to the cell grid, centred on the camera within half a cell
+            assert abs((k.min[i] + size / 2) - cam) <= cell / 2 + 1e-5
+            assert k.decision_min[i] == pytest.approx(cam - size / 2 + 1.5 * cell, abs=1e-5)
+        size *= 2
+    assert v.voxel_size_in_world == pytest.approx((v.volume_world_max[0] - v.volume_world_min[0]) / cfg.voxel_resolution)
+
+
+def test_cascade_choice_and_transition():
+    cfg, v = _volume()
+    n = cfg.cav_cascades
+    rng = np.random.default_rng(3)
+    for p in rng.uniform(-40, 40, size=(3000, 3)):
+        c = L().orc_cascade(C.byref(v), n, _abi.fvec(p, 3))
+        inside = [all(v.cascades[k].decision_min[i] <= p[i] <= v.cascades[k].decision_max[i] for i in range(3))
+                  for k in range(n - 1)]
+        expect = inside.index(True) if any(inside) else n - 1
+        assert c == expect
+        t = L().orc_cascade_transition(C.byref(v), c, _abi.fvec(p, 3))
+        k = v.cascades[c]
+        m = min(min(k.decision_max[i] - p[i], p[i] - k.decision_min[i]) for i in range(3))
+        assert t == pytest.approx(min(1.0, max(0.0, 1 - m / (k.world_voxel_size * v.cav_transition_zone_size))),
+                                  abs=1e-6)
+
+
+def test_unproject_identity_matrix():
+    ident = _abi.fvec([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], 16)
+    out = (C.c_float * 3)()
+    for x, y, d in [(0, 0, 0.5), (255, 0, 0.25), (17, 200, 0.75)]:
+        L().orc_unproject_pixel(ident, 256, 256, x, y, d, out)
+        assert out[0] == pytest.approx((x + 0.5) / 256 * 2 - 1)
+        assert out[1] == pytest.approx((y + 0.5) / 256 * 2 - 1)
+        assert out[2] == pytest.approx(d)
+
+
+# ---- leader rule (cacheGather.comp:141-162) ---------------------------------
+
+def _leaders(own, own2):
+    lead = np.zeros(256, np.uint8)
+    lead2 = np.zeros(256, np.uint8)
+    a = np.ascontiguousarray(own, np.int32)
+    b = np.ascontiguousarray(own2, np.int32)
+    L().orc_tile_leaders(a.ctypes.data, b.ctypes.data, lead.ctypes.data, lead2.ctypes.data)
+    return lead.reshape(16, 16), lead2.reshape(16, 16)
+
+
+def test_leader_rule_brute_force():
+    rng = np.random.default_rng(4)
+    for trial in range(200):
+        own = rng.integers(-1, 6, size=(16, 16)).astype(np.int32)
+        own2 = rng.integers(-1, 6, size=(16, 16)).astype(np.int32)
+        lead, lead2 = _leaders(own.reshape(-1), own2.reshape(-1))
+        for ly in range(16):
+            for lx in range(16):
+                o = own[ly, lx]
+                ux, uy = max(0, lx - 1), max(0, ly - 1)
+                e = ((own[uy, lx] != o and own[ly, ux] != o and own[uy, ux] != o) or (lx == 0 and ly == 0)) and o != -1
+                assert bool(lead[ly, lx]) == e
+                o2 = own2[ly, lx]
+                vx, vy = min(15, lx + 1), min(15, ly + 1)
+                e2 = ((own2[vy, lx] != o2 and own2[ly, vx] != o2 and own2[vy, vx] != o2)
+                      or (vx == 15 and vy == 15)) and o2 != -1
+                assert bool(lead2[ly, lx]) == e2
+
+
+def test_leader_rule_edge_quirk_leaves_six_of_sixteen_cells_unled():
+    # A tile of 4x4-pixel cells: cells whose top-left pixel sits on the left
+    # column or top row (other than (0,0)) get no leader from this tile.
+    own = np.array([[(y // 4) * 4 + x // 4 for x in range(16)] for y in range(16)], np.int32)
+    lead, _ = _leaders(own.reshape(-1), np.full(256, -1, np.int32))
+    led = {int(own[y, x]) for y in range(16) for x in range(16) if lead[y, x]}
+    assert len(led) == 10 and 16 - len(led) == 6
+
+
+# ---- allocation invariants (cacheGather.comp:32-91) -------------------------
+
+@pytest.mark.parametrize("overrides", [{}, {"cav_cascades": 1}, {"transition_size": 0.0}])
+def test_allocation_invariants(overrides):
+    fi = frame_inputs("cornell_256")
+    w = fi.workload.with_(**overrides)
+    cfg = w.config()
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    count, cav, entries = orc.allocate(cfg, vol, fi.frame(), fi.depth)
+    cells, pos, phantoms, ok = canonical(count, cav, entries, cfg.max_caches)
+    assert ok and count > 0
+    # every append is a claimed cell or a phantom (corner outside the volume)
+    assert len(cells) + len(phantoms) == count
+    # Position = cellLocal * cellSize + Min (float32, unfused)
+    R, Cn = cfg.cav_resolution, cfg.cav_cascades
+    z, y, gx = np.unravel_index(cells, (R, R, Cn * R))
+    c = gx // R
+    x = gx % R
+    for i, coord in enumerate((x, y, z)):
+        cs = np.array([vol.cascades[k].world_voxel_size for k in range(Cn)], np.float32)[c]
+        mn = np.array([vol.cascades[k].min[i] for k in range(Cn)], np.float32)[c]
+        expect = (coord.astype(np.float32) * cs + mn).astype(np.float32)
+        np.testing.assert_array_equal(pos[:, i].view(np.float32), expect)
+    # SH fields start at zero
+    assert not entries[:count, 4:].any()
+    # claimed cells are exactly the 8-corner closures of the pixels' own cells
+    # as allocated by some leader: every claimed cell is a corner of a pixel cell
+    W, H = cfg.width, cfg.height
+    ys, xs = np.nonzero(fi.depth > 1e-4)
+    base = set()
+    wpo = (C.c_float * 3)()
+    for yy, xx in zip(ys[::7], xs[::7]):
+        L().orc_unproject_pixel(fi.frame().inverse_view_projection, W, H, int(xx), int(yy), float(fi.depth[yy, xx]),
+                                wpo)
+        cc = L().orc_cascade(C.byref(vol), Cn, wpo)
+        k = vol.cascades[cc]
+        g = [min(R - 1, max(0, int((wpo[i] - k.min[i]) / k.world_voxel_size))) for i in range(3)]
+        base.add((cc, *g))
+    claimed = {(int(gx_ // R), int(gx_ % R), int(y_), int(z_)) for gx_, y_, z_ in zip(gx, y, z)}
+    corners = {(cc, gx_ + dx, gy_ + dy, gz_ + dz) for cc, gx_, gy_, gz_ in base
+               for dx in (0, 1) for dy in (0, 1) for dz in (0, 1)}
+    assert claimed <= corners | {(cc, gx_, gy_, gz_) for cc, gx_, gy_, gz_ in claimed if cc > 0}
+
+
+# ---- voxel mips (voxelmipmap.comp) and sampling (GL 4.5 8.14) ---------------
+
+def test_voxel_mips_exact_integer_mean():
+    rng = np.random.default_rng(5)
+    res = 16
+    mips = np.zeros(orc.mip_bytes(res), np.uint8)
+    mips[: res ** 3] = rng.integers(0, 256, res ** 3)
+    L().orc_voxel_mips(res, mips.ctypes.data)
+    prev = mips[: res ** 3].reshape(res, res, res).astype(np.int64)
+    level = 1
+    while (res >> level) >= 1:
+        w = res >> level
+        s = prev.reshape(w, 2, w, 2, w, 2).sum(axis=(1, 3, 5))
+        expect = (s + 4) // 8
+        np.testing.assert_array_equal(orc.mip_level(mips, res, level), expect)
+        prev = expect
+        level += 1
+
+
+def _np_sample(levels, p, lod):
+    """Independent numpy statement of GL LINEAR_MIPMAP_LINEAR sampling (clamp to edge)."""
+    def tri(lvl):
+        w = lvl.shape[0]
+        u = np.asarray(p) * w - 0.5
+        i0 = np.floor(u).astype(int)
+        a = u - i0
+        val = 0.0
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    idx = np.clip(i0 + (dx, dy, dz), 0, w - 1)
+                    wgt = (a[0] if dx else 1 - a[0]) * (a[1] if dy else 1 - a[1]) * (a[2] if dz else 1 - a[2])
+                    val += wgt * lvl[idx[2], idx[1], idx[0]] / 255.0
+        return val
+    q = len(levels) - 1
+    if lod <= 0:
+        return tri(levels[0])
+    if lod >= q:
+        return tri(levels[q])
+    d = int(math.floor(lod))
+    f = lod - d
+    return (1 - f) * tri(levels[d]) + f * tri(levels[d + 1])
+
+
+def test_quadrilinear_sampling_brute_force():
+    rng = np.random.default_rng(6)
+    res = 8
+    mips = np.zeros(orc.mip_bytes(res), np.uint8)
+    vol = np.zeros((res, res, res), np.uint8)
+    vol[3, 4, 2] = 255
+    vol[rng.integers(0, 8, 20), rng.integers(0, 8, 20), rng.integers(0, 8, 20)] = rng.integers(1, 256, 20)
+    mips[: res ** 3] = vol.reshape(-1)
+    L().orc_voxel_mips(res, mips.ctypes.data)
+    levels = [orc.mip_level(mips, res, lv) for lv in range(4)]
+    pd = (C.c_double * 3)()
+    pf = (C.c_float * 3)()
+    for _ in range(400):
+        p = rng.uniform(-0.2, 1.2, 3)
+        lod = float(rng.uniform(-1.5, 4.0))
+        for i in range(3):
+            pd[i] = p[i]
+            pf[i] = p[i]
+        e = _np_sample(levels, p, lod)
+        assert L().orc_sample_voxels_f64(res, mips.ctypes.data, pd, lod) == pytest.approx(e, abs=1e-12)
+        assert L().orc_sample_voxels_f32(res, mips.ctypes.data, pf, lod) == pytest.approx(e, abs=1e-5)
+    # texel centres at lod 0 return the texel exactly
+    for (z, y, x) in [(3, 4, 2), (0, 0, 0)]:
+        for i, v in enumerate((x, y, z)):
+            pd[i] = (v + 0.5) / res
+        assert L().orc_sample_voxels_f64(res, mips.ctypes.data, pd, 0.0) == pytest.approx(vol[z, y, x] / 255.0)
+
+
+# ---- cone trace (cacheLightingRSM.comp:167-231) -----------------------------
+
+def _cone_setup(fill: int, rho_scale=1.0):
+    fi = frame_inputs("cornell_256")
+    w = fi.workload
+    cfg = w.config()
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    res = w.voxel_resolution
+    mips = np.zeros(orc.mip_bytes(res), np.uint8)
+    mips[: res ** 3] = fill
+    L().orc_voxel_mips(res, mips.ctypes.data)
+    rsm, keep = orc.orc_rsm(fi)
+    return fi, vol, res, mips, rsm, keep
+
+
+@pytest.mark.parametrize("fill,expect", [(0, 1.0), (255, 0.0)])
+def test_cone_trace_empty_and_full_grids(fill, expect):
+    fi, vol, res, mips, rsm, keep = _cone_setup(fill)
+    steps = C.c_int32()
+    for wp in [(0.0, -2.0, 0.0), (1.5, 0.5, -1.0)]:
+        for blk in (0, 16, 1024, 4080):
+            s = L().orc_cone_trace_f32(C.byref(vol), res, mips.ctypes.data, C.byref(fi.light), C.byref(rsm),
+                                       _abi.fvec(wp, 3), blk, C.byref(steps))
+            assert s == expect
+            assert 1 <= steps.value <= 32
+
+
+def _march(vol, res, mips, wp, avg, rho):
+    steps = C.c_int32()
+    v = L().orc_cone_march_f32(C.byref(vol), res, mips.ctypes.data, _abi.fvec(wp, 3), _abi.fvec(avg, 3), rho,
+                               C.byref(steps))
+    return v, steps.value
+
+
+def test_cone_trace_step_count_closed_form():
+    """With a cone so thin that every step stays one voxel, the march takes
+    min(32, max(1, ceil(goal))) steps, goal = distance / voxel - 2."""
+    fi, vol, res, mips, rsm, keep = _cone_setup(0)
+    vs = vol.voxel_size_in_world
+    rng = np.random.default_rng(10)
+    for _ in range(60):
+        wp = rng.uniform(-3.5, 3.5, 3)
+        direction = rng.normal(size=3)
+        direction /= np.linalg.norm(direction)
+        dist_vox = float(rng.uniform(0.5, 40.0))
+        avg = wp + direction * dist_vox * vs
+        # the oracle measures the distance in float32; compute goal the same way
+        d32 = np.float32(np.linalg.norm((avg.astype(np.float32) - wp.astype(np.float32)).astype(np.float32)))
+        goal = float(np.float32(d32 / np.float32(vs)) - np.float32(2.0))
+        vis, st = _march(vol, res, mips, wp, avg, 1e-7)
+        assert vis == 1.0
+        assert st == min(32, max(1, math.ceil(goal)))
+
+
+def test_cone_trace_through_single_voxel_matches_product_of_samples():
+    """One occupied voxel on an 8^3 grid: visibility = prod(1 - o_s) over the
+    samples the march takes, each o_s from the independent GL-spec sampler."""
+    res = 8
+    mips = np.zeros(orc.mip_bytes(res), np.uint8)
+    vol3 = np.zeros((res, res, res), np.uint8)
+    vol3[4, 4, 4] = 255
+    mips[: res ** 3] = vol3.reshape(-1)
+    L().orc_voxel_mips(res, mips.ctypes.data)
+    levels = [orc.mip_level(mips, res, lv) for lv in range(4)]
+    v = _abi.DrvVolume()
+    for i in range(3):
+        v.volume_world_min[i] = 0.0
+        v.volume_world_max[i] = 8.0
+    v.voxel_size_in_world = 1.0
+    for wp, avg, rho in [((4.5, 4.5, 0.3), (4.5, 4.5, 7.9), 0.01), ((0.6, 4.4, 4.6), (7.7, 4.6, 4.4), 0.05),
+                         ((1.0, 1.0, 1.0), (7.0, 7.0, 7.0), 0.2)]:
+        vis, st = _march(v, res, mips, wp, avg, rho)
+        wp_ = np.array(wp)
+        to = np.array(avg) - wp_
+        ld = np.linalg.norm(to)
+        to /= ld
+        cur = wp_ / res + to / res * 2.0
+        dist, step, trans = 0.0, 1.0, 1.0
+        k = 2.0 / (1.0 - rho)
+        goal = ld / 1.0 - 2.0
+        n = 0
+        for s_ in range(32):
+            cur = cur + to / res * step
+            dist += step
+            r = dist * rho
+            trans *= 1.0 - _np_sample(levels, cur, math.log2(r))
+            n += 1
+            if dist >= goal:
+                break
+            step = max(1.0, r * k)
+        assert st == n
+        assert vis == pytest.approx(min(1.0, max(0.0, trans)), abs=2e-5)
+        assert vis < 1.0 or rho > 0.1  # the rays pass next to or through the voxel
+
+
+# ---- SH projection against textbook real SH (scipy) -------------------------
+
+def _single_vpl_rsm(fi, texel):
+    flux, rn, dls = (a.copy() for a in fi.read_level)
+    keep_texel = flux[texel[1], texel[0]].copy()
+    flux[...] = 0
+    flux[texel[1], texel[0]] = keep_texel
+    return flux, rn, dls
+
+
+def test_single_vpl_projection_is_real_sh_of_its_direction():
+    fi = frame_inputs("cornell_256")
+    w = fi.workload.with_(sh_bands=2, indirect_shadow=0)
+    cfg = w.config()
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    shc = orc.sh_constants()
+    flux, rn, dls = _single_vpl_rsm(fi, (40, 22))
+    keep = [flux, rn, dls, fi.shadow_dls]
+    rsm = orc.OrcRsm(flux.ctypes.data, rn.ctypes.data, dls.ctypes.data, fi.shadow_dls.ctypes.data,
+                     fi.read_resolution)
+    pos = np.array([[0.5, -1.0, 0.2], [-2.0, 2.5, 1.0], [2.5, -3.0, -2.5]], np.float32)
+    sh = np.zeros((3, 27))
+    L().orc_gather_f64(C.byref(cfg), C.byref(vol), C.byref(shc), C.byref(fi.light), C.byref(rsm), None,
+                       pos.ctypes.data, 3, sh.ctypes.data, None, 1)
+    del keep
+    nz = sh.reshape(3, 9, 3)
+    checked = 0
+    for i in range(3):
+        c = nz[i, :, 0]
+        if c[0] == 0.0:
+            continue  # VPL faces away from this cache
+        checked += 1
+        # direction from the cache to the VPL, from the ratios of band 1
+        d = np.array([-c[3], -c[1], c[2]]) / (shc.eva1 / shc.eva0 * c[0])
+        assert np.linalg.norm(d) == pytest.approx(1.0, abs=1e-4)
+        th = math.acos(max(-1.0, min(1.0, d[2])))
+        ph = math.atan2(d[1], d[0])
+        def real_y(l_, m):
+            # real SH with the (-1)^m Condon-Shortley correction (all-positive x, y, z forms)
+            if m == 0:
+                return sps.sph_harm_y(l_, 0, th, ph).real
+            z = sps.sph_harm_y(l_, abs(m), th, ph)
+            return math.sqrt(2) * (-1) ** abs(m) * (z.real if m > 0 else z.imag)
+
+        rad = c[0] / shc.eva0
+        # canonical order SH00, SH1-1, SH10, SH1+1, SH2-2, SH2-1, SH20, SH2+1, SH2+2 and the
+        # reference's sign per field (cacheLightingRSM.comp:267-277 subtracts SH1neg1,
+        # SH1pos1 and SH2neg2)
+        order = [(0, 0), (1, -1), (1, 0), (1, 1), (2, -2), (2, -1), (2, 0), (2, 1), (2, 2)]
+        sign = [1, -1, 1, -1, -1, 1, 1, 1, 1]
+        y = {lm: real_y(*lm) for lm in order}
+        for k, lm in enumerate(order):
+            assert c[k] == pytest.approx(sign[k] * y[lm] * rad, rel=1e-6, abs=1e-9 * abs(rad))
+    assert checked >= 2
+
+
+def test_unshadowed_sh_irradiance_matches_brute_force_sum():
+    """Projected coefficients reconstruct the cosine-weighted VPL sum
+    (shader/bruteforcersm.frag's diffuse term) up to the SH2 truncation."""
+    fi = frame_inputs("cornell_256")
+    w = fi.workload.with_(sh_bands=2, indirect_shadow=0)
+    cfg = w.config()
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    shc = orc.sh_constants()
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(-3.5, 3.5, size=(40, 3)).astype(np.float32)
+    sh, _ = orc.gather_f64(cfg, vol, shc, fi.light, fi, None, pos)
+    # brute force: sum over VPLs of rad * max(0, n . w)
+    R = fi.read_resolution
+    flux, rn, dls = fi.read_level
+    fl = flux.view(np.float16).astype(np.float64).reshape(-1, 3)
+    d = dls.view(np.float16).astype(np.float64).reshape(-1, 2)[:, 0]
+    ys, xs = np.mgrid[0:R, 0:R]
+    s = ((xs + 0.5) / R * 2 - 1).reshape(-1)
+    t = ((ys + 0.5) / R * 2 - 1).reshape(-1)
+    M = np.array(fi.light.inverse_light_view_projection[:], np.float64).reshape(4, 4)
+    clip = np.stack([s, t, np.zeros_like(s), np.ones_like(s)], 1) @ M.T
+    lp = np.array(fi.light.light_position[:], np.float64)
+    dirs = clip[:, :3] / clip[:, 3:] - lp
+    vpos = lp + dirs / np.linalg.norm(dirs, axis=1, keepdims=True) * d[:, None]
+    nrm = np.zeros((R * R, 3))
+    nn = (C.c_float * 3)()
+    for i, (a, b) in enumerate(rn.reshape(-1, 2)):
+        L().orc_unpack_normal16i(int(a), int(b), nn)
+        nrm[i] = nn[:]
+    disc = d * d * fi.light.val_area_factor
+    errs = []
+    for i in range(len(pos)):
+        tv = vpos - pos[i]
+        d2 = (tv * tv).sum(1)
+        tv /= np.sqrt(d2)[:, None]
+        k = np.clip(-(nrm * tv).sum(1), 0, 1)
+        rad = fl * (k / (d2 + disc))[:, None]
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        exact = (rad * np.maximum(0, tv @ n)[:, None]).sum(0)
+        c = sh[i].reshape(9, 3)
+        x, y, z = n
+        # correct-sign SH2 irradiance (A_l * coefficients * Y_lm(n))
+        e = (shc.cos_lobe0 * c[0] - shc.cos_lobe1 * c[1] * y + shc.cos_lobe1 * c[2] * z - shc.cos_lobe1 * c[3] * x
+             + abs(shc.cos_lobe2n2_p1_n1) * (-c[4] * x * y + c[5] * y * z + c[7] * x * z)
+             + shc.cos_lobe20 * c[6] * (3 * z * z - 1) + shc.cos_lobe2p2 * c[8] * (x * x - y * y))
+        if exact.sum() > 1e-6:
+            errs.append(np.abs(e - exact).sum() / exact.sum())
+    assert len(errs) > 10
+    # SH2 captures a clamped cosine to ~3% (Ramamoorthi & Hanrahan); sums of
+    # lobes stay well inside 15%.
+    assert np.median(errs) < 0.15
+
+
+def test_fp32_and_fp64_gathers_agree():
+    fi = frame_inputs("cornell_256")
+    r = orc.run_frame(fi, gather=False)
+    cfg, vol = r["cfg"], r["vol"]
+    ids = np.linspace(0, r["live"] - 1, 64).astype(np.uint32)
+    e = r["entries"].copy()
+    orc.gather_f32(cfg, vol, r["shc"], fi.light, fi, r["mips"], e, ids=ids)
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    sh64, _ = orc.gather_f64(cfg, vol, r["shc"], fi.light, fi, r["mips"], e[ids, :3])
+    # FP32 accumulation over 4096 VPLs and FP32 cone marching stay well inside 1e-4.
+    assert rel_l2(entries_sh(e[ids], cfg.sh_bands), sh64) < 1e-4
+
+
+# ---- voxelization (voxelize.geom / .frag) -----------------------------------
+
+def _vox(res, tris, vmin=(-1.0, -1.0, -1.0), vmax=(1.0, 1.0, 1.0)):
+    v = _abi.DrvVolume()
+    for i in range(3):
+        v.volume_world_min[i] = vmin[i]
+        v.volume_world_max[i] = vmax[i]
+    target, _ = orc.voxelize(res, v, np.asarray(tris, np.float32).reshape(-1, 9), adaption_factor=0)
+    return target
+
+
+def test_voxelize_axis_aligned_quad_is_exact_slab():
+    res = 16
+    # z = 0.3 quad spanning x in [-0.47, 0.41], y in [-0.2, 0.7] (world == NDC);
+    # no edge lies on a voxel boundary (touching squares count as covered)
+    a, b, c, d = (-0.47, -0.2, 0.3), (0.41, -0.2, 0.3), (0.41, 0.7, 0.3), (-0.47, 0.7, 0.3)
+    t = _vox(res, [a + b + c, a + c + d])
+    z0 = int((0.3 * 0.5 + 0.5) * res)
+    zs, ys, xs = np.nonzero(t)
+    assert set(zs.tolist()) == {z0}
+    # conservative: every voxel column whose square touches the closed rectangle
+    lo = lambda v: int(math.floor((v * 0.5 + 0.5) * res))  # noqa: E731
+    hi = lambda v: int(math.ceil((v * 0.5 + 0.5) * res)) - 1  # noqa: E731
+    assert xs.min() == lo(-0.47) and xs.max() == hi(0.41)
+    assert ys.min() == lo(-0.2) and ys.max() == hi(0.7)
+    assert len(xs) == (hi(0.41) - lo(-0.47) + 1) * (hi(0.7) - lo(-0.2) + 1)
+
+
+def _tri_box_overlap(tri, centre, half):
+    """Akenine-Moller triangle/AABB separating-axis test."""
+    v = np.asarray(tri, float).reshape(3, 3) - centre
+    e = [v[1] - v[0], v[2] - v[1], v[0] - v[2]]
+    axes = [np.array(a, float) for a in np.eye(3)]
+    n = np.cross(e[0], e[1])
+    cand = axes + [n] + [np.cross(a, ed) for a in axes for ed in e]
+    for ax in cand:
+        if np.dot(ax, ax) < 1e-20:
+            continue
+        p = v @ ax
+        r = half * np.abs(ax).sum()
+        if p.min() > r + 1e-6 or p.max() < -r - 1e-6:
+            return False
+    return True
+
+
+def test_voxelize_footprint_is_conservative_in_the_dominant_projection():
+    """voxelize.geom expands the triangle conservatively in its dominant-axis
+    projection, so every voxel the triangle overlaps (Akenine-Moller SAT) has a
+    marked voxel in its column along that axis.  Depth is only 'conservative'
+    to +-1 voxel around a plane that the expansion distorts (moved vertices keep
+    their depth), so marked voxels stay within two voxels of the overlap range
+    but may miss some of it."""
+    rng = np.random.default_rng(8)
+    res = 8
+    for _ in range(40):
+        tri = rng.uniform(-0.9, 0.9, 9)
+        t = _vox(res, [tri])
+        P = tri.reshape(3, 3)
+        n = np.abs(np.cross(P[1] - P[0], P[2] - P[0]))
+        axis = int(np.argmax(n))
+        h = 1.0 / res
+        overl = np.zeros((res, res, res), bool)
+        for z in range(res):
+            for y in range(res):
+                for x in range(res):
+                    c = np.array([(x + 0.5) / res * 2 - 1, (y + 0.5) / res * 2 - 1, (z + 0.5) / res * 2 - 1])
+                    overl[z, y, x] = _tri_box_overlap(tri, c, h * 0.98)
+        # move the dominant axis last: [z, y, x] array axes are (2, 1, 0) of the world
+        ax_arr = 2 - axis
+        o = np.moveaxis(overl, ax_arr, -1)
+        m = np.moveaxis(t == 255, ax_arr, -1)
+        for i in range(res):
+            for j in range(res):
+                if o[i, j].any():
+                    assert m[i, j].any(), (tri, i, j)
+                    lo_, hi_ = np.nonzero(o[i, j])[0][[0, -1]]
+                    mk = np.nonzero(m[i, j])[0]
+                    assert mk.min() >= lo_ - 2 and mk.max() <= hi_ + 2
+
+
+# ---- apply (cacheApply.frag / lightcache.glsl:137-183) -----------------------
+
+def test_apply_with_equal_caches_is_one_irradiance_evaluation():
+    fi = frame_inputs("cornell_256")
+    w = fi.workload.with_(transition_size=0.0, sh_bands=2)
+    cfg = w.config()
+    vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
+    count, cav, entries = orc.allocate(cfg, vol, fi.frame(), fi.depth)
+    rng = np.random.default_rng(9)
+    sh = rng.uniform(0.1, 1.0, 27).astype(np.float32)
+    from dynamicradiancevolume_amd.pipeline import SH_OFFSETS
+
+    for k in range(9):
+        entries[:count, SH_OFFSETS[k]] = sh[3 * k: 3 * k + 3]
+    cav_full = cav.copy()
+    cav_full[cav_full == 0] = 1  # every corner points at an (identical) cache
+    out = orc.apply(cfg, vol, fi.frame(), orc.sh_constants(), fi, cav_full, entries, count)
+    shc = orc.sh_constants()
+    nn = (C.c_float * 3)()
+    lut = np.array([L().orc_srgb_to_linear(i) for i in range(256)])
+    for (y, x) in [(100, 100), (30, 200), (220, 17)]:
+        L().orc_unpack_normal16i(int(fi.gnormal[y, x, 0]), int(fi.gnormal[y, x, 1]), nn)
+        nx, ny, nz = nn[:]
+        c = sh.reshape(9, 3).astype(np.float64)
+        irr = (c[0] * shc.cos_lobe0 - c[1] * shc.cos_lobe1 * ny + c[2] * shc.cos_lobe1 * nz
+               - c[3] * shc.cos_lobe1 * nx - c[4] * shc.cos_lobe2n2_p1_n1 * nx * ny
+               + c[5] * shc.cos_lobe2n2_p1_n1 * ny * nz + c[6] * shc.cos_lobe20 * (3 * nz * nz - 1)
+               + c[7] * shc.cos_lobe2n2_p1_n1 * nx * nz + c[8] * shc.cos_lobe2p2 * (nx * nx - ny * ny))
+        expect = np.maximum(irr, 0) * lut[fi.galbedo[y, x]] / math.pi
+        np.testing.assert_allclose(out[y, x], expect, rtol=2e-5, atol=1e-7)
+
+
+def test_apply_reads_unclaimed_cells_as_zero():
+    fi = frame_inputs("cornell_256")
+    cfg = fi.workload.config()
+    vol = orc.update_volume(cfg, fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    count, cav, entries = orc.allocate(cfg, vol, fi.frame(), fi.depth)
+    entries[:count, 4:] = 1.0
+    out = orc.apply(cfg, vol, fi.frame(), orc.sh_constants(), fi, np.zeros_like(cav), entries, count)
+    assert not out.any()
