@@ -63,6 +63,9 @@ __device__ __forceinline__ int cache_1d_coord(const drv_volume& vol, int R, f3 w
 
 // AllocateCaches, cacheGather.comp:32-91, called by every lane of the wave
 // (inactive lanes pass active = false) so the appends can be aggregated.
+// A lane issues its 8 corner CASes back to back and waits once; then one
+// ballot per corner numbers the winners (corner-major, lane-minor) and a
+// single atomicAdd per wave reserves the whole run of cache indices.
 __device__ void allocate_corners(const AllocArgs& a, bool active, int coord, int c) {
   const int R = a.R, R2 = R * R, R3 = R2 * R;
   int bx = 0, by = 0, bz = 0;
@@ -72,37 +75,50 @@ __device__ void allocate_corners(const AllocArgs& a, bool active, int coord, int
     by = (coord - bz * R2) / R;
     bx = coord % R;
   }
-  const unsigned lane = __lane_id();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll 1
+  uint32_t cell[8];
+  uint32_t old[8];
+  uint32_t inside_mask = 0;
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     // offsets (0,0,0),(0,1,0),(0,0,1),(0,1,1),(1,0,0),(1,1,0),(1,0,1),(1,1,1)
     const int lx = bx + (i >> 2), ly = by + (i & 1), lz = bz + ((i >> 1) & 1);
     const int gx = lx + c * R;
-    bool won = false;
-    size_t ci = 0;
-    bool inside = false;
-    if (active) {
-      inside = gx >= 0 && gx < a.C * R && ly >= 0 && ly < R && lz >= 0 && lz < R;
-      if (inside) {
-        ci = (static_cast<size_t>(lz) * R + ly) * (static_cast<size_t>(a.C) * R) + gx;
-        won = atomicCAS(&a.cav[ci], 0u, 0xFFFFFFFFu) == 0u;
-      } else {
-        // An image atomic outside the image returns 0: the lock "succeeds"
-        // and a cache is appended that no cell points to.
-        won = true;
-      }
+    const bool inside = active && gx >= 0 && gx < a.C * R && ly >= 0 && ly < R && lz >= 0 && lz < R;
+    cell[i] = inside ? static_cast<uint32_t>((lz * R + ly) * (a.C * R) + gx) : 0u;
+    old[i] = 1u;
+    if (inside) {
+      inside_mask |= 1u << i;
+      old[i] = atomicCAS(&a.cav[cell[i]], 0u, 0xFFFFFFFFu);
     }
-    const uint64_t mask = __ballot(won);
-    if (mask == 0ull) continue;
-    const int first = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (static_cast<int>(lane) == first) base = atomicAdd(&a.counter->total, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, first);
-    if (won) {
-      const uint32_t idx = base + static_cast<uint32_t>(__popcll(mask & lt_mask));
+  }
+  // A lock on a cell outside the image "succeeds" (image atomics outside the
+  // image return 0) and appends a cache that no cell points to.
+  uint32_t won = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const bool inside = (inside_mask >> i) & 1u;
+    if (active && (!inside || old[i] == 0u)) won |= 1u << i;
+  }
+  const unsigned lane = __lane_id();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t masks[8];
+  uint32_t total = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    masks[i] = __ballot((won >> i) & 1u);
+    total += static_cast<uint32_t>(__popcll(masks[i]));
+  }
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&a.counter->total, total);
+  base = __shfl(base, 0);
+  const drv_cascade& k = a.vol.cascades[c < 0 ? 0 : c];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if ((won >> i) & 1u) {
+      const uint32_t idx = base + static_cast<uint32_t>(__popcll(masks[i] & lt_mask));
       if (idx < a.capacity) {
-        const drv_cascade& k = a.vol.cascades[c];
+        const int lx = bx + (i >> 2), ly = by + (i & 1), lz = bz + ((i >> 1) & 1);
         const float px = static_cast<float>(lx) * k.world_voxel_size + k.min[0];
         const float py = static_cast<float>(ly) * k.world_voxel_size + k.min[1];
         const float pz = static_cast<float>(lz) * k.world_voxel_size + k.min[2];
@@ -110,8 +126,9 @@ __device__ void allocate_corners(const AllocArgs& a, bool active, int coord, int
         e[0] = make_float4(px, py, pz, 0.0f);
         for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       }
-      if (inside) a.cav[ci] = idx + 1u;  // the cell stays locked for others
+      if ((inside_mask >> i) & 1u) a.cav[cell[i]] = idx + 1u;  // the cell stays locked for others
     }
+    base += static_cast<uint32_t>(__popcll(masks[i]));
   }
 }
 

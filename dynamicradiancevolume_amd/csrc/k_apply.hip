@@ -30,50 +30,49 @@ struct ApplyArgs {
   int width, height, R, C, stride, sh2, transitions, accumulate;
 };
 
-// SampleCacheIrradiance, lightcache.glsl:137-183.
-__device__ __forceinline__ void sample_cache(const ApplyArgs& a, uint32_t addr, uint32_t live, f3 n, float* irr) {
-  if (addr >= live) {
-    irr[0] = irr[1] = irr[2] = 0.0f;
-    return;
-  }
-  const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(addr) * a.stride);
-  const float4 q1 = e4[1], q2 = e4[2], q3 = e4[3];  // {SH1neg1, SH00_r}, {SH10, SH00_g}, {SH1pos1, SH00_b}
-  const float sh00[3] = {q1.w, q2.w, q3.w};
-  const float s1n1[3] = {q1.x, q1.y, q1.z};
-  const float s10[3] = {q2.x, q2.y, q2.z};
-  const float s1p1[3] = {q3.x, q3.y, q3.z};
+// SampleCacheIrradiance, lightcache.glsl:137-183, on an entry already in
+// registers (q[k] = float4 k of the std430 LightCacheEntry).
+__device__ __forceinline__ void cache_irradiance(const ApplyArgs& a, const float4* q, f3 n, float* irr) {
+  // {SH1neg1, SH00_r}, {SH10, SH00_g}, {SH1pos1, SH00_b}
   const float CL0 = a.shc.cos_lobe0, CL1 = a.shc.cos_lobe1;
-  float v[3];
-#pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    v[ch] = sh00[ch] * CL0;
-    v[ch] = v[ch] - s1n1[ch] * (CL1 * n.y);
-    v[ch] = v[ch] + s10[ch] * (CL1 * n.z);
-    v[ch] = v[ch] - s1p1[ch] * (CL1 * n.x);
-  }
+  float v0 = q[1].w * CL0, v1 = q[2].w * CL0, v2 = q[3].w * CL0;
+  v0 = v0 - q[1].x * (CL1 * n.y);
+  v1 = v1 - q[1].y * (CL1 * n.y);
+  v2 = v2 - q[1].z * (CL1 * n.y);
+  v0 = v0 + q[2].x * (CL1 * n.z);
+  v1 = v1 + q[2].y * (CL1 * n.z);
+  v2 = v2 + q[2].z * (CL1 * n.z);
+  v0 = v0 - q[3].x * (CL1 * n.x);
+  v1 = v1 - q[3].y * (CL1 * n.x);
+  v2 = v2 - q[3].z * (CL1 * n.x);
   if (a.sh2) {
-    const float4 q4 = e4[4], q5 = e4[5], q6 = e4[6], q7 = e4[7];
-    const float s2n2[3] = {q4.x, q4.y, q4.z};
-    const float s2n1[3] = {q5.x, q5.y, q5.z};
-    const float s20[3] = {q4.w, q5.w, q6.w};
-    const float s2p1[3] = {q6.x, q6.y, q6.z};
-    const float s2p2[3] = {q7.x, q7.y, q7.z};
+    // {SH2neg2, SH20_r}, {SH2neg1, SH20_g}, {SH2pos1, SH20_b}, {SH2pos2, -}
     const float CL2 = a.shc.cos_lobe2n2_p1_n1, CL20 = a.shc.cos_lobe20, CL22 = a.shc.cos_lobe2p2;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      v[ch] = v[ch] - s2n2[ch] * (CL2 * n.x * n.y);
-      v[ch] = v[ch] + s2n1[ch] * (CL2 * n.y * n.z);
-      v[ch] = v[ch] + s20[ch] * (CL20 * (n.z * n.z * 3.0f - 1.0f));
-      v[ch] = v[ch] + s2p1[ch] * (CL2 * n.x * n.z);
-      v[ch] = v[ch] + s2p2[ch] * (CL22 * (n.x * n.x - n.y * n.y));
-    }
+    const float fxy = CL2 * n.x * n.y, fyz = CL2 * n.y * n.z, fxz = CL2 * n.x * n.z;
+    const float f20 = CL20 * (n.z * n.z * 3.0f - 1.0f), f22 = CL22 * (n.x * n.x - n.y * n.y);
+    v0 = v0 - q[4].x * fxy;
+    v1 = v1 - q[4].y * fxy;
+    v2 = v2 - q[4].z * fxy;
+    v0 = v0 + q[5].x * fyz;
+    v1 = v1 + q[5].y * fyz;
+    v2 = v2 + q[5].z * fyz;
+    v0 = v0 + q[4].w * f20;
+    v1 = v1 + q[5].w * f20;
+    v2 = v2 + q[6].w * f20;
+    v0 = v0 + q[6].x * fxz;
+    v1 = v1 + q[6].y * fxz;
+    v2 = v2 + q[6].z * fxz;
+    v0 = v0 + q[7].x * f22;
+    v1 = v1 + q[7].y * f22;
+    v2 = v2 + q[7].z * f22;
   }
-  irr[0] = fmaxf(v[0], 0.0f);
-  irr[1] = fmaxf(v[1], 0.0f);
-  irr[2] = fmaxf(v[2], 0.0f);
+  irr[0] = fmaxf(v0, 0.0f);
+  irr[1] = fmaxf(v1, 0.0f);
+  irr[2] = fmaxf(v2, 0.0f);
 }
 
-// ComputeLightingFromCaches, cacheApply.frag:28-118.
+// ComputeLightingFromCaches, cacheApply.frag:28-118.  All 8 addresses are
+// fetched first, then all 8 entries, so one pixel keeps 16 loads in flight.
 __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c, const float* diffuse, uint32_t live,
                                          float* out) {
   const drv_cascade& k = a.vol.cascades[c];
@@ -86,16 +85,28 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c,
   const float w[8] = {jx * jy * jz, ix * jy * jz, jx * iy * jz, ix * iy * jz,
                       jx * jy * iz, ix * jy * iz, jx * iy * iz, ix * iy * iz};
   const int R = a.R, CR = a.C * a.R;
-  float acc[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t addr[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     // offsets x fastest: (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),(1,0,1),(0,1,1),(1,1,1)
     const int x = ax + (i & 1) + R * c, y = ay + ((i >> 1) & 1), z = az + (i >> 2);
-    uint32_t addr = 0u;
-    if (x >= 0 && y >= 0 && z >= 0 && x < CR && y < R && z < R) addr = a.cav[(static_cast<size_t>(z) * R + y) * CR + x];
-    addr -= 1u;
+    // texelFetch outside the CAV reads 0 -> address 0xFFFFFFFF
+    uint32_t v = 0u;
+    if (x >= 0 && y >= 0 && z >= 0 && x < CR && y < R && z < R) v = a.cav[(static_cast<uint32_t>(z) * R + y) * CR + x];
+    addr[i] = v - 1u;
+  }
+  const int nq = a.sh2 ? 8 : 4;
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    // An address past the live caches (unclaimed cell, overflow) reads zeros.
+    const bool ok = addr[i] < live;
+    const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(ok ? addr[i] : 0u) * a.stride);
+    float4 q[8];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) q[j] = (ok && j < nq) ? e4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float irr[3];
-    sample_cache(a, addr, live, n, irr);
+    cache_irradiance(a, q, n, irr);
     acc[0] = acc[0] + irr[0] * w[i];
     acc[1] = acc[1] + irr[1] * w[i];
     acc[2] = acc[2] + irr[2] * w[i];
