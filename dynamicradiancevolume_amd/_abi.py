@@ -137,6 +137,17 @@ class DrvStageTimes(C.Structure):
     ] + [("cone_traces", C.c_uint64), ("cone_steps", C.c_uint64), ("cone_steps_executed", C.c_uint64)]
 
 
+class DrvSurrogate(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("spot_fraction", C.c_float), ("fallback_threshold", C.c_float)]
+
+
+class DrvSurrogateStats(C.Structure):
+    _fields_ = [("spot_error", C.c_float), ("fell_back", C.c_int32), ("spot_caches", C.c_uint32)]
+
+
+DRV_MLP_PARAMS = 10497
+DRV_MLP_FEATURES = 32
+
 P = C.POINTER
 vp = C.c_void_p
 
@@ -164,6 +175,16 @@ DRV_FUNCTIONS = {
     "drv_read_cav": (C.c_int, [vp, vp]),
     "drv_read_voxels": (C.c_int, [vp, C.c_int32, vp]),
     "drv_read_visibility": (C.c_int, [vp, vp, C.c_uint64]),
+    "drv_mlp_set_params": (C.c_int, [vp, vp, C.c_uint32]),
+    "drv_mlp_get_params": (C.c_int, [vp, vp, C.c_uint32]),
+    "drv_set_surrogate": (C.c_int, [vp, P(DrvSurrogate)]),
+    "drv_get_surrogate_stats": (C.c_int, [vp, P(DrvSurrogateStats)]),
+    "drv_mlp_collect": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp]),
+    "drv_mlp_features": (C.c_int, [vp, vp, C.c_uint32, vp]),
+    "drv_mlp_forward": (C.c_int, [vp, vp, C.c_uint32, vp]),
+    "drv_mlp_gradient": (C.c_int, [vp, vp, vp, C.c_uint32, vp]),
+    "drv_mlp_train": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, vp]),
+    "drv_mlp_finetune": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, vp]),
     "drv_set_profiling": (C.c_int, [vp, C.c_int32]),
     "drv_get_stage_times": (C.c_int, [vp, P(DrvStageTimes)]),
 }

@@ -51,6 +51,41 @@ struct drv_ctx {
   uint32_t last_n_blocks = 0;
   bool counted_steps = false;
 
+  // Surrogate visibility network.
+  float* mlp_params = nullptr;  // FP32 master [kMlpParams]
+  float* mlp_m = nullptr;       // Adam moments
+  float* mlp_v = nullptr;
+  float* mlp_grad = nullptr;  // reduced gradient [kMlpParams + 1] (last: loss)
+  void* mlp_pack = nullptr;   // inference operands
+  void* mlp_train_w = nullptr;  // training LDS images
+  float* mlp_partial = nullptr;
+  size_t mlp_partial_waves = 0;
+  uint32_t* mlp_idx = nullptr;
+  size_t mlp_idx_cap = 0;
+  uint32_t* mlp_pairs = nullptr;
+  size_t mlp_pairs_cap = 0;
+  uint32_t* mlp_n = nullptr;
+  float* ft_x = nullptr;  // fine-tuning samples
+  float* ft_y = nullptr;
+  size_t ft_x_cap = 0, ft_y_cap = 0;
+  uint32_t adam_t = 0;
+  bool mlp_ready = false;
+  drv_surrogate sur{0, 0.01f, 0.05f};
+  // Spot check: two compact cache sets, their visibility, counters
+  // [traced set, network set, fallback gate] and the device-side stats.
+  float* spot_caches = nullptr;
+  size_t spot_caches_size = 0;  // floats
+  size_t spot_cap = 0;          // caches per set
+  float* spot_vis = nullptr;
+  size_t spot_vis_size = 0;
+  Counter* spot_ctr = nullptr;
+  uint32_t* spot_stats = nullptr;
+  // Where drv_mlp_collect finds cone-traced labels of the last light pass.
+  const float* label_caches = nullptr;
+  const Counter* label_counter = nullptr;
+  const float* label_vis = nullptr;
+  uint32_t label_pitch = 0;
+
   bool profiling = false;
   int profiling_level = 0;
   uint64_t* step_counter = nullptr;  // [traces, reference steps, executed steps] when counting
@@ -136,6 +171,68 @@ drvk::VoxelView voxel_view(const drv_ctx* ctx) {
 }
 
 int persistent_grid(const drv_ctx* ctx) { return ctx->cu_count * 8; }
+
+template <typename T>
+int ensure(drv_ctx* ctx, T** p, size_t* cap, size_t need) {
+  if (need <= *cap && *p) return DRV_OK;
+  dev_free(*p);
+  *cap = 0;
+  int rc = dev_alloc(ctx, p, need);
+  if (rc == DRV_OK) *cap = need;
+  return rc;
+}
+
+// Visibility of every (live cache, shadow block) from the network, checked on
+// a spot set of caches against the cone trace; the full cone trace runs
+// (device-gated) when the spot check fails.
+int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32_t interval, int grid) {
+  const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
+  const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
+  const drvk::VoxelView vox = voxel_view(ctx);
+  const drvk::FeatureArgsHost fh = drvk::make_feature_args(ctx->vol, ctx->frame.camera_position, vox);
+  const bool spot = ctx->sur.spot_fraction > 0.0f;
+  int rc;
+  if (spot) {
+    const uint32_t k = std::max<uint32_t>(1u, static_cast<uint32_t>(std::lround(1.0 / ctx->sur.spot_fraction)));
+    const size_t spot_cap = (cap + k - 1) / k;
+    if ((rc = ensure(ctx, &ctx->spot_caches, &ctx->spot_caches_size, 2 * spot_cap * stride)) != DRV_OK) return rc;
+    if ((rc = ensure(ctx, &ctx->spot_vis, &ctx->spot_vis_size, 2 * spot_cap * n_blocks)) != DRV_OK) return rc;
+    if (!ctx->spot_ctr && (rc = dev_alloc(ctx, &ctx->spot_ctr, 3)) != DRV_OK) return rc;
+    if (!ctx->spot_stats && (rc = dev_alloc(ctx, &ctx->spot_stats, 4)) != DRV_OK) return rc;
+    ctx->spot_cap = spot_cap;
+    const uint32_t sc = static_cast<uint32_t>(spot_cap);
+    float* set_a = ctx->spot_caches;
+    float* set_b = ctx->spot_caches + spot_cap * stride;
+    float* vis_a = ctx->spot_vis;
+    float* vis_b = ctx->spot_vis + spot_cap * n_blocks;
+    drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, ctx->spot_caches, sc, ctx->spot_ctr, ctx->stream);
+    drvk::launch_visibility(ctx->vol, vox, set_a, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, vis_a, sc, nullptr,
+                            grid, ctx->stream);
+    drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, set_b, stride, &ctx->spot_ctr[1], ctx->blocks,
+                                n_blocks, vis_b, sc, grid, ctx->stream);
+    for (int set = 0; set < 2; ++set) {
+      float* cs = set == 0 ? set_a : set_b;
+      drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->vpls, n_vpl,
+                          set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, grid, ctx->stream);
+      drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->partial, sc,
+                             ctx->cu_count * 4, ctx->stream);
+    }
+    drvk::launch_spot_error(ctx->spot_caches, sc, stride, ctx->cfg.sh_bands, ctx->spot_ctr, ctx->sur.fallback_threshold,
+                            ctx->counter, ctx->spot_stats, &ctx->spot_ctr[2], ctx->stream);
+    ctx->label_caches = set_a;
+    ctx->label_counter = &ctx->spot_ctr[0];
+    ctx->label_vis = vis_a;
+    ctx->label_pitch = sc;
+  } else {
+    ctx->label_caches = nullptr;
+  }
+  drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, ctx->caches, stride, ctx->counter, ctx->blocks,
+                              n_blocks, ctx->visibility, static_cast<uint32_t>(cap), grid, ctx->stream);
+  if (spot)  // gate counter: live = 0 unless the spot check failed
+    drvk::launch_visibility(ctx->vol, vox, ctx->caches, stride, &ctx->spot_ctr[2], ctx->blocks, n_blocks,
+                            ctx->visibility, static_cast<uint32_t>(cap), nullptr, grid, ctx->stream);
+  return DRV_OK;
+}
 
 }  // namespace
 
@@ -245,6 +342,22 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->visibility);
   dev_free(ctx->partial);
   dev_free(ctx->step_counter);
+  dev_free(ctx->mlp_params);
+  dev_free(ctx->mlp_m);
+  dev_free(ctx->mlp_v);
+  dev_free(ctx->mlp_grad);
+  dev_free(ctx->mlp_pack);
+  dev_free(ctx->mlp_train_w);
+  dev_free(ctx->mlp_partial);
+  dev_free(ctx->mlp_idx);
+  dev_free(ctx->mlp_pairs);
+  dev_free(ctx->mlp_n);
+  dev_free(ctx->ft_x);
+  dev_free(ctx->ft_y);
+  dev_free(ctx->spot_caches);
+  dev_free(ctx->spot_vis);
+  dev_free(ctx->spot_ctr);
+  dev_free(ctx->spot_stats);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   delete ctx;
@@ -412,6 +525,10 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
   drvk::launch_prepare_dispatch(ctx->counter, static_cast<uint32_t>(ctx->cfg.max_caches), ctx->stream);
   record(ctx, 4);
   ctx->allocated = true;
+  // A new cache list: visibility and training labels of earlier passes no
+  // longer refer to these caches.
+  ctx->last_n_blocks = 0;
+  ctx->label_caches = nullptr;
   return check_hip(ctx, hipGetLastError(), "allocate");
 }
 
@@ -465,9 +582,18 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   record(ctx, 5);
   drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, ctx->stream);
   record(ctx, 6);
-  if (shadow)
+  const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
+  if (use_net) {
+    if ((rc = surrogate_visibility(ctx, n_blocks, n_vpl, interval, grid)) != DRV_OK) return rc;
+    steps = nullptr;
+  } else if (shadow) {
     drvk::launch_visibility(ctx->vol, voxel_view(ctx), ctx->caches, stride, ctx->counter, ctx->blocks, n_blocks,
                             ctx->visibility, static_cast<uint32_t>(cap), steps, grid, ctx->stream);
+    ctx->label_caches = ctx->caches;
+    ctx->label_counter = ctx->counter;
+    ctx->label_vis = ctx->visibility;
+    ctx->label_pitch = static_cast<uint32_t>(cap);
+  }
   record(ctx, 7);
   drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
                       ctx->visibility, static_cast<uint32_t>(cap), shadow ? interval : 64u, shadow ? 1 : 0,
@@ -554,6 +680,196 @@ int drv_read_visibility(drv_ctx* ctx, float* host_out, uint64_t count) {
     host_out[i] = buf[b * live + c];
   }
   return DRV_OK;
+}
+
+// ---- surrogate visibility network -------------------------------------------
+
+namespace {
+
+int mlp_alloc(drv_ctx* ctx) {
+  if (ctx->mlp_params) return DRV_OK;
+  int rc;
+  if ((rc = dev_alloc(ctx, &ctx->mlp_params, drvk::kMlpParams)) != DRV_OK) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->mlp_m, drvk::kMlpParams)) != DRV_OK) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->mlp_v, drvk::kMlpParams)) != DRV_OK) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->mlp_grad, drvk::kMlpParams + 1)) != DRV_OK) return rc;
+  if ((rc = dev_alloc(ctx, reinterpret_cast<uint8_t**>(&ctx->mlp_pack), drvk::kMlpPackBytes)) != DRV_OK) return rc;
+  if ((rc = dev_alloc(ctx, reinterpret_cast<uint16_t**>(&ctx->mlp_train_w), drvk::kMlpTrainW)) != DRV_OK) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->mlp_n, 1)) != DRV_OK) return rc;
+  return DRV_OK;
+}
+
+int require_net(drv_ctx* ctx) {
+  if (!ctx->mlp_ready) return fail(ctx, DRV_ERR_STATE, "surrogate: network parameters not set");
+  return DRV_OK;
+}
+
+// Features (and labels) need the shadow blocks and voxels of the last light pass.
+int require_features(drv_ctx* ctx) {
+  if (ctx->last_n_blocks == 0) return fail(ctx, DRV_ERR_STATE, "surrogate: no shadowed light pass yet");
+  return DRV_OK;
+}
+
+// Gradient of the mean loss over the `batch` samples listed in mlp_idx: the
+// training kernel's per-wave partials, reduced in a fixed order into mlp_grad
+// (last entry: summed loss), and for lr > 0 one Adam step plus a repack.
+int mlp_step(drv_ctx* ctx, const float* x, const float* y, uint32_t batch, float lr) {
+  const uint32_t waves = drvk::mlp_train_waves(batch, ctx->cu_count);
+  int rc;
+  if ((rc = ensure(ctx, &ctx->mlp_partial, &ctx->mlp_partial_waves, static_cast<size_t>(waves) * (drvk::kMlpParams + 1))) !=
+      DRV_OK)
+    return rc;
+  drvk::launch_mlp_train(x, y, ctx->mlp_idx, batch, ctx->mlp_params, ctx->mlp_train_w, ctx->mlp_partial, waves,
+                         ctx->stream);
+  float bc1 = 1.0f, bc2 = 1.0f;
+  if (lr > 0.0f) {
+    ctx->adam_t += 1;
+    bc1 = 1.0f - static_cast<float>(std::pow(0.9, ctx->adam_t));
+    bc2 = 1.0f - static_cast<float>(std::pow(0.999, ctx->adam_t));
+  }
+  drvk::launch_mlp_adam(ctx->mlp_partial, waves, ctx->mlp_params, ctx->mlp_m, ctx->mlp_v, ctx->mlp_grad, lr, 0.9f,
+                        0.999f, 1e-8f, bc1, bc2, ctx->stream);
+  if (lr > 0.0f) drvk::launch_mlp_pack(ctx->mlp_params, ctx->mlp_pack, ctx->mlp_train_w, ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "mlp_step");
+}
+
+}  // namespace
+
+int drv_mlp_set_params(drv_ctx* ctx, const float* host_params, uint32_t count) {
+  if (!ctx || !host_params || count != static_cast<uint32_t>(drvk::kMlpParams)) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = mlp_alloc(ctx)) != DRV_OK) return rc;
+  DRV_HIP(ctx, hipMemcpyAsync(ctx->mlp_params, host_params, sizeof(float) * count, hipMemcpyHostToDevice, ctx->stream));
+  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->mlp_m), drvk::kMlpParams, 0u, ctx->stream);
+  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->mlp_v), drvk::kMlpParams, 0u, ctx->stream);
+  drvk::launch_mlp_pack(ctx->mlp_params, ctx->mlp_pack, ctx->mlp_train_w, ctx->stream);
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->adam_t = 0;
+  ctx->mlp_ready = true;
+  return check_hip(ctx, hipGetLastError(), "mlp_set_params");
+}
+
+int drv_mlp_get_params(drv_ctx* ctx, float* host_params, uint32_t count) {
+  if (!ctx || !host_params || count != static_cast<uint32_t>(drvk::kMlpParams)) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_net(ctx)) != DRV_OK) return rc;
+  DRV_HIP(ctx, hipMemcpyAsync(host_params, ctx->mlp_params, sizeof(float) * count, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_set_surrogate(drv_ctx* ctx, const drv_surrogate* settings) {
+  if (!ctx || !settings) return DRV_ERR_INVALID_ARG;
+  if (!(settings->spot_fraction >= 0.0f && settings->spot_fraction <= 1.0f) || !(settings->fallback_threshold >= 0.0f))
+    return fail(ctx, DRV_ERR_INVALID_ARG, "set_surrogate: spot fraction must lie in [0, 1], threshold >= 0");
+  if (settings->enabled) {
+    int rc;
+    if ((rc = require_net(ctx)) != DRV_OK) return rc;
+  }
+  ctx->sur = *settings;
+  return DRV_OK;
+}
+
+int drv_get_surrogate_stats(drv_ctx* ctx, drv_surrogate_stats* out) {
+  if (!ctx || !out) return DRV_ERR_INVALID_ARG;
+  std::memset(out, 0, sizeof(*out));
+  if (!ctx->spot_stats) return DRV_OK;
+  uint32_t st[4] = {0, 0, 0, 0};
+  DRV_HIP(ctx, hipMemcpyAsync(st, ctx->spot_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(&out->spot_error, &st[0], 4);
+  out->fell_back = static_cast<int32_t>(st[1]);
+  out->spot_caches = st[2];
+  return DRV_OK;
+}
+
+int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float* y_dev) {
+  if (!ctx || (n && (!x_dev || !y_dev))) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_features(ctx)) != DRV_OK) return rc;
+  if (!ctx->label_caches) return fail(ctx, DRV_ERR_STATE, "mlp_collect: the last light pass has no cone-traced labels");
+  if (n == 0) return DRV_OK;
+  // Pairs are drawn over the label set's live caches (read on the device).
+  Counter c{};
+  DRV_HIP(ctx, hipMemcpyAsync(&c, ctx->label_counter, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (c.live == 0) return fail(ctx, DRV_ERR_STATE, "mlp_collect: no live caches");
+  if ((rc = ensure(ctx, &ctx->mlp_pairs, &ctx->mlp_pairs_cap, 2 * static_cast<size_t>(n))) != DRV_OK) return rc;
+  if (!ctx->mlp_n && (rc = dev_alloc(ctx, &ctx->mlp_n, 1)) != DRV_OK) return rc;
+  drvk::launch_mlp_sample_pairs(ctx->label_counter, ctx->last_n_blocks, n, seed, ctx->mlp_pairs, ctx->mlp_n,
+                                ctx->stream);
+  const drvk::FeatureArgsHost fh = drvk::make_feature_args(ctx->vol, ctx->frame.camera_position, voxel_view(ctx));
+  drvk::launch_mlp_features(fh, ctx->label_caches, drvk::entry_stride(ctx->cfg.sh_bands), c.live, ctx->blocks,
+                            ctx->last_n_blocks, ctx->mlp_pairs, n, x_dev, ctx->label_vis, ctx->label_pitch, y_dev,
+                            ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "mlp_collect");
+}
+
+int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float* x_dev) {
+  if (!ctx || (n && (!pairs_dev || !x_dev))) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_features(ctx)) != DRV_OK) return rc;
+  const drvk::FeatureArgsHost fh = drvk::make_feature_args(ctx->vol, ctx->frame.camera_position, voxel_view(ctx));
+  // Pairs name caches of the current list (live count read on the host).
+  uint32_t total = 0, live = 0;
+  if ((rc = drv_get_cache_count(ctx, &total, &live)) != DRV_OK) return rc;
+  drvk::launch_mlp_features(fh, ctx->caches, drvk::entry_stride(ctx->cfg.sh_bands), live, ctx->blocks,
+                            ctx->last_n_blocks, pairs_dev, n, x_dev, nullptr, 0, nullptr, ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "mlp_features");
+}
+
+int drv_mlp_forward(drv_ctx* ctx, const float* x_dev, uint32_t n, float* p_dev) {
+  if (!ctx || (n && (!x_dev || !p_dev))) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_net(ctx)) != DRV_OK) return rc;
+  drvk::launch_mlp_forward(ctx->mlp_pack, ctx->mlp_params, x_dev, n, p_dev, persistent_grid(ctx), ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "mlp_forward");
+}
+
+int drv_mlp_gradient(drv_ctx* ctx, const float* x_dev, const float* y_dev, uint32_t n, float* grad_host) {
+  if (!ctx || !x_dev || !y_dev || !grad_host || n == 0) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_net(ctx)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->mlp_idx, &ctx->mlp_idx_cap, n)) != DRV_OK) return rc;
+  drvk::launch_mlp_batch_indices(n, n, 0u, ctx->mlp_idx, true, ctx->stream);
+  if ((rc = mlp_step(ctx, x_dev, y_dev, n, 0.0f)) != DRV_OK) return rc;
+  DRV_HIP(ctx, hipMemcpyAsync(grad_host, ctx->mlp_grad, sizeof(float) * (drvk::kMlpParams + 1), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_mlp_train(drv_ctx* ctx, const float* x_dev, const float* y_dev, uint32_t n, uint32_t batch, uint32_t steps,
+                  float lr, uint32_t seed, float* mean_loss_host) {
+  if (!ctx || !x_dev || !y_dev || n == 0 || batch == 0 || !(lr > 0.0f)) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_net(ctx)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->mlp_idx, &ctx->mlp_idx_cap, batch)) != DRV_OK) return rc;
+  for (uint32_t t = 0; t < steps; ++t) {
+    drvk::launch_mlp_batch_indices(n, batch, seed * 0x9e3779b1u + t, ctx->mlp_idx, false, ctx->stream);
+    if ((rc = mlp_step(ctx, x_dev, y_dev, batch, lr)) != DRV_OK) return rc;
+  }
+  if (mean_loss_host) {
+    float loss = 0.0f;
+    if (steps > 0) {
+      DRV_HIP(ctx, hipMemcpyAsync(&loss, ctx->mlp_grad + drvk::kMlpParams, sizeof(float), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    *mean_loss_host = loss / static_cast<float>(batch);
+  }
+  return check_hip(ctx, hipGetLastError(), "mlp_train");
+}
+
+int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t steps, float lr, uint32_t seed,
+                     float* mean_loss_host) {
+  if (!ctx || n_samples == 0) return DRV_ERR_INVALID_ARG;
+  int rc;
+  if ((rc = require_net(ctx)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->ft_x, &ctx->ft_x_cap, static_cast<size_t>(n_samples) * 32)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->ft_y, &ctx->ft_y_cap, static_cast<size_t>(n_samples))) != DRV_OK) return rc;
+  if ((rc = drv_mlp_collect(ctx, n_samples, seed, ctx->ft_x, ctx->ft_y)) != DRV_OK) return rc;
+  return drv_mlp_train(ctx, ctx->ft_x, ctx->ft_y, n_samples, batch, steps, lr, seed ^ 0x5bd1e995u, mean_loss_host);
 }
 
 // ---- profiling ------------------------------------------------------------

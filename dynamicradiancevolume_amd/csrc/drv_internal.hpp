@@ -38,9 +38,11 @@ struct Vpl {
 };
 
 // One shadow block (cacheLightingRSM.comp:171-192): averaged VAL position and
-// the cone aperture distToSphereRad.
+// the cone aperture distToSphereRad, plus the mean VPL normal of the block
+// (an input of the surrogate visibility network only).
 struct ShadowBlock {
   float x, y, z, rho;
+  float nx, ny, nz, pad;
 };
 
 // Voxel volume layout: per mip level l (w = res >> l) the R8 texels, x
@@ -62,6 +64,28 @@ struct VoxelView {
   uint32_t corner_offset[16];
   int res;
   int levels;
+};
+
+// Surrogate MLP parameter layout (FP32 master): W1 [32][64] (row = input),
+// b1 [64], W2 [64][64], b2, W3 [64][64], b3, w4 [64], b4.
+constexpr int kMlpW1 = 0;
+constexpr int kMlpB1 = 2048;
+constexpr int kMlpW2 = 2112;
+constexpr int kMlpB2 = 6208;
+constexpr int kMlpW3 = 6272;
+constexpr int kMlpB3 = 10368;
+constexpr int kMlpW4 = 10432;
+constexpr int kMlpB4 = 10496;
+constexpr int kMlpParams = 10497;
+constexpr int kMlpPackBytes = 1280 * 16;       // inference A operands
+constexpr int kMlpTrainW = 64 * 40 + 4 * 64 * 72;  // bf16 LDS images for training
+
+// Inputs of the surrogate's feature extraction.
+struct FeatureArgsHost {
+  float vmin[3];
+  float voxel_size;
+  float cam[3];
+  VoxelView vox;
 };
 
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
@@ -88,6 +112,31 @@ void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* cache
                    uint32_t partial_pitch, int grid, hipStream_t s);
 void launch_reduce_sh(const drv_sh_constants& shc, int sh_bands, float* caches, int stride, const Counter* counter,
                       const float* partial, uint32_t partial_pitch, int grid, hipStream_t s);
+
+FeatureArgsHost make_feature_args(const drv_volume& vol, const float* cam, const VoxelView& vox);
+size_t mlp_train_lds_bytes();
+void launch_mlp_pack(const float* params, void* pack, void* train_w, hipStream_t s);
+void launch_visibility_mlp(const FeatureArgsHost& fh, const void* pack, const float* params, const float* caches,
+                           int stride, const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks,
+                           float* vis, uint32_t pitch, int grid, hipStream_t s);
+void launch_mlp_sample_pairs(const Counter* counter, uint32_t n_blocks, uint32_t n, uint32_t seed, uint32_t* pairs,
+                             uint32_t* n_out, hipStream_t s);
+void launch_mlp_features(const FeatureArgsHost& fh, const float* caches, int stride, uint32_t n_caches,
+                         const ShadowBlock* blocks, uint32_t n_blocks, const uint32_t* pairs, uint32_t n, float* x,
+                         const float* vis, uint32_t pitch, float* y, hipStream_t s);
+void launch_mlp_forward(const void* pack, const float* params, const float* x, uint32_t n, float* p, int grid,
+                        hipStream_t s);
+void launch_mlp_batch_indices(uint32_t n, uint32_t batch, uint32_t seed, uint32_t* idx, bool all, hipStream_t s);
+uint32_t mlp_train_waves(uint32_t batch, int cu_count);
+void launch_mlp_train(const float* x, const float* y, const uint32_t* idx, uint32_t batch, const float* params,
+                      const void* train_w, float* partial, uint32_t n_waves, hipStream_t s);
+void launch_mlp_adam(const float* partial, uint32_t n_waves, float* params, float* m, float* v, float* grad_out,
+                     float lr, float beta1, float beta2, float eps, float bc1, float bc2, hipStream_t s);
+
+void launch_spot_select(const Counter* counter, uint32_t k, const float* caches, int stride, float* spot,
+                        uint32_t spot_cap, Counter* spot_ctr, hipStream_t s);
+void launch_spot_error(const float* spot, uint32_t spot_cap, int stride, int sh_bands, const Counter* spot_ctr,
+                       float threshold, const Counter* counter, uint32_t* stats, Counter* gate, hipStream_t s);
 
 void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const drv_sh_constants& shc,
                   const drv_gbuffer& gb, const uint32_t* cav, const float* caches, const Counter* counter,

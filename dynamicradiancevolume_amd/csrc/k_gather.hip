@@ -30,6 +30,7 @@
 
 #include "drv_internal.hpp"
 #include "k_common.hpp"
+#include "k_sample.hpp"
 
 namespace drvk {
 namespace {
@@ -47,6 +48,7 @@ struct PrepArgs {
   int R;
   int R_shadow;
   uint32_t interval;
+  int block;  // B = 2^lod texels per block side
   Vpl* vpls;
   ShadowBlock* blocks;
 };
@@ -118,7 +120,20 @@ __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
     const float cand = sqrtf(variance) * 2.0f / dd[0];
     // GLSL max(x, y) = (x < y) ? y : x, so a NaN candidate keeps SuperValWidth.
     const float rho = (a.super_val_width < cand) ? cand : a.super_val_width;
-    a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho};
+    // Mean normal of the block's VPLs (the B x B texel square at (x, y)).
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    const int B = a.block;
+    for (int dy = 0; dy < B; ++dy)
+      for (int dx = 0; dx < B; ++dx) {
+        const size_t t = static_cast<size_t>(y + dy) * a.R + (x + dx);
+        const f3 vn = unpack_normal16i(a.normal[2 * t], a.normal[2 * t + 1]);
+        sx += vn.x;
+        sy += vn.y;
+        sz += vn.z;
+      }
+    const float nl = sqrtf(sx * sx + sy * sy + sz * sz);
+    const float inv = nl > 0.0f ? 1.0f / nl : 0.0f;
+    a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho, sx * inv, sy * inv, sz * inv, 0.0f};
   }
 }
 
@@ -139,53 +154,6 @@ struct VisArgs {
 };
 
 constexpr uint32_t kVisBlocksPerItem = 8;
-
-// Corner-expanded fetch address of point p at mip level l and its trilinear
-// weights (GL 4.5 section 8.14.2, clamp to edge).  p is in level-0 texel
-// units; the lower filter corner floor(p * 2^-l - 0.5) is clamped to
-// [-1, w - 1], which selects the same clamped texels.
-struct Footprint {
-  uint32_t index;
-  float ax, ay, az;
-};
-
-__device__ __forceinline__ Footprint footprint(const VoxelView& vx, int l, float px, float py, float pz) {
-  const int w = vx.res >> l;
-  const float scale = __builtin_amdgcn_ldexpf(1.0f, -l);
-  const float u = __builtin_fmaf(px, scale, -0.5f);
-  const float v = __builtin_fmaf(py, scale, -0.5f);
-  const float q = __builtin_fmaf(pz, scale, -0.5f);
-  const float fu = floorf(u), fv = floorf(v), fq = floorf(q);
-  const float hi = static_cast<float>(w - 1);
-  // Clamped in float first so the conversion is always in range (NaN -> -1).
-  const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fu, -1.0f, hi) + 1.0f);
-  const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fv, -1.0f, hi) + 1.0f);
-  const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fq, -1.0f, hi) + 1.0f);
-  const uint32_t wp = static_cast<uint32_t>(w + 1);
-  Footprint f;
-  f.index = vx.corner_offset[l] + __umul24(__umul24(k, wp) + j, wp) + i;
-  f.ax = u - fu;
-  f.ay = v - fv;
-  f.az = q - fq;
-  return f;
-}
-
-// Trilinear blend of the 8 packed unorm8 texels (byte dx + 2 dy + 4 dz),
-// in units of 1/255.
-__device__ __forceinline__ float blend8(uint64_t c, float ax, float ay, float az) {
-  const uint32_t lo = static_cast<uint32_t>(c), hi = static_cast<uint32_t>(c >> 32);
-  const float t000 = static_cast<float>(lo & 0xffu), t100 = static_cast<float>((lo >> 8) & 0xffu);
-  const float t010 = static_cast<float>((lo >> 16) & 0xffu), t110 = static_cast<float>(lo >> 24);
-  const float t001 = static_cast<float>(hi & 0xffu), t101 = static_cast<float>((hi >> 8) & 0xffu);
-  const float t011 = static_cast<float>((hi >> 16) & 0xffu), t111 = static_cast<float>(hi >> 24);
-  const float x00 = __builtin_fmaf(ax, t100 - t000, t000);
-  const float x10 = __builtin_fmaf(ax, t110 - t010, t010);
-  const float x01 = __builtin_fmaf(ax, t101 - t001, t001);
-  const float x11 = __builtin_fmaf(ax, t111 - t011, t011);
-  const float y0 = __builtin_fmaf(ay, x10 - x00, x00);
-  const float y1 = __builtin_fmaf(ay, x11 - x01, x01);
-  return __builtin_fmaf(az, y1 - y0, y0);
-}
 
 // Below this transmittance a ray counts as fully occluded: the remaining
 // samples can lower the returned visibility by at most this much.
@@ -501,6 +469,7 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.R = rsm.read_resolution;
   a.R_shadow = rsm.read_resolution >> static_cast<int>(light.indirect_shadow_lod);
   a.interval = static_cast<uint32_t>(light.indirect_shadow_sample_interval);
+  a.block = 1 << static_cast<int>(light.indirect_shadow_lod);
   a.vpls = vpls;
   a.blocks = blocks;
   const uint32_t n = static_cast<uint32_t>(a.R) * a.R;

@@ -188,6 +188,59 @@ class Renderer:
         self._check(self.lib.drv_read_visibility(self.ctx, out.ctypes.data, out.size), "drv_read_visibility")
         return out
 
+    # --- surrogate visibility ---------------------------------------------------
+    def mlp_set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (_abi.DRV_MLP_PARAMS,)
+        self._check(self.lib.drv_mlp_set_params(self.ctx, p.ctypes.data, p.size), "drv_mlp_set_params")
+
+    def mlp_params(self) -> np.ndarray:
+        out = np.empty(_abi.DRV_MLP_PARAMS, np.float32)
+        self._check(self.lib.drv_mlp_get_params(self.ctx, out.ctypes.data, out.size), "drv_mlp_get_params")
+        return out
+
+    def set_surrogate(self, enabled: bool, spot_fraction: float = 0.01, fallback_threshold: float = 0.05):
+        s = _abi.DrvSurrogate(int(enabled), spot_fraction, fallback_threshold)
+        self._check(self.lib.drv_set_surrogate(self.ctx, C.byref(s)), "drv_set_surrogate")
+
+    def surrogate_stats(self) -> dict:
+        s = _abi.DrvSurrogateStats()
+        self._check(self.lib.drv_get_surrogate_stats(self.ctx, C.byref(s)), "drv_get_surrogate_stats")
+        return {"spot_error": s.spot_error, "fell_back": bool(s.fell_back), "spot_caches": s.spot_caches}
+
+    def mlp_collect(self, n: int, seed: int, x: torch.Tensor, y: torch.Tensor):
+        assert x.dtype == torch.float32 and x.is_contiguous() and x.numel() >= 32 * n and y.numel() >= n
+        self._check(self.lib.drv_mlp_collect(self.ctx, n, seed, x.data_ptr(), y.data_ptr()), "drv_mlp_collect")
+
+    def mlp_features(self, pairs: torch.Tensor, x: torch.Tensor):
+        n = pairs.shape[0]
+        assert pairs.dtype == torch.int32 and pairs.is_contiguous() and x.numel() >= 32 * n
+        self._check(self.lib.drv_mlp_features(self.ctx, pairs.data_ptr(), n, x.data_ptr()), "drv_mlp_features")
+
+    def mlp_forward(self, x: torch.Tensor, p: torch.Tensor):
+        n = x.shape[0]
+        assert x.dtype == torch.float32 and x.is_contiguous() and p.numel() >= n
+        self._check(self.lib.drv_mlp_forward(self.ctx, x.data_ptr(), n, p.data_ptr()), "drv_mlp_forward")
+
+    def mlp_gradient(self, x: torch.Tensor, y: torch.Tensor) -> np.ndarray:
+        out = np.empty(_abi.DRV_MLP_PARAMS + 1, np.float32)
+        self._check(self.lib.drv_mlp_gradient(self.ctx, x.data_ptr(), y.data_ptr(), x.shape[0], out.ctypes.data),
+                    "drv_mlp_gradient")
+        return out
+
+    def mlp_train(self, x: torch.Tensor, y: torch.Tensor, n: int, batch: int, steps: int, lr: float,
+                  seed: int = 0) -> float:
+        loss = C.c_float()
+        self._check(self.lib.drv_mlp_train(self.ctx, x.data_ptr(), y.data_ptr(), n, batch, steps, lr, seed,
+                                           C.byref(loss)), "drv_mlp_train")
+        return loss.value
+
+    def mlp_finetune(self, n_samples: int, batch: int, steps: int, lr: float, seed: int) -> float:
+        loss = C.c_float()
+        self._check(self.lib.drv_mlp_finetune(self.ctx, n_samples, batch, steps, lr, seed, C.byref(loss)),
+                    "drv_mlp_finetune")
+        return loss.value
+
     def set_profiling(self, on: bool):
         self._check(self.lib.drv_set_profiling(self.ctx, int(on)), "drv_set_profiling")
 

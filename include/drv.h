@@ -239,6 +239,61 @@ int drv_read_voxels(drv_ctx* ctx, int32_t level, uint8_t* host_out);
  * drv_light_caches call, float[live * n_blocks], cache-major. */
 int drv_read_visibility(drv_ctx* ctx, float* host_out, uint64_t count);
 
+/* ---- surrogate visibility -------------------------------------------- */
+
+/* A 32-64-64-64-1 MLP (ReLU hidden layers, sigmoid output; BF16 operands,
+ * FP32 accumulation) that predicts the cone-traced shadowing of a (cache,
+ * shadow block) pair from 32 features: cache and VAL geometry and 12
+ * voxel-mip occupancy samples along the segment.  It has no counterpart in
+ * the reference; when enabled it replaces the cone trace of
+ * shader/cacheLightingRSM.comp:167-231 inside drv_light_caches. */
+#define DRV_MLP_PARAMS 10497 /* W1[32][64], b1[64], W2[64][64], b2, W3[64][64], b3, w4[64], b4 */
+#define DRV_MLP_FEATURES 32
+
+typedef struct drv_surrogate {
+  int32_t enabled;          /* 1: drv_light_caches predicts visibility with the MLP */
+  float spot_fraction;      /* fraction of caches cone-traced each light as a spot check (0 = off) */
+  float fallback_threshold; /* spot-check SH relative L2 error above which the light is re-gathered with cone tracing */
+} drv_surrogate;
+
+typedef struct drv_surrogate_stats {
+  float spot_error;     /* SH relative L2 error of the MLP on the spot-check caches (last light) */
+  int32_t fell_back;    /* 1 if the last light fell back to cone tracing */
+  uint32_t spot_caches; /* caches in the spot check */
+} drv_surrogate_stats;
+
+/* Network parameters (FP32 master copy, layout of DRV_MLP_PARAMS).  Setting
+ * them resets the optimizer state. */
+int drv_mlp_set_params(drv_ctx* ctx, const float* host_params, uint32_t count);
+int drv_mlp_get_params(drv_ctx* ctx, float* host_params, uint32_t count);
+int drv_set_surrogate(drv_ctx* ctx, const drv_surrogate* settings);
+int drv_get_surrogate_stats(drv_ctx* ctx, drv_surrogate_stats* out);
+/* Training samples from the last cone-traced drv_light_caches call: n random
+ * (cache, shadow block) pairs -> features x_dev[n][32] and the cone-traced
+ * visibility y_dev[n] (device pointers). */
+int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float* y_dev);
+/* Features of explicit (cache, block) pairs (pairs_dev[n][2], device) of the
+ * last drv_light_caches call; a pair naming no live cache or block gets NaN
+ * features. */
+int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float* x_dev);
+/* Network output for features x_dev[n][32] -> p_dev[n] (device). */
+int drv_mlp_forward(drv_ctx* ctx, const float* x_dev, uint32_t n, float* p_dev);
+/* Gradient of the mean binary cross-entropy over all n samples, computed by
+ * the training kernels: grad_host[DRV_MLP_PARAMS + 1], the last entry being
+ * the summed loss. */
+int drv_mlp_gradient(drv_ctx* ctx, const float* x_dev, const float* y_dev, uint32_t n, float* grad_host);
+/* `steps` Adam steps (beta 0.9 / 0.999) on random minibatches of `batch`
+ * samples; mean_loss_host (optional) receives the last step's mean loss. */
+int drv_mlp_train(drv_ctx* ctx, const float* x_dev, const float* y_dev, uint32_t n, uint32_t batch, uint32_t steps,
+                  float lr, uint32_t seed, float* mean_loss_host);
+
+/* Online fine-tuning: collects n_samples fresh cone-traced samples from the
+ * last drv_light_caches call (its spot-check caches when the surrogate was
+ * on) into library-owned buffers and runs drv_mlp_train on them.  A render
+ * loop calls it every few frames (the bench and tests use 16). */
+int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t steps, float lr, uint32_t seed,
+                     float* mean_loss_host);
+
 /* ---- profiling -------------------------------------------------------- */
 
 /* When on, each stage is bracketed by HIP events on the context stream;

@@ -817,6 +817,90 @@ float orc_cone_march_f32(const drv_volume* vol, int32_t res, const uint8_t* mips
   return s;
 }
 
+void orc_shadow_blocks(const drv_spot_light* light, const orc_rsm* rsm, float* out) {
+  std::vector<Block<float>> blocks;
+  prepare_blocks<float>(light, rsm, blocks);
+  const int R = rsm->read_resolution;
+  const int B = 1 << static_cast<int>(light->indirect_shadow_lod);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    uint32_t x, y;
+    morton_decode(static_cast<uint32_t>(b) * static_cast<uint32_t>(light->indirect_shadow_sample_interval), &x, &y);
+    // Block b covers the B x B texel square with lower corner (x, y).
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int dy = 0; dy < B; ++dy)
+      for (int dx = 0; dx < B; ++dx) {
+        const size_t t = static_cast<size_t>(y + dy) * R + (x + dx);
+        const V3<float> n = unpack_normal16i<float>(rsm->normal[2 * t], rsm->normal[2 * t + 1]);
+        sx += n.x;
+        sy += n.y;
+        sz += n.z;
+      }
+    const float nl = std::sqrt(sx * sx + sy * sy + sz * sz);
+    const float inv = nl > 0.0f ? 1.0f / nl : 0.0f;
+    float* o = out + 8 * b;
+    o[0] = blocks[b].avg_pos.x;
+    o[1] = blocks[b].avg_pos.y;
+    o[2] = blocks[b].avg_pos.z;
+    o[3] = blocks[b].rho;
+    o[4] = sx * inv;
+    o[5] = sy * inv;
+    o[6] = sz * inv;
+    o[7] = 0.0f;
+  }
+}
+
+void orc_mlp_features(const drv_volume* vol, int32_t res, const uint8_t* mips, const float cam[3], const float wp[3],
+                      const float blk[8], float out[32]) {
+  const float fres = static_cast<float>(res);
+  const float vsz = vol->voxel_size_in_world * fres;
+  const float* vmin = vol->volume_world_min;
+  // Geometry: cache position in the volume, unit direction to the camera,
+  // VAL position in the volume, block normal, unit direction to the VAL,
+  // distance, aperture, march goal and two cosines.
+  const float vp[3] = {(wp[0] - vmin[0]) / vsz, (wp[1] - vmin[1]) / vsz, (wp[2] - vmin[2]) / vsz};
+  float c[3] = {cam[0] - wp[0], cam[1] - wp[1], cam[2] - wp[2]};
+  const float cl = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  const float ci = cl > 0.0f ? 1.0f / cl : 0.0f;
+  for (float& v : c) v *= ci;
+  float t[3] = {blk[0] - wp[0], blk[1] - wp[1], blk[2] - wp[2]};
+  const float ld = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  const float li = ld > 0.0f ? 1.0f / ld : 0.0f;
+  for (float& v : t) v *= li;
+  const float dist = ld / vol->voxel_size_in_world;
+  const float goal = dist - 2.0f;
+  const float G[20] = {vp[0], vp[1], vp[2], c[0], c[1], c[2],
+                       (blk[0] - vmin[0]) / vsz, (blk[1] - vmin[1]) / vsz, (blk[2] - vmin[2]) / vsz,
+                       blk[4], blk[5], blk[6], t[0], t[1], t[2],
+                       dist * (1.0f / 64.0f), blk[3] * 16.0f, goal * (1.0f / 64.0f),
+                       -(blk[4] * t[0] + blk[5] * t[1] + blk[6] * t[2]), c[0] * t[0] + c[1] * t[1] + c[2] * t[2]};
+  // Occupancy samples along the segment from 2 voxels off the cache towards
+  // the VAL: 8 at (i + 0.5) / 8 of it on the level of the cone radius there,
+  // 4 at (i + 0.5) / 4 on level 3 (single-level trilinear, no mip blend).
+  const int q = log2i(res);
+  const float span = goal > 0.0f ? goal : 0.0f;
+  float S[12];
+  for (int k = 0; k < 12; ++k) {
+    float d;
+    int lvl;
+    if (k < 8) {
+      d = 2.0f + span * ((static_cast<float>(k) + 0.5f) * 0.125f);
+      const float r = d * blk[3];
+      const float lod = std::log2(r > 1.0f ? r : 1.0f);
+      lvl = std::min(static_cast<int>(lod), q);
+    } else {
+      d = 2.0f + span * ((static_cast<float>(k - 8) + 0.5f) * 0.25f);
+      lvl = std::min(3, q);
+    }
+    const float p[3] = {(vp[0] * fres + t[0] * d) / fres, (vp[1] * fres + t[1] * d) / fres,
+                        (vp[2] * fres + t[2] * d) / fres};
+    S[k] = sample_level<float>(mips + level_offset(res, lvl), res >> lvl, p);
+  }
+  for (int g = 0; g < 4; ++g) {
+    for (int e = 0; e < 5; ++e) out[8 * g + e] = G[5 * g + e];
+    for (int e = 0; e < 3; ++e) out[8 * g + 5 + e] = S[3 * g + e];
+  }
+}
+
 void orc_entry_sh(const float* e, int32_t sh_bands, float out[27]) {
   const int n = sh_bands >= 2 ? 9 : 4;
   for (int k = 0; k < 9; ++k)

@@ -114,6 +114,17 @@ float orc_cone_trace_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t
  * (cacheLightingRSM.comp:195-230), FP32. */
 float orc_cone_march_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t* voxel_mips, const float wp[3],
                          const float avg_pos[3], float rho, int32_t* steps_out);
+/* ---- surrogate visibility features (no counterpart in the reference) ---- */
+/* Shadow blocks of one light: per block the averaged VAL position and cone
+ * aperture of cacheLightingRSM.comp:169-192, plus the normalized mean VPL
+ * normal of its B x B texels; out[8 * n_blocks] = x, y, z, rho, nx, ny, nz, 0. */
+void orc_shadow_blocks(const drv_spot_light* light, const orc_rsm* rsm, float* out);
+/* The 32 network inputs of the pair (cache at wp, block blk[8]), in the
+ * network's slot order (slot 8g + e: e < 5 geometry feature 5g + e, else
+ * voxel sample 3g + e - 5). */
+void orc_mlp_features(const drv_volume* vol, int32_t voxel_res, const uint8_t* voxel_mips, const float cam[3],
+                      const float wp[3], const float blk[8], float out[32]);
+
 /* Extracts the 27 SH coefficients (canonical order above) from an entry. */
 void orc_entry_sh(const float* entry, int32_t sh_bands, float out27[27]);
 

@@ -58,6 +58,9 @@ _FUNCS = {
     "orc_cone_march_f32": (C.c_float, [P(_abi.DrvVolume), C.c_int32, vp, P(C.c_float), P(C.c_float), C.c_float,
                                        P(C.c_int32)]),
     "orc_entry_sh": (None, [P(C.c_float), C.c_int32, P(C.c_float)]),
+    "orc_shadow_blocks": (None, [P(_abi.DrvSpotLight), P(OrcRsm), vp]),
+    "orc_mlp_features": (None, [P(_abi.DrvVolume), C.c_int32, vp, P(C.c_float), P(C.c_float), P(C.c_float),
+                                P(C.c_float)]),
     "orc_apply": (None, [P(_abi.DrvConfig), P(_abi.DrvVolume), P(_abi.DrvFrame), P(_abi.DrvShConstants), vp, vp, vp,
                          vp, vp, C.c_uint32, vp, C.c_int32]),
 }
@@ -177,6 +180,31 @@ def gather_f64(cfg, vol, shc, light, fi, mips: np.ndarray, positions: np.ndarray
                          _ptr(vis) if vis is not None else None, n_threads)
     del keep
     return sh, vis
+
+
+def shadow_blocks(light, fi) -> np.ndarray:
+    """[n_blocks][8] = VAL position, aperture, mean VPL normal, 0."""
+    rsm, keep = orc_rsm(fi)
+    nb = (fi.read_resolution ** 2) // int(light.indirect_shadow_sample_interval)
+    out = np.zeros((nb, 8), np.float32)
+    lib().orc_shadow_blocks(C.byref(light), C.byref(rsm), _ptr(out))
+    del keep
+    return out
+
+
+def mlp_features(vol, res: int, mips: np.ndarray, cam, positions: np.ndarray, blocks: np.ndarray,
+                 pairs: np.ndarray) -> np.ndarray:
+    """Network inputs [n][32] of (cache, block) pairs[n][2]."""
+    mips = np.ascontiguousarray(mips, np.uint8)
+    cam_a = (C.c_float * 3)(*[float(v) for v in cam])
+    pos = np.ascontiguousarray(positions, np.float32)
+    blk = np.ascontiguousarray(blocks, np.float32)
+    out = np.zeros((len(pairs), 32), np.float32)
+    fp = P(C.c_float)
+    for i, (c, b) in enumerate(np.asarray(pairs)):
+        lib().orc_mlp_features(C.byref(vol), res, _ptr(mips), cam_a, pos[c].ctypes.data_as(fp),
+                               blk[b].ctypes.data_as(fp), out[i].ctypes.data_as(fp))
+    return out
 
 
 def apply(cfg, vol, frame, shc, fi, cav: np.ndarray, entries: np.ndarray, live: int, n_threads: int = 0):

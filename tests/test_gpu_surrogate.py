@@ -1,0 +1,233 @@
+"""Surrogate visibility on the GPU, against the CPU oracle.
+
+* forward pass of the BF16 matrix-core kernel vs the numpy network with the
+  same bfloat16 roundings (this also pins the assumed MFMA result layout);
+* gradient of the training kernels vs the numpy backward pass;
+* one Adam step of drv_mlp_train vs numpy Adam on the same minibatch;
+* the 32 features of (cache, block) pairs vs the C restatement;
+* the surrogate path inside drv_light_caches: the spot check reports the
+  error the host recomputes, a failing check falls back to the bit-exact
+  cone-traced result, and collection / fine-tuning work on the spot set.
+"""
+import numpy as np
+import pytest
+import torch
+
+from canon import rel_l2
+from conftest import announce, frame_inputs
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _renderer(fi, **overrides):
+    from dynamicradiancevolume_amd.pipeline import Renderer
+
+    cfg = fi.workload.with_(**overrides).config() if overrides else fi.workload.config()
+    return Renderer(cfg)
+
+
+def _traced_frame(fi, r=None):
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame
+
+    r = r or _renderer(fi)
+    df = DeviceFrame(fi)
+    r.render_frame(df)
+    return r, df
+
+
+def _random_features(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0.0, 1.0, (n, 32)).astype(np.float32)
+    x[:, 5:8] = rng.uniform(0, 1, (n, 3))  # occupancy-like slots
+    return x
+
+
+def _hash32(x):
+    x = np.asarray(x, np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def _batch_indices(n, batch, seed):
+    i = np.arange(batch, dtype=np.uint64)
+    return (_hash32((i * 0x9E3779B9 + seed * 0x85EBCA6B + 7) & 0xFFFFFFFF) % n).astype(np.int64)
+
+
+def test_mlp_forward_matches_oracle(cornell):
+    from dynamicradiancevolume_amd.surrogate import init_params
+    from oracle import mlp
+
+    announce("mlp forward vs oracle")
+    r = _renderer(cornell)
+    params = init_params(11)
+    rng = np.random.default_rng(5)
+    params[2048:2112] = rng.normal(0, 0.2, 64)   # non-zero biases
+    params[6208:6272] = rng.normal(0, 0.2, 64)
+    params[10368:10432] = rng.normal(0, 0.2, 64)
+    r.mlp_set_params(params)
+    np.testing.assert_array_equal(r.mlp_params(), params)
+    n = 1000  # not a multiple of 16
+    x = _random_features(n, 7)
+    xd = torch.from_numpy(x).to(DEV)
+    pd = torch.full((n,), -1.0, device=DEV)
+    r.mlp_forward(xd, pd)
+    p = pd.cpu().numpy()
+    ref, _ = mlp.forward(params, x, bf16=True)
+    assert np.all(np.isfinite(p))
+    err = np.abs(p - ref)
+    assert err.max() <= 2e-3, err.max()
+    assert np.mean(err) <= 2e-4, np.mean(err)
+
+
+def test_mlp_gradient_matches_oracle(cornell):
+    from dynamicradiancevolume_amd.surrogate import OFFSETS, init_params
+    from oracle import mlp
+
+    announce("mlp gradient vs oracle")
+    r = _renderer(cornell)
+    params = init_params(12)
+    rng = np.random.default_rng(6)
+    params[2048:2112] = rng.normal(0, 0.2, 64)
+    params[6208:6272] = rng.normal(0, 0.2, 64)
+    params[10368:10432] = rng.normal(0, 0.2, 64)
+    r.mlp_set_params(params)
+    n = 3000
+    x = _random_features(n, 8)
+    y = (rng.uniform(0, 1, n) < 0.3).astype(np.float32)
+    g = r.mlp_gradient(torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV))
+    loss, ref = mlp.loss_and_grad(params, x, y, bf16=True)
+    assert g[-1] == pytest.approx(loss, rel=1e-3)
+    names = list(OFFSETS) + ["end"]
+    bounds = list(OFFSETS.values()) + [mlp.N_PARAMS]
+    for k in range(8):
+        lo, hi = bounds[k], bounds[k + 1]
+        e = rel_l2(g[lo:hi], ref[lo:hi])
+        assert e <= 1e-2, (names[k], e)
+
+
+def test_mlp_adam_step_matches_oracle(cornell):
+    from dynamicradiancevolume_amd.surrogate import init_params
+    from oracle import mlp
+
+    announce("mlp adam step vs oracle")
+    r = _renderer(cornell)
+    params = init_params(13)
+    r.mlp_set_params(params)
+    n, batch, lr, seed = 2000, 512, 1e-3, 5
+    x = _random_features(n, 9)
+    y = np.random.default_rng(10).uniform(0, 1, n).astype(np.float32)
+    xd, yd = torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV)
+    r.mlp_train(xd, yd, n, batch, 1, lr, seed)
+    p1 = r.mlp_params().astype(np.float64)
+    # the same minibatch through the gradient entry point (same kernel, same order)
+    idx = _batch_indices(n, batch, (seed * 0x9E3779B1) & 0xFFFFFFFF)
+    r.mlp_set_params(params)
+    g = r.mlp_gradient(xd[torch.from_numpy(idx).to(DEV)].contiguous(),
+                       yd[torch.from_numpy(idx).to(DEV)].contiguous())[:-1].astype(np.float64)
+    ref, _, _ = mlp.adam_step(params.astype(np.float64), g, np.zeros_like(g), np.zeros_like(g), 1, lr)
+    np.testing.assert_allclose(p1, ref, rtol=0, atol=1e-3 * lr + 1e-7)
+
+
+def test_mlp_features_match_oracle(cornell):
+    from oracle import orc
+
+    announce("mlp features vs oracle (cornell)")
+    r, df = _traced_frame(cornell)
+    _, live = r.cache_count()
+    pos = r.read_caches(live)[:, :3].copy()
+    light = cornell.light
+    blocks = orc.shadow_blocks(light, cornell)
+    res = cornell.workload.voxel_resolution
+    mips = np.concatenate([r.read_voxels(l).reshape(-1) for l in range(int(np.log2(res)) + 1)])
+    rng = np.random.default_rng(3)
+    n = 4000
+    pairs = np.stack([rng.integers(0, live, n), rng.integers(0, len(blocks), n)], 1).astype(np.int32)
+    xd = torch.empty((n, 32), device=DEV)
+    r.mlp_features(torch.from_numpy(pairs).to(DEV), xd)
+    x = xd.cpu().numpy()
+    ref = orc.mlp_features(r.volume(), res, mips, cornell.workload.eye, pos, blocks, pairs)
+    close = np.abs(x - ref) <= 1e-5 + 1e-5 * np.abs(ref)
+    # The level of a fine sample is floor(log2(radius)); the hardware log2
+    # may round across an integer on rare pairs.
+    assert close.mean() >= 0.999, close.mean()
+    rows_off = np.flatnonzero(~close.all(1))
+    assert len(rows_off) <= 0.01 * n, len(rows_off)
+
+
+def _sh(r, live, bands):
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    return entries_sh(r.read_caches(live), bands)
+
+
+def _light_again(r, df):
+    # One more drv_light_caches on the same cache list: its contribution is
+    # added to the entries (no re-allocation, so cache order is unchanged).
+    r.light_caches_rsm(df.rsm(), df.inputs.light)
+
+
+def test_surrogate_fallback_is_bit_exact(cornell):
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    announce("surrogate fallback (threshold 0) == traced")
+    r, df = _traced_frame(cornell)
+    _, live = r.cache_count()
+    traced = r.read_caches(live)
+    r.mlp_set_params(init_params(1))
+    r.set_surrogate(True, spot_fraction=0.05, fallback_threshold=0.0)
+    _light_again(r, df)
+    st = r.surrogate_stats()
+    assert st["fell_back"] and st["spot_caches"] == -(-live // 20)
+    both = r.read_caches(live)
+    # the fallback repeats the cone-traced pass exactly: x + x = 2x
+    np.testing.assert_array_equal(both[:, :4], traced[:, :4])
+    np.testing.assert_array_equal(both[:, 4:], 2.0 * traced[:, 4:])
+
+
+def test_surrogate_spot_error_matches_host(cornell):
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    announce("surrogate spot error vs host recomputation")
+    r, df = _traced_frame(cornell)
+    _, live = r.cache_count()
+    bands = cornell.workload.sh_bands
+    traced = _sh(r, live, bands).astype(np.float64)
+    r.mlp_set_params(init_params(2))
+    r.set_surrogate(True, spot_fraction=0.1, fallback_threshold=1e9)
+    _light_again(r, df)
+    st = r.surrogate_stats()
+    assert not st["fell_back"]
+    net = _sh(r, live, bands).astype(np.float64) - traced
+    idx = np.arange(0, live, 10)
+    assert st["spot_caches"] == len(idx)
+    host = rel_l2(net[idx], traced[idx])
+    # the device spot set is lit with the same kernels on compact copies
+    assert st["spot_error"] == pytest.approx(host, rel=1e-3, abs=1e-6)
+    assert host > 0.0
+
+
+def test_surrogate_collect_and_finetune_on_spot_set(cornell):
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    announce("surrogate collect + finetune on the spot set")
+    r, df = _traced_frame(cornell)
+    n = 5000
+    x = torch.empty((n, 32), device=DEV)
+    y = torch.empty((n,), device=DEV)
+    r.mlp_set_params(init_params(3))
+    r.mlp_collect(n, 1, x, y)          # labels of the cone-traced pass
+    yt = y.cpu().numpy()
+    assert np.all((yt >= 0) & (yt <= 1)) and (yt < 0.999).mean() > 0.01
+    assert np.isfinite(x.cpu().numpy()).all()
+    r.set_surrogate(True, spot_fraction=0.05, fallback_threshold=1e9)
+    r.render_frame(df)
+    before = r.mlp_params()
+    loss = r.mlp_finetune(4096, 1024, 5, 1e-3, 7)
+    assert np.isfinite(loss) and loss > 0
+    assert not np.array_equal(r.mlp_params(), before)
