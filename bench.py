@@ -32,6 +32,9 @@ FLOP_SH = {1: 49, 2: 93}          # per cache.VPL interaction
 FLOP_CONE_SETUP = 90              # per (cache, shadow block) trace
 FLOP_CONE_STEP = 60               # per march step
 
+# Online fine-tuning of the surrogate (every 16 frames).
+FINETUNE_SAMPLES, FINETUNE_BATCH, FINETUNE_STEPS, FINETUNE_LR = 8192, 2048, 4, 1e-4
+
 
 def _cpu_baseline(fi, r, n_threads: int, budget_s: float):
     """The oracle's FP32 gather (VPL sum + cone tracing) on a bounded sample of
@@ -68,6 +71,72 @@ def _cpu_baseline(fi, r, n_threads: int, budget_s: float):
     }
 
 
+def _time_frames(r, df, stream, steps: int, finetune_every: int = 0):
+    """ms per frame over `steps` whole frames (HIP events on the library's
+    stream), optionally with an online fine-tuning step every N frames."""
+    import torch
+
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for i in range(steps):
+        r.render_frame(df)
+        if finetune_every and (i + 1) % finetune_every == 0:
+            r.mlp_finetune(FINETUNE_SAMPLES, FINETUNE_BATCH, FINETUNE_STEPS, FINETUNE_LR, 1000 + i)
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / steps, time.perf_counter() - t_wall
+
+
+def _sh_by_position(r, bands: int):
+    """{cache position bits: 27 SH coefficients} of the current cache list
+    (the append order of the allocation differs between frames)."""
+    import numpy as np
+
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    _, live = r.cache_count()
+    e = r.read_caches(live)
+    keys = [tuple(k) for k in e[:, :3].view(np.uint32).tolist()]
+    return dict(zip(keys, entries_sh(e, bands).astype(np.float64)))
+
+
+def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, threshold: float):
+    """The same frames with the surrogate visibility network (committed
+    weights, 1 % spot check, fine-tuning every 16 frames)."""
+    import numpy as np
+    import torch
+
+    from dynamicradiancevolume_amd import surrogate as S
+
+    if not S.WEIGHTS.exists():
+        return None
+    ref = _sh_by_position(r, w.sh_bands)
+    out = {"weights": S.WEIGHTS.name, "spot_fraction": 0.01, "finetune_every": 16}
+    for label, thr in (("no_fallback", 1e30), ("fallback", threshold)):
+        r.mlp_set_params(S.load_weights())
+        r.set_surrogate(True, spot_fraction=0.01, fallback_threshold=thr)
+        for _ in range(warmup):
+            r.render_frame(df)
+        torch.cuda.synchronize()
+        ms, _ = _time_frames(r, df, stream, steps, finetune_every=16)
+        st = r.surrogate_stats()
+        out[label] = {"ms_per_frame": ms, "speedup_vs_traced": traced_ms / ms, "threshold": thr,
+                      "spot_error_last_frame": st["spot_error"], "fell_back_last_frame": st["fell_back"]}
+    # SH error of one surrogate frame against the cone-traced frame
+    r.set_surrogate(True, spot_fraction=0.01, fallback_threshold=1e30)
+    r.mlp_set_params(S.load_weights())
+    r.render_frame(df)
+    sur = _sh_by_position(r, w.sh_bands)
+    keys = [k for k in ref if k in sur]
+    a = np.stack([sur[k] for k in keys])
+    b = np.stack([ref[k] for k in keys])
+    out["sh_rel_l2_vs_traced"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    r.set_surrogate(False)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +145,8 @@ def main() -> int:
     ap.add_argument("--workload", default="boxes_1080p")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
@@ -101,16 +172,7 @@ def main() -> int:
         r.render_frame(df)
     torch.cuda.synchronize()
 
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        r.render_frame(df)
-    end.record(stream)
-    end.synchronize()
-    wall = time.perf_counter() - t_wall
-    ms = start.elapsed_time(end) / args.steps
+    ms, wall = _time_frames(r, df, stream, args.steps)
     demanded, live = r.cache_count()
     interactions = live * n_vpl
     print(f"[bench] timed {args.steps} frames: {ms:.3f} ms/frame (wall {wall:.2f} s), caches {live}",
@@ -143,7 +205,8 @@ def main() -> int:
     dms, dflop = kernels[dom]
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
     roofline = {
-        "bound": "valu_fp32",
+        "bound": "mfma",
+        "pipe": "fp32 vector ALU (compute-bound; the FP32 peak is the roofline)",
         "kernel": dom,
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
@@ -153,6 +216,11 @@ def main() -> int:
         "kernel_ms": dms,
         "algorithmic_flop": dflop,
     }
+
+    sur = None
+    if not args.no_surrogate:
+        print("[bench] surrogate", file=sys.stderr, flush=True)
+        sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -191,6 +259,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(line))
     return 0

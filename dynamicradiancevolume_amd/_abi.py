@@ -145,6 +145,11 @@ class DrvSurrogateStats(C.Structure):
     _fields_ = [("spot_error", C.c_float), ("fell_back", C.c_int32), ("spot_caches", C.c_uint32)]
 
 
+class DrvTuning(C.Structure):
+    _fields_ = [("visibility_kernel", C.c_int32), ("blocks_per_item", C.c_int32),
+                ("gather_max_splits", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_min_lanes", C.c_int32)]
+
+
 DRV_MLP_PARAMS = 10497
 DRV_MLP_FEATURES = 32
 
@@ -185,6 +190,8 @@ DRV_FUNCTIONS = {
     "drv_mlp_gradient": (C.c_int, [vp, vp, vp, C.c_uint32, vp]),
     "drv_mlp_train": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, vp]),
     "drv_mlp_finetune": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, vp]),
+    "drv_set_tuning": (C.c_int, [vp, P(DrvTuning)]),
+    "drv_get_tuning": (C.c_int, [vp, P(DrvTuning)]),
     "drv_set_profiling": (C.c_int, [vp, C.c_int32]),
     "drv_get_stage_times": (C.c_int, [vp, P(DrvStageTimes)]),
 }

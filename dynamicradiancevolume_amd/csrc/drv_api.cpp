@@ -86,16 +86,19 @@ struct drv_ctx {
   const float* label_vis = nullptr;
   uint32_t label_pitch = 0;
 
+  drvk::Tuning tune;
+  size_t partial_splits = 0;  // splits the partial buffer holds
+
   bool profiling = false;
   int profiling_level = 0;
   uint64_t* step_counter = nullptr;  // [traces, reference steps, executed steps] when counting
   hipEvent_t ev[16] = {};
+  uint32_t recorded = 0;  // events recorded at least once
   drv_stage_times times{};
 };
 
 namespace {
 
-constexpr uint32_t kMaxSplits = 32;
 
 int fail(drv_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -152,12 +155,20 @@ void light_matrices(const drv_spot_params& p, float* vp_out, float* ivp_out) {
 }
 
 void record(drv_ctx* ctx, int i) {
-  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+  if (ctx->profiling && hipEventRecord(ctx->ev[i], ctx->stream) == hipSuccess) ctx->recorded |= 1u << i;
 }
 
+// Time between two recorded events; 0 for a pair that was never recorded.
+// A failed query must not leave an error behind for the next entry point's
+// hipGetLastError check.
 float elapsed(drv_ctx* ctx, int a, int b) {
+  const uint32_t need = (1u << a) | (1u << b);
+  if ((ctx->recorded & need) != need) return 0.0f;
   float ms = 0.0f;
-  if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.0f;
+  if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0f;
+  }
   return ms;
 }
 
@@ -170,7 +181,8 @@ drvk::VoxelView voxel_view(const drv_ctx* ctx) {
   return v;
 }
 
-int persistent_grid(const drv_ctx* ctx) { return ctx->cu_count * 8; }
+// Persistent kernels: waves_per_cu waves per CU in 256-thread workgroups.
+int persistent_grid(const drv_ctx* ctx) { return std::max(1, ctx->cu_count * ctx->tune.waves_per_cu / 4); }
 
 template <typename T>
 int ensure(drv_ctx* ctx, T** p, size_t* cap, size_t need) {
@@ -207,13 +219,17 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     float* vis_b = ctx->spot_vis + spot_cap * n_blocks;
     drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, ctx->spot_caches, sc, ctx->spot_ctr, ctx->stream);
     drvk::launch_visibility(ctx->vol, vox, set_a, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, vis_a, sc, nullptr,
-                            grid, ctx->stream);
+                            ctx->tune, grid, ctx->stream);
     drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, set_b, stride, &ctx->spot_ctr[1], ctx->blocks,
                                 n_blocks, vis_b, sc, grid, ctx->stream);
+    // The split partials of a small set fit many more splits than the
+    // main list's (the partial buffer holds partial_splits x capacity).
+    const uint32_t spot_splits = static_cast<uint32_t>(std::min<size_t>(128, ctx->partial_splits * cap / spot_cap));
     for (int set = 0; set < 2; ++set) {
       float* cs = set == 0 ? set_a : set_b;
       drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->vpls, n_vpl,
-                          set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, grid, ctx->stream);
+                          set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, spot_splits, grid,
+                          ctx->stream);
       drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->partial, sc,
                              ctx->cu_count * 4, ctx->stream);
     }
@@ -230,7 +246,7 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
                               n_blocks, ctx->visibility, static_cast<uint32_t>(cap), grid, ctx->stream);
   if (spot)  // gate counter: live = 0 unless the spot check failed
     drvk::launch_visibility(ctx->vol, vox, ctx->caches, stride, &ctx->spot_ctr[2], ctx->blocks, n_blocks,
-                            ctx->visibility, static_cast<uint32_t>(cap), nullptr, grid, ctx->stream);
+                            ctx->visibility, static_cast<uint32_t>(cap), nullptr, ctx->tune, grid, ctx->stream);
   return DRV_OK;
 }
 
@@ -569,7 +585,13 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     if ((rc = dev_alloc(ctx, &ctx->visibility, cap * n_blocks)) != DRV_OK) return rc;
     ctx->vis_blocks_cap = n_blocks;
   }
-  if (!ctx->partial && (rc = dev_alloc(ctx, &ctx->partial, cap * 27 * kMaxSplits)) != DRV_OK) return rc;
+  const size_t splits = static_cast<size_t>(ctx->tune.gather_max_splits);
+  if (ctx->partial_splits < splits) {
+    dev_free(ctx->partial);
+    ctx->partial_splits = 0;
+    if ((rc = dev_alloc(ctx, &ctx->partial, cap * 27 * splits)) != DRV_OK) return rc;
+    ctx->partial_splits = splits;
+  }
 
   const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
   const int grid = persistent_grid(ctx);
@@ -588,7 +610,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     steps = nullptr;
   } else if (shadow) {
     drvk::launch_visibility(ctx->vol, voxel_view(ctx), ctx->caches, stride, ctx->counter, ctx->blocks, n_blocks,
-                            ctx->visibility, static_cast<uint32_t>(cap), steps, grid, ctx->stream);
+                            ctx->visibility, static_cast<uint32_t>(cap), steps, ctx->tune, grid, ctx->stream);
     ctx->label_caches = ctx->caches;
     ctx->label_counter = ctx->counter;
     ctx->label_vis = ctx->visibility;
@@ -597,7 +619,8 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   record(ctx, 7);
   drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
                       ctx->visibility, static_cast<uint32_t>(cap), shadow ? interval : 64u, shadow ? 1 : 0,
-                      ctx->partial, static_cast<uint32_t>(cap), grid, ctx->stream);
+                      ctx->partial, static_cast<uint32_t>(cap), static_cast<uint32_t>(ctx->tune.gather_max_splits),
+                      grid, ctx->stream);
   record(ctx, 8);
   drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->partial,
                          static_cast<uint32_t>(cap), ctx->cu_count * 4, ctx->stream);
@@ -785,6 +808,7 @@ int drv_get_surrogate_stats(drv_ctx* ctx, drv_surrogate_stats* out) {
 
 int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float* y_dev) {
   if (!ctx || (n && (!x_dev || !y_dev))) return DRV_ERR_INVALID_ARG;
+  if (reinterpret_cast<uintptr_t>(x_dev) % 16 != 0) return fail(ctx, DRV_ERR_INVALID_ARG, "mlp_collect: x must be 16-byte aligned");
   int rc;
   if ((rc = require_features(ctx)) != DRV_OK) return rc;
   if (!ctx->label_caches) return fail(ctx, DRV_ERR_STATE, "mlp_collect: the last light pass has no cone-traced labels");
@@ -807,6 +831,7 @@ int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float
 
 int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float* x_dev) {
   if (!ctx || (n && (!pairs_dev || !x_dev))) return DRV_ERR_INVALID_ARG;
+  if (reinterpret_cast<uintptr_t>(x_dev) % 16 != 0) return fail(ctx, DRV_ERR_INVALID_ARG, "mlp_features: x must be 16-byte aligned");
   int rc;
   if ((rc = require_features(ctx)) != DRV_OK) return rc;
   const drvk::FeatureArgsHost fh = drvk::make_feature_args(ctx->vol, ctx->frame.camera_position, voxel_view(ctx));
@@ -870,6 +895,32 @@ int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t 
   if ((rc = ensure(ctx, &ctx->ft_y, &ctx->ft_y_cap, static_cast<size_t>(n_samples))) != DRV_OK) return rc;
   if ((rc = drv_mlp_collect(ctx, n_samples, seed, ctx->ft_x, ctx->ft_y)) != DRV_OK) return rc;
   return drv_mlp_train(ctx, ctx->ft_x, ctx->ft_y, n_samples, batch, steps, lr, seed ^ 0x5bd1e995u, mean_loss_host);
+}
+
+// ---- tuning ---------------------------------------------------------------
+
+int drv_set_tuning(drv_ctx* ctx, const drv_tuning* t) {
+  if (!ctx || !t) return DRV_ERR_INVALID_ARG;
+  if ((t->visibility_kernel != 0 && t->visibility_kernel != 1) || t->blocks_per_item < 1 || t->blocks_per_item > 64 ||
+      t->gather_max_splits < 1 || t->gather_max_splits > 128 || !is_pow2(t->gather_max_splits) ||
+      t->waves_per_cu < 4 || t->waves_per_cu > 64 || t->refill_min_lanes < 1 || t->refill_min_lanes > 64)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "set_tuning: value out of range");
+  ctx->tune.visibility_kernel = t->visibility_kernel;
+  ctx->tune.blocks_per_item = t->blocks_per_item;
+  ctx->tune.gather_max_splits = t->gather_max_splits;
+  ctx->tune.waves_per_cu = t->waves_per_cu;
+  ctx->tune.refill_min_lanes = t->refill_min_lanes;
+  return DRV_OK;
+}
+
+int drv_get_tuning(const drv_ctx* ctx, drv_tuning* out) {
+  if (!ctx || !out) return DRV_ERR_INVALID_ARG;
+  out->visibility_kernel = ctx->tune.visibility_kernel;
+  out->blocks_per_item = ctx->tune.blocks_per_item;
+  out->gather_max_splits = ctx->tune.gather_max_splits;
+  out->waves_per_cu = ctx->tune.waves_per_cu;
+  out->refill_min_lanes = ctx->tune.refill_min_lanes;
+  return DRV_OK;
 }
 
 // ---- profiling ------------------------------------------------------------

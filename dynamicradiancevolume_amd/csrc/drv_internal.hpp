@@ -88,6 +88,16 @@ struct FeatureArgsHost {
   VoxelView vox;
 };
 
+// Launch shapes chosen per context (drv_set_tuning).
+// Defaults measured on boxes_1080p (6352 caches x 16384 VPLs).
+struct Tuning {
+  int visibility_kernel = 0;   // 0: lane per cache, one block at a time; 1: refilling lane pool
+  int blocks_per_item = 16;    // shadow blocks per visibility work item
+  int gather_max_splits = 128; // VPL splits of the gather
+  int waves_per_cu = 64;       // persistent grid size
+  int refill_min_lanes = 16;   // lane pool: idle lanes that trigger a refill
+};
+
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 
@@ -105,11 +115,11 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
                          hipStream_t s);
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
-                       uint32_t vis_pitch, uint64_t* step_counter, int grid, hipStream_t s);
+                       uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid, hipStream_t s);
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,
                    const Counter* counter, const Vpl* vpls, uint32_t n_vpl, const float* visibility,
                    uint32_t vis_pitch, uint32_t sample_interval, int indirect_shadow, float* partial,
-                   uint32_t partial_pitch, int grid, hipStream_t s);
+                   uint32_t partial_pitch, uint32_t max_splits, int grid, hipStream_t s);
 void launch_reduce_sh(const drv_sh_constants& shc, int sh_bands, float* caches, int stride, const Counter* counter,
                       const float* partial, uint32_t partial_pitch, int grid, hipStream_t s);
 

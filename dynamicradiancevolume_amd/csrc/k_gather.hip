@@ -151,9 +151,9 @@ struct VisArgs {
   float* vis;
   uint32_t pitch;
   uint64_t* steps;  // optional {traces, reference steps, executed steps}
+  uint32_t blocks_per_item;
+  uint32_t refill_min;  // lane pool: idle lanes that trigger a refill
 };
-
-constexpr uint32_t kVisBlocksPerItem = 8;
 
 // Below this transmittance a ray counts as fully occluded: the remaining
 // samples can lower the returned visibility by at most this much.
@@ -278,17 +278,41 @@ __device__ __forceinline__ int reference_steps(Trace t) {
   return s;
 }
 
-// A wave owns 64 caches x kVisBlocksPerItem shadow blocks and traces one
+// Shadow blocks per work item: the requested count, or one when there are
+// too few items to occupy the grid (a small spot-check set).
+__device__ __forceinline__ uint32_t items_blocks(uint32_t tiles, uint32_t n_blocks, uint32_t want,
+                                                 uint32_t n_waves) {
+  return tiles * ((n_blocks + want - 1) / want) >= 2u * n_waves ? want : 1u;
+}
+
+__device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint32_t my_traces, uint32_t my_steps,
+                                            uint32_t my_exec) {
+  // One atomic triple per wave after a butterfly sum.
+  uint64_t tr = my_traces, sv = my_steps, ex = my_exec;
+  for (int off = 32; off > 0; off >>= 1) {
+    tr += __shfl_xor(tr, off);
+    sv += __shfl_xor(sv, off);
+    ex += __shfl_xor(ex, off);
+  }
+  if (lane == 0 && (tr | sv)) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&steps[0]), static_cast<unsigned long long>(tr));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&steps[1]), static_cast<unsigned long long>(sv));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&steps[2]), static_cast<unsigned long long>(ex));
+  }
+}
+
+// A wave owns 64 caches x blocks_per_item shadow blocks and traces one
 // block at a time for all its lanes (one uniform block record, coherent
 // rays from neighbouring caches towards the same VPL block).
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
-  const uint32_t groups = (a.n_blocks + kVisBlocksPerItem - 1) / kVisBlocksPerItem;
-  const uint32_t items = tiles * groups;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t bpi = items_blocks(tiles, a.n_blocks, a.blocks_per_item, n_waves);
+  const uint32_t groups = (a.n_blocks + bpi - 1) / bpi;
+  const uint32_t items = tiles * groups;
   const float fres = static_cast<float>(a.vox.res);
   const float vsz = a.voxel_size * fres;
   uint32_t my_traces = 0, my_steps = 0, my_exec = 0;
@@ -300,8 +324,8 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     const f3 wp = make3(e[0], e[1], e[2]);
     // cacheLightingRSM.comp:103-104
     const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
-    const uint32_t b0 = group * kVisBlocksPerItem;
-    const uint32_t b1 = min(a.n_blocks, b0 + kVisBlocksPerItem);
+    const uint32_t b0 = group * bpi;
+    const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
       Trace t = trace_setup(a, wp, vpos, a.blocks[b]);
       bool finished = false;
@@ -312,20 +336,76 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
       if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
     }
   }
-  if (a.steps) {
-    // One atomic triple per wave after a butterfly sum.
-    uint64_t tr = my_traces, sv = my_steps, ex = my_exec;
-    for (int off = 32; off > 0; off >>= 1) {
-      tr += __shfl_xor(tr, off);
-      sv += __shfl_xor(sv, off);
-      ex += __shfl_xor(ex, off);
-    }
-    if (lane == 0 && (tr | sv)) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[0]), static_cast<unsigned long long>(tr));
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[1]), static_cast<unsigned long long>(sv));
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.steps[2]), static_cast<unsigned long long>(ex));
+  if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
+}
+
+// The same traces with lane refilling: a work item's n_valid x n_blocks
+// traces form a pool (block-major, so lanes that refill together mostly
+// share a block), every lane holds one trace, and after each batch of steps
+// the lanes whose trace ended take the next pool entries (ballot + mbcnt
+// ranks), so lanes do not idle behind the longest ray of the wave.
+__global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
+  const uint32_t live = a.counter->live;
+  const uint32_t tiles = (live + 63u) / 64u;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t bpi = items_blocks(tiles, a.n_blocks, a.blocks_per_item, n_waves);
+  const uint32_t groups = (a.n_blocks + bpi - 1) / bpi;
+  const uint32_t items = tiles * groups;
+  const float fres = static_cast<float>(a.vox.res);
+  const float vsz = a.voxel_size * fres;
+  uint32_t my_traces = 0, my_steps = 0, my_exec = 0;
+  for (uint32_t item = wave; item < items; item += n_waves) {
+    const uint32_t tile = item / groups, group = item % groups;
+    const uint32_t n_valid = min(64u, live - tile * 64u);
+    const uint32_t b0 = group * bpi;
+    const uint32_t pool = n_valid * (min(a.n_blocks, b0 + bpi) - b0);
+    uint32_t next = 64u;  // wave-uniform: first pool entry not yet taken
+    uint32_t idx = lane;
+    bool has = idx < pool;
+    uint32_t cache = 0u, blk = 0u;
+    Trace t{};  // an idle lane: finished, takes no samples
+    t.done = true;
+    t.s = 32;
+    auto start = [&](uint32_t i) {
+      const uint32_t bl = i / n_valid;
+      cache = tile * 64u + (i - bl * n_valid);
+      blk = b0 + bl;
+      const float* e = a.caches + static_cast<size_t>(cache) * a.stride;
+      const f3 wp = make3(e[0], e[1], e[2]);
+      // cacheLightingRSM.comp:103-104
+      const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
+      t = trace_setup(a, wp, vpos, a.blocks[blk]);
+    };
+    if (has) start(idx);
+    while (true) {
+      const bool ended = trace_batch(a.vox, t);
+      const bool fin = has && ended;
+      if (fin) {
+        a.vis[static_cast<size_t>(blk) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
+        my_traces += 1u;
+        my_exec += static_cast<uint32_t>(t.s);
+        if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
+        has = false;
+      }
+      if (!__any(has) && next >= pool) break;
+      const uint64_t want = __ballot(!has);
+      // Refill once enough lanes are idle (or all are), so the setup of new
+      // traces is shared by several lanes.
+      const uint32_t idle = static_cast<uint32_t>(__builtin_popcountll(want));
+      if (next < pool && idle < a.refill_min && idle < 64u) continue;
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(want >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(want), 0u));
+      if (!has && next + rank < pool) {
+        idx = next + rank;
+        has = true;
+        start(idx);
+      }
+      next += static_cast<uint32_t>(__builtin_popcountll(want));
     }
   }
+  if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
 }
 
 // ---- SH gather ----------------------------------------------------------
@@ -432,11 +512,23 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
     const uint32_t j = jc / 3u, ch = jc % 3u;
     const float* p = partial + static_cast<size_t>(jc) * partial_pitch + c;
     const float* p0 = partial + static_cast<size_t>(ch) * partial_pitch + c;  // raw sum of rad
-    float v = 0.0f, v0 = 0.0f;
-    for (uint32_t sp = 0; sp < splits; ++sp) {
-      v += p[sp * split_stride];
-      if (j == 6u) v0 += p0[sp * split_stride];
+    // Four interleaved partial sums (fixed order) keep several loads in
+    // flight when a small cache set was split many ways.
+    float v4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t sp = 0;
+    for (; sp + 4u <= splits; sp += 4u) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v4[q] += p[(sp + q) * split_stride];
+        if (j == 6u) w4[q] += p0[(sp + q) * split_stride];
+      }
     }
+    for (int q = 0; sp < splits; ++sp, ++q) {
+      v4[q] += p[sp * split_stride];
+      if (j == 6u) w4[q] += p0[sp * split_stride];
+    }
+    const float v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
+    const float v0 = (w4[0] + w4[1]) + (w4[2] + w4[3]);
     float* e = caches + static_cast<size_t>(c) * stride;
     switch (j) {
       case 0: e[7 + 4 * ch] += shc.eva0 * v; break;                       // SH00_r/g/b at 7, 11, 15
@@ -478,7 +570,8 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
 
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
-                       uint32_t vis_pitch, uint64_t* step_counter, int grid, hipStream_t s) {
+                       uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid,
+                       hipStream_t s) {
   VisArgs a;
   a.vox = vox;
   for (int i = 0; i < 3; ++i) a.vmin[i] = vol.volume_world_min[i];
@@ -491,18 +584,23 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
   a.vis = visibility;
   a.pitch = vis_pitch;
   a.steps = step_counter;
-  hipLaunchKernelGGL(k_visibility, dim3(grid), dim3(256), 0, s, a);
+  a.blocks_per_item = static_cast<uint32_t>(tune.blocks_per_item);
+  a.refill_min = static_cast<uint32_t>(tune.refill_min_lanes);
+  if (tune.visibility_kernel == 1)
+    hipLaunchKernelGGL(k_visibility_pool, dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_visibility, dim3(grid), dim3(256), 0, s, a);
 }
 
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,
                    const Counter* counter, const Vpl* vpls, uint32_t n_vpl, const float* visibility,
                    uint32_t vis_pitch, uint32_t interval, int indirect_shadow, float* partial,
-                   uint32_t partial_pitch, int grid, hipStream_t s) {
+                   uint32_t partial_pitch, uint32_t max_splits, int grid, hipStream_t s) {
   (void)shc;
   // Split planning: enough (tile, split) items for ~16 waves per CU, each
   // split a whole number of shadow blocks.
   const uint32_t quantum = std::max<uint32_t>(interval, 64u);
-  hipLaunchKernelGGL(k_plan_gather, dim3(1), dim3(1), 0, s, const_cast<Counter*>(counter), n_vpl, quantum, 32u,
+  hipLaunchKernelGGL(k_plan_gather, dim3(1), dim3(1), 0, s, const_cast<Counter*>(counter), n_vpl, quantum, max_splits,
                      static_cast<uint32_t>(grid) * 4u);
   const bool sh2 = sh_bands >= 2;
   if (sh2 && indirect_shadow)

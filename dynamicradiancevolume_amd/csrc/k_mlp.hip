@@ -40,8 +40,8 @@ __device__ __forceinline__ float from_bf16(__bf16 v) { return static_cast<float>
 
 // ---- features ---------------------------------------------------------------
 
-// Feature slot k = 8g + e of a (cache, block) sample: e < 5 -> geometry
-// feature G[5g + e], e >= 5 -> voxel sample S[3g + e - 5].
+// The 32 features of a (cache, block) sample in slot order: slot 8g + e holds
+// geometry feature G[5g + e] for e < 5 and voxel sample S[3g + e - 5] else.
 //   G: 0-2 cache position in the voxel volume [0,1]^3; 3-5 unit direction to
 //      the camera (caches carry no normal); 6-8 averaged VAL position in the
 //      volume; 9-11 mean VPL normal of the block; 12-14 unit direction to the
@@ -56,7 +56,7 @@ struct FeatureCtx {
   VoxelView vox;
 };
 
-__device__ __forceinline__ void feature_group(const FeatureCtx& fc, f3 wp, const ShadowBlock& b, int g, float* out) {
+__device__ __forceinline__ void features_all(const FeatureCtx& fc, f3 wp, const ShadowBlock& b, float* out) {
   const float res = static_cast<float>(fc.vox.res);
   const float vsz = fc.voxel_size * res;
   const float vpx = (wp.x - fc.vmin[0]) / vsz, vpy = (wp.y - fc.vmin[1]) / vsz, vpz = (wp.z - fc.vmin[2]) / vsz;
@@ -74,42 +74,18 @@ __device__ __forceinline__ void feature_group(const FeatureCtx& fc, f3 wp, const
   tz *= li;
   const float dist = ld / fc.voxel_size;
   const float goal = dist - 2.0f;
-  float G[5];
-#pragma unroll
-  for (int e = 0; e < 5; ++e) {
-    const int k = 5 * g + e;
-    float v;
-    switch (k) {
-      case 0: v = vpx; break;
-      case 1: v = vpy; break;
-      case 2: v = vpz; break;
-      case 3: v = cx; break;
-      case 4: v = cy; break;
-      case 5: v = cz; break;
-      case 6: v = (b.x - fc.vmin[0]) / vsz; break;
-      case 7: v = (b.y - fc.vmin[1]) / vsz; break;
-      case 8: v = (b.z - fc.vmin[2]) / vsz; break;
-      case 9: v = b.nx; break;
-      case 10: v = b.ny; break;
-      case 11: v = b.nz; break;
-      case 12: v = tx; break;
-      case 13: v = ty; break;
-      case 14: v = tz; break;
-      case 15: v = dist * (1.0f / 64.0f); break;
-      case 16: v = b.rho * 16.0f; break;
-      case 17: v = goal * (1.0f / 64.0f); break;
-      case 18: v = -(b.nx * tx + b.ny * ty + b.nz * tz); break;
-      default: v = cx * tx + cy * ty + cz * tz; break;
-    }
-    G[e] = v;
-  }
+  const float G[20] = {vpx, vpy, vpz, cx, cy, cz,
+                       (b.x - fc.vmin[0]) / vsz, (b.y - fc.vmin[1]) / vsz, (b.z - fc.vmin[2]) / vsz,
+                       b.nx, b.ny, b.nz, tx, ty, tz,
+                       dist * (1.0f / 64.0f), b.rho * 16.0f, goal * (1.0f / 64.0f),
+                       -(b.nx * tx + b.ny * ty + b.nz * tz), cx * tx + cy * ty + cz * tz};
   const int q = fc.vox.levels - 1;
   const float span = fmaxf(goal, 0.0f);
   const float p0x = vpx * res, p0y = vpy * res, p0z = vpz * res;
-  float S[3];
+  // All 12 fetch addresses first, then the loads, then the blends.
+  Footprint fp[12];
 #pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    const int k = 3 * g + e;
+  for (int k = 0; k < 12; ++k) {
     float d;
     int lvl;
     if (k < 8) {
@@ -120,12 +96,21 @@ __device__ __forceinline__ void feature_group(const FeatureCtx& fc, f3 wp, const
       d = 2.0f + span * ((static_cast<float>(k - 8) + 0.5f) * 0.25f);
       lvl = min(3, q);
     }
-    S[e] = sample_level(fc.vox, lvl, p0x + tx * d, p0y + ty * d, p0z + tz * d);
+    fp[k] = footprint(fc.vox, lvl, p0x + tx * d, p0y + ty * d, p0z + tz * d);
   }
+  uint64_t c[12];
 #pragma unroll
-  for (int e = 0; e < 5; ++e) out[e] = G[e];
+  for (int k = 0; k < 12; ++k) c[k] = fc.vox.corners[fp[k].index];
 #pragma unroll
-  for (int e = 0; e < 3; ++e) out[5 + e] = S[e];
+  for (int g = 0; g < 4; ++g) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) out[8 * g + e] = G[5 * g + e];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const int k = 3 * g + e;
+      out[8 * g + 5 + e] = blend8(c[k], fp[k].ax, fp[k].ay, fp[k].az) * (1.0f / 255.0f);
+    }
+  }
 }
 
 // ---- register-resident inference ------------------------------------------
@@ -225,37 +210,63 @@ struct InferArgs {
 };
 
 constexpr uint32_t kMlpBlocksPerItem = 8;
+constexpr int kFeatLd = 40;  // bf16 per staged feature row (32 + pad)
 
 // Surrogate visibility for every (live cache, shadow block): persistent waves
-// over (16-cache tile, 8-block group) items, 16 samples per MFMA chain.
+// over (64-cache tile, 8-block group) items.  Per block, each lane builds the
+// features of its own cache (the block record is wave-uniform) and stages
+// them in LDS as bf16; four 16-sample MFMA chains then evaluate the network,
+// and chain c leaves the result of sample 16c + j in lane (j, g = c), i.e.
+// in the lane that owns that cache, so the store is one coalesced row.
 __global__ __launch_bounds__(256) void k_visibility_mlp(InferArgs a) {
+  __shared__ __align__(16) __bf16 stage[4][64 * kFeatLd];
   const uint32_t live = a.counter->live;
-  const uint32_t tiles = (live + 15u) / 16u;
-  const uint32_t groups = (a.n_blocks + kMlpBlocksPerItem - 1) / kMlpBlocksPerItem;
-  const uint32_t items = tiles * groups;
+  const uint32_t tiles = (live + 63u) / 64u;
   const int lane = static_cast<int>(threadIdx.x & 63u);
   const int j = lane & 15, g = lane >> 4;
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  // Few caches (a spot-check set): one block per item, so more waves work.
+  const uint32_t bpi = tiles * ((a.n_blocks + kMlpBlocksPerItem - 1) / kMlpBlocksPerItem) >= 2u * n_waves
+                           ? kMlpBlocksPerItem
+                           : 1u;
+  const uint32_t groups = (a.n_blocks + bpi - 1) / bpi;
+  const uint32_t items = tiles * groups;
   if (wave >= items) return;
+  __bf16* st = stage[threadIdx.x >> 6];
   InferPack w;
   load_pack(a.pack, lane, w);
   for (uint32_t item = wave; item < items; item += n_waves) {
     const uint32_t tile = item / groups, group = item % groups;
-    const uint32_t cache = tile * 16u + static_cast<uint32_t>(j);
+    const uint32_t cache = tile * 64u + static_cast<uint32_t>(lane);
     const bool valid = cache < live;
     const float* e = a.caches + static_cast<size_t>(valid ? cache : 0u) * a.stride;
     const f3 wp = make3(e[0], e[1], e[2]);
-    const uint32_t b0 = group * kMlpBlocksPerItem;
-    const uint32_t b1 = min(a.n_blocks, b0 + kMlpBlocksPerItem);
+    const uint32_t b0 = group * bpi;
+    const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
-      float f[8];
-      feature_group(a.fc, wp, a.blocks[b], g, f);
-      bf16x8 x;
+      float f[32];
+      features_all(a.fc, wp, a.blocks[b], f);
+      bf16x8 row[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) x[k] = to_bf16(f[k]);
-      const float p = mlp_forward16(w, a.params, x, g);
-      if (g == 0 && valid) a.vis[static_cast<size_t>(b) * a.pitch + cache] = p;
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) row[q][k] = to_bf16(f[8 * q + k]);
+      // the wave's own rows: all lanes store, then read other lanes' rows
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) *reinterpret_cast<bf16x8*>(st + lane * kFeatLd + 8 * q) = row[q];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      float mine = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bf16x8 x = *reinterpret_cast<const bf16x8*>(st + (16 * c + j) * kFeatLd + 8 * g);
+        const float p = mlp_forward16(w, a.params, x, g);
+        mine = (g == c) ? p : mine;
+      }
+      if (valid) a.vis[static_cast<size_t>(b) * a.pitch + cache] = mine;
     }
   }
 }
@@ -287,12 +298,11 @@ __global__ __launch_bounds__(256) void k_mlp_features(FeatArgs a) {
   }
   const float* e = a.caches + static_cast<size_t>(cache) * a.stride;
   const f3 wp = make3(e[0], e[1], e[2]);
-  const ShadowBlock blk = a.blocks[b];
-  for (int g = 0; g < 4; ++g) {
-    float f[8];
-    feature_group(a.fc, wp, blk, g, f);
-    for (int k = 0; k < 8; ++k) a.x[static_cast<size_t>(i) * 32 + 8 * g + k] = f[k];
-  }
+  float f[32];
+  features_all(a.fc, wp, a.blocks[b], f);
+  float4* xo = reinterpret_cast<float4*>(a.x + static_cast<size_t>(i) * 32);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) xo[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
   if (a.y) a.y[i] = a.vis[static_cast<size_t>(b) * a.pitch + cache];
 }
 

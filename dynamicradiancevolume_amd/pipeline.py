@@ -241,6 +241,17 @@ class Renderer:
                     "drv_mlp_finetune")
         return loss.value
 
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(self.lib.drv_set_tuning(self.ctx, C.byref(t)), "drv_set_tuning")
+
+    def tuning(self) -> _abi.DrvTuning:
+        t = _abi.DrvTuning()
+        self._check(self.lib.drv_get_tuning(self.ctx, C.byref(t)), "drv_get_tuning")
+        return t
+
     def set_profiling(self, on: bool):
         self._check(self.lib.drv_set_profiling(self.ctx, int(on)), "drv_set_profiling")
 

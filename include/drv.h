@@ -269,12 +269,12 @@ int drv_mlp_get_params(drv_ctx* ctx, float* host_params, uint32_t count);
 int drv_set_surrogate(drv_ctx* ctx, const drv_surrogate* settings);
 int drv_get_surrogate_stats(drv_ctx* ctx, drv_surrogate_stats* out);
 /* Training samples from the last cone-traced drv_light_caches call: n random
- * (cache, shadow block) pairs -> features x_dev[n][32] and the cone-traced
- * visibility y_dev[n] (device pointers). */
+ * (cache, shadow block) pairs -> features x_dev[n][32] (16-byte aligned) and
+ * the cone-traced visibility y_dev[n] (device pointers). */
 int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float* y_dev);
 /* Features of explicit (cache, block) pairs (pairs_dev[n][2], device) of the
- * last drv_light_caches call; a pair naming no live cache or block gets NaN
- * features. */
+ * last drv_light_caches call into x_dev[n][32] (16-byte aligned); a pair
+ * naming no live cache or block gets NaN features. */
 int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float* x_dev);
 /* Network output for features x_dev[n][32] -> p_dev[n] (device). */
 int drv_mlp_forward(drv_ctx* ctx, const float* x_dev, uint32_t n, float* p_dev);
@@ -293,6 +293,23 @@ int drv_mlp_train(drv_ctx* ctx, const float* x_dev, const float* y_dev, uint32_t
  * loop calls it every few frames (the bench and tests use 16). */
 int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t steps, float lr, uint32_t seed,
                      float* mean_loss_host);
+
+/* ---- launch tuning ---------------------------------------------------- */
+
+/* Launch shapes of the gather stage; the defaults suit one 1080p view, and
+ * the batched path's autotuner picks them per workload.  No counterpart in
+ * the reference (its dispatch is fixed at 64 caches per workgroup,
+ * shader/cacheLightingRSM.comp:7). */
+typedef struct drv_tuning {
+  int32_t visibility_kernel; /* 0: lane per cache, one shadow block at a time (default); 1: refilling lane pool */
+  int32_t blocks_per_item;   /* shadow blocks per visibility work item, 1..64 (default 16) */
+  int32_t gather_max_splits; /* largest VPL split count of the gather, power of two 1..128 (default 128) */
+  int32_t waves_per_cu;      /* persistent grid size, 4..64 waves per CU (default 64) */
+  int32_t refill_min_lanes;  /* lane pool: idle lanes of a wave that trigger a refill, 1..64 (default 16) */
+} drv_tuning;
+
+int drv_set_tuning(drv_ctx* ctx, const drv_tuning* tuning);
+int drv_get_tuning(const drv_ctx* ctx, drv_tuning* out);
 
 /* ---- profiling -------------------------------------------------------- */
 

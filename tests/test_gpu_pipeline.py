@@ -310,3 +310,54 @@ def test_full_frame_matches_oracle(name):
         osh = entries_sh(ro["entries"][(ro["cav"].reshape(-1)[oc] - 1)], cfg.sh_bands)
         assert rel_l2(gsh, osh) < SH_TOL
         assert rel_l2(got, ro["rgb"]) < RGB_TOL
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(visibility_kernel=0, blocks_per_item=8),
+    dict(visibility_kernel=1, blocks_per_item=8),
+    dict(visibility_kernel=1, blocks_per_item=1, waves_per_cu=8),
+    dict(visibility_kernel=1, blocks_per_item=32, gather_max_splits=128, waves_per_cu=64),
+    dict(visibility_kernel=0, blocks_per_item=3, gather_max_splits=1, waves_per_cu=4),
+])
+def test_tuning_variants_agree(tuning):
+    """Every launch shape traces the same rays: visibility is bit-identical to
+    the default's; SH differ only by the split summation order."""
+    from canon import rel_l2
+
+    fi = frame_inputs("boxes_1080p")
+    df = _device(fi)
+    announce(f"tuning variant {tuning}")
+    res = []
+    for t in ({}, tuning):
+        r = _renderer(fi)
+        if t:
+            r.set_tuning(**t)
+        r.render_frame(df)
+        _, live = r.cache_count()
+        e = r.read_caches(live)
+        nb = (fi.read_resolution ** 2) // int(fi.light.indirect_shadow_sample_interval)
+        order = np.lexsort(e[:, :3].view(np.uint32).T)
+        vis = r.read_visibility(live, nb)
+        res.append((e[order], vis[order]))
+        r.close()
+    (e0, v0), (e1, v1) = res
+    np.testing.assert_array_equal(e0[:, :3], e1[:, :3])
+    np.testing.assert_array_equal(v0, v1)
+    assert rel_l2(e1[:, 4:], e0[:, 4:]) <= 1e-5
+
+
+def test_stage_times_after_late_profiling():
+    """Turning profiling on after unprofiled frames: querying stages that were
+    never timed reports 0 and leaves no error for the next call."""
+    fi = frame_inputs("cornell_256")
+    df = _device(fi)
+    announce("stage times after late profiling")
+    r = _renderer(fi)
+    r.render_frame(df)
+    r.set_profiling(True)
+    r.light_caches_rsm(df.rsm(), fi.light)
+    t = r.stage_times()
+    assert t["voxelize_ms"] == 0.0 and t["light_caches_ms"] > 0.0
+    r.light_caches_rsm(df.rsm(), fi.light)
+    r.apply_caches(df.gbuffer(), df.out)
+    assert r.stage_times()["apply_ms"] > 0.0
