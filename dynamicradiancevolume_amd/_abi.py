@@ -192,6 +192,16 @@ DRV_FUNCTIONS = {
     "drv_mlp_finetune": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, vp]),
     "drv_set_tuning": (C.c_int, [vp, P(DrvTuning)]),
     "drv_get_tuning": (C.c_int, [vp, P(DrvTuning)]),
+    "drv_batch_create": (C.c_int, [vp, C.c_int32, P(vp)]),
+    "drv_batch_destroy": (C.c_int, [vp]),
+    "drv_batch_last_error": (C.c_char_p, [vp]),
+    "drv_batch_set_tuning": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+    "drv_batch_get_tuning": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+    "drv_batch_light_caches": (C.c_int, [vp, vp, vp]),
+    "drv_batch_autotune": (C.c_int, [vp, vp, vp, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_float)]),
+    "drv_graph_begin": (C.c_int, [vp]),
+    "drv_graph_end": (C.c_int, [vp]),
+    "drv_graph_launch": (C.c_int, [vp]),
     "drv_set_profiling": (C.c_int, [vp, C.c_int32]),
     "drv_get_stage_times": (C.c_int, [vp, P(DrvStageTimes)]),
 }

@@ -94,6 +94,7 @@ struct drv_ctx {
   uint64_t* step_counter = nullptr;  // [traces, reference steps, executed steps] when counting
   hipEvent_t ev[16] = {};
   uint32_t recorded = 0;  // events recorded at least once
+  hipGraphExec_t graph_exec = nullptr;
   drv_stage_times times{};
 };
 
@@ -376,6 +377,7 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->spot_stats);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   delete ctx;
   return DRV_OK;
 }
@@ -550,12 +552,18 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
 
 // ---- stage C ----------------------------------------------------------
 
-int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light) {
+}  // extern "C"
+
+namespace {
+
+// Validation of one light's RSM (cacheLightingRSM.comp:80-163 needs a square
+// power-of-two read resolution; the 64-VPL chunking needs R >= 8) and the
+// VPL / shadow block buffers it needs.
+int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light, uint32_t* n_vpl_out,
+                  uint32_t* interval_out, uint32_t* n_blocks_out) {
   if (!ctx || !rsm || !light || !rsm->flux || !rsm->normal || !rsm->depth_lin_sq) return DRV_ERR_INVALID_ARG;
   if (!ctx->allocated) return fail(ctx, DRV_ERR_STATE, "light_caches: no allocation yet");
   const int R = rsm->read_resolution;
-  // The Morton decode of cacheLightingRSM.comp:142 needs a square
-  // power-of-two read resolution; the 64-VPL chunking needs R >= 8.
   if (!is_pow2(R) || R < 8 || R > 65536 || R != light->rsm_read_resolution)
     return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: read resolution must be a power of two >= 8 matching the light");
   const uint32_t n_vpl = static_cast<uint32_t>(R) * static_cast<uint32_t>(R);
@@ -579,6 +587,21 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     if ((rc = dev_alloc(ctx, &ctx->blocks, n_blocks)) != DRV_OK) return rc;
     ctx->block_cap = n_blocks;
   }
+  *n_vpl_out = n_vpl;
+  *interval_out = shadow ? interval : 64u;
+  *n_blocks_out = n_blocks;
+  return DRV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light) {
+  uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
+  int rc = prepare_light(ctx, rsm, light, &n_vpl, &interval, &n_blocks);
+  if (rc != DRV_OK) return rc;
+  const bool shadow = ctx->cfg.indirect_shadow != 0;
   const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
   if (shadow && n_blocks > ctx->vis_blocks_cap) {
     dev_free(ctx->visibility);
@@ -618,7 +641,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   }
   record(ctx, 7);
   drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
-                      ctx->visibility, static_cast<uint32_t>(cap), shadow ? interval : 64u, shadow ? 1 : 0,
+                      ctx->visibility, static_cast<uint32_t>(cap), interval, shadow ? 1 : 0,
                       ctx->partial, static_cast<uint32_t>(cap), static_cast<uint32_t>(ctx->tune.gather_max_splits),
                       grid, ctx->stream);
   record(ctx, 8);
@@ -895,6 +918,278 @@ int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t 
   if ((rc = ensure(ctx, &ctx->ft_y, &ctx->ft_y_cap, static_cast<size_t>(n_samples))) != DRV_OK) return rc;
   if ((rc = drv_mlp_collect(ctx, n_samples, seed, ctx->ft_x, ctx->ft_y)) != DRV_OK) return rc;
   return drv_mlp_train(ctx, ctx->ft_x, ctx->ft_y, n_samples, batch, steps, lr, seed ^ 0x5bd1e995u, mean_loss_host);
+}
+
+// ---- batched views ----------------------------------------------------------
+
+}  // extern "C"
+
+struct drv_batch {
+  std::vector<drv_ctx*> views;
+  hipStream_t stream = nullptr;
+  int stride = 16;
+  int sh_bands = 1;
+  int shadow = 0;
+  int cu_count = 256;
+  uint32_t cpw = 64;
+  uint32_t vpl_tile = 4096;
+  int waves_per_cu = 64;
+  drvk::BatchView* d_views = nullptr;
+  std::vector<drvk::BatchView> h_views;  // as last uploaded
+  drvk::BatchQueue* d_queue = nullptr;
+  float* partial = nullptr;
+  size_t partial_size = 0;  // floats
+  std::string err;
+};
+
+namespace {
+
+constexpr size_t kBatchPartialFloats = size_t(1) << 28;  // 1 GiB of split partials at most
+
+int bfail(drv_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+bool is_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+}  // namespace
+
+extern "C" {
+
+int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out) {
+  if (!views || n_views <= 0 || n_views > 65535 || !out) return DRV_ERR_INVALID_ARG;
+  *out = nullptr;
+  for (int i = 0; i < n_views; ++i) {
+    if (!views[i]) return DRV_ERR_INVALID_ARG;
+    if (views[i]->cfg.sh_bands != views[0]->cfg.sh_bands ||
+        (views[i]->cfg.indirect_shadow != 0) != (views[0]->cfg.indirect_shadow != 0) ||
+        views[i]->stream != views[0]->stream)
+      return DRV_ERR_INVALID_ARG;
+  }
+  drv_batch* b = new (std::nothrow) drv_batch();
+  if (!b) return DRV_ERR_NO_MEMORY;
+  b->views.assign(views, views + n_views);
+  b->stream = views[0]->stream;
+  b->sh_bands = views[0]->cfg.sh_bands;
+  b->stride = drvk::entry_stride(b->sh_bands);
+  b->shadow = views[0]->cfg.indirect_shadow != 0 ? 1 : 0;
+  b->cu_count = views[0]->cu_count;
+  b->h_views.resize(n_views);
+  std::memset(b->h_views.data(), 0, sizeof(drvk::BatchView) * n_views);
+  if (hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(drvk::BatchView) * n_views) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&b->d_queue), sizeof(drvk::BatchQueue)) != hipSuccess) {
+    drv_batch_destroy(b);
+    return DRV_ERR_NO_MEMORY;
+  }
+  *out = b;
+  return DRV_OK;
+}
+
+int drv_batch_destroy(drv_batch* b) {
+  if (!b) return DRV_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  else (void)hipDeviceSynchronize();
+  dev_free(b->d_views);
+  dev_free(b->d_queue);
+  dev_free(b->partial);
+  delete b;
+  return DRV_OK;
+}
+
+const char* drv_batch_last_error(const drv_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t waves_per_cu) {
+  if (!b) return DRV_ERR_INVALID_ARG;
+  if ((cpw != 64 && cpw != 32 && cpw != 16) || !is_pow2(vpl_tile) || vpl_tile < 16 || waves_per_cu < 4 ||
+      waves_per_cu > 64)
+    return bfail(b, DRV_ERR_INVALID_ARG, "batch_set_tuning: caches per wave 64/32/16, VPL tile a power of two >= 16");
+  b->cpw = static_cast<uint32_t>(cpw);
+  b->vpl_tile = static_cast<uint32_t>(vpl_tile);
+  b->waves_per_cu = waves_per_cu;
+  return DRV_OK;
+}
+
+int drv_batch_get_tuning(const drv_batch* b, int32_t* cpw, int32_t* vpl_tile, int32_t* waves_per_cu) {
+  if (!b) return DRV_ERR_INVALID_ARG;
+  if (cpw) *cpw = static_cast<int32_t>(b->cpw);
+  if (vpl_tile) *vpl_tile = static_cast<int32_t>(b->vpl_tile);
+  if (waves_per_cu) *waves_per_cu = b->waves_per_cu;
+  return DRV_OK;
+}
+
+int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights) {
+  if (!b || !rsms || !lights) return DRV_ERR_INVALID_ARG;
+  const size_t n = b->views.size();
+  const bool capturing = is_capturing(b->stream);
+  size_t partial_need = 1;
+  std::vector<drvk::BatchView> hv(n);
+  std::memset(hv.data(), 0, sizeof(drvk::BatchView) * n);
+  for (size_t v = 0; v < n; ++v) {
+    drv_ctx* ctx = b->views[v];
+    uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
+    const size_t vpl_had = ctx->vpl_cap, blk_had = ctx->block_cap;
+    int rc = prepare_light(ctx, &rsms[v], &lights[v], &n_vpl, &interval, &n_blocks);
+    if (rc != DRV_OK) return bfail(b, rc, "batch view " + std::to_string(v) + ": " + ctx->err);
+    if (capturing && (ctx->vpl_cap != vpl_had || ctx->block_cap != blk_had))
+      return bfail(b, DRV_ERR_STATE, "batch_light_caches: buffers grew during graph capture");
+    if (interval > b->vpl_tile)
+      return bfail(b, DRV_ERR_INVALID_ARG, "batch_light_caches: VPL tile shorter than a shadow block");
+    drvk::BatchView& d = hv[v];
+    d.caches = ctx->caches;
+    d.counter = ctx->counter;
+    d.vpls = ctx->vpls;
+    d.blocks = b->shadow ? ctx->blocks : nullptr;
+    d.vox = voxel_view(ctx);
+    for (int i = 0; i < 3; ++i) d.vmin[i] = ctx->vol.volume_world_min[i];
+    d.voxel_size = ctx->vol.voxel_size_in_world;
+    d.n_vpl = n_vpl;
+    d.interval = interval;
+    const size_t splits = n_vpl / std::min<uint32_t>(b->vpl_tile, n_vpl);
+    const size_t tiles = (static_cast<size_t>(ctx->cfg.max_caches) + b->cpw - 1) / b->cpw;
+    if (splits > 1) partial_need += tiles * splits * 27 * b->cpw;
+  }
+  // Split partials are bounded (the plan kernel lengthens the VPL tile
+  // when the live caches would need more).
+  partial_need = std::min(partial_need, kBatchPartialFloats);
+  if (partial_need > b->partial_size) {
+    if (capturing) return bfail(b, DRV_ERR_STATE, "batch_light_caches: buffers grew during graph capture");
+    dev_free(b->partial);
+    b->partial_size = 0;
+    if (hipMalloc(reinterpret_cast<void**>(&b->partial), partial_need * sizeof(float)) != hipSuccess)
+      return bfail(b, DRV_ERR_NO_MEMORY, "batch_light_caches: partial sums do not fit");
+    b->partial_size = partial_need;
+  }
+  if (std::memcmp(hv.data(), b->h_views.data(), sizeof(drvk::BatchView) * n) != 0) {
+    if (capturing) return bfail(b, DRV_ERR_STATE, "batch_light_caches: view descriptors changed during graph capture");
+    if (hipMemcpy(b->d_views, hv.data(), sizeof(drvk::BatchView) * n, hipMemcpyHostToDevice) != hipSuccess)
+      return bfail(b, DRV_ERR_HIP, "batch_light_caches: descriptor upload failed");
+    b->h_views = hv;
+  }
+  for (size_t v = 0; v < n; ++v) {
+    drv_ctx* ctx = b->views[v];
+    drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, b->stream);
+    // The batch leaves no per-pair visibility behind.
+    ctx->last_n_blocks = 0;
+    ctx->label_caches = nullptr;
+    ctx->counted_steps = false;
+  }
+  const int grid = std::max(1, b->cu_count * b->waves_per_cu / 4);
+  drvk::launch_batch(b->d_views, static_cast<uint32_t>(n), b->d_queue, b->stride, b->sh_bands, b->shadow, b->cpw,
+                     b->vpl_tile, b->partial, b->partial_size, b->views[0]->shc, grid, b->stream);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, DRV_ERR_HIP, std::string("batch_light_caches: ") + hipGetErrorString(e));
+  return DRV_OK;
+}
+
+int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights, int32_t reps,
+                       int32_t* cpw_out, int32_t* tile_out, float* ms_out) {
+  if (!b || !rsms || !lights || reps < 1) return DRV_ERR_INVALID_ARG;
+  if (is_capturing(b->stream)) return bfail(b, DRV_ERR_STATE, "batch_autotune: not while capturing");
+  const size_t n = b->views.size();
+  // Snapshot of every view's entries (restored at the end).
+  std::vector<float*> saved(n, nullptr);
+  std::vector<size_t> bytes(n, 0);
+  int rc = DRV_OK;
+  auto cleanup = [&]() {
+    for (float* p : saved)
+      if (p) (void)hipFree(p);
+  };
+  for (size_t v = 0; v < n; ++v) {
+    bytes[v] = sizeof(float) * b->stride * static_cast<size_t>(b->views[v]->cfg.max_caches);
+    if (hipMalloc(reinterpret_cast<void**>(&saved[v]), bytes[v]) != hipSuccess) {
+      cleanup();
+      return bfail(b, DRV_ERR_NO_MEMORY, "batch_autotune: snapshot does not fit");
+    }
+    (void)hipMemcpyAsync(saved[v], b->views[v]->caches, bytes[v], hipMemcpyDeviceToDevice, b->stream);
+  }
+  uint32_t max_vpl = 16;
+  for (size_t v = 0; v < n; ++v) {
+    const uint32_t r = static_cast<uint32_t>(rsms[v].read_resolution);
+    max_vpl = std::max(max_vpl, r * r);
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const uint32_t keep_cpw = b->cpw, keep_tile = b->vpl_tile;
+  float best = 1e30f;
+  uint32_t best_cpw = keep_cpw, best_tile = keep_tile;
+  const uint32_t cpws[3] = {64, 32, 16};
+  for (uint32_t cpw : cpws) {
+    for (uint32_t tile = 256; tile <= max_vpl; tile *= 2) {
+      b->cpw = cpw;
+      b->vpl_tile = tile;
+      std::vector<float> t;
+      for (int r = 0; r <= reps && rc == DRV_OK; ++r) {  // first run warms up (buffer growth)
+        (void)hipEventRecord(e0, b->stream);
+        rc = drv_batch_light_caches(b, rsms, lights);
+        (void)hipEventRecord(e1, b->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) t.push_back(ms);
+      }
+      if (rc != DRV_OK) break;
+      std::sort(t.begin(), t.end());
+      const float med = t[t.size() / 2];
+      if (med < best) {
+        best = med;
+        best_cpw = cpw;
+        best_tile = tile;
+      }
+    }
+    if (rc != DRV_OK) break;
+  }
+  b->cpw = rc == DRV_OK ? best_cpw : keep_cpw;
+  b->vpl_tile = rc == DRV_OK ? best_tile : keep_tile;
+  for (size_t v = 0; v < n; ++v)
+    (void)hipMemcpyAsync(b->views[v]->caches, saved[v], bytes[v], hipMemcpyDeviceToDevice, b->stream);
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cleanup();
+  if (rc != DRV_OK) return rc;
+  if (cpw_out) *cpw_out = static_cast<int32_t>(b->cpw);
+  if (tile_out) *tile_out = static_cast<int32_t>(b->vpl_tile);
+  if (ms_out) *ms_out = best;
+  return DRV_OK;
+}
+
+// ---- HIP graph -------------------------------------------------------------
+
+int drv_graph_begin(drv_ctx* ctx) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  if (!ctx->stream) return fail(ctx, DRV_ERR_STATE, "graph_begin: the null stream cannot be captured");
+  if (ctx->profiling) return fail(ctx, DRV_ERR_STATE, "graph_begin: turn profiling off before capturing");
+  DRV_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  return DRV_OK;
+}
+
+int drv_graph_end(drv_ctx* ctx) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  hipGraph_t g = nullptr;
+  DRV_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
+  if (ctx->graph_exec) {
+    (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+  }
+  const hipError_t e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    ctx->graph_exec = nullptr;
+    return check_hip(ctx, e, "hipGraphInstantiate");
+  }
+  return DRV_OK;
+}
+
+int drv_graph_launch(drv_ctx* ctx) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  if (!ctx->graph_exec) return fail(ctx, DRV_ERR_STATE, "graph_launch: nothing captured");
+  DRV_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return DRV_OK;
 }
 
 // ---- tuning ---------------------------------------------------------------

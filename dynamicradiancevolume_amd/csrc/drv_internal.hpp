@@ -88,6 +88,28 @@ struct FeatureArgsHost {
   VoxelView vox;
 };
 
+// One view of a batched light pass (device descriptor).  The plan kernel
+// fills tiles, splits and item_begin from the live cache count.
+struct BatchView {
+  float* caches;
+  const Counter* counter;
+  const Vpl* vpls;
+  const ShadowBlock* blocks;
+  VoxelView vox;
+  float vmin[3];
+  float voxel_size;
+  uint32_t n_vpl;
+  uint32_t interval;  // VPLs per shadow block
+  uint32_t tiles, splits, item_begin, partial_begin;
+};
+
+struct BatchQueue {
+  uint32_t head;      // next item to hand out
+  uint32_t total;     // items of all views
+  uint32_t vpl_tile;  // VPL tile length in effect (grown until the partials fit)
+  uint32_t pad;
+};
+
 // Launch shapes chosen per context (drv_set_tuning).
 // Defaults measured on boxes_1080p (6352 caches x 16384 VPLs).
 struct Tuning {
@@ -142,6 +164,10 @@ void launch_mlp_train(const float* x, const float* y, const uint32_t* idx, uint3
                       const void* train_w, float* partial, uint32_t n_waves, hipStream_t s);
 void launch_mlp_adam(const float* partial, uint32_t n_waves, float* params, float* m, float* v, float* grad_out,
                      float lr, float beta1, float beta2, float eps, float bc1, float bc2, hipStream_t s);
+
+void launch_batch(BatchView* views, uint32_t n_views, BatchQueue* q, int stride, int sh_bands, int shadow,
+                  uint32_t cpw, uint32_t vpl_tile, float* partial, size_t partial_floats,
+                  const drv_sh_constants& shc, int grid, hipStream_t s);
 
 void launch_spot_select(const Counter* counter, uint32_t k, const float* caches, int stride, float* spot,
                         uint32_t spot_cap, Counter* spot_ctr, hipStream_t s);

@@ -31,6 +31,8 @@
 #include "drv_internal.hpp"
 #include "k_common.hpp"
 #include "k_sample.hpp"
+#include "k_sh.hpp"
+#include "k_trace.hpp"
 
 namespace drvk {
 namespace {
@@ -155,129 +157,6 @@ struct VisArgs {
   uint32_t refill_min;  // lane pool: idle lanes that trigger a refill
 };
 
-// Below this transmittance a ray counts as fully occluded: the remaining
-// samples can lower the returned visibility by at most this much.
-constexpr float kOpaque = 1.0f / 1048576.0f;
-constexpr int kStepBatch = 4;
-
-// Per-lane state of one cone trace (cacheLightingRSM.comp:195-210).
-struct Trace {
-  float px, py, pz;  // current sample position, level-0 texel units
-  float dx, dy, dz;  // unit direction, texel units per step unit
-  float dist, step, goal, rho, k;
-  float occ;
-  int s;             // samples taken
-  bool done;         // dist reached goal (the reference's break)
-};
-
-__device__ __forceinline__ Trace trace_setup(const VisArgs& a, f3 wp, f3 vpos, const ShadowBlock& b) {
-  const float fres = static_cast<float>(a.vox.res);
-  float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
-  const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
-  tx = tx / ld;
-  ty = ty / ld;
-  tz = tz / ld;
-  Trace t;
-  // dirInVoxel = toAverageVal / volumeSize; positions are kept scaled by res.
-  t.dx = tx;
-  t.dy = ty;
-  t.dz = tz;
-  t.px = vpos.x * fres + tx * 2.0f;
-  t.py = vpos.y * fres + ty * 2.0f;
-  t.pz = vpos.z * fres + tz * 2.0f;
-  t.dist = 0.0f;
-  t.step = 1.0f;
-  t.goal = ld / a.voxel_size - 2.0f;
-  t.rho = b.rho;
-  t.k = 2.0f / (1.0f - b.rho);
-  t.occ = 0.0f;
-  t.s = 0;
-  t.done = false;
-  return t;
-}
-
-// Advances one trace by up to kStepBatch samples (cacheLightingRSM.comp:211-228).
-// The sample positions depend only on the geometry, so all samples of the
-// batch are planned first and their loads issued together; occlusion is then
-// accumulated in step order.  Returns true when the trace has finished.
-__device__ __forceinline__ bool trace_batch(const VoxelView& vx, Trace& t) {
-  const int q = vx.levels - 1;
-  const float qf = static_cast<float>(q);
-  float sx[kStepBatch], sy[kStepBatch], sz[kStepBatch], frac[kStepBatch];
-  int lv[kStepBatch];
-  Footprint f1[kStepBatch];
-  bool take[kStepBatch];
-  bool need2 = false;
-#pragma unroll
-  for (int i = 0; i < kStepBatch; ++i) {
-    take[i] = !t.done && (t.s + i) < 32;
-    t.px = __builtin_fmaf(t.dx, t.step, t.px);
-    t.py = __builtin_fmaf(t.dy, t.step, t.py);
-    t.pz = __builtin_fmaf(t.dz, t.step, t.pz);
-    t.dist = t.dist + t.step;
-    const float r = t.dist * t.rho;
-    // LINEAR_MIPMAP_LINEAR level pair for lod = log2(r), clamped to [0, q]
-    // (GL 4.5 section 8.14.3); frac = 0 means a single level.
-    const float lod = __builtin_amdgcn_fmed3f(__builtin_amdgcn_logf(r), 0.0f, qf);
-    const float fl = floorf(lod);
-    lv[i] = static_cast<int>(fl);
-    frac[i] = take[i] ? lod - fl : 0.0f;
-    need2 = need2 || frac[i] > 0.0f;
-    sx[i] = t.px;
-    sy[i] = t.py;
-    sz[i] = t.pz;
-    f1[i] = footprint(vx, lv[i], t.px, t.py, t.pz);
-    if (take[i]) {
-      if (t.dist >= t.goal) t.done = true;
-      else t.step = fmaxf(1.0f, r * t.k);
-    }
-  }
-  uint64_t c1[kStepBatch], c2[kStepBatch];
-#pragma unroll
-  for (int i = 0; i < kStepBatch; ++i) c1[i] = take[i] ? vx.corners[f1[i].index] : 0ull;
-  float o[kStepBatch];
-  // The second level is only planned when some lane of the wave blends two.
-  if (__any(need2)) {
-    Footprint f2[kStepBatch];
-#pragma unroll
-    for (int i = 0; i < kStepBatch; ++i) {
-      f2[i] = footprint(vx, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
-      c2[i] = frac[i] > 0.0f ? vx.corners[f2[i].index] : 0ull;
-    }
-#pragma unroll
-    for (int i = 0; i < kStepBatch; ++i) {
-      const float o1 = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
-      const float o2 = blend8(c2[i], f2[i].ax, f2[i].ay, f2[i].az);
-      o[i] = __builtin_fmaf(frac[i], o2 - o1, o1);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < kStepBatch; ++i) o[i] = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
-  }
-#pragma unroll
-  for (int i = 0; i < kStepBatch; ++i) {
-    if (take[i]) {
-      const float v = o[i] * (1.0f / 255.0f);
-      t.occ = t.occ + (1.0f - t.occ) * v;
-      ++t.s;
-    }
-  }
-  return t.done || t.s >= 32 || (1.0f - t.occ) <= kOpaque;
-}
-
-// Steps the reference still takes after a trace stopped early (geometry only).
-__device__ __forceinline__ int reference_steps(Trace t) {
-  int s = t.s;
-  if (t.done) return s;
-  while (s < 32) {
-    t.dist = t.dist + t.step;
-    ++s;
-    if (t.dist >= t.goal) break;
-    t.step = fmaxf(1.0f, t.dist * t.rho * t.k);
-  }
-  return s;
-}
-
 // Shadow blocks per work item: the requested count, or one when there are
 // too few items to occupy the grid (a small spot-check set).
 __device__ __forceinline__ uint32_t items_blocks(uint32_t tiles, uint32_t n_blocks, uint32_t want,
@@ -327,7 +206,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     const uint32_t b0 = group * bpi;
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
-      Trace t = trace_setup(a, wp, vpos, a.blocks[b]);
+      Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[b]);
       bool finished = false;
       while (!finished) finished = trace_batch(a.vox, t);
       a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
@@ -376,7 +255,7 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
       const f3 wp = make3(e[0], e[1], e[2]);
       // cacheLightingRSM.comp:103-104
       const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
-      t = trace_setup(a, wp, vpos, a.blocks[blk]);
+      t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[blk]);
     };
     if (has) start(idx);
     while (true) {
@@ -442,55 +321,22 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
     const float* e = caches + static_cast<size_t>(c) * stride;
     const float wx = e[0], wy = e[1], wz = e[2];
     // Raw sums; the SH evaluation factors are applied at write-out.
-    float s0r = 0, s0g = 0, s0b = 0;     // sum rad              (SH00 / c0)
-    float s1r = 0, s1g = 0, s1b = 0;     // sum t.y rad          (-SH1neg1 / c1)
-    float s2r = 0, s2g = 0, s2b = 0;     // sum t.z rad          (SH10 / c1)
-    float s3r = 0, s3g = 0, s3b = 0;     // sum t.x rad          (-SH1pos1 / c1)
-    float s4r = 0, s4g = 0, s4b = 0;     // sum t.x t.y rad      (-SH2neg2 / c2)
-    float s5r = 0, s5g = 0, s5b = 0;     // sum t.y t.z rad      (SH2neg1 / c2)
-    float s6r = 0, s6g = 0, s6b = 0;     // sum t.z^2 rad        (SH20 = c20 (3 s6 - s0))
-    float s7r = 0, s7g = 0, s7b = 0;     // sum t.x t.z rad      (SH2pos1 / c2)
-    float s8r = 0, s8g = 0, s8b = 0;     // sum (t.x^2 - t.y^2) rad (SH2pos2 / c22)
+    ShSums acc;
+    sh_zero(acc);
     const uint32_t m0 = split * chunk, m1 = m0 + chunk;
     for (uint32_t mb = m0; mb < m1; mb += interval) {
       float shadowing = 1.0f;
       if (kShadow) shadowing = vis[static_cast<size_t>(mb / interval) * vis_pitch + c];
 #pragma unroll 4
-      for (uint32_t m = mb; m < mb + interval; ++m) {
-        const Vpl v = vpls[m];
-        const float tx0 = v.px - wx, ty0 = v.py - wy, tz0 = v.pz - wz;
-        const float d2 = __builtin_fmaf(tz0, tz0, __builtin_fmaf(ty0, ty0, tx0 * tx0));
-        const float inv = __builtin_amdgcn_rsqf(d2);
-        const float tx = tx0 * inv, ty = ty0 * inv, tz = tz0 * inv;
-        const float nd = -__builtin_fmaf(v.nz, tz, __builtin_fmaf(v.ny, ty, v.nx * tx));
-        float k = fminf(1.0f, fmaxf(0.0f, nd));
-        k = k * shadowing;
-        const float f = k * __builtin_amdgcn_rcpf(d2 + v.disc_area);
-        const float rr = v.fr * f, rg = v.fg * f, rb = v.fb * f;
-        s0r += rr; s0g += rg; s0b += rb;
-        s1r = __builtin_fmaf(ty, rr, s1r); s1g = __builtin_fmaf(ty, rg, s1g); s1b = __builtin_fmaf(ty, rb, s1b);
-        s2r = __builtin_fmaf(tz, rr, s2r); s2g = __builtin_fmaf(tz, rg, s2g); s2b = __builtin_fmaf(tz, rb, s2b);
-        s3r = __builtin_fmaf(tx, rr, s3r); s3g = __builtin_fmaf(tx, rg, s3g); s3b = __builtin_fmaf(tx, rb, s3b);
-        if (kSh2) {
-          const float bxy = tx * ty, byz = ty * tz, bzz = tz * tz, bxz = tx * tz;
-          const float bdd = __builtin_fmaf(tx, tx, -(ty * ty));
-          s4r = __builtin_fmaf(bxy, rr, s4r); s4g = __builtin_fmaf(bxy, rg, s4g); s4b = __builtin_fmaf(bxy, rb, s4b);
-          s5r = __builtin_fmaf(byz, rr, s5r); s5g = __builtin_fmaf(byz, rg, s5g); s5b = __builtin_fmaf(byz, rb, s5b);
-          s6r = __builtin_fmaf(bzz, rr, s6r); s6g = __builtin_fmaf(bzz, rg, s6g); s6b = __builtin_fmaf(bzz, rb, s6b);
-          s7r = __builtin_fmaf(bxz, rr, s7r); s7g = __builtin_fmaf(bxz, rg, s7g); s7b = __builtin_fmaf(bxz, rb, s7b);
-          s8r = __builtin_fmaf(bdd, rr, s8r); s8g = __builtin_fmaf(bdd, rg, s8g); s8b = __builtin_fmaf(bdd, rb, s8b);
-        }
-      }
+      for (uint32_t m = mb; m < mb + interval; ++m) sh_accumulate<kSh2>(acc, vpls[m], wx, wy, wz, shadowing);
     }
     if (!valid) continue;
     // Raw sums, coefficient-major: partial[(split * 27 + k) * pitch + cache].
     float* p = partial + static_cast<size_t>(split) * 27u * partial_pitch + cache;
-    const float out[27] = {s0r, s0g, s0b, s1r, s1g, s1b, s2r, s2g, s2b, s3r, s3g, s3b, s4r, s4g,
-                           s4b, s5r, s5g, s5b, s6r, s6g, s6b, s7r, s7g, s7b, s8r, s8g, s8b};
     const int n_out = kSh2 ? 27 : 12;
 #pragma unroll
     for (int k = 0; k < 27; ++k)
-      if (k < n_out) p[static_cast<size_t>(k) * partial_pitch] = out[k];
+      if (k < n_out) p[static_cast<size_t>(k) * partial_pitch] = acc.s[k];
   }
 }
 
@@ -529,18 +375,7 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
     }
     const float v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
     const float v0 = (w4[0] + w4[1]) + (w4[2] + w4[3]);
-    float* e = caches + static_cast<size_t>(c) * stride;
-    switch (j) {
-      case 0: e[7 + 4 * ch] += shc.eva0 * v; break;                       // SH00_r/g/b at 7, 11, 15
-      case 1: e[4 + ch] += -(shc.eva1 * v); break;                        // SH1neg1
-      case 2: e[8 + ch] += shc.eva1 * v; break;                           // SH10
-      case 3: e[12 + ch] += -(shc.eva1 * v); break;                       // SH1pos1
-      case 4: e[16 + ch] += -(shc.eva2n2_p1_n1 * v); break;               // SH2neg2
-      case 5: e[20 + ch] += shc.eva2n2_p1_n1 * v; break;                  // SH2neg1
-      case 6: e[19 + 4 * ch] += shc.eva20 * (3.0f * v - v0); break;       // SH20_r/g/b at 19, 23, 27
-      case 7: e[24 + ch] += shc.eva2n2_p1_n1 * v; break;                  // SH2pos1
-      default: e[28 + ch] += shc.eva2p2 * v; break;                       // SH2pos2
-    }
+    sh_add_coefficient(caches + static_cast<size_t>(c) * stride, j, ch, v, v0, shc);
   }
 }
 

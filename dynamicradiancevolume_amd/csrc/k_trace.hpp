@@ -1,0 +1,136 @@
+// Cone-traced indirect shadow of one (cache, shadow block) pair
+// (shader/cacheLightingRSM.comp:195-231), shared by the visibility kernels
+// and the batched gather.
+#pragma once
+
+#include "drv_internal.hpp"
+#include "k_common.hpp"
+#include "k_sample.hpp"
+
+namespace drvk {
+
+// Below this transmittance a ray counts as fully occluded: the remaining
+// samples can lower the returned visibility by at most this much.
+constexpr float kOpaque = 1.0f / 1048576.0f;
+constexpr int kStepBatch = 4;
+
+// Per-lane state of one cone trace (cacheLightingRSM.comp:195-210).
+struct Trace {
+  float px, py, pz;  // current sample position, level-0 texel units
+  float dx, dy, dz;  // unit direction, texel units per step unit
+  float dist, step, goal, rho, k;
+  float occ;
+  int s;             // samples taken
+  bool done;         // dist reached goal (the reference's break)
+};
+
+__device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_size, f3 wp, f3 vpos,
+                                             const ShadowBlock& b) {
+  const float fres = static_cast<float>(vox.res);
+  float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
+  const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
+  tx = tx / ld;
+  ty = ty / ld;
+  tz = tz / ld;
+  Trace t;
+  // dirInVoxel = toAverageVal / volumeSize; positions are kept scaled by res.
+  t.dx = tx;
+  t.dy = ty;
+  t.dz = tz;
+  t.px = vpos.x * fres + tx * 2.0f;
+  t.py = vpos.y * fres + ty * 2.0f;
+  t.pz = vpos.z * fres + tz * 2.0f;
+  t.dist = 0.0f;
+  t.step = 1.0f;
+  t.goal = ld / voxel_size - 2.0f;
+  t.rho = b.rho;
+  t.k = 2.0f / (1.0f - b.rho);
+  t.occ = 0.0f;
+  t.s = 0;
+  t.done = false;
+  return t;
+}
+
+// Advances one trace by up to kStepBatch samples (cacheLightingRSM.comp:211-228).
+// The sample positions depend only on the geometry, so all samples of the
+// batch are planned first and their loads issued together; occlusion is then
+// accumulated in step order.  Returns true when the trace has finished.
+__device__ __forceinline__ bool trace_batch(const VoxelView& vx, Trace& t) {
+  const int q = vx.levels - 1;
+  const float qf = static_cast<float>(q);
+  float sx[kStepBatch], sy[kStepBatch], sz[kStepBatch], frac[kStepBatch];
+  int lv[kStepBatch];
+  Footprint f1[kStepBatch];
+  bool take[kStepBatch];
+  bool need2 = false;
+#pragma unroll
+  for (int i = 0; i < kStepBatch; ++i) {
+    take[i] = !t.done && (t.s + i) < 32;
+    t.px = __builtin_fmaf(t.dx, t.step, t.px);
+    t.py = __builtin_fmaf(t.dy, t.step, t.py);
+    t.pz = __builtin_fmaf(t.dz, t.step, t.pz);
+    t.dist = t.dist + t.step;
+    const float r = t.dist * t.rho;
+    // LINEAR_MIPMAP_LINEAR level pair for lod = log2(r), clamped to [0, q]
+    // (GL 4.5 section 8.14.3); frac = 0 means a single level.
+    const float lod = __builtin_amdgcn_fmed3f(__builtin_amdgcn_logf(r), 0.0f, qf);
+    const float fl = floorf(lod);
+    lv[i] = static_cast<int>(fl);
+    frac[i] = take[i] ? lod - fl : 0.0f;
+    need2 = need2 || frac[i] > 0.0f;
+    sx[i] = t.px;
+    sy[i] = t.py;
+    sz[i] = t.pz;
+    f1[i] = footprint(vx, lv[i], t.px, t.py, t.pz);
+    if (take[i]) {
+      if (t.dist >= t.goal) t.done = true;
+      else t.step = fmaxf(1.0f, r * t.k);
+    }
+  }
+  uint64_t c1[kStepBatch], c2[kStepBatch];
+#pragma unroll
+  for (int i = 0; i < kStepBatch; ++i) c1[i] = take[i] ? vx.corners[f1[i].index] : 0ull;
+  float o[kStepBatch];
+  // The second level is only planned when some lane of the wave blends two.
+  if (__any(need2)) {
+    Footprint f2[kStepBatch];
+#pragma unroll
+    for (int i = 0; i < kStepBatch; ++i) {
+      f2[i] = footprint(vx, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
+      c2[i] = frac[i] > 0.0f ? vx.corners[f2[i].index] : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < kStepBatch; ++i) {
+      const float o1 = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+      const float o2 = blend8(c2[i], f2[i].ax, f2[i].ay, f2[i].az);
+      o[i] = __builtin_fmaf(frac[i], o2 - o1, o1);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kStepBatch; ++i) o[i] = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+  }
+#pragma unroll
+  for (int i = 0; i < kStepBatch; ++i) {
+    if (take[i]) {
+      const float v = o[i] * (1.0f / 255.0f);
+      t.occ = t.occ + (1.0f - t.occ) * v;
+      ++t.s;
+    }
+  }
+  return t.done || t.s >= 32 || (1.0f - t.occ) <= kOpaque;
+}
+
+// Steps the reference still takes after a trace stopped early (geometry only).
+__device__ __forceinline__ int reference_steps(Trace t) {
+  int s = t.s;
+  if (t.done) return s;
+  while (s < 32) {
+    t.dist = t.dist + t.step;
+    ++s;
+    if (t.dist >= t.goal) break;
+    t.step = fmaxf(1.0f, t.dist * t.rho * t.k);
+  }
+  return s;
+}
+
+}  // namespace drvk

@@ -259,3 +259,90 @@ class Renderer:
         t = _abi.DrvStageTimes()
         self._check(self.lib.drv_get_stage_times(self.ctx, C.byref(t)), "drv_get_stage_times")
         return {n: getattr(t, n) for n, _ in t._fields_}
+
+
+class Batch:
+    """Several views lit by one persistent work-queue kernel
+    (drv_batch_light_caches), each view a Renderer on the same stream."""
+
+    def __init__(self, renderers: list):
+        self.lib = _abi.load_drv()
+        self.renderers = renderers
+        arr = (C.c_void_p * len(renderers))(*[r.ctx.value for r in renderers])
+        self.handle = C.c_void_p()
+        rc = self.lib.drv_batch_create(arr, len(renderers), C.byref(self.handle))
+        if rc != _abi.DRV_OK:
+            raise DrvError(f"drv_batch_create failed ({rc}): views must share SH bands, shadow mode and stream")
+
+    def close(self):
+        if self.handle:
+            self.lib.drv_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != _abi.DRV_OK:
+            raise DrvError(f"{what} failed ({rc}): {self.lib.drv_batch_last_error(self.handle).decode()}")
+
+    def set_tuning(self, caches_per_wave: int, vpl_tile: int, waves_per_cu: int = 64):
+        self._check(self.lib.drv_batch_set_tuning(self.handle, caches_per_wave, vpl_tile, waves_per_cu),
+                    "drv_batch_set_tuning")
+
+    def tuning(self) -> tuple:
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.drv_batch_get_tuning(self.handle, C.byref(a), C.byref(b), C.byref(c)),
+                    "drv_batch_get_tuning")
+        return a.value, b.value, c.value
+
+    @staticmethod
+    def _arrays(frames):
+        rsms = (_abi.DrvRsm * len(frames))(*[df.rsm() for df in frames])
+        lights = (_abi.DrvSpotLight * len(frames))(*[df.inputs.light for df in frames])
+        return rsms, lights
+
+    def light_caches(self, frames: list):
+        rsms, lights = self._arrays(frames)
+        self._check(self.lib.drv_batch_light_caches(self.handle, rsms, lights), "drv_batch_light_caches")
+
+    def autotune(self, frames: list, reps: int = 3) -> tuple:
+        rsms, lights = self._arrays(frames)
+        cpw, tile, ms = C.c_int32(), C.c_int32(), C.c_float()
+        self._check(self.lib.drv_batch_autotune(self.handle, rsms, lights, reps, C.byref(cpw), C.byref(tile),
+                                                C.byref(ms)), "drv_batch_autotune")
+        return cpw.value, tile.value, ms.value
+
+    def render_frame(self, frames: list, voxelize: bool = True):
+        """Every view's Renderer::Draw, with the light pass batched."""
+        for r, df in zip(self.renderers, frames):
+            fi = df.inputs
+            w = fi.workload
+            r.set_frame(fi.frame())
+            r.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+            if voxelize and r.cfg.indirect_shadow:
+                r.voxelize_scene(df.positions, len(fi.positions), adaption_factor=w.adaption_factor)
+            r.allocate_caches(df.gbuffer())
+        self.light_caches(frames)
+        for r, df in zip(self.renderers, frames):
+            r.apply_caches(df.gbuffer(), df.out)
+
+
+class FrameGraph:
+    """A HIP graph of whatever a Renderer's stream receives between
+    begin() and end() (drv_graph_begin / drv_graph_end), replayed by launch()."""
+
+    def __init__(self, renderer: Renderer):
+        self.r = renderer
+
+    def begin(self):
+        self.r._check(self.r.lib.drv_graph_begin(self.r.ctx), "drv_graph_begin")
+
+    def end(self):
+        self.r._check(self.r.lib.drv_graph_end(self.r.ctx), "drv_graph_end")
+
+    def launch(self):
+        self.r._check(self.r.lib.drv_graph_launch(self.r.ctx), "drv_graph_launch")

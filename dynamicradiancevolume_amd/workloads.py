@@ -94,6 +94,31 @@ BOXES_1080P = Workload(
 
 WORKLOADS = {w.name: w for w in (CORNELL_256, BOXES_1080P)}
 
+# Batched views: resolutions of 16:9 frames from 720p to 4K, and RSM reads of
+# 4096 or 16384 VPLs (square power-of-two RSM reads, which the Morton order of
+# cacheLightingRSM.comp:142 requires).
+BATCH_RESOLUTIONS = ((1280, 720), (1600, 900), (1920, 1080), (2560, 1440), (3200, 1800), (3840, 2160))
+BATCH_RSM = ((128, 1), (256, 1))  # (render resolution, read LOD) -> 64^2 or 128^2 VPLs
+
+
+def batch_views(n: int, seed: int = 0, family: Workload = BOXES_1080P) -> list:
+    """`n` views of seeded scenes of `family`, each with its own scene seed,
+    camera, resolution and VPL count (drawn with numpy's PCG64 from `seed`).
+    Capacities are sized for the resolution."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        w_, h_ = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+        rsm_res, lod = BATCH_RSM[int(rng.integers(len(BATCH_RSM)))]
+        ang = float(rng.uniform(-0.6, 0.6))
+        eye = (6.0 * math.sin(ang), float(rng.uniform(4.0, 8.0)), 6.0 * math.cos(ang))
+        target = (float(rng.uniform(-3, 3)), 0.0, float(rng.uniform(-8, -4)))
+        light = replace(family.light, rsm_resolution=rsm_res, rsm_read_lod=lod)
+        cap = 65536 if w_ * h_ <= 1920 * 1080 else 131072
+        out.append(family.with_(name=f"{family.name}_view{i}", seed=int(rng.integers(1 << 30)), width=w_, height=h_,
+                                eye=eye, target=target, light=light, max_caches=cap))
+    return out
+
 
 @dataclass
 class FrameInputs:

@@ -311,6 +311,47 @@ typedef struct drv_tuning {
 int drv_set_tuning(drv_ctx* ctx, const drv_tuning* tuning);
 int drv_get_tuning(const drv_ctx* ctx, drv_tuning* out);
 
+/* ---- batched views ----------------------------------------------------- */
+
+/* A batch lights the caches of several views (one drv_ctx each, e.g. views
+ * of different scenes, resolutions and VPL counts) with one persistent
+ * work-queue kernel instead of a pass per view: drv_batch_light_caches is
+ * drv_light_caches for every view at once (same results up to the order of
+ * the VPL sums).  All contexts must share sh_bands, indirect_shadow and their
+ * HIP stream.  No counterpart in the reference (one view per frame,
+ * rendering/renderer.cpp:539-594). */
+typedef struct drv_batch drv_batch;
+
+int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out);
+int drv_batch_destroy(drv_batch* batch);
+const char* drv_batch_last_error(const drv_batch* batch);
+/* Work decomposition: caches per wavefront (64, 32 or 16; lane groups of a
+ * wave split the shadow blocks of a cache when < 64), VPL tile length (VPLs
+ * per work item, a power of two >= the shadow block size; a view with more
+ * VPLs is split into partial sums), persistent grid size in waves per CU. */
+int drv_batch_set_tuning(drv_batch* batch, int32_t caches_per_wave, int32_t vpl_tile, int32_t waves_per_cu);
+int drv_batch_get_tuning(const drv_batch* batch, int32_t* caches_per_wave, int32_t* vpl_tile, int32_t* waves_per_cu);
+/* rsms[i], lights[i] belong to view i. */
+int drv_batch_light_caches(drv_batch* batch, const drv_rsm* rsms, const drv_spot_light* lights);
+/* Times drv_batch_light_caches over the candidate decompositions (median of
+ * `reps` runs each), keeps the fastest and returns it; the views' cache
+ * entries are restored afterwards. */
+int drv_batch_autotune(drv_batch* batch, const drv_rsm* rsms, const drv_spot_light* lights, int32_t reps,
+                       int32_t* caches_per_wave, int32_t* vpl_tile, float* ms);
+
+/* ---- HIP graph of a frame ----------------------------------------------- */
+
+/* Captures everything the context's stream receives between begin and end
+ * (any entry points of this or other contexts on the same stream) into a
+ * HIP graph, replayed by drv_graph_launch.  The stream must not be the null
+ * stream; calls that read back to the host (drv_get_cache_count, drv_read_*)
+ * and first-time buffer growth are not allowed while capturing, so run the
+ * frame once before capturing it.  Kernel arguments (matrices, volumes) are
+ * those of the captured calls. */
+int drv_graph_begin(drv_ctx* ctx);
+int drv_graph_end(drv_ctx* ctx);
+int drv_graph_launch(drv_ctx* ctx);
+
 /* ---- profiling -------------------------------------------------------- */
 
 /* When on, each stage is bracketed by HIP events on the context stream;

@@ -1,0 +1,185 @@
+"""Batched light pass and HIP graph capture.
+
+* drv_batch_light_caches over uneven views equals each view's own
+  drv_light_caches (the single-view path, itself pinned against the oracle)
+  up to the order of the VPL sums, for every work decomposition; one view is
+  also checked against the FP32 oracle directly;
+* the autotuner returns a valid decomposition and leaves the entries as it
+  found them;
+* a captured frame (allocate, batched light, apply for every view) replays
+  bit-identically to the eager frame, several times, with eager work and
+  synchronisations between the replays.
+"""
+import numpy as np
+import pytest
+import torch
+
+from canon import rel_l2
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+SH_TOL = 1e-5   # batch vs single view (same terms, different summation order)
+
+
+def _views():
+    from dynamicradiancevolume_amd import workloads as W
+
+    c = W.CORNELL_256
+    small_rsm = W.replace(c.light, rsm_resolution=128, rsm_read_lod=1)   # 64^2 VPLs from a 128^2 RSM
+    b = W.BOXES_1080P
+    return [
+        c,
+        c.with_(name="cornell_wide", seed=3, width=320, height=180, light=small_rsm),
+        b.with_(name="boxes_720p", seed=5, width=1280, height=720, max_caches=65536,
+                light=W.replace(b.light, rsm_resolution=128, rsm_read_lod=1)),
+        b.with_(name="boxes_900p", seed=6, width=1600, height=900, max_caches=65536),
+    ]
+
+
+_cache = {}
+
+
+def _frames(names=None):
+    from dynamicradiancevolume_amd.workloads import make_inputs
+
+    out = []
+    for w in _views():
+        if names and w.name not in names:
+            continue
+        if w.name not in _cache:
+            _cache[w.name] = make_inputs(w)
+        out.append(_cache[w.name])
+    return out
+
+
+def _by_position(r):
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    _, live = r.cache_count()
+    e = r.read_caches(live)
+    order = np.lexsort(e[:, :3].view(np.uint32).T)
+    return e[order, :3].copy(), entries_sh(e[order], r.cfg.sh_bands).astype(np.float64)
+
+
+def _group(frames, sh_bands, stream=None):
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame, Renderer
+
+    rs = [Renderer(fi.workload.with_(sh_bands=sh_bands).config(), stream) for fi in frames]
+    dfs = [DeviceFrame(fi) for fi in frames]
+    return rs, dfs
+
+
+def _single_view_sh(frames, sh_bands):
+    rs, dfs = _group(frames, sh_bands)
+    out = []
+    for r, df in zip(rs, dfs):
+        r.render_frame(df)
+        out.append(_by_position(r))
+        r.close()
+    return out
+
+
+@pytest.mark.parametrize("sh_bands", [1, 2])
+def test_batch_matches_single_views(sh_bands):
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce(f"batch of {len(frames)} views vs single-view path, SH{sh_bands}")
+    ref = _single_view_sh(frames, sh_bands)
+    rs, dfs = _group(frames, sh_bands)
+    b = Batch(rs)
+    b.render_frame(dfs)
+    for (p0, s0), r, fi in zip(ref, rs, frames):
+        p1, s1 = _by_position(r)
+        np.testing.assert_array_equal(p0, p1)
+        assert rel_l2(s1, s0) <= SH_TOL, fi.workload.name
+    b.close()
+
+
+def test_batch_view_matches_oracle():
+    from dynamicradiancevolume_amd.pipeline import Batch
+    from oracle import orc
+
+    frames = _frames()
+    announce("batch view 1 vs FP32 oracle")
+    rs, dfs = _group(frames, 1)
+    b = Batch(rs)
+    b.render_frame(dfs)
+    fi, r = frames[1], rs[1]
+    res = orc.run_frame(fi)
+    _, live = r.cache_count()
+    assert live == res["live"]
+    p1, s1 = _by_position(r)
+    e = res["entries"][: res["live"]]
+    order = np.lexsort(e[:, :3].view(np.uint32).T)
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    np.testing.assert_array_equal(e[order, :3], p1)
+    assert rel_l2(s1, entries_sh(e[order], 1)) <= 1e-4
+
+
+@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536)])
+def test_batch_decompositions_agree(cpw, tile):
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce(f"batch decomposition cpw={cpw} tile={tile}")
+    res = []
+    for t in (None, (cpw, tile)):
+        rs, dfs = _group(frames, 2)
+        b = Batch(rs)
+        if t:
+            b.set_tuning(*t)
+        b.render_frame(dfs)
+        res.append([_by_position(r) for r in rs])
+        b.close()
+    for (p0, s0), (p1, s1) in zip(*res):
+        np.testing.assert_array_equal(p0, p1)
+        assert rel_l2(s1, s0) <= SH_TOL
+
+
+def test_batch_autotune_restores_entries():
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce("batch autotune")
+    rs, dfs = _group(frames, 2)
+    b = Batch(rs)
+    b.render_frame(dfs)
+    before = [r.read_caches(r.cache_count()[1]) for r in rs]
+    cpw, tile, ms = b.autotune(dfs, reps=1)
+    assert cpw in (64, 32, 16) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
+    assert b.tuning()[:2] == (cpw, tile)
+    for r, e in zip(rs, before):
+        np.testing.assert_array_equal(r.read_caches(len(e)), e)
+
+
+def test_graph_replay_matches_eager():
+    from dynamicradiancevolume_amd.pipeline import Batch, FrameGraph
+
+    frames = _frames()
+    announce("graph capture of a batched frame, 3 replays")
+    stream = torch.cuda.Stream()
+    rs, dfs = _group(frames, 2, stream)
+    b = Batch(rs)
+    with torch.cuda.stream(stream):
+        b.render_frame(dfs)           # eager frame (also grows every buffer)
+        stream.synchronize()
+        eager = [(_by_position(r), df.out.clone()) for r, df in zip(rs, dfs)]
+        g = FrameGraph(rs[0])
+        g.begin()
+        b.render_frame(dfs)
+        g.end()
+        for rep in range(3):
+            for df in dfs:
+                df.out.fill_(-1.0)    # eager work between replays
+            stream.synchronize()
+            g.launch()
+            stream.synchronize()
+            for ((p0, s0), img0), r, df in zip(eager, rs, dfs):
+                p1, s1 = _by_position(r)
+                np.testing.assert_array_equal(p1, p0)
+                np.testing.assert_array_equal(s1, s0)
+                assert torch.equal(df.out, img0), f"replay {rep}"
+    b.close()
