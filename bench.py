@@ -71,6 +71,17 @@ def _cpu_baseline(fi, r, n_threads: int, budget_s: float):
     }
 
 
+def _traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed byte-counter
+    summary (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload,
+    scaled by a known-size copy); None when absent."""
+    p = ROOT / "profiles" / "pmc_boxes_1080p.json"
+    if not p.exists():
+        return None
+    k = json.loads(p.read_text()).get("kernels", {}).get(kernel)
+    return k["hbm_bytes"] if k else None
+
+
 def _time_frames(r, df, stream, steps: int, finetune_every: int = 0):
     """ms per frame over `steps` whole frames (HIP events on the library's
     stream), optionally with an online fine-tuning step every N frames."""
@@ -137,6 +148,78 @@ def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, thre
     return out
 
 
+def _batch(n_views: int, steps: int, stream):
+    """The batched-views workload: `n_views` seeded views of the boxes family
+    (720p to 4K, 4096 or 16384 VPLs, own scene each) rendered (a) view after
+    view through the single-view path, (b) as one batch (autotuned work
+    decomposition), (c) as a replayed HIP graph of the batched frame."""
+    import torch
+
+    from dynamicradiancevolume_amd import workloads as W
+    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, FrameGraph, Renderer
+
+    views = W.batch_views(n_views, seed=0)
+    frames = []
+    for i, w in enumerate(views):
+        frames.append(W.make_inputs(w, n_threads=min(16, os.cpu_count() or 1)))
+        if (i + 1) % 8 == 0:
+            print(f"[bench] batch inputs {i + 1}/{n_views}", file=sys.stderr, flush=True)
+    rs = [Renderer(w.config(), stream) for w in views]
+    for r in rs:
+        r.set_tuning(gather_max_splits=32)
+    dfs = [DeviceFrame(fi) for fi in frames]
+    del frames
+    torch.cuda.synchronize()
+
+    def timed(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    k = max(2, steps // 4)
+    seq_ms = timed(lambda: [r.render_frame(df) for r, df in zip(rs, dfs)], k)
+    lives = [r.cache_count()[1] for r in rs]
+    interactions = sum(lv * df.inputs.read_resolution ** 2 for lv, df in zip(lives, dfs))
+    b = Batch(rs)
+    b.render_frame(dfs)
+    cpw, tile, tune_ms = b.autotune(dfs, reps=2)
+    batch_ms = timed(lambda: b.render_frame(dfs), k)
+    g = FrameGraph(rs[0])
+    with torch.cuda.stream(stream):
+        b.render_frame(dfs)
+        stream.synchronize()
+        g.begin()
+        b.render_frame(dfs)
+        g.end()
+    graph_ms = timed(g.launch, k)
+    print(f"[bench] batch of {n_views}: sequential {seq_ms:.2f} ms, batched {batch_ms:.2f} ms, graph {graph_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+    out = {
+        "views": n_views,
+        "resolutions": sorted({f"{w.width}x{w.height}" for w in views}),
+        "vpls_per_view": sorted({df.inputs.read_resolution ** 2 for df in dfs}),
+        "live_caches": [min(lives), max(lives), sum(lives)],
+        "interactions_per_batch": interactions,
+        "sequential_ms": seq_ms,
+        "batched_ms": batch_ms,
+        "graph_ms": graph_ms,
+        "speedup_graph_vs_sequential": seq_ms / graph_ms,
+        "interactions_per_s_graph": interactions / (graph_ms * 1e-3),
+        "autotuned": {"caches_per_wave": cpw, "vpl_tile": tile, "light_pass_ms": tune_ms},
+        "frames_timed": k,
+    }
+    b.close()
+    for r in rs:
+        r.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +229,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--batch-views", type=int, default=64, help="views of the batched workload (0 = skip)")
     ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -179,27 +263,32 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     # Per-kernel durations: HIP events on the library's stream, averaged over
-    # a profiled pass of the same frames (step counting on).
-    r.set_profiling(2)
+    # a profiled pass of the same frames; the march-step counts come from one
+    # more frame with step counting on (which slows the trace, so it is not
+    # timed).
+    r.set_profiling(1)
     keys = ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
             "gather_ms", "reduce_ms", "apply_ms")
     acc = {k: 0.0 for k in keys}
     n_prof = max(3, min(10, args.steps))
-    traces = steps_ = steps_exec = 0
     for _ in range(n_prof):
         r.render_frame(df)
         t = r.stage_times()
         for k in keys:
             acc[k] += t[k]
-        traces, steps_, steps_exec = t["cone_traces"], t["cone_steps"], t["cone_steps_executed"]
     stages = {k: round(v / n_prof, 4) for k, v in acc.items()}
+    r.set_profiling(2)
+    r.render_frame(df)
+    t = r.stage_times()
+    traces, steps_, steps_exec = t["cone_traces"], t["cone_steps"], t["cone_steps_executed"]
     r.set_profiling(0)
 
     vis_flop = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_
     gather_flop = FLOP_SH[w.sh_bands] * interactions
+    gather_name = f"k_gather<{'true' if w.sh_bands >= 2 else 'false'}, {'true' if w.indirect_shadow else 'false'}>"
     kernels = {
         "k_visibility": (stages["visibility_ms"], vis_flop),
-        "k_gather": (stages["gather_ms"], gather_flop),
+        gather_name: (stages["gather_ms"], gather_flop),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
     dms, dflop = kernels[dom]
@@ -212,7 +301,8 @@ def main() -> int:
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-        "traffic": None,
+        "traffic": _traffic(dom) if w.name == "boxes_1080p" else None,
+        "traffic_source": "profiles/hbm_bytes_boxes_1080p.md (rocprofv3 --pmc, copy-calibrated)",
         "kernel_ms": dms,
         "algorithmic_flop": dflop,
     }
@@ -221,6 +311,11 @@ def main() -> int:
     if not args.no_surrogate:
         print("[bench] surrogate", file=sys.stderr, flush=True)
         sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold)
+
+    batch = None
+    if args.batch_views > 0:
+        print(f"[bench] batch of {args.batch_views} views", file=sys.stderr, flush=True)
+        batch = _batch(args.batch_views, args.steps, stream)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -260,6 +355,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "batch": batch,
     }
     print(json.dumps(line))
     return 0

@@ -232,7 +232,7 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
                           set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, spot_splits, grid,
                           ctx->stream);
       drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->partial, sc,
-                             ctx->cu_count * 4, ctx->stream);
+                             ctx->cu_count * 8, ctx->stream);
     }
     drvk::launch_spot_error(ctx->spot_caches, sc, stride, ctx->cfg.sh_bands, ctx->spot_ctr, ctx->sur.fallback_threshold,
                             ctx->counter, ctx->spot_stats, &ctx->spot_ctr[2], ctx->stream);
@@ -646,7 +646,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
                       grid, ctx->stream);
   record(ctx, 8);
   drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->partial,
-                         static_cast<uint32_t>(cap), ctx->cu_count * 4, ctx->stream);
+                         static_cast<uint32_t>(cap), ctx->cu_count * 8, ctx->stream);
   record(ctx, 9);
   ctx->counted_steps = steps != nullptr;
   ctx->last_n_blocks = n_blocks;

@@ -184,6 +184,8 @@ __device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint
 // block at a time for all its lanes (one uniform block record, coherent
 // rays from neighbouring caches towards the same VPL block).
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
+  __shared__ uint32_t off[16];
+  stage_offsets(a.vox, off);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t lane = threadIdx.x & 63u;
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     for (uint32_t b = b0; b < b1; ++b) {
       Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[b]);
       bool finished = false;
-      while (!finished) finished = trace_batch(a.vox, t);
+      while (!finished) finished = trace_batch(a.vox, off, t);
       a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
       my_traces += 1u;
       my_exec += static_cast<uint32_t>(t.s);
@@ -224,6 +226,8 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
 // the lanes whose trace ended take the next pool entries (ballot + mbcnt
 // ranks), so lanes do not idle behind the longest ray of the wave.
 __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
+  __shared__ uint32_t off[16];
+  stage_offsets(a.vox, off);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t lane = threadIdx.x & 63u;
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
     };
     if (has) start(idx);
     while (true) {
-      const bool ended = trace_batch(a.vox, t);
+      const bool ended = trace_batch(a.vox, off, t);
       const bool fin = has && ended;
       if (fin) {
         a.vis[static_cast<size_t>(blk) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
@@ -340,42 +344,44 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
   }
 }
 
-// Sums the split partials in split order, applies the SH evaluation factors
+// Sums the split partials, applies the SH evaluation factors
 // (cacheLightingRSM.comp:267-277) and adds into the entry
-// (cacheLightingRSM.comp:357-373).  One lane per (cache, coefficient,
-// channel); every lane owns one entry field.
+// (cacheLightingRSM.comp:357-373).  A workgroup owns one (64-cache tile,
+// coefficient, channel); its four waves sum interleaved quarters of the
+// splits (lane = cache, so every load is a coalesced row) and the quarters
+// are combined in a fixed order through LDS.
 __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const Counter* counter,
                                                 const float* partial, uint32_t partial_pitch, drv_sh_constants shc,
                                                 int sh2) {
+  __shared__ float part[2][4][64];
   const uint32_t live = counter->live;
   const uint32_t splits = counter->splits;
-  const uint32_t n_coef = sh2 ? 9u : 4u;
-  const uint32_t total = live * n_coef * 3u;
+  const uint32_t n_jc = sh2 ? 27u : 12u;
+  const uint32_t tiles = (live + 63u) / 64u;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const size_t split_stride = static_cast<size_t>(27u) * partial_pitch;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const uint32_t c = t % live;
-    const uint32_t jc = t / live;  // coefficient * 3 + channel
+  for (uint32_t item = blockIdx.x; item < tiles * n_jc; item += gridDim.x) {
+    const uint32_t tile = item % tiles, jc = item / tiles;
     const uint32_t j = jc / 3u, ch = jc % 3u;
-    const float* p = partial + static_cast<size_t>(jc) * partial_pitch + c;
-    const float* p0 = partial + static_cast<size_t>(ch) * partial_pitch + c;  // raw sum of rad
-    // Four interleaved partial sums (fixed order) keep several loads in
-    // flight when a small cache set was split many ways.
-    float v4[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    uint32_t sp = 0;
-    for (; sp + 4u <= splits; sp += 4u) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        v4[q] += p[(sp + q) * split_stride];
-        if (j == 6u) w4[q] += p0[(sp + q) * split_stride];
-      }
+    const uint32_t c = tile * 64u + lane;
+    const uint32_t cc = c < live ? c : 0u;
+    const float* p = partial + static_cast<size_t>(jc) * partial_pitch + cc;
+    const float* p0 = partial + static_cast<size_t>(ch) * partial_pitch + cc;  // raw sum of rad
+    float v = 0.0f, v0 = 0.0f;
+#pragma unroll 8
+    for (uint32_t sp = w; sp < splits; sp += 4u) {
+      v += p[sp * split_stride];
+      if (j == 6u) v0 += p0[sp * split_stride];
     }
-    for (int q = 0; sp < splits; ++sp, ++q) {
-      v4[q] += p[sp * split_stride];
-      if (j == 6u) w4[q] += p0[sp * split_stride];
+    part[0][w][lane] = v;
+    part[1][w][lane] = v0;
+    __syncthreads();
+    if (w == 0u && c < live) {
+      const float s = ((part[0][0][lane] + part[0][1][lane]) + part[0][2][lane]) + part[0][3][lane];
+      const float s0 = ((part[1][0][lane] + part[1][1][lane]) + part[1][2][lane]) + part[1][3][lane];
+      sh_add_coefficient(caches + static_cast<size_t>(c) * stride, j, ch, s, s0, shc);
     }
-    const float v = (v4[0] + v4[1]) + (v4[2] + v4[3]);
-    const float v0 = (w4[0] + w4[1]) + (w4[2] + w4[3]);
-    sh_add_coefficient(caches + static_cast<size_t>(c) * stride, j, ch, v, v0, shc);
+    __syncthreads();
   }
 }
 
@@ -454,6 +460,7 @@ void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* cache
 
 void launch_reduce_sh(const drv_sh_constants& shc, int sh_bands, float* caches, int stride, const Counter* counter,
                       const float* partial, uint32_t partial_pitch, int grid, hipStream_t s) {
+  // grid: persistent workgroups over (tile, coefficient, channel) items
   hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(256), 0, s, caches, stride, counter, partial, partial_pitch, shc,
                      sh_bands >= 2 ? 1 : 0);
 }

@@ -16,7 +16,10 @@ struct Footprint {
   float ax, ay, az;
 };
 
-__device__ __forceinline__ Footprint footprint(const VoxelView& vx, int l, float px, float py, float pz) {
+// `off` is the view's corner_offset table staged in LDS (a per-lane level
+// index into the kernel-argument copy would be a memory load per sample).
+__device__ __forceinline__ Footprint footprint(const VoxelView& vx, const uint32_t* off, int l, float px, float py,
+                                               float pz) {
   const int w = vx.res >> l;
   const float scale = __builtin_amdgcn_ldexpf(1.0f, -l);
   const float u = __builtin_fmaf(px, scale, -0.5f);
@@ -30,7 +33,7 @@ __device__ __forceinline__ Footprint footprint(const VoxelView& vx, int l, float
   const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fq, -1.0f, hi) + 1.0f);
   const uint32_t wp = static_cast<uint32_t>(w + 1);
   Footprint f;
-  f.index = vx.corner_offset[l] + __umul24(__umul24(k, wp) + j, wp) + i;
+  f.index = off[l] + __umul24(__umul24(k, wp) + j, wp) + i;
   f.ax = u - fu;
   f.ay = v - fv;
   f.az = q - fq;
@@ -55,9 +58,17 @@ __device__ __forceinline__ float blend8(uint64_t c, float ax, float ay, float az
 }
 
 // Single-level sample (0..1) of the voxel mips at level l, p in level-0 texel units.
-__device__ __forceinline__ float sample_level(const VoxelView& vx, int l, float px, float py, float pz) {
-  const Footprint f = footprint(vx, l, px, py, pz);
+__device__ __forceinline__ float sample_level(const VoxelView& vx, const uint32_t* off, int l, float px, float py,
+                                              float pz) {
+  const Footprint f = footprint(vx, off, l, px, py, pz);
   return blend8(vx.corners[f.index], f.ax, f.ay, f.az) * (1.0f / 255.0f);
+}
+
+// Stages the corner_offset table in LDS; every thread of the workgroup must
+// call it (it synchronises the workgroup).
+__device__ __forceinline__ void stage_offsets(const VoxelView& vx, uint32_t* off) {
+  if (threadIdx.x < 16u) off[threadIdx.x] = vx.corner_offset[threadIdx.x];
+  __syncthreads();
 }
 
 }  // namespace drvk

@@ -55,7 +55,7 @@ __device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_s
 // The sample positions depend only on the geometry, so all samples of the
 // batch are planned first and their loads issued together; occlusion is then
 // accumulated in step order.  Returns true when the trace has finished.
-__device__ __forceinline__ bool trace_batch(const VoxelView& vx, Trace& t) {
+__device__ __forceinline__ bool trace_batch(const VoxelView& vx, const uint32_t* off, Trace& t) {
   const int q = vx.levels - 1;
   const float qf = static_cast<float>(q);
   float sx[kStepBatch], sy[kStepBatch], sz[kStepBatch], frac[kStepBatch];
@@ -81,7 +81,7 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, Trace& t) {
     sx[i] = t.px;
     sy[i] = t.py;
     sz[i] = t.pz;
-    f1[i] = footprint(vx, lv[i], t.px, t.py, t.pz);
+    f1[i] = footprint(vx, off, lv[i], t.px, t.py, t.pz);
     if (take[i]) {
       if (t.dist >= t.goal) t.done = true;
       else t.step = fmaxf(1.0f, r * t.k);
@@ -96,7 +96,7 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, Trace& t) {
     Footprint f2[kStepBatch];
 #pragma unroll
     for (int i = 0; i < kStepBatch; ++i) {
-      f2[i] = footprint(vx, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
+      f2[i] = footprint(vx, off, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
       c2[i] = frac[i] > 0.0f ? vx.corners[f2[i].index] : 0ull;
     }
 #pragma unroll
