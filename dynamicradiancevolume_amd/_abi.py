@@ -147,7 +147,8 @@ class DrvSurrogateStats(C.Structure):
 
 class DrvTuning(C.Structure):
     _fields_ = [("visibility_kernel", C.c_int32), ("blocks_per_item", C.c_int32),
-                ("gather_max_splits", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_min_lanes", C.c_int32)]
+                ("gather_max_splits", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_min_lanes", C.c_int32),
+                ("steps_per_batch", C.c_int32)]
 
 
 DRV_MLP_PARAMS = 10497

@@ -1198,13 +1198,15 @@ int drv_set_tuning(drv_ctx* ctx, const drv_tuning* t) {
   if (!ctx || !t) return DRV_ERR_INVALID_ARG;
   if ((t->visibility_kernel != 0 && t->visibility_kernel != 1) || t->blocks_per_item < 1 || t->blocks_per_item > 64 ||
       t->gather_max_splits < 1 || t->gather_max_splits > 128 || !is_pow2(t->gather_max_splits) ||
-      t->waves_per_cu < 4 || t->waves_per_cu > 64 || t->refill_min_lanes < 1 || t->refill_min_lanes > 64)
+      t->waves_per_cu < 4 || t->waves_per_cu > 64 || t->refill_min_lanes < 1 || t->refill_min_lanes > 64 ||
+      (t->steps_per_batch != 2 && t->steps_per_batch != 4 && t->steps_per_batch != 8))
     return fail(ctx, DRV_ERR_INVALID_ARG, "set_tuning: value out of range");
   ctx->tune.visibility_kernel = t->visibility_kernel;
   ctx->tune.blocks_per_item = t->blocks_per_item;
   ctx->tune.gather_max_splits = t->gather_max_splits;
   ctx->tune.waves_per_cu = t->waves_per_cu;
   ctx->tune.refill_min_lanes = t->refill_min_lanes;
+  ctx->tune.steps_per_batch = t->steps_per_batch;
   return DRV_OK;
 }
 
@@ -1215,6 +1217,7 @@ int drv_get_tuning(const drv_ctx* ctx, drv_tuning* out) {
   out->gather_max_splits = ctx->tune.gather_max_splits;
   out->waves_per_cu = ctx->tune.waves_per_cu;
   out->refill_min_lanes = ctx->tune.refill_min_lanes;
+  out->steps_per_batch = ctx->tune.steps_per_batch;
   return DRV_OK;
 }
 

@@ -118,6 +118,7 @@ struct Tuning {
   int gather_max_splits = 128; // VPL splits of the gather
   int waves_per_cu = 64;       // persistent grid size
   int refill_min_lanes = 16;   // lane pool: idle lanes that trigger a refill
+  int steps_per_batch = 4;     // cone-trace steps planned (and loaded) together: 2, 4 or 8
 };
 
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);

@@ -183,6 +183,7 @@ __device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint
 // A wave owns 64 caches x blocks_per_item shadow blocks and traces one
 // block at a time for all its lanes (one uniform block record, coherent
 // rays from neighbouring caches towards the same VPL block).
+template <int kBatch>
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   __shared__ uint32_t off[16];
   stage_offsets(a.vox, off);
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     for (uint32_t b = b0; b < b1; ++b) {
       Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[b]);
       bool finished = false;
-      while (!finished) finished = trace_batch(a.vox, off, t);
+      while (!finished) finished = trace_batch<kBatch>(a.vox, off, t);
       a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
       my_traces += 1u;
       my_exec += static_cast<uint32_t>(t.s);
@@ -429,8 +430,12 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
   a.refill_min = static_cast<uint32_t>(tune.refill_min_lanes);
   if (tune.visibility_kernel == 1)
     hipLaunchKernelGGL(k_visibility_pool, dim3(grid), dim3(256), 0, s, a);
+  else if (tune.steps_per_batch == 2)
+    hipLaunchKernelGGL(k_visibility<2>, dim3(grid), dim3(256), 0, s, a);
+  else if (tune.steps_per_batch == 8)
+    hipLaunchKernelGGL(k_visibility<8>, dim3(grid), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL(k_visibility, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_visibility<4>, dim3(grid), dim3(256), 0, s, a);
 }
 
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,

@@ -55,16 +55,17 @@ __device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_s
 // The sample positions depend only on the geometry, so all samples of the
 // batch are planned first and their loads issued together; occlusion is then
 // accumulated in step order.  Returns true when the trace has finished.
+template <int kBatch = kStepBatch>
 __device__ __forceinline__ bool trace_batch(const VoxelView& vx, const uint32_t* off, Trace& t) {
   const int q = vx.levels - 1;
   const float qf = static_cast<float>(q);
-  float sx[kStepBatch], sy[kStepBatch], sz[kStepBatch], frac[kStepBatch];
-  int lv[kStepBatch];
-  Footprint f1[kStepBatch];
-  bool take[kStepBatch];
+  float sx[kBatch], sy[kBatch], sz[kBatch], frac[kBatch];
+  int lv[kBatch];
+  Footprint f1[kBatch];
+  bool take[kBatch];
   bool need2 = false;
 #pragma unroll
-  for (int i = 0; i < kStepBatch; ++i) {
+  for (int i = 0; i < kBatch; ++i) {
     take[i] = !t.done && (t.s + i) < 32;
     t.px = __builtin_fmaf(t.dx, t.step, t.px);
     t.py = __builtin_fmaf(t.dy, t.step, t.py);
@@ -87,30 +88,30 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, const uint32_t*
       else t.step = fmaxf(1.0f, r * t.k);
     }
   }
-  uint64_t c1[kStepBatch], c2[kStepBatch];
+  uint64_t c1[kBatch], c2[kBatch];
 #pragma unroll
-  for (int i = 0; i < kStepBatch; ++i) c1[i] = take[i] ? vx.corners[f1[i].index] : 0ull;
-  float o[kStepBatch];
+  for (int i = 0; i < kBatch; ++i) c1[i] = take[i] ? vx.corners[f1[i].index] : 0ull;
+  float o[kBatch];
   // The second level is only planned when some lane of the wave blends two.
   if (__any(need2)) {
-    Footprint f2[kStepBatch];
+    Footprint f2[kBatch];
 #pragma unroll
-    for (int i = 0; i < kStepBatch; ++i) {
+    for (int i = 0; i < kBatch; ++i) {
       f2[i] = footprint(vx, off, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
       c2[i] = frac[i] > 0.0f ? vx.corners[f2[i].index] : 0ull;
     }
 #pragma unroll
-    for (int i = 0; i < kStepBatch; ++i) {
+    for (int i = 0; i < kBatch; ++i) {
       const float o1 = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
       const float o2 = blend8(c2[i], f2[i].ax, f2[i].ay, f2[i].az);
       o[i] = __builtin_fmaf(frac[i], o2 - o1, o1);
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < kStepBatch; ++i) o[i] = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+    for (int i = 0; i < kBatch; ++i) o[i] = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
   }
 #pragma unroll
-  for (int i = 0; i < kStepBatch; ++i) {
+  for (int i = 0; i < kBatch; ++i) {
     if (take[i]) {
       const float v = o[i] * (1.0f / 255.0f);
       t.occ = t.occ + (1.0f - t.occ) * v;

@@ -318,10 +318,13 @@ def test_full_frame_matches_oracle(name):
     dict(visibility_kernel=1, blocks_per_item=1, waves_per_cu=8),
     dict(visibility_kernel=1, blocks_per_item=32, gather_max_splits=128, waves_per_cu=64),
     dict(visibility_kernel=0, blocks_per_item=3, gather_max_splits=1, waves_per_cu=4),
+    dict(visibility_kernel=0, steps_per_batch=2),
+    dict(visibility_kernel=0, steps_per_batch=8, blocks_per_item=5),
 ])
 def test_tuning_variants_agree(tuning):
     """Every launch shape traces the same rays: visibility is bit-identical to
-    the default's; SH differ only by the split summation order."""
+    the default's (up to the early-exit granularity when the steps per batch
+    change); SH differ only by the split summation order."""
     from canon import rel_l2
 
     fi = frame_inputs("boxes_1080p")
@@ -342,7 +345,13 @@ def test_tuning_variants_agree(tuning):
         r.close()
     (e0, v0), (e1, v1) = res
     np.testing.assert_array_equal(e0[:, :3], e1[:, :3])
-    np.testing.assert_array_equal(v0, v1)
+    if tuning.get("steps_per_batch", 4) == 4:
+        np.testing.assert_array_equal(v0, v1)
+    else:
+        # A ray stops at the first batch boundary after its transmittance
+        # drops below 2^-20, so the visibility of opaque rays may differ by
+        # less than that.
+        assert np.abs(v0 - v1).max() <= 2.0 ** -19
     assert rel_l2(e1[:, 4:], e0[:, 4:]) <= 1e-5
 
 
