@@ -145,6 +145,10 @@ class DrvSurrogateStats(C.Structure):
     _fields_ = [("spot_error", C.c_float), ("fell_back", C.c_int32), ("spot_caches", C.c_uint32)]
 
 
+class DrvScene(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("albedo", C.c_void_p), ("normals", C.c_void_p), ("n_tri", C.c_uint32)]
+
+
 class DrvTuning(C.Structure):
     _fields_ = [("visibility_kernel", C.c_int32), ("blocks_per_item", C.c_int32),
                 ("gather_max_splits", C.c_int32), ("waves_per_cu", C.c_int32), ("refill_min_lanes", C.c_int32),
@@ -181,6 +185,10 @@ DRV_FUNCTIONS = {
     "drv_read_cav": (C.c_int, [vp, vp]),
     "drv_read_voxels": (C.c_int, [vp, C.c_int32, vp]),
     "drv_read_visibility": (C.c_int, [vp, vp, C.c_uint64]),
+    "drv_render_gbuffer": (C.c_int, [vp, P(DrvScene), P(C.c_float), P(DrvGBuffer)]),
+    "drv_render_rsm": (C.c_int, [vp, P(DrvScene), P(DrvSpotLight), P(DrvRsm)]),
+    "drv_read_rsm": (C.c_int, [vp, C.c_int32, vp, vp, vp]),
+    "drv_read_gbuffer": (C.c_int, [vp, vp, vp, vp]),
     "drv_mlp_set_params": (C.c_int, [vp, vp, C.c_uint32]),
     "drv_mlp_get_params": (C.c_int, [vp, vp, C.c_uint32]),
     "drv_set_surrogate": (C.c_int, [vp, P(DrvSurrogate)]),

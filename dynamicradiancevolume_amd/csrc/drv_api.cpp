@@ -89,6 +89,19 @@ struct drv_ctx {
   drvk::Tuning tune;
   size_t partial_splits = 0;  // splits the partial buffer holds
 
+  // Device-rendered inputs: raster scratch, G-buffer, RSM mip chain.
+  unsigned long long* rz = nullptr;
+  uint32_t* rtri = nullptr;
+  size_t raster_cap = 0, raster_tri_cap = 0;
+  float* g_depth = nullptr;
+  int16_t* g_normal = nullptr;
+  uint8_t* g_albedo = nullptr;
+  uint16_t* rsm_flux = nullptr;
+  int16_t* rsm_normal = nullptr;
+  uint16_t* rsm_dls = nullptr;
+  size_t rsm_cap = 0, rsm_normal_cap = 0, rsm_dls_cap = 0;  // texels of the chain
+  int rsm_res = 0;                                           // render resolution of the last RSM
+
   bool profiling = false;
   int profiling_level = 0;
   uint64_t* step_counter = nullptr;  // [traces, reference steps, executed steps] when counting
@@ -375,6 +388,14 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->spot_vis);
   dev_free(ctx->spot_ctr);
   dev_free(ctx->spot_stats);
+  dev_free(ctx->rz);
+  dev_free(ctx->rtri);
+  dev_free(ctx->g_depth);
+  dev_free(ctx->g_normal);
+  dev_free(ctx->g_albedo);
+  dev_free(ctx->rsm_flux);
+  dev_free(ctx->rsm_normal);
+  dev_free(ctx->rsm_dls);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -725,6 +746,100 @@ int drv_read_visibility(drv_ctx* ctx, float* host_out, uint64_t count) {
     const uint64_t c = i / nb, b = i % nb;
     host_out[i] = buf[b * live + c];
   }
+  return DRV_OK;
+}
+
+// ---- inputs rendered on the device -------------------------------------------
+
+namespace {
+
+size_t rsm_chain_texels(int res) {
+  size_t n = 0;
+  for (int r = res; r >= 1; r /= 2) n += static_cast<size_t>(r) * r;
+  return n;
+}
+
+int ensure_raster(drv_ctx* ctx, size_t pixels) {
+  int rc;
+  if ((rc = ensure(ctx, &ctx->rz, &ctx->raster_cap, pixels)) != DRV_OK) return rc;
+  return ensure(ctx, &ctx->rtri, &ctx->raster_tri_cap, pixels);
+}
+
+bool valid_scene(const drv_scene* s) {
+  return s && (s->n_tri == 0 || (s->positions && s->albedo && s->normals));
+}
+
+}  // namespace
+
+int drv_render_gbuffer(drv_ctx* ctx, const drv_scene* scene, const float view_projection[16], drv_gbuffer* out) {
+  if (!ctx || !valid_scene(scene) || !view_projection || !out) return DRV_ERR_INVALID_ARG;
+  const int w = ctx->cfg.width, h = ctx->cfg.height;
+  const size_t n = static_cast<size_t>(w) * h;
+  int rc;
+  if ((rc = ensure_raster(ctx, n)) != DRV_OK) return rc;
+  if (!ctx->g_depth) {
+    if ((rc = dev_alloc(ctx, &ctx->g_depth, n)) != DRV_OK) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->g_normal, 2 * n)) != DRV_OK) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->g_albedo, 3 * n)) != DRV_OK) return rc;
+  }
+  drvk::launch_render_gbuffer(scene->positions, scene->albedo, scene->normals, scene->n_tri, view_projection, w, h,
+                              ctx->rz, ctx->rtri, ctx->g_depth, ctx->g_normal, ctx->g_albedo, ctx->stream);
+  out->depth = ctx->g_depth;
+  out->normal = ctx->g_normal;
+  out->albedo = ctx->g_albedo;
+  out->width = w;
+  out->height = h;
+  return check_hip(ctx, hipGetLastError(), "render_gbuffer");
+}
+
+int drv_render_rsm(drv_ctx* ctx, const drv_scene* scene, const drv_spot_light* light, drv_rsm* out) {
+  if (!ctx || !valid_scene(scene) || !light || !out) return DRV_ERR_INVALID_ARG;
+  const int res = light->rsm_render_resolution;
+  if (!is_pow2(res) || res > 8192 || light->rsm_read_resolution < 1 || res % light->rsm_read_resolution != 0)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "render_rsm: render resolution must be a power of two <= 8192");
+  const size_t chain = rsm_chain_texels(res);
+  int rc;
+  if ((rc = ensure_raster(ctx, static_cast<size_t>(res) * res)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->rsm_flux, &ctx->rsm_cap, 3 * chain)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->rsm_normal, &ctx->rsm_normal_cap, 2 * chain)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->rsm_dls, &ctx->rsm_dls_cap, 2 * chain)) != DRV_OK) return rc;
+  drvk::launch_render_rsm(scene->positions, scene->albedo, scene->normals, scene->n_tri, *light, ctx->rz, ctx->rtri,
+                          ctx->rsm_flux, ctx->rsm_normal, ctx->rsm_dls, ctx->stream);
+  ctx->rsm_res = res;
+  // Level l starts after the texels of levels 0..l-1.
+  const int read_lod = log2i(res / light->rsm_read_resolution);
+  const int shadow_lod = std::min(read_lod + static_cast<int>(light->indirect_shadow_lod), log2i(res));
+  const size_t o_read = rsm_chain_texels(res) - rsm_chain_texels(res >> read_lod);
+  const size_t o_shadow = rsm_chain_texels(res) - rsm_chain_texels(res >> shadow_lod);
+  out->flux = ctx->rsm_flux + 3 * o_read;
+  out->normal = ctx->rsm_normal + 2 * o_read;
+  out->depth_lin_sq = ctx->rsm_dls + 2 * o_read;
+  out->depth_lin_sq_shadow = ctx->rsm_dls + 2 * o_shadow;
+  out->read_resolution = light->rsm_read_resolution;
+  return check_hip(ctx, hipGetLastError(), "render_rsm");
+}
+
+int drv_read_gbuffer(drv_ctx* ctx, float* depth, int16_t* normal, uint8_t* albedo) {
+  if (!ctx || !depth || !normal || !albedo) return DRV_ERR_INVALID_ARG;
+  if (!ctx->g_depth) return fail(ctx, DRV_ERR_STATE, "read_gbuffer: no G-buffer rendered yet");
+  const size_t n = static_cast<size_t>(ctx->cfg.width) * ctx->cfg.height;
+  DRV_HIP(ctx, hipMemcpyAsync(depth, ctx->g_depth, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipMemcpyAsync(normal, ctx->g_normal, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipMemcpyAsync(albedo, ctx->g_albedo, 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+int drv_read_rsm(drv_ctx* ctx, int32_t level, uint16_t* flux, int16_t* normal, uint16_t* depth_lin_sq) {
+  if (!ctx || !flux || !normal || !depth_lin_sq) return DRV_ERR_INVALID_ARG;
+  if (ctx->rsm_res == 0) return fail(ctx, DRV_ERR_STATE, "read_rsm: no RSM rendered yet");
+  if (level < 0 || (ctx->rsm_res >> level) < 1) return fail(ctx, DRV_ERR_INVALID_ARG, "read_rsm: bad level");
+  const size_t off = rsm_chain_texels(ctx->rsm_res) - rsm_chain_texels(ctx->rsm_res >> level);
+  const size_t n = static_cast<size_t>(ctx->rsm_res >> level) * static_cast<size_t>(ctx->rsm_res >> level);
+  DRV_HIP(ctx, hipMemcpyAsync(flux, ctx->rsm_flux + 3 * off, 6 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipMemcpyAsync(normal, ctx->rsm_normal + 2 * off, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipMemcpyAsync(depth_lin_sq, ctx->rsm_dls + 2 * off, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return DRV_OK;
 }
 

@@ -175,6 +175,13 @@ void launch_spot_select(const Counter* counter, uint32_t k, const float* caches,
 void launch_spot_error(const float* spot, uint32_t spot_cap, int stride, int sh_bands, const Counter* spot_ctr,
                        float threshold, const Counter* counter, uint32_t* stats, Counter* gate, hipStream_t s);
 
+void launch_render_gbuffer(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
+                           const float* view_proj, int w, int h, unsigned long long* zbits, uint32_t* tri,
+                           float* depth, int16_t* normal, uint8_t* albedo_srgb, hipStream_t s);
+void launch_render_rsm(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
+                       const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint16_t* flux,
+                       int16_t* normal, uint16_t* dls, hipStream_t s);
+
 void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const drv_sh_constants& shc,
                   const drv_gbuffer& gb, const uint32_t* cav, const float* caches, const Counter* counter,
                   const float* srgb_lut, float* out_rgb, int accumulate, hipStream_t s);

@@ -48,6 +48,8 @@ class DeviceFrame:
         dev = torch.device(device)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
         self.positions = t(fi.positions)
+        self.albedo = t(fi.albedo)
+        self.normals = t(fi.normals)
         self.depth = t(fi.depth)
         self.gnormal = t(fi.gnormal)
         self.galbedo = t(fi.galbedo)
@@ -62,6 +64,10 @@ class DeviceFrame:
         w = self.inputs.workload
         return _abi.DrvGBuffer(self.depth.data_ptr(), self.gnormal.data_ptr(), self.galbedo.data_ptr(), w.width,
                                w.height)
+
+    def scene(self) -> _abi.DrvScene:
+        return _abi.DrvScene(self.positions.data_ptr(), self.albedo.data_ptr(), self.normals.data_ptr(),
+                             len(self.inputs.positions))
 
     def rsm(self) -> _abi.DrvRsm:
         return _abi.DrvRsm(self.flux.data_ptr(), self.rnormal.data_ptr(), self.dls.data_ptr(),
@@ -154,6 +160,38 @@ class Renderer:
         self.allocate_caches(df.gbuffer())
         self.light_caches_rsm(df.rsm(), fi.light)
         self.apply_caches(df.gbuffer(), df.out, accumulate)
+
+    # --- inputs rendered on the device ------------------------------------------
+    def render_gbuffer(self, scene: _abi.DrvScene, view_proj) -> _abi.DrvGBuffer:
+        out = _abi.DrvGBuffer()
+        self._check(self.lib.drv_render_gbuffer(self.ctx, C.byref(scene), _abi.fvec(view_proj, 16), C.byref(out)),
+                    "drv_render_gbuffer")
+        return out
+
+    def render_rsm(self, scene: _abi.DrvScene, light: _abi.DrvSpotLight) -> _abi.DrvRsm:
+        out = _abi.DrvRsm()
+        self._check(self.lib.drv_render_rsm(self.ctx, C.byref(scene), C.byref(light), C.byref(out)),
+                    "drv_render_rsm")
+        return out
+
+    def read_rsm(self, level: int, render_resolution: int):
+        r = render_resolution >> level
+        flux = np.empty((r, r, 3), np.uint16)
+        nrm = np.empty((r, r, 2), np.int16)
+        dls = np.empty((r, r, 2), np.uint16)
+        self._check(self.lib.drv_read_rsm(self.ctx, level, flux.ctypes.data, nrm.ctypes.data, dls.ctypes.data),
+                    "drv_read_rsm")
+        return flux, nrm, dls
+
+    def read_gbuffer(self):
+        """Host copies (depth, normal, albedo) of the last rendered G-buffer."""
+        w, h = self.cfg.width, self.cfg.height
+        depth = np.empty((h, w), np.float32)
+        nrm = np.empty((h, w, 2), np.int16)
+        alb = np.empty((h, w, 3), np.uint8)
+        self._check(self.lib.drv_read_gbuffer(self.ctx, depth.ctypes.data, nrm.ctypes.data, alb.ctypes.data),
+                    "drv_read_gbuffer")
+        return depth, nrm, alb
 
     # --- read-back -----------------------------------------------------------
     def synchronize(self):

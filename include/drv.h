@@ -239,6 +239,34 @@ int drv_read_voxels(drv_ctx* ctx, int32_t level, uint8_t* host_out);
  * drv_light_caches call, float[live * n_blocks], cache-major. */
 int drv_read_visibility(drv_ctx* ctx, float* host_out, uint64_t count);
 
+/* ---- inputs rendered on the device ------------------------------------ */
+
+/* Scene geometry for the device-side raster passes: a triangle soup on the
+ * device, positions[n_tri][9] (world space), albedo[n_tri][3] (linear) and
+ * normals[n_tri][3] per triangle. */
+typedef struct drv_scene {
+  const float* positions;
+  const float* albedo;
+  const float* normals;
+  uint32_t n_tri;
+} drv_scene;
+
+/* G-buffer fill (shader/fillgbuffer.frag; Renderer::Draw's G-buffer pass,
+ * rendering/renderer.cpp:539-560) into library-owned buffers of the
+ * configured width x height; *out receives their device pointers. */
+int drv_render_gbuffer(drv_ctx* ctx, const drv_scene* scene, const float view_projection[16], drv_gbuffer* out);
+/* RSM of one spot light (Renderer::PrepareRSM, rendering/renderer.cpp:1300-
+ * 1339: shader/fillrsm.frag at the render resolution, then the
+ * shader/downsamplersm.frag mip chain) into library-owned buffers; *out
+ * receives the read level (rsm_read_lod) and the indirect-shadow level, as
+ * drv_light_caches expects them. */
+int drv_render_rsm(drv_ctx* ctx, const drv_scene* scene, const drv_spot_light* light, drv_rsm* out);
+/* Host copies of the last rendered G-buffer (width x height texels). */
+int drv_read_gbuffer(drv_ctx* ctx, float* depth, int16_t* normal, uint8_t* albedo);
+/* Host copy of mip `level` of the last rendered RSM (flux RGB16F,
+ * normal RG16I, depth/depth^2 RG16F; (render resolution >> level)^2 texels). */
+int drv_read_rsm(drv_ctx* ctx, int32_t level, uint16_t* flux, int16_t* normal, uint16_t* depth_lin_sq);
+
 /* ---- surrogate visibility -------------------------------------------- */
 
 /* A 32-64-64-64-1 MLP (ReLU hidden layers, sigmoid output; BF16 operands,
