@@ -189,6 +189,7 @@ DRV_FUNCTIONS = {
     "drv_render_rsm": (C.c_int, [vp, P(DrvScene), P(DrvSpotLight), P(DrvRsm)]),
     "drv_read_rsm": (C.c_int, [vp, C.c_int32, vp, vp, vp]),
     "drv_read_gbuffer": (C.c_int, [vp, vp, vp, vp]),
+    "drv_bruteforce_rsm": (C.c_int, [vp, P(DrvGBuffer), vp, P(DrvRsm), P(DrvSpotLight), C.c_int32, vp, C.c_int32]),
     "drv_mlp_set_params": (C.c_int, [vp, vp, C.c_uint32]),
     "drv_mlp_get_params": (C.c_int, [vp, vp, C.c_uint32]),
     "drv_set_surrogate": (C.c_int, [vp, P(DrvSurrogate)]),

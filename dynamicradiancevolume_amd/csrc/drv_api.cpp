@@ -843,6 +843,35 @@ int drv_read_rsm(drv_ctx* ctx, int32_t level, uint16_t* flux, int16_t* normal, u
   return DRV_OK;
 }
 
+// ---- brute-force RSM reference ------------------------------------------------
+
+int drv_bruteforce_rsm(drv_ctx* ctx, const drv_gbuffer* gb, const uint8_t* roughness_metallic, const drv_rsm* rsm,
+                       const drv_spot_light* light, int32_t diffuse_only, float* out_rgb, int32_t accumulate) {
+  if (!ctx || !gb || !gb->depth || !gb->normal || !gb->albedo || !rsm || !rsm->flux || !rsm->normal ||
+      !rsm->depth_lin_sq || !light || !out_rgb)
+    return DRV_ERR_INVALID_ARG;
+  if (gb->width != ctx->cfg.width || gb->height != ctx->cfg.height)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "bruteforce_rsm: G-buffer size differs from the configured backbuffer");
+  if (!ctx->have_frame) return fail(ctx, DRV_ERR_STATE, "bruteforce_rsm: frame not set");
+  const int R = rsm->read_resolution;
+  if (!is_pow2(R) || R > 65536 || R != light->rsm_read_resolution)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "bruteforce_rsm: read resolution must be a power of two matching the light");
+  const uint32_t n = static_cast<uint32_t>(R) * static_cast<uint32_t>(R);
+  int rc;
+  if (n > ctx->vpl_cap) {
+    dev_free(ctx->vpls);
+    ctx->vpl_cap = 0;
+    if ((rc = dev_alloc(ctx, &ctx->vpls, n)) != DRV_OK) return rc;
+    ctx->vpl_cap = n;
+  }
+  // VAL records (position, flux, normal) of the read level; the VPL order
+  // does not matter to a plain sum.
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, ctx->stream);
+  drvk::launch_bruteforce(ctx->cfg, ctx->frame, *light, *gb, roughness_metallic, ctx->srgb_lut, ctx->vpls, n,
+                          diffuse_only, out_rgb, accumulate, ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "bruteforce_rsm");
+}
+
 // ---- surrogate visibility network -------------------------------------------
 
 namespace {

@@ -182,6 +182,11 @@ void launch_render_rsm(const float* positions, const float* albedo, const float*
                        const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint16_t* flux,
                        int16_t* normal, uint16_t* dls, hipStream_t s);
 
+void launch_bruteforce(const drv_config& cfg, const drv_frame& frame, const drv_spot_light& light,
+                       const drv_gbuffer& gb, const uint8_t* roughness_metallic, const float* srgb_lut,
+                       const Vpl* vals, uint32_t n_vals, int diffuse_only, float* out_rgb, int accumulate,
+                       hipStream_t s);
+
 void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const drv_sh_constants& shc,
                   const drv_gbuffer& gb, const uint32_t* cav, const float* caches, const Counter* counter,
                   const float* srgb_lut, float* out_rgb, int accumulate, hipStream_t s);

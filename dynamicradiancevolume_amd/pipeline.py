@@ -161,6 +161,15 @@ class Renderer:
         self.light_caches_rsm(df.rsm(), fi.light)
         self.apply_caches(df.gbuffer(), df.out, accumulate)
 
+    def bruteforce_rsm(self, gbuffer: _abi.DrvGBuffer, rsm: _abi.DrvRsm, light: _abi.DrvSpotLight,
+                       out: torch.Tensor, diffuse_only: bool = True, roughness_metallic: torch.Tensor | None = None,
+                       accumulate: bool = False):
+        """Renderer::ApplyRSMsBruteForce (renderer.cpp:851-874) for one light."""
+        rm = roughness_metallic.data_ptr() if roughness_metallic is not None else None
+        self._check(self.lib.drv_bruteforce_rsm(self.ctx, C.byref(gbuffer), rm, C.byref(rsm), C.byref(light),
+                                                int(diffuse_only), out.data_ptr(), int(accumulate)),
+                    "drv_bruteforce_rsm")
+
     # --- inputs rendered on the device ------------------------------------------
     def render_gbuffer(self, scene: _abi.DrvScene, view_proj) -> _abi.DrvGBuffer:
         out = _abi.DrvGBuffer()

@@ -267,6 +267,19 @@ int drv_read_gbuffer(drv_ctx* ctx, float* depth, int16_t* normal, uint8_t* albed
  * normal RG16I, depth/depth^2 RG16F; (render resolution >> level)^2 texels). */
 int drv_read_rsm(drv_ctx* ctx, int32_t level, uint16_t* flux, int16_t* normal, uint16_t* depth_lin_sq);
 
+/* ---- brute-force RSM reference ----------------------------------------- */
+
+/* Renderer::ApplyRSMsBruteForce (rendering/renderer.cpp:851-874,
+ * shader/bruteforcersm.frag): every pixel sums the light of every VAL of the
+ * RSM with the BRDF of shader/lightingfunctions.glsl:46-58, unshadowed - the
+ * reference's ground truth for the cache path.  roughness_metallic is the
+ * RG8 G-buffer texture (NULL: roughness 1, metallic 0); diffuse_only = 1
+ * keeps only the diffuse term (what the caches represent).  Background
+ * pixels (depth < 1e-5) receive 0. */
+int drv_bruteforce_rsm(drv_ctx* ctx, const drv_gbuffer* gbuffer, const uint8_t* roughness_metallic,
+                       const drv_rsm* rsm, const drv_spot_light* light, int32_t diffuse_only, float* out_rgb,
+                       int32_t accumulate);
+
 /* ---- surrogate visibility -------------------------------------------- */
 
 /* A 32-64-64-64-1 MLP (ReLU hidden layers, sigmoid output; BF16 operands,

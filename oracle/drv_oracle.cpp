@@ -960,6 +960,78 @@ void orc_gather_f64(const drv_config* cfg, const drv_volume* vol, const drv_sh_c
   }
 }
 
+void orc_bruteforce_rsm(const drv_config* cfg, const drv_frame* frame, const drv_spot_light* light,
+                        const orc_rsm* rsm, const float* depth, const int16_t* normal, const uint8_t* albedo,
+                        const uint8_t* rm, int32_t diffuse_only, const uint32_t* pixels, int64_t n, float* out,
+                        int32_t n_threads) {
+  const int W = cfg->width, H = cfg->height, R = rsm->read_resolution;
+  if (!pixels) n = static_cast<int64_t>(W) * H;
+  // The VALs of the read level (bruteforcersm.frag:49-53), shader order.
+  std::vector<Vpl<float>> vals(static_cast<size_t>(R) * R);
+  for (int x = 0; x < R; ++x)
+    for (int y = 0; y < R; ++y) {
+      const size_t i = static_cast<size_t>(y) * R + x;
+      Vpl<float>& v = vals[static_cast<size_t>(x) * R + y];
+      const float s = (static_cast<float>(x) + 0.5f) / static_cast<float>(R);
+      const float t = (static_cast<float>(y) + 0.5f) / static_cast<float>(R);
+      const float d = half_to_float(rsm->depth_lin_sq[2 * i]);
+      v.pos = rsm_position<float>(light, s, t, d);
+      v.flux = mk<float>(half_to_float(rsm->flux[3 * i]), half_to_float(rsm->flux[3 * i + 1]),
+                         half_to_float(rsm->flux[3 * i + 2]));
+      v.normal = unpack_normal16i<float>(rsm->normal[2 * i], rsm->normal[2 * i + 1]);
+    }
+  float lut[256];
+  for (int i = 0; i < 256; ++i) lut[i] = orc_srgb_to_linear(static_cast<uint8_t>(i));
+  const V3<float> lpos = mk<float>(light->light_position[0], light->light_position[1], light->light_position[2]);
+  const V3<float> cam = mk<float>(frame->camera_position[0], frame->camera_position[1], frame->camera_position[2]);
+  const int nt = resolve_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+  for (int64_t k = 0; k < n; ++k) {
+    const size_t p = pixels ? pixels[k] : static_cast<size_t>(k);
+    const int x = static_cast<int>(p % W), y = static_cast<int>(p / W);
+    float* o = out + 3 * k;
+    o[0] = o[1] = o[2] = 0.0f;
+    if (depth[p] < 0.00001f) continue;
+    float wpa[3];
+    unproject(frame->inverse_view_projection, W, H, x, y, depth[p], wpa);
+    const V3<float> wp = mk<float>(wpa[0], wpa[1], wpa[2]);
+    const V3<float> nrm = unpack_normal16i<float>(normal[2 * p], normal[2 * p + 1]);
+    const V3<float> to_cam = normalize(cam - wp);
+    const float base[3] = {lut[albedo[3 * p]], lut[albedo[3 * p + 1]], lut[albedo[3 * p + 2]]};
+    const float rough = rm ? static_cast<float>(rm[2 * p]) / 255.0f : 1.0f;
+    const float metal = rm ? static_cast<float>(rm[2 * p + 1]) / 255.0f : 0.0f;
+    // lightingfunctions.glsl:26-44
+    const float rsq = rough * rough;
+    const float blinn = 2.0f / (rsq * rsq + 0.0005f);
+    const float norm_f = (blinn + 8.0f) / (8.0f * kPi);
+    float diff[3], spec[3];
+    for (int c = 0; c < 3; ++c) {
+      diff[c] = base[c] * (1.0f - metal) + 0.02f * metal;     // mix(baseColor, 0.02, metallic)
+      spec[c] = 0.04f * (1.0f - metal) + base[c] * metal;     // mix(0.04, baseColor, metallic)
+    }
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (const Vpl<float>& v : vals) {
+      // bruteforcersm.frag:58-80
+      V3<float> to_val = v.pos - wp;
+      const float dsq = dot(to_val, to_val);
+      to_val = to_val * (1.0f / std::sqrt(dsq));
+      const float cos_theta = std::min(1.0f, std::max(0.0f, dot(to_val, nrm)));
+      const V3<float> vl = v.pos - lpos;
+      const float val_area = dot(vl, vl) * light->val_area_factor;
+      const float to_intensity = std::min(1.0f, std::max(0.0f, dot(v.normal, to_val * -1.0f)));
+      const float f = to_intensity * cos_theta / (dsq + val_area);
+      float bs = 0.0f;
+      if (!diffuse_only) {  // lightingfunctions.glsl:54-57
+        const V3<float> h = normalize(to_cam + to_val);
+        bs = norm_f * std::pow(std::min(1.0f, std::max(0.0f, dot(nrm, h))), blinn);
+      }
+      const float fl[3] = {v.flux.x, v.flux.y, v.flux.z};
+      for (int c = 0; c < 3; ++c) acc[c] += (diff[c] / kPi + bs * spec[c]) * (fl[c] * f);
+    }
+    for (int c = 0; c < 3; ++c) o[c] = acc[c];
+  }
+}
+
 void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* frame, const drv_sh_constants* shc,
                const float* depth, const int16_t* normal, const uint8_t* albedo, const uint32_t* cav,
                const float* entries, uint32_t live, float* out, int32_t n_threads) {

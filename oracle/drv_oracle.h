@@ -135,6 +135,17 @@ void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* fr
                const float* depth, const int16_t* normal, const uint8_t* albedo, const uint32_t* cav,
                const float* entries, uint32_t live_count, float* out_rgb, int32_t n_threads);
 
+/* ---- brute-force RSM reference ----------------------------------------- */
+/* bruteforcersm.frag:15-88 with BRDF of lightingfunctions.glsl:46-58 for the
+ * pixels listed in `pixels` (y * width + x; NULL = all), VALs visited in the
+ * shader's order (x outer, y inner); roughness_metallic (RG8) may be NULL
+ * (roughness 1, metallic 0); diffuse_only drops the Blinn-Phong term.
+ * Pixels with depth < 1e-5 get 0.  out_rgb[3 * n]. */
+void orc_bruteforce_rsm(const drv_config* cfg, const drv_frame* frame, const drv_spot_light* light,
+                        const orc_rsm* rsm, const float* depth, const int16_t* normal, const uint8_t* albedo,
+                        const uint8_t* roughness_metallic, int32_t diffuse_only, const uint32_t* pixels, int64_t n,
+                        float* out_rgb, int32_t n_threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,8 @@ _FUNCS = {
                                        P(C.c_int32)]),
     "orc_entry_sh": (None, [P(C.c_float), C.c_int32, P(C.c_float)]),
     "orc_shadow_blocks": (None, [P(_abi.DrvSpotLight), P(OrcRsm), vp]),
+    "orc_bruteforce_rsm": (None, [P(_abi.DrvConfig), P(_abi.DrvFrame), P(_abi.DrvSpotLight), P(OrcRsm), vp, vp, vp,
+                                  vp, C.c_int32, vp, C.c_int64, vp, C.c_int32]),
     "orc_mlp_features": (None, [P(_abi.DrvVolume), C.c_int32, vp, P(C.c_float), P(C.c_float), P(C.c_float),
                                 P(C.c_float)]),
     "orc_apply": (None, [P(_abi.DrvConfig), P(_abi.DrvVolume), P(_abi.DrvFrame), P(_abi.DrvShConstants), vp, vp, vp,
@@ -204,6 +206,21 @@ def mlp_features(vol, res: int, mips: np.ndarray, cam, positions: np.ndarray, bl
     for i, (c, b) in enumerate(np.asarray(pairs)):
         lib().orc_mlp_features(C.byref(vol), res, _ptr(mips), cam_a, pos[c].ctypes.data_as(fp),
                                blk[b].ctypes.data_as(fp), out[i].ctypes.data_as(fp))
+    return out
+
+
+def bruteforce_rsm(fi, diffuse_only=True, roughness_metallic=None, pixels=None, n_threads: int = 0):
+    """bruteforcersm.frag for the pixels listed (all when None): [n][3]."""
+    cfg = fi.workload.config()
+    rsm, keep = orc_rsm(fi)
+    px = np.ascontiguousarray(pixels, np.uint32) if pixels is not None else None
+    n = len(px) if px is not None else cfg.width * cfg.height
+    out = np.zeros((n, 3), np.float32)
+    rm = np.ascontiguousarray(roughness_metallic, np.uint8) if roughness_metallic is not None else None
+    lib().orc_bruteforce_rsm(C.byref(cfg), C.byref(fi.frame()), C.byref(fi.light), C.byref(rsm), _ptr(fi.depth),
+                             _ptr(fi.gnormal), _ptr(fi.galbedo), _ptr(rm) if rm is not None else None,
+                             int(diffuse_only), _ptr(px) if px is not None else None, n, _ptr(out), n_threads)
+    del keep
     return out
 
 

@@ -633,3 +633,82 @@ def test_apply_reads_unclaimed_cells_as_zero():
     entries[:count, 4:] = 1.0
     out = orc.apply(cfg, vol, fi.frame(), orc.sh_constants(), fi, np.zeros_like(cav), entries, count)
     assert not out.any()
+
+
+def _np_bruteforce(fi, pixels, diffuse_only, rm=None):
+    """bruteforcersm.frag restated with numpy in float64, vectorized over VALs."""
+    from oracle import orc
+
+    w = fi.workload
+    W, H = w.width, w.height
+    R = fi.read_resolution
+    flux, rn, dls = fi.read_level
+    lut = np.array([orc.lib().orc_srgb_to_linear(i) for i in range(256)])
+    light = fi.light
+    lp = np.array(light.light_position[:], np.float64)
+    ivl = np.array(light.inverse_light_view_projection[:], np.float64).reshape(4, 4)
+    xs, ys = np.meshgrid(np.arange(R), np.arange(R), indexing="ij")
+    s = (xs.ravel() + 0.5) / R
+    t = (ys.ravel() + 0.5) / R
+    clip = np.stack([s * 2 - 1, t * 2 - 1, np.zeros_like(s), np.ones_like(s)], 1) @ ivl.T
+    dirs = clip[:, :3] / clip[:, 3:4] - lp
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    f16 = lambda a: a.view(np.float16).astype(np.float64)  # noqa: E731
+    d = f16(dls[..., 0])[ys.ravel(), xs.ravel()]
+    vp = lp + dirs * d[:, None]
+    fl = f16(flux)[ys.ravel(), xs.ravel()]
+    ang = rn[..., 0][ys.ravel(), xs.ravel()] * (np.pi / 32768.0)
+    z = rn[..., 1][ys.ravel(), xs.ravel()] / 32768.0
+    sp = np.sqrt(1 - z * z)
+    vn = np.stack([np.cos(ang) * sp, np.sin(ang) * sp, z], 1)
+    vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+    ivp = np.array(fi.inv_view_proj, np.float64).reshape(4, 4)
+    cam = np.array(w.eye, np.float64)
+    out = []
+    for p in pixels:
+        x, y = p % W, p // W
+        dep = float(fi.depth[y, x])
+        if dep < 1e-5:
+            out.append(np.zeros(3))
+            continue
+        c = np.array([(x + 0.5) / W * 2 - 1, (y + 0.5) / H * 2 - 1, dep, 1.0]) @ ivp.T
+        wp = c[:3] / c[3]
+        a, b = fi.gnormal[y, x]
+        ph, zz = a * np.pi / 32768.0, b / 32768.0
+        n = np.array([np.cos(ph) * np.sqrt(1 - zz * zz), np.sin(ph) * np.sqrt(1 - zz * zz), zz])
+        n /= np.linalg.norm(n)
+        base = lut[fi.galbedo[y, x]]
+        rough, metal = (rm[y, x] / 255.0) if rm is not None else (1.0, 0.0)
+        tv = vp - wp
+        dsq = (tv * tv).sum(1)
+        tv /= np.sqrt(dsq)[:, None]
+        cos_t = np.clip(tv @ n, 0, 1)
+        area = ((vp - lp) ** 2).sum(1) * light.val_area_factor
+        fi_ = np.clip(-(vn * tv).sum(1), 0, 1) * cos_t / (dsq + area)
+        diff = base * (1 - metal) + 0.02 * metal
+        spec = 0.04 * (1 - metal) + base * metal
+        brdf = np.tile(diff / np.pi, (len(vp), 1))
+        if not diffuse_only:
+            blinn = 2 / (rough ** 4 + 0.0005)
+            tc = (cam - wp) / np.linalg.norm(cam - wp)
+            h = tc + tv
+            h /= np.linalg.norm(h, axis=1, keepdims=True)
+            brdf = brdf + ((blinn + 8) / (8 * np.pi) * np.clip(h @ n, 0, 1) ** blinn)[:, None] * spec
+        out.append((brdf * fl * fi_[:, None]).sum(0))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("diffuse_only", [True, False])
+def test_bruteforce_oracle_matches_numpy(diffuse_only):
+    from oracle import orc
+
+    fi = frame_inputs("cornell_256")
+    rng = np.random.default_rng(4)
+    W, H = fi.workload.width, fi.workload.height
+    rm = rng.integers(80, 256, (H, W, 2)).astype(np.uint8)
+    rm[..., 1] = rng.integers(0, 256, (H, W))
+    px = rng.choice(W * H, 40, replace=False).astype(np.uint32)
+    got = orc.bruteforce_rsm(fi, diffuse_only, rm if not diffuse_only else None, px)
+    ref = _np_bruteforce(fi, px, diffuse_only, rm if not diffuse_only else None)
+    assert np.abs(ref).sum() > 0
+    np.testing.assert_allclose(got, ref, rtol=2e-4, atol=1e-7 * np.abs(ref).max())
