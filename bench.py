@@ -220,6 +220,55 @@ def _batch(n_views: int, steps: int, stream):
     return out
 
 
+def _inputs_and_reference(fi, df, stream, w):
+    """The gather's inputs rendered on the device (G-buffer, RSM + mips) and
+    the brute-force RSM reference (unshadowed, diffuse) of the same frame,
+    with the end-to-end error of the unshadowed cache path against it."""
+    import numpy as np
+    import torch
+
+    from dynamicradiancevolume_amd.pipeline import Renderer
+
+    r = Renderer(w.with_(indirect_shadow=0).config(), stream)
+    scene = df.scene()
+
+    def timed(fn, k=5):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    gb_ms = timed(lambda: r.render_gbuffer(scene, fi.view_proj))
+    rsm_ms = timed(lambda: r.render_rsm(scene, fi.light))
+    r.render_frame(df)
+    stream.synchronize()  # torch's own work below runs on its default stream
+    cache_rgb = df.out.clone()
+    bf = torch.zeros_like(cache_rgb)
+    torch.cuda.synchronize()
+    bf_ms = timed(lambda: r.bruteforce_rsm(df.gbuffer(), df.rsm(), fi.light, bf, True), k=2)
+    stream.synchronize()
+    lit = torch.from_numpy(fi.depth >= 1e-5).to(bf.device)
+    a = cache_rgb[lit].double()
+    b = bf[lit].double()
+    err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    n_px = int(np.count_nonzero(fi.depth >= 1e-5))
+    r.close()
+    return {
+        "gbuffer_ms": gb_ms,
+        "rsm_ms": rsm_ms,
+        "triangles": int(len(fi.positions)),
+        "bruteforce_ms": bf_ms,
+        "bruteforce_pixel_val_interactions_per_s": w.width * w.height * fi.read_resolution ** 2 / (bf_ms * 1e-3),
+        "unshadowed_cache_path_vs_bruteforce_rel_l2": err,
+        "lit_pixels": n_px,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,7 +293,9 @@ def main() -> int:
 
     w = W.WORKLOADS[args.workload]
     print(f"[bench] building inputs for {w.name}", file=sys.stderr, flush=True)
+    t_in = time.perf_counter()
     fi = W.make_inputs(w, n_threads=min(16, os.cpu_count() or 1))
+    host_inputs_s = time.perf_counter() - t_in
     df = DeviceFrame(fi)
     stream = torch.cuda.Stream()
     r = Renderer(w.config(), stream)
@@ -312,6 +363,10 @@ def main() -> int:
         print("[bench] surrogate", file=sys.stderr, flush=True)
         sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold)
 
+    print("[bench] device inputs and brute-force reference", file=sys.stderr, flush=True)
+    ref = _inputs_and_reference(fi, df, stream, w)
+    ref["host_producer_s"] = host_inputs_s
+
     batch = None
     if args.batch_views > 0:
         print(f"[bench] batch of {args.batch_views} views", file=sys.stderr, flush=True)
@@ -356,6 +411,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "surrogate": sur,
         "batch": batch,
+        "inputs_and_reference": ref,
     }
     print(json.dumps(line))
     return 0

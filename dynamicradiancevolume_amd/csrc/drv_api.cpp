@@ -92,7 +92,8 @@ struct drv_ctx {
   // Device-rendered inputs: raster scratch, G-buffer, RSM mip chain.
   unsigned long long* rz = nullptr;
   uint32_t* rtri = nullptr;
-  size_t raster_cap = 0, raster_tri_cap = 0;
+  uint32_t* rbig = nullptr;  // large-triangle list: count + indices
+  size_t raster_cap = 0, raster_tri_cap = 0, rbig_cap = 0;
   float* g_depth = nullptr;
   int16_t* g_normal = nullptr;
   uint8_t* g_albedo = nullptr;
@@ -390,6 +391,7 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->spot_stats);
   dev_free(ctx->rz);
   dev_free(ctx->rtri);
+  dev_free(ctx->rbig);
   dev_free(ctx->g_depth);
   dev_free(ctx->g_normal);
   dev_free(ctx->g_albedo);
@@ -759,9 +761,10 @@ size_t rsm_chain_texels(int res) {
   return n;
 }
 
-int ensure_raster(drv_ctx* ctx, size_t pixels) {
+int ensure_raster(drv_ctx* ctx, size_t pixels, size_t n_tri) {
   int rc;
   if ((rc = ensure(ctx, &ctx->rz, &ctx->raster_cap, pixels)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->rbig, &ctx->rbig_cap, n_tri + 1)) != DRV_OK) return rc;
   return ensure(ctx, &ctx->rtri, &ctx->raster_tri_cap, pixels);
 }
 
@@ -776,14 +779,14 @@ int drv_render_gbuffer(drv_ctx* ctx, const drv_scene* scene, const float view_pr
   const int w = ctx->cfg.width, h = ctx->cfg.height;
   const size_t n = static_cast<size_t>(w) * h;
   int rc;
-  if ((rc = ensure_raster(ctx, n)) != DRV_OK) return rc;
+  if ((rc = ensure_raster(ctx, n, scene->n_tri)) != DRV_OK) return rc;
   if (!ctx->g_depth) {
     if ((rc = dev_alloc(ctx, &ctx->g_depth, n)) != DRV_OK) return rc;
     if ((rc = dev_alloc(ctx, &ctx->g_normal, 2 * n)) != DRV_OK) return rc;
     if ((rc = dev_alloc(ctx, &ctx->g_albedo, 3 * n)) != DRV_OK) return rc;
   }
   drvk::launch_render_gbuffer(scene->positions, scene->albedo, scene->normals, scene->n_tri, view_projection, w, h,
-                              ctx->rz, ctx->rtri, ctx->g_depth, ctx->g_normal, ctx->g_albedo, ctx->stream);
+                              ctx->rz, ctx->rtri, ctx->rbig, ctx->g_depth, ctx->g_normal, ctx->g_albedo, ctx->stream);
   out->depth = ctx->g_depth;
   out->normal = ctx->g_normal;
   out->albedo = ctx->g_albedo;
@@ -799,12 +802,12 @@ int drv_render_rsm(drv_ctx* ctx, const drv_scene* scene, const drv_spot_light* l
     return fail(ctx, DRV_ERR_INVALID_ARG, "render_rsm: render resolution must be a power of two <= 8192");
   const size_t chain = rsm_chain_texels(res);
   int rc;
-  if ((rc = ensure_raster(ctx, static_cast<size_t>(res) * res)) != DRV_OK) return rc;
+  if ((rc = ensure_raster(ctx, static_cast<size_t>(res) * res, scene->n_tri)) != DRV_OK) return rc;
   if ((rc = ensure(ctx, &ctx->rsm_flux, &ctx->rsm_cap, 3 * chain)) != DRV_OK) return rc;
   if ((rc = ensure(ctx, &ctx->rsm_normal, &ctx->rsm_normal_cap, 2 * chain)) != DRV_OK) return rc;
   if ((rc = ensure(ctx, &ctx->rsm_dls, &ctx->rsm_dls_cap, 2 * chain)) != DRV_OK) return rc;
   drvk::launch_render_rsm(scene->positions, scene->albedo, scene->normals, scene->n_tri, *light, ctx->rz, ctx->rtri,
-                          ctx->rsm_flux, ctx->rsm_normal, ctx->rsm_dls, ctx->stream);
+                          ctx->rbig, ctx->rsm_flux, ctx->rsm_normal, ctx->rsm_dls, ctx->stream);
   ctx->rsm_res = res;
   // Level l starts after the texels of levels 0..l-1.
   const int read_lod = log2i(res / light->rsm_read_resolution);

@@ -177,10 +177,11 @@ void launch_spot_error(const float* spot, uint32_t spot_cap, int stride, int sh_
 
 void launch_render_gbuffer(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
                            const float* view_proj, int w, int h, unsigned long long* zbits, uint32_t* tri,
-                           float* depth, int16_t* normal, uint8_t* albedo_srgb, hipStream_t s);
+                           uint32_t* big, float* depth, int16_t* normal, uint8_t* albedo_srgb, hipStream_t s);
+// big: 1 + n_tri uint32 (count, then the list of large triangles)
 void launch_render_rsm(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
-                       const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint16_t* flux,
-                       int16_t* normal, uint16_t* dls, hipStream_t s);
+                       const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint32_t* big,
+                       uint16_t* flux, int16_t* normal, uint16_t* dls, hipStream_t s);
 
 void launch_bruteforce(const drv_config& cfg, const drv_frame& frame, const drv_spot_light& light,
                        const drv_gbuffer& gb, const uint8_t* roughness_metallic, const float* srgb_lut,

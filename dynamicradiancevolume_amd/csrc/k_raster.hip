@@ -12,7 +12,8 @@
 // 1 takes the 64-bit atomic max of the depth bits (non-negative doubles
 // order like their bit patterns), pass 2 the atomic min of the index among
 // the fragments that reached that depth; a resolve pass then shades each
-// pixel from its winning triangle.  One thread rasterizes one triangle.
+// pixel from its winning triangle.  One thread rasterizes a small triangle;
+// triangles with a large pixel box are listed and rasterized by a workgroup.
 #include <algorithm>
 
 #include "drv_internal.hpp"
@@ -80,38 +81,95 @@ struct RasterArgs {
   int w, h;
   unsigned long long* zbits;  // [h][w] depth bits (0 = empty)
   uint32_t* tri;              // [h][w] winning triangle (0xffffffff = none)
+  uint32_t* big;              // triangles whose screen box exceeds kBigPixels
+  uint32_t* n_big;
 };
 
-// Visits every kept fragment of triangle t: f(pixel index, depth).
+// Triangles covering more pixel-box area than this are rasterized by a whole
+// workgroup instead of one thread (a ground plane's few large triangles would
+// otherwise serialize the pass).
+constexpr uint32_t kBigPixels = 512;
+
+// Screen-space setup of one clipped triangle piece: pixel box and edge data.
+struct Setup {
+  ScreenTri q;
+  int px0, py0, bw, bh;
+  double inv;
+  bool ok;
+};
+
+__device__ __forceinline__ Setup setup_piece(const ScreenTri& q, int w, int h) {
+  Setup st;
+  st.q = q;
+  st.ok = false;
+  const double miny = fmin(fmin(q.y[0], q.y[1]), q.y[2]), maxy = fmax(fmax(q.y[0], q.y[1]), q.y[2]);
+  const double minx = fmin(fmin(q.x[0], q.x[1]), q.x[2]), maxx = fmax(fmax(q.x[0], q.x[1]), q.x[2]);
+  // Bounds in double first so that off-screen triangles cannot overflow int.
+  const double fy0 = fmax(0.0, ceil(miny - 0.5)), fy1 = fmin(static_cast<double>(h - 1), floor(maxy - 0.5));
+  const double fx0 = fmax(0.0, ceil(minx - 0.5)), fx1 = fmin(static_cast<double>(w - 1), floor(maxx - 0.5));
+  if (!(fy0 <= fy1) || !(fx0 <= fx1)) return st;
+  const double area = (q.x[1] - q.x[0]) * (q.y[2] - q.y[0]) - (q.x[2] - q.x[0]) * (q.y[1] - q.y[0]);
+  if (area == 0.0) return st;
+  st.inv = 1.0 / area;
+  st.px0 = static_cast<int>(fx0);
+  st.py0 = static_cast<int>(fy0);
+  st.bw = static_cast<int>(fx1) - st.px0 + 1;
+  st.bh = static_cast<int>(fy1) - st.py0 + 1;
+  st.ok = true;
+  return st;
+}
+
+// Visits pixels start, start + step, ... of the piece's box (row-major) and
+// calls f(pixel index, depth) for every kept fragment.
 template <typename F>
-__device__ __forceinline__ void raster_triangle(const RasterArgs& a, uint32_t t, F&& f) {
-  ScreenTri st[2];
-  const int m = project_clip(a.positions + 9 * static_cast<size_t>(t), a.vp, a.w, a.h, st);
+__device__ __forceinline__ void raster_piece(const Setup& st, int w, uint32_t start, uint32_t step, F&& f) {
+  const ScreenTri& q = st.q;
+  const uint32_t npix = static_cast<uint32_t>(st.bw) * static_cast<uint32_t>(st.bh);
+  for (uint32_t k = start; k < npix; k += step) {
+    const int px = st.px0 + static_cast<int>(k % static_cast<uint32_t>(st.bw));
+    const int py = st.py0 + static_cast<int>(k / static_cast<uint32_t>(st.bw));
+    const double cy = py + 0.5, cx = px + 0.5;
+    const double e0 = ((q.x[2] - q.x[1]) * (cy - q.y[1]) - (q.y[2] - q.y[1]) * (cx - q.x[1])) * st.inv;
+    const double e1 = ((q.x[0] - q.x[2]) * (cy - q.y[2]) - (q.y[0] - q.y[2]) * (cx - q.x[2])) * st.inv;
+    const double e2 = 1.0 - e0 - e1;
+    if (e0 < 0 || e1 < 0 || e2 < 0) continue;
+    const double z = e0 * q.z[0] + e1 * q.z[1] + e2 * q.z[2];
+    if (!(z > 0.0) || z > 1.0) continue;  // a fragment must beat the cleared depth 0
+    f(static_cast<size_t>(py) * w + px, z);
+  }
+}
+
+// Small triangles: one thread each.  Large ones are listed (first pass) or
+// skipped (second pass) for the workgroup kernel.
+template <typename F>
+__device__ __forceinline__ void raster_small(const RasterArgs& a, uint32_t t, bool list_big, F&& f) {
+  ScreenTri sc[2];
+  const int m = project_clip(a.positions + 9 * static_cast<size_t>(t), a.vp, a.w, a.h, sc);
+  Setup st[2];
+  bool big = false;
   for (int s = 0; s < m; ++s) {
-    const ScreenTri& q = st[s];
-    const double miny = fmin(fmin(q.y[0], q.y[1]), q.y[2]), maxy = fmax(fmax(q.y[0], q.y[1]), q.y[2]);
-    const double minx = fmin(fmin(q.x[0], q.x[1]), q.x[2]), maxx = fmax(fmax(q.x[0], q.x[1]), q.x[2]);
-    // Bounds in double first so that off-screen triangles cannot overflow int.
-    const double fy0 = fmax(0.0, ceil(miny - 0.5)), fy1 = fmin(static_cast<double>(a.h - 1), floor(maxy - 0.5));
-    const double fx0 = fmax(0.0, ceil(minx - 0.5)), fx1 = fmin(static_cast<double>(a.w - 1), floor(maxx - 0.5));
-    if (!(fy0 <= fy1) || !(fx0 <= fx1)) continue;
-    const int py0 = static_cast<int>(fy0), py1 = static_cast<int>(fy1);
-    const int px0 = static_cast<int>(fx0), px1 = static_cast<int>(fx1);
-    const double area = (q.x[1] - q.x[0]) * (q.y[2] - q.y[0]) - (q.x[2] - q.x[0]) * (q.y[1] - q.y[0]);
-    if (area == 0.0) continue;
-    const double inv = 1.0 / area;
-    for (int py = py0; py <= py1; ++py) {
-      const double cy = py + 0.5;
-      for (int px = px0; px <= px1; ++px) {
-        const double cx = px + 0.5;
-        const double e0 = ((q.x[2] - q.x[1]) * (cy - q.y[1]) - (q.y[2] - q.y[1]) * (cx - q.x[1])) * inv;
-        const double e1 = ((q.x[0] - q.x[2]) * (cy - q.y[2]) - (q.y[0] - q.y[2]) * (cx - q.x[2])) * inv;
-        const double e2 = 1.0 - e0 - e1;
-        if (e0 < 0 || e1 < 0 || e2 < 0) continue;
-        const double z = e0 * q.z[0] + e1 * q.z[1] + e2 * q.z[2];
-        if (!(z > 0.0) || z > 1.0) continue;  // a fragment must beat the cleared depth 0
-        f(static_cast<size_t>(py) * a.w + px, z);
-      }
+    st[s] = setup_piece(sc[s], a.w, a.h);
+    if (st[s].ok && static_cast<uint32_t>(st[s].bw) * static_cast<uint32_t>(st[s].bh) > kBigPixels) big = true;
+  }
+  if (big) {
+    if (list_big) a.big[atomicAdd(a.n_big, 1u)] = t;
+    return;
+  }
+  for (int s = 0; s < m; ++s)
+    if (st[s].ok) raster_piece(st[s], a.w, 0u, 1u, f);
+}
+
+// Large triangles: workgroup-strided pixel loops, persistent over the list.
+template <typename F>
+__device__ __forceinline__ void raster_big(const RasterArgs& a, F&& f) {
+  const uint32_t n = *a.n_big;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t t = a.big[i];
+    ScreenTri sc[2];
+    const int m = project_clip(a.positions + 9 * static_cast<size_t>(t), a.vp, a.w, a.h, sc);
+    for (int s = 0; s < m; ++s) {
+      const Setup st = setup_piece(sc[s], a.w, a.h);
+      if (st.ok) raster_piece(st, a.w, threadIdx.x, blockDim.x, [&](size_t px, double z) { f(t, px, z); });
     }
   }
 }
@@ -119,7 +177,13 @@ __device__ __forceinline__ void raster_triangle(const RasterArgs& a, uint32_t t,
 __global__ __launch_bounds__(256) void k_raster_depth(RasterArgs a) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n_tri) return;
-  raster_triangle(a, t, [&](size_t i, double z) {
+  raster_small(a, t, true, [&](size_t i, double z) {
+    atomicMax(&a.zbits[i], static_cast<unsigned long long>(__double_as_longlong(z)));
+  });
+}
+
+__global__ __launch_bounds__(256) void k_raster_depth_big(RasterArgs a) {
+  raster_big(a, [&](uint32_t, size_t i, double z) {
     atomicMax(&a.zbits[i], static_cast<unsigned long long>(__double_as_longlong(z)));
   });
 }
@@ -127,7 +191,13 @@ __global__ __launch_bounds__(256) void k_raster_depth(RasterArgs a) {
 __global__ __launch_bounds__(256) void k_raster_id(RasterArgs a) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n_tri) return;
-  raster_triangle(a, t, [&](size_t i, double z) {
+  raster_small(a, t, false, [&](size_t i, double z) {
+    if (static_cast<unsigned long long>(__double_as_longlong(z)) == a.zbits[i]) atomicMin(&a.tri[i], t);
+  });
+}
+
+__global__ __launch_bounds__(256) void k_raster_id_big(RasterArgs a) {
+  raster_big(a, [&](uint32_t t, size_t i, double z) {
     if (static_cast<unsigned long long>(__double_as_longlong(z)) == a.zbits[i]) atomicMin(&a.tri[i], t);
   });
 }
@@ -292,7 +362,7 @@ __global__ __launch_bounds__(256) void k_rsm_downsample(int res, const uint16_t*
 }
 
 void raster(const float* positions, uint32_t n_tri, const float* vp, int w, int h, unsigned long long* zbits,
-            uint32_t* tri, hipStream_t s) {
+            uint32_t* tri, uint32_t* big, hipStream_t s) {
   RasterArgs a;
   a.positions = positions;
   a.n_tri = n_tri;
@@ -301,21 +371,26 @@ void raster(const float* positions, uint32_t n_tri, const float* vp, int w, int 
   a.h = h;
   a.zbits = zbits;
   a.tri = tri;
+  a.n_big = big;      // big[0] counts, the list follows
+  a.big = big + 1;
   const size_t n = static_cast<size_t>(w) * h;
   launch_fill_u32(reinterpret_cast<uint32_t*>(zbits), 2 * n, 0u, s);
   launch_fill_u32(tri, n, 0xffffffffu, s);
+  launch_fill_u32(big, 1, 0u, s);
   if (n_tri == 0) return;
   const dim3 grid((n_tri + 255) / 256);
   hipLaunchKernelGGL(k_raster_depth, grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_raster_depth_big, dim3(1024), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_raster_id, grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_raster_id_big, dim3(1024), dim3(256), 0, s, a);
 }
 
 }  // namespace
 
 void launch_render_gbuffer(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
                            const float* view_proj, int w, int h, unsigned long long* zbits, uint32_t* tri,
-                           float* depth, int16_t* normal, uint8_t* albedo_srgb, hipStream_t s) {
-  raster(positions, n_tri, view_proj, w, h, zbits, tri, s);
+                           uint32_t* big, float* depth, int16_t* normal, uint8_t* albedo_srgb, hipStream_t s) {
+  raster(positions, n_tri, view_proj, w, h, zbits, tri, big, s);
   GbufArgs a;
   a.w = w;
   a.h = h;
@@ -331,10 +406,10 @@ void launch_render_gbuffer(const float* positions, const float* albedo, const fl
 }
 
 void launch_render_rsm(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
-                       const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint16_t* flux,
-                       int16_t* normal, uint16_t* dls, hipStream_t s) {
+                       const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint32_t* big,
+                       uint16_t* flux, int16_t* normal, uint16_t* dls, hipStream_t s) {
   const int res = light.rsm_render_resolution;
-  raster(positions, n_tri, light.light_view_projection, res, res, zbits, tri, s);
+  raster(positions, n_tri, light.light_view_projection, res, res, zbits, tri, big, s);
   RsmArgs a;
   a.res = res;
   a.zbits = zbits;
