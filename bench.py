@@ -309,6 +309,24 @@ def main() -> int:
 
     ms, wall = _time_frames(r, df, stream, args.steps)
     demanded, live = r.cache_count()
+    # The same frame captured once into a HIP graph and replayed (no host
+    # launch overhead; a static camera, or a graph updated per frame).
+    from dynamicradiancevolume_amd.pipeline import FrameGraph
+
+    g = FrameGraph(r)
+    with torch.cuda.stream(stream):
+        g.begin()
+        r.render_frame(df)
+        g.end()
+        g.launch()
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            g.launch()
+        e1.record(stream)
+        e1.synchronize()
+    graph_ms = e0.elapsed_time(e1) / args.steps
     interactions = live * n_vpl
     print(f"[bench] timed {args.steps} frames: {ms:.3f} ms/frame (wall {wall:.2f} s), caches {live}",
           file=sys.stderr, flush=True)
@@ -406,6 +424,7 @@ def main() -> int:
             "cone_steps_executed": steps_exec,
             "stage_ms": stages,
             "path": "plain cone-traced",
+            "graph_replay_ms_per_frame": graph_ms,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -20,6 +20,11 @@ using drvk::Counter;
 struct drv_ctx {
   drv_config cfg{};
   hipStream_t stream = nullptr;
+  // Stage B (voxelization) runs on its own stream, forked from and joined
+  // back into `stream`, so it overlaps stage A (allocation).
+  hipStream_t aux = nullptr;
+  hipEvent_t fork_ev = nullptr, vox_done_ev = nullptr;
+  bool vox_pending = false;
   int device = 0;
   int cu_count = 256;
   std::string err;
@@ -169,8 +174,17 @@ void light_matrices(const drv_spot_params& p, float* vp_out, float* ivp_out) {
   drvm::to_float(ivp, ivp_out);
 }
 
-void record(drv_ctx* ctx, int i) {
-  if (ctx->profiling && hipEventRecord(ctx->ev[i], ctx->stream) == hipSuccess) ctx->recorded |= 1u << i;
+void record(drv_ctx* ctx, int i, hipStream_t s) {
+  if (ctx->profiling && hipEventRecord(ctx->ev[i], s) == hipSuccess) ctx->recorded |= 1u << i;
+}
+
+void record(drv_ctx* ctx, int i) { record(ctx, i, ctx->stream); }
+
+// Makes `stream` wait for a voxelization enqueued on the side stream.
+void join_voxels(drv_ctx* ctx) {
+  if (!ctx->vox_pending) return;
+  (void)hipStreamWaitEvent(ctx->stream, ctx->vox_done_ev, 0);
+  ctx->vox_pending = false;
 }
 
 // Time between two recorded events; 0 for a pair that was never recorded.
@@ -352,6 +366,10 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->vox_corners), L.corner_count * 2, 0u, nullptr);
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(DRV_ERR_HIP);
+  if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->vox_done_ev, hipEventDisableTiming) != hipSuccess)
+    return bail(DRV_ERR_HIP);
   if (hipDeviceSynchronize() != hipSuccess) return bail(DRV_ERR_HIP);
   *out_ctx = ctx;
   return DRV_OK;
@@ -361,6 +379,12 @@ int drv_destroy(drv_ctx* ctx) {
   if (!ctx) return DRV_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
+  if (ctx->aux) {
+    (void)hipStreamSynchronize(ctx->aux);
+    (void)hipStreamDestroy(ctx->aux);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->vox_done_ev) (void)hipEventDestroy(ctx->vox_done_ev);
   dev_free(ctx->caches);
   dev_free(ctx->counter);
   dev_free(ctx->cav);
@@ -415,6 +439,7 @@ int drv_set_stream(drv_ctx* ctx, void* hip_stream) {
 
 int drv_synchronize(drv_ctx* ctx) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
+  join_voxels(ctx);
   DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return DRV_OK;
 }
@@ -523,9 +548,14 @@ int drv_prepare_spot_light(const drv_spot_params* p, drv_spot_light* o) {
 
 int drv_voxelize_begin(drv_ctx* ctx) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
-  record(ctx, 0);
+  // Fork: the side stream starts after everything already on `stream` (the
+  // previous frame still reads the voxels).
+  DRV_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+  DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->fork_ev, 0));
+  record(ctx, 0, ctx->aux);
   const size_t res3 = static_cast<size_t>(ctx->vox.res) * ctx->vox.res * ctx->vox.res;
-  drvk::launch_fill_u8(ctx->vox_target, res3, 0, ctx->stream);  // voxelization.cpp:105
+  drvk::launch_fill_u8(ctx->vox_target, res3, 0, ctx->aux);  // voxelization.cpp:105
+  ctx->vox_pending = true;
   return check_hip(ctx, hipGetLastError(), "voxelize_begin");
 }
 
@@ -533,21 +563,25 @@ int drv_voxelize_mesh(drv_ctx* ctx, const float* positions, const uint32_t* indi
                       const float world[16]) {
   if (!ctx || (!positions && n_tri) || !world) return DRV_ERR_INVALID_ARG;
   if (!ctx->have_volume) return fail(ctx, DRV_ERR_STATE, "voxelize: volume not set");
-  drvk::launch_voxelize(ctx->vol, ctx->vox.res, positions, indices, n_tri, world, ctx->vox_target, ctx->stream);
+  if (!ctx->vox_pending) return fail(ctx, DRV_ERR_STATE, "voxelize_mesh: drv_voxelize_begin first");
+  drvk::launch_voxelize(ctx->vol, ctx->vox.res, positions, indices, n_tri, world, ctx->vox_target, ctx->aux);
   return check_hip(ctx, hipGetLastError(), "voxelize_mesh");
 }
 
 int drv_voxelize_end(drv_ctx* ctx, float adaption_factor) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
-  record(ctx, 1);
+  if (!ctx->vox_pending) return fail(ctx, DRV_ERR_STATE, "voxelize_end: drv_voxelize_begin first");
+  record(ctx, 1, ctx->aux);
   // voxelization.cpp:100: nothing is blended or mipmapped when the floored
   // adaption is zero.
   if (adaption_factor > 0.0f) {
-    drvk::launch_voxel_blend(ctx->vox.res, ctx->vox_target, ctx->vox_mips, adaption_factor, ctx->stream);
-    drvk::launch_voxel_mips(ctx->vox, ctx->vox_mips, ctx->stream);
-    drvk::launch_voxel_corners(ctx->vox, ctx->vox_mips, ctx->vox_corners, ctx->stream);
+    drvk::launch_voxel_blend(ctx->vox.res, ctx->vox_target, ctx->vox_mips, adaption_factor, ctx->aux);
+    drvk::launch_voxel_mips(ctx->vox, ctx->vox_mips, ctx->aux);
+    drvk::launch_voxel_corners(ctx->vox, ctx->vox_mips, ctx->vox_corners, ctx->aux);
   }
-  record(ctx, 2);
+  record(ctx, 2, ctx->aux);
+  // Join point: the next pass that reads the voxels waits for this event.
+  DRV_HIP(ctx, hipEventRecord(ctx->vox_done_ev, ctx->aux));
   return check_hip(ctx, hipGetLastError(), "voxelize_end");
 }
 
@@ -624,6 +658,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
   int rc = prepare_light(ctx, rsm, light, &n_vpl, &interval, &n_blocks);
   if (rc != DRV_OK) return rc;
+  join_voxels(ctx);
   const bool shadow = ctx->cfg.indirect_shadow != 0;
   const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
   if (shadow && n_blocks > ctx->vis_blocks_cap) {
@@ -722,6 +757,7 @@ int drv_read_voxels(drv_ctx* ctx, int32_t level, uint8_t* host_out) {
   if (!ctx || !host_out) return DRV_ERR_INVALID_ARG;
   if (level < -1 || level >= ctx->vox.levels) return fail(ctx, DRV_ERR_INVALID_ARG, "read_voxels: bad level");
   // level -1 reads the voxelization target (before the blend).
+  join_voxels(ctx);
   const uint8_t* src = level < 0 ? ctx->vox_target : ctx->vox_mips + ctx->vox.mip_offset[level];
   const size_t w = static_cast<size_t>(ctx->vox.res >> (level < 0 ? 0 : level));
   DRV_HIP(ctx, hipMemcpyAsync(host_out, src, w * w * w, hipMemcpyDeviceToHost, ctx->stream));
@@ -982,6 +1018,7 @@ int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float
   int rc;
   if ((rc = require_features(ctx)) != DRV_OK) return rc;
   if (!ctx->label_caches) return fail(ctx, DRV_ERR_STATE, "mlp_collect: the last light pass has no cone-traced labels");
+  join_voxels(ctx);
   if (n == 0) return DRV_OK;
   // Pairs are drawn over the label set's live caches (read on the device).
   Counter c{};
@@ -1005,6 +1042,7 @@ int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float*
   int rc;
   if ((rc = require_features(ctx)) != DRV_OK) return rc;
   const drvk::FeatureArgsHost fh = drvk::make_feature_args(ctx->vol, ctx->frame.camera_position, voxel_view(ctx));
+  join_voxels(ctx);
   // Pairs name caches of the current list (live count read on the host).
   uint32_t total = 0, live = 0;
   if ((rc = drv_get_cache_count(ctx, &total, &live)) != DRV_OK) return rc;
@@ -1218,6 +1256,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   }
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
+    join_voxels(ctx);  // the batch runs on the views' shared stream
     drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, b->stream);
     // The batch leaves no per-pair visibility behind.
     ctx->last_n_blocks = 0;
@@ -1317,6 +1356,7 @@ int drv_graph_begin(drv_ctx* ctx) {
 
 int drv_graph_end(drv_ctx* ctx) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
+  join_voxels(ctx);  // a capture must end with every forked stream joined
   hipGraph_t g = nullptr;
   DRV_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
   if (ctx->graph_exec) {
