@@ -189,6 +189,12 @@ DRV_FUNCTIONS = {
     "drv_render_rsm": (C.c_int, [vp, P(DrvScene), P(DrvSpotLight), P(DrvRsm)]),
     "drv_read_rsm": (C.c_int, [vp, C.c_int32, vp, vp, vp]),
     "drv_read_gbuffer": (C.c_int, [vp, vp, vp, vp]),
+    "drv_get_shadow_map": (C.c_int, [vp, P(vp), P(C.c_int32)]),
+    "drv_read_shadow_map": (C.c_int, [vp, vp]),
+    "drv_direct_lighting": (C.c_int, [vp, P(DrvGBuffer), vp, P(DrvSpotLight), vp, C.c_int32, C.c_float, C.c_float,
+                                      vp, C.c_int32]),
+    "drv_tonemap": (C.c_int, [vp, vp, C.c_float, C.c_float, vp]),
+    "drv_save_pfm": (C.c_int, [vp, vp, C.c_char_p]),
     "drv_bruteforce_rsm": (C.c_int, [vp, P(DrvGBuffer), vp, P(DrvRsm), P(DrvSpotLight), C.c_int32, vp, C.c_int32]),
     "drv_mlp_set_params": (C.c_int, [vp, vp, C.c_uint32]),
     "drv_mlp_get_params": (C.c_int, [vp, vp, C.c_uint32]),

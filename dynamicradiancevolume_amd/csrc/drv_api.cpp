@@ -106,6 +106,8 @@ struct drv_ctx {
   int16_t* rsm_normal = nullptr;
   uint16_t* rsm_dls = nullptr;
   size_t rsm_cap = 0, rsm_normal_cap = 0, rsm_dls_cap = 0;  // texels of the chain
+  float* rsm_shadow = nullptr;  // the light's depth buffer at render resolution
+  size_t rsm_shadow_cap = 0;
   int rsm_res = 0;                                           // render resolution of the last RSM
 
   bool profiling = false;
@@ -422,6 +424,7 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->rsm_flux);
   dev_free(ctx->rsm_normal);
   dev_free(ctx->rsm_dls);
+  dev_free(ctx->rsm_shadow);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -842,8 +845,9 @@ int drv_render_rsm(drv_ctx* ctx, const drv_scene* scene, const drv_spot_light* l
   if ((rc = ensure(ctx, &ctx->rsm_flux, &ctx->rsm_cap, 3 * chain)) != DRV_OK) return rc;
   if ((rc = ensure(ctx, &ctx->rsm_normal, &ctx->rsm_normal_cap, 2 * chain)) != DRV_OK) return rc;
   if ((rc = ensure(ctx, &ctx->rsm_dls, &ctx->rsm_dls_cap, 2 * chain)) != DRV_OK) return rc;
+  if ((rc = ensure(ctx, &ctx->rsm_shadow, &ctx->rsm_shadow_cap, static_cast<size_t>(res) * res)) != DRV_OK) return rc;
   drvk::launch_render_rsm(scene->positions, scene->albedo, scene->normals, scene->n_tri, *light, ctx->rz, ctx->rtri,
-                          ctx->rbig, ctx->rsm_flux, ctx->rsm_normal, ctx->rsm_dls, ctx->stream);
+                          ctx->rbig, ctx->rsm_flux, ctx->rsm_normal, ctx->rsm_dls, ctx->rsm_shadow, ctx->stream);
   ctx->rsm_res = res;
   // Level l starts after the texels of levels 0..l-1.
   const int read_lod = log2i(res / light->rsm_read_resolution);
@@ -880,6 +884,63 @@ int drv_read_rsm(drv_ctx* ctx, int32_t level, uint16_t* flux, int16_t* normal, u
   DRV_HIP(ctx, hipMemcpyAsync(depth_lin_sq, ctx->rsm_dls + 2 * off, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
   DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return DRV_OK;
+}
+
+int drv_get_shadow_map(drv_ctx* ctx, const float** shadow_map, int32_t* resolution) {
+  if (!ctx || !shadow_map || !resolution) return DRV_ERR_INVALID_ARG;
+  if (ctx->rsm_res == 0) return fail(ctx, DRV_ERR_STATE, "get_shadow_map: no RSM rendered yet");
+  *shadow_map = ctx->rsm_shadow;
+  *resolution = ctx->rsm_res;
+  return DRV_OK;
+}
+
+int drv_read_shadow_map(drv_ctx* ctx, float* host_out) {
+  if (!ctx || !host_out) return DRV_ERR_INVALID_ARG;
+  if (ctx->rsm_res == 0) return fail(ctx, DRV_ERR_STATE, "read_shadow_map: no RSM rendered yet");
+  const size_t n = static_cast<size_t>(ctx->rsm_res) * ctx->rsm_res;
+  DRV_HIP(ctx, hipMemcpyAsync(host_out, ctx->rsm_shadow, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
+}
+
+// ---- direct lighting and HDR output ---------------------------------------------
+
+int drv_direct_lighting(drv_ctx* ctx, const drv_gbuffer* gb, const uint8_t* roughness_metallic,
+                        const drv_spot_light* light, const float* shadow_map, int32_t shadow_resolution,
+                        float normal_offset_bias, float shadow_bias, float* out_rgb, int32_t accumulate) {
+  if (!ctx || !gb || !gb->depth || !gb->normal || !gb->albedo || !light || !shadow_map || shadow_resolution <= 0 ||
+      !out_rgb)
+    return DRV_ERR_INVALID_ARG;
+  if (gb->width != ctx->cfg.width || gb->height != ctx->cfg.height)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "direct_lighting: G-buffer size differs from the configured backbuffer");
+  if (!ctx->have_frame) return fail(ctx, DRV_ERR_STATE, "direct_lighting: frame not set");
+  drvk::launch_direct(ctx->cfg, ctx->frame, *light, *gb, roughness_metallic, ctx->srgb_lut, shadow_map,
+                      shadow_resolution, normal_offset_bias, shadow_bias, out_rgb, accumulate, ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "direct_lighting");
+}
+
+int drv_tonemap(drv_ctx* ctx, const float* hdr_rgb, float exposure, float lmax, float* out_rgb) {
+  if (!ctx || !hdr_rgb || !out_rgb || !(lmax > 0.0f)) return DRV_ERR_INVALID_ARG;
+  const size_t n = static_cast<size_t>(ctx->cfg.width) * ctx->cfg.height * 3;
+  // Renderer::SetTonemapLMax (renderer.cpp:1223-1227): DragoDivider = log2(LMax + 1)
+  drvk::launch_tonemap(hdr_rgb, n, exposure, std::log2(lmax + 1.0f), out_rgb, ctx->stream);
+  return check_hip(ctx, hipGetLastError(), "tonemap");
+}
+
+int drv_save_pfm(drv_ctx* ctx, const float* hdr_rgb, const char* path) {
+  if (!ctx || !hdr_rgb || !path) return DRV_ERR_INVALID_ARG;
+  const int w = ctx->cfg.width, h = ctx->cfg.height;
+  std::vector<float> img(static_cast<size_t>(w) * h * 3);
+  DRV_HIP(ctx, hipMemcpyAsync(img.data(), hdr_rgb, img.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // WritePfm (rendering/hdrimage.cpp:6-30): "PF", size, scale -1 (little
+  // endian), then RGB floats row by row from the bottom row up.
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(ctx, DRV_ERR_INVALID_ARG, std::string("save_pfm: cannot open ") + path);
+  std::fprintf(f, "PF\n%d %d\n-1.000000\n", w, h);
+  const size_t wrote = std::fwrite(img.data(), sizeof(float), img.size(), f);
+  const bool ok = std::fclose(f) == 0 && wrote == img.size();
+  return ok ? DRV_OK : fail(ctx, DRV_ERR_HIP, std::string("save_pfm: write failed: ") + path);
 }
 
 // ---- brute-force RSM reference ------------------------------------------------

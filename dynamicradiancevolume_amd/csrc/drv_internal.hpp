@@ -181,7 +181,11 @@ void launch_render_gbuffer(const float* positions, const float* albedo, const fl
 // big: 1 + n_tri uint32 (count, then the list of large triangles)
 void launch_render_rsm(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
                        const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint32_t* big,
-                       uint16_t* flux, int16_t* normal, uint16_t* dls, hipStream_t s);
+                       uint16_t* flux, int16_t* normal, uint16_t* dls, float* shadow, hipStream_t s);
+void launch_direct(const drv_config& cfg, const drv_frame& frame, const drv_spot_light& light, const drv_gbuffer& gb,
+                   const uint8_t* roughness_metallic, const float* srgb_lut, const float* shadow, int shadow_res,
+                   float normal_offset, float bias, float* out_rgb, int accumulate, hipStream_t s);
+void launch_tonemap(const float* hdr, size_t n, float exposure, float divider, float* out, hipStream_t s);
 
 void launch_bruteforce(const drv_config& cfg, const drv_frame& frame, const drv_spot_light& light,
                        const drv_gbuffer& gb, const uint8_t* roughness_metallic, const float* srgb_lut,

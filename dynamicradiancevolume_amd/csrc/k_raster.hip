@@ -288,6 +288,7 @@ struct RsmArgs {
   uint16_t* flux;
   int16_t* normal;
   uint16_t* dls;
+  float* shadow;  // the light's depth buffer (the direct-lighting shadow map)
 };
 
 // shader/fillrsm.frag:32-61 at one texel: flux = albedo * I * spot falloff *
@@ -299,6 +300,7 @@ __global__ __launch_bounds__(256) void k_rsm_resolve(RsmArgs a) {
   if (i >= static_cast<size_t>(res) * res) return;
   const int x = static_cast<int>(i % res), y = static_cast<int>(i / res);
   const uint32_t t = a.tri[i];
+  a.shadow[i] = static_cast<float>(__longlong_as_double(static_cast<long long>(a.zbits[i])));
   if (t == 0xffffffffu) {
     a.flux[3 * i] = a.flux[3 * i + 1] = a.flux[3 * i + 2] = 0;
     a.normal[2 * i] = a.normal[2 * i + 1] = 0;
@@ -407,7 +409,7 @@ void launch_render_gbuffer(const float* positions, const float* albedo, const fl
 
 void launch_render_rsm(const float* positions, const float* albedo, const float* normals, uint32_t n_tri,
                        const drv_spot_light& light, unsigned long long* zbits, uint32_t* tri, uint32_t* big,
-                       uint16_t* flux, int16_t* normal, uint16_t* dls, hipStream_t s) {
+                       uint16_t* flux, int16_t* normal, uint16_t* dls, float* shadow, hipStream_t s) {
   const int res = light.rsm_render_resolution;
   raster(positions, n_tri, light.light_view_projection, res, res, zbits, tri, big, s);
   RsmArgs a;
@@ -426,6 +428,7 @@ void launch_render_rsm(const float* positions, const float* albedo, const float*
   a.flux = flux;
   a.normal = normal;
   a.dls = dls;
+  a.shadow = shadow;
   const size_t n = static_cast<size_t>(res) * res;
   hipLaunchKernelGGL(k_rsm_resolve, dim3((n + 255) / 256), dim3(256), 0, s, a);
   // Mip chain: level l + 1 from level l, packed one after another.

@@ -170,6 +170,34 @@ class Renderer:
                                                 int(diffuse_only), out.data_ptr(), int(accumulate)),
                     "drv_bruteforce_rsm")
 
+    def shadow_map(self) -> tuple:
+        """(device pointer, resolution) of the last rendered RSM's depth buffer."""
+        ptr, res = C.c_void_p(), C.c_int32()
+        self._check(self.lib.drv_get_shadow_map(self.ctx, C.byref(ptr), C.byref(res)), "drv_get_shadow_map")
+        return ptr.value, res.value
+
+    def read_shadow_map(self) -> np.ndarray:
+        _, res = self.shadow_map()
+        out = np.empty((res, res), np.float32)
+        self._check(self.lib.drv_read_shadow_map(self.ctx, out.ctypes.data), "drv_read_shadow_map")
+        return out
+
+    def direct_lighting(self, gbuffer: _abi.DrvGBuffer, light: _abi.DrvSpotLight, out: torch.Tensor,
+                        roughness_metallic: torch.Tensor | None = None, normal_offset_bias: float = 0.01,
+                        shadow_bias: float = 0.0001, accumulate: bool = False):
+        """Renderer::ApplyDirectLighting (renderer.cpp:827-849) with the last rendered RSM's depth."""
+        ptr, res = self.shadow_map()
+        rm = roughness_metallic.data_ptr() if roughness_metallic is not None else None
+        self._check(self.lib.drv_direct_lighting(self.ctx, C.byref(gbuffer), rm, C.byref(light), ptr, res,
+                                                 normal_offset_bias, shadow_bias, out.data_ptr(), int(accumulate)),
+                    "drv_direct_lighting")
+
+    def tonemap(self, hdr: torch.Tensor, out: torch.Tensor, exposure: float = 1.0, lmax: float = 1.2):
+        self._check(self.lib.drv_tonemap(self.ctx, hdr.data_ptr(), exposure, lmax, out.data_ptr()), "drv_tonemap")
+
+    def save_pfm(self, hdr: torch.Tensor, path: str):
+        self._check(self.lib.drv_save_pfm(self.ctx, hdr.data_ptr(), str(path).encode()), "drv_save_pfm")
+
     # --- inputs rendered on the device ------------------------------------------
     def render_gbuffer(self, scene: _abi.DrvScene, view_proj) -> _abi.DrvGBuffer:
         out = _abi.DrvGBuffer()

@@ -267,6 +267,31 @@ int drv_read_gbuffer(drv_ctx* ctx, float* depth, int16_t* normal, uint8_t* albed
  * normal RG16I, depth/depth^2 RG16F; (render resolution >> level)^2 texels). */
 int drv_read_rsm(drv_ctx* ctx, int32_t level, uint16_t* flux, int16_t* normal, uint16_t* depth_lin_sq);
 
+/* The depth buffer of the last drv_render_rsm (reverse-Z, resolution^2
+ * floats, device): the light's shadow map for drv_direct_lighting. */
+int drv_get_shadow_map(drv_ctx* ctx, const float** shadow_map, int32_t* resolution);
+/* Host copy of that depth buffer (resolution^2 floats). */
+int drv_read_shadow_map(drv_ctx* ctx, float* host_out);
+
+/* ---- direct lighting and HDR output -------------------------------------- */
+
+/* Renderer::ApplyDirectLighting (rendering/renderer.cpp:827-849,
+ * shader/directdeferredlighting.frag): one spot light with a normal-offset
+ * PCF shadow lookup (LINEAR compare-GREATER sampler, border depth 0) into
+ * its shadow map; normal_offset_bias / shadow_bias are Light::
+ * normalOffsetShadowBias / shadowBias (0.01 / 0.0001, scene/light.hpp:13).
+ * Background pixels receive 0. */
+int drv_direct_lighting(drv_ctx* ctx, const drv_gbuffer* gbuffer, const uint8_t* roughness_metallic,
+                        const drv_spot_light* light, const float* shadow_map, int32_t shadow_resolution,
+                        float normal_offset_bias, float shadow_bias, float* out_rgb, int32_t accumulate);
+/* shader/tonemapping.frag: out = log2(hdr * exposure + 1) / log2(lmax + 1)
+ * per channel (Renderer defaults: exposure 1, LMax 1.2; renderer.cpp:38-39);
+ * device buffers of width x height x 3 floats. */
+int drv_tonemap(drv_ctx* ctx, const float* hdr_rgb, float exposure, float lmax, float* out_rgb);
+/* Renderer::SaveToPFM (rendering/renderer.cpp:1229-1235, hdrimage.cpp:6-30):
+ * writes the device HDR image (width x height x 3 floats) as a PFM file. */
+int drv_save_pfm(drv_ctx* ctx, const float* hdr_rgb, const char* path);
+
 /* ---- brute-force RSM reference ----------------------------------------- */
 
 /* Renderer::ApplyRSMsBruteForce (rendering/renderer.cpp:851-874,

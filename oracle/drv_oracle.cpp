@@ -1032,6 +1032,78 @@ void orc_bruteforce_rsm(const drv_config* cfg, const drv_frame* frame, const drv
   }
 }
 
+void orc_direct_lighting(const drv_config* cfg, const drv_frame* frame, const drv_spot_light* light,
+                         const float* depth, const int16_t* normal, const uint8_t* albedo, const uint8_t* rm,
+                         const float* shadow, int32_t sres, float normal_offset, float bias, float* out,
+                         int32_t n_threads) {
+  const int W = cfg->width, H = cfg->height;
+  float lut[256];
+  for (int i = 0; i < 256; ++i) lut[i] = orc_srgb_to_linear(static_cast<uint8_t>(i));
+  const V3<float> lpos = load3<float>(light->light_position);
+  const V3<float> ldir = load3<float>(light->light_direction);
+  const V3<float> cam = load3<float>(frame->camera_position);
+  // textureProjLod on the shadow sampler: per-texel GREATER comparisons,
+  // bilinearly weighted, texels outside the map read the border depth 0.
+  auto pcf = [&](float s, float t, float ref) {
+    const float u = s * static_cast<float>(sres) - 0.5f, v = t * static_cast<float>(sres) - 0.5f;
+    const float fu = std::floor(u), fv = std::floor(v);
+    const int i0 = to_int(fu), j0 = to_int(fv);
+    const float al = u - fu, be = v - fv;
+    auto lit = [&](int i, int j) {
+      const float d = (i >= 0 && j >= 0 && i < sres && j < sres) ? shadow[static_cast<size_t>(j) * sres + i] : 0.0f;
+      return ref > d ? 1.0f : 0.0f;
+    };
+    return (1.0f - al) * (1.0f - be) * lit(i0, j0) + al * (1.0f - be) * lit(i0 + 1, j0) +
+           (1.0f - al) * be * lit(i0, j0 + 1) + al * be * lit(i0 + 1, j0 + 1);
+  };
+  const int nt = resolve_threads(n_threads);
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const size_t p = static_cast<size_t>(y) * W + x;
+      float* o = out + 3 * p;
+      o[0] = o[1] = o[2] = 0.0f;
+      if (depth[p] < 0.00001f) continue;
+      float wpa[3];
+      unproject(frame->inverse_view_projection, W, H, x, y, depth[p], wpa);
+      const V3<float> wp = load3<float>(wpa);
+      const V3<float> n = unpack_normal16i<float>(normal[2 * p], normal[2 * p + 1]);
+      // directdeferredlighting.frag:20-25
+      V3<float> to_light = lpos - wp;
+      const float dsq = dot(to_light, to_light);
+      to_light = to_light * (1.0f / std::sqrt(dsq));
+      const float cos_theta = std::min(1.0f, std::max(0.0f, dot(to_light, n)));
+      // directdeferredlighting.frag:27-36
+      const float off = normal_offset * (1.0f - cos_theta);
+      const V3<float> wo = wp + n * off;
+      float po[4], pw[4];
+      mat_vec<float>(light->light_view_projection, wo.x, wo.y, wo.z, 1.0f, po);
+      mat_vec<float>(light->light_view_projection, wp.x, wp.y, wp.z, 1.0f, pw);
+      const float sx = po[0] * 0.5f + 0.5f * pw[3], sy = po[1] * 0.5f + 0.5f * pw[3];
+      const float sz = pw[2] + bias;
+      const float shadowing = pcf(sx / pw[3], sy / pw[3], sz / pw[3]);
+      if (shadowing == 0.0f) continue;  // discard
+      const V3<float> to_cam = normalize(cam - wp);
+      const float rough = rm ? static_cast<float>(rm[2 * p]) / 255.0f : 1.0f;
+      const float metal = rm ? static_cast<float>(rm[2 * p + 1]) / 255.0f : 0.0f;
+      // lightingfunctions.glsl:3-10, 26-58
+      const float falloff = std::min(1.0f, std::max(0.0f, dot(to_light * -1.0f, ldir) - light->light_cos_half_angle)) /
+                            (1.0f - light->light_cos_half_angle);
+      const float rsq = rough * rough;
+      const float blinn = 2.0f / (rsq * rsq + 0.0005f);
+      const float norm = (blinn + 8.0f) / (8.0f * kPi);
+      const V3<float> h = normalize(to_cam + to_light);
+      const float bp = norm * std::pow(std::min(1.0f, std::max(0.0f, dot(n, h))), blinn);
+      const float k = shadowing * falloff * cos_theta / dsq;
+      for (int c = 0; c < 3; ++c) {
+        const float base = lut[albedo[3 * p + c]];
+        const float diff = base * (1.0f - metal) + 0.02f * metal;
+        const float spec = 0.04f * (1.0f - metal) + base * metal;
+        o[c] = (diff / kPi + bp * spec) * (light->light_intensity[c] * k);
+      }
+    }
+}
+
 void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* frame, const drv_sh_constants* shc,
                const float* depth, const int16_t* normal, const uint8_t* albedo, const uint32_t* cav,
                const float* entries, uint32_t live, float* out, int32_t n_threads) {

@@ -146,6 +146,15 @@ void orc_bruteforce_rsm(const drv_config* cfg, const drv_frame* frame, const drv
                         const uint8_t* roughness_metallic, int32_t diffuse_only, const uint32_t* pixels, int64_t n,
                         float* out_rgb, int32_t n_threads);
 
+/* ---- direct lighting and HDR output -------------------------------------- */
+/* directdeferredlighting.frag:13-52 with the LINEAR compare-GREATER PCF of
+ * the shadow sampler (renderer.cpp:33-34, border depth 0); out_rgb[3*W*H],
+ * background pixels 0.  roughness_metallic may be NULL (1, 0). */
+void orc_direct_lighting(const drv_config* cfg, const drv_frame* frame, const drv_spot_light* light,
+                         const float* depth, const int16_t* normal, const uint8_t* albedo,
+                         const uint8_t* roughness_metallic, const float* shadow_map, int32_t shadow_res,
+                         float normal_offset_bias, float shadow_bias, float* out_rgb, int32_t n_threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,8 @@ _FUNCS = {
                                        P(C.c_int32)]),
     "orc_entry_sh": (None, [P(C.c_float), C.c_int32, P(C.c_float)]),
     "orc_shadow_blocks": (None, [P(_abi.DrvSpotLight), P(OrcRsm), vp]),
+    "orc_direct_lighting": (None, [P(_abi.DrvConfig), P(_abi.DrvFrame), P(_abi.DrvSpotLight), vp, vp, vp, vp, vp,
+                                   C.c_int32, C.c_float, C.c_float, vp, C.c_int32]),
     "orc_bruteforce_rsm": (None, [P(_abi.DrvConfig), P(_abi.DrvFrame), P(_abi.DrvSpotLight), P(OrcRsm), vp, vp, vp,
                                   vp, C.c_int32, vp, C.c_int64, vp, C.c_int32]),
     "orc_mlp_features": (None, [P(_abi.DrvVolume), C.c_int32, vp, P(C.c_float), P(C.c_float), P(C.c_float),
@@ -206,6 +208,19 @@ def mlp_features(vol, res: int, mips: np.ndarray, cam, positions: np.ndarray, bl
     for i, (c, b) in enumerate(np.asarray(pairs)):
         lib().orc_mlp_features(C.byref(vol), res, _ptr(mips), cam_a, pos[c].ctypes.data_as(fp),
                                blk[b].ctypes.data_as(fp), out[i].ctypes.data_as(fp))
+    return out
+
+
+def direct_lighting(fi, shadow_map: np.ndarray, roughness_metallic=None, normal_offset=0.01, bias=0.0001,
+                    n_threads: int = 0):
+    """directdeferredlighting.frag over the frame: [H][W][3]."""
+    cfg = fi.workload.config()
+    out = np.zeros((cfg.height, cfg.width, 3), np.float32)
+    sm = np.ascontiguousarray(shadow_map, np.float32)
+    rm = np.ascontiguousarray(roughness_metallic, np.uint8) if roughness_metallic is not None else None
+    lib().orc_direct_lighting(C.byref(cfg), C.byref(fi.frame()), C.byref(fi.light), _ptr(fi.depth), _ptr(fi.gnormal),
+                              _ptr(fi.galbedo), _ptr(rm) if rm is not None else None, _ptr(sm), sm.shape[0],
+                              normal_offset, bias, _ptr(out), n_threads)
     return out
 
 

@@ -712,3 +712,47 @@ def test_bruteforce_oracle_matches_numpy(diffuse_only):
     ref = _np_bruteforce(fi, px, diffuse_only, rm if not diffuse_only else None)
     assert np.abs(ref).sum() > 0
     np.testing.assert_allclose(got, ref, rtol=2e-4, atol=1e-7 * np.abs(ref).max())
+
+
+def test_direct_lighting_oracle_closed_forms():
+    """Unoccluded (shadow map all 0 = border depth): every lit pixel gets
+    I * falloff * cos / d^2 * BRDF; with every texel at the near plane
+    (depth 1) only lookups that leave the map (the normal offset near the
+    cone's edge reads the border) keep light."""
+    from oracle import orc
+
+    fi = frame_inputs("cornell_256")
+    w = fi.workload
+    lit = orc.direct_lighting(fi, np.zeros((64, 64), np.float32))
+    dark = orc.direct_lighting(fi, np.ones((64, 64), np.float32))
+    assert np.all(dark <= lit) and dark.sum() <= 0.02 * lit.sum()
+    # numpy at a few pixels (diffuse + Blinn-Phong with roughness 1, metallic 0)
+    rng = np.random.default_rng(8)
+    ys, xs = np.nonzero(fi.depth >= 1e-5)
+    pick = rng.choice(len(ys), 25, replace=False)
+    lut = np.array([orc.lib().orc_srgb_to_linear(i) for i in range(256)])
+    ivp = np.array(fi.inv_view_proj, np.float64).reshape(4, 4)
+    L = fi.light
+    lp = np.array(L.light_position[:]); ld = np.array(L.light_direction[:]); I = np.array(L.light_intensity[:])
+    for k in pick:
+        y, x = ys[k], xs[k]
+        c = np.array([(x + 0.5) / w.width * 2 - 1, (y + 0.5) / w.height * 2 - 1, fi.depth[y, x], 1.0]) @ ivp.T
+        wp = c[:3] / c[3]
+        a, b = fi.gnormal[y, x]
+        ph, zz = a * np.pi / 32768, b / 32768
+        n = np.array([np.cos(ph) * np.sqrt(1 - zz * zz), np.sin(ph) * np.sqrt(1 - zz * zz), zz])
+        n /= np.linalg.norm(n)
+        tl = lp - wp
+        dsq = tl @ tl
+        tl /= np.sqrt(dsq)
+        cos_t = np.clip(tl @ n, 0, 1)
+        fall = np.clip(-(tl @ ld) - L.light_cos_half_angle, 0, 1) / (1 - L.light_cos_half_angle)
+        tc = np.array(w.eye) - wp
+        tc /= np.linalg.norm(tc)
+        h = tc + tl
+        h /= np.linalg.norm(h)
+        blinn = 2 / (1 + 0.0005)
+        bp = (blinn + 8) / (8 * np.pi) * np.clip(n @ h, 0, 1) ** blinn
+        base = lut[fi.galbedo[y, x]]
+        ref = (base / np.pi + bp * 0.04) * I * fall * cos_t / dsq
+        np.testing.assert_allclose(lit[y, x], ref, rtol=2e-4, atol=1e-9)
