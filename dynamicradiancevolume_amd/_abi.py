@@ -36,6 +36,9 @@ class DrvConfig(C.Structure):
         ("indirect_shadow", C.c_int32),
         ("cav_transition_size", C.c_float),
         ("cav_cascade0_world_size", C.c_float),
+        ("indirect_specular", C.c_int32),
+        ("specular_envmap_size", C.c_int32),
+        ("specular_hole_fill_level", C.c_int32),
     ]
 
 
@@ -116,6 +119,7 @@ class DrvGBuffer(C.Structure):
         ("albedo", C.c_void_p),
         ("width", C.c_int32),
         ("height", C.c_int32),
+        ("roughness_metallic", C.c_void_p),
     ]
 
 
@@ -134,7 +138,8 @@ class DrvStageTimes(C.Structure):
         (n, C.c_float)
         for n in ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
                   "gather_ms", "reduce_ms", "apply_ms")
-    ] + [("cone_traces", C.c_uint64), ("cone_steps", C.c_uint64), ("cone_steps_executed", C.c_uint64)]
+    ] + [("cone_traces", C.c_uint64), ("cone_steps", C.c_uint64), ("cone_steps_executed", C.c_uint64),
+         ("specular_ms", C.c_float)]
 
 
 class DrvSurrogate(C.Structure):
@@ -180,6 +185,8 @@ DRV_FUNCTIONS = {
     "drv_allocate_caches": (C.c_int, [vp, P(DrvGBuffer)]),
     "drv_light_caches": (C.c_int, [vp, P(DrvRsm), P(DrvSpotLight)]),
     "drv_apply_caches": (C.c_int, [vp, P(DrvGBuffer), vp, C.c_int32]),
+    "drv_get_specular_atlas": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+    "drv_read_specular_atlas": (C.c_int, [vp, C.c_int32, vp]),
     "drv_get_cache_count": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32)]),
     "drv_read_caches": (C.c_int, [vp, vp, C.c_uint32]),
     "drv_read_cav": (C.c_int, [vp, vp]),

@@ -94,6 +94,11 @@ struct drv_ctx {
   drvk::Tuning tune;
   size_t partial_splits = 0;  // splits the partial buffer holds
 
+  // Indirect specular: atlas, work sums, spill; spec_first is set by the
+  // allocation pass so that the frame's first light overwrites the sums.
+  drvk::SpecAtlas spec;
+  bool spec_first = true;
+
   // Device-rendered inputs: raster scratch, G-buffer, RSM mip chain.
   unsigned long long* rz = nullptr;
   uint32_t* rtri = nullptr;
@@ -296,9 +301,21 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   // Address arithmetic of the CAV is 32-bit (cacheGather.comp:25-28).
   if (static_cast<double>(c.cav_cascades) * c.cav_resolution * c.cav_resolution * c.cav_resolution > 2147483647.0)
     return DRV_ERR_INVALID_ARG;
+  if (c.indirect_specular) {
+    const int size = c.specular_envmap_size == 0 ? 16 : c.specular_envmap_size;
+    if (!is_pow2(size) || size > 32 || c.specular_hole_fill_level < 0) return DRV_ERR_INVALID_ARG;
+  }
   drv_ctx* ctx = new (std::nothrow) drv_ctx();
   if (!ctx) return DRV_ERR_NO_MEMORY;
   ctx->cfg = c;
+  if (c.indirect_specular) {
+    drv_config& k = ctx->cfg;
+    k.indirect_specular = 1;
+    if (k.specular_envmap_size == 0) k.specular_envmap_size = 16;
+    k.specular_hole_fill_level = std::min(k.specular_hole_fill_level, log2i(k.specular_envmap_size));
+  } else {
+    ctx->cfg.indirect_specular = 0;
+  }
   int rc = DRV_OK;
   auto bail = [&](int code) {
     drv_destroy(ctx);
@@ -360,6 +377,32 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   if ((rc = dev_alloc(ctx, &ctx->srgb_lut, 256)) != DRV_OK) return bail(rc);
   if (hipMemcpy(ctx->srgb_lut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) return bail(DRV_ERR_HIP);
 
+  // Specular atlas (renderer.cpp:253-286): side 2^ceil(log2(ceil(sqrt(max
+  // caches)) * S)), log2(S) + 1 levels; work sums and spill per cache.
+  if (ctx->cfg.indirect_specular) {
+    drvk::SpecAtlas& A = ctx->spec;
+    A.S = ctx->cfg.specular_envmap_size;
+    A.levels = log2i(A.S) + 1;
+    A.fill = ctx->cfg.specular_hole_fill_level;
+    const double side = std::ceil(std::sqrt(static_cast<double>(c.max_caches))) * A.S;
+    const double w = std::pow(2.0, std::ceil(std::log2(side)));
+    if (w > 65536.0) return bail(DRV_ERR_INVALID_ARG);
+    A.width = static_cast<uint32_t>(w);
+    A.per_row = A.width / static_cast<uint32_t>(A.S);
+    uint64_t off = 0;
+    for (int l = 0; l < A.levels; ++l) {
+      A.level_offset[l] = off;
+      const uint64_t wl = A.width >> l;
+      off += wl * wl;
+    }
+    A.texels = off;
+    const size_t n = static_cast<size_t>(c.max_caches);
+    if ((rc = dev_alloc(ctx, &A.atlas, A.texels)) != DRV_OK) return bail(rc);
+    if ((rc = dev_alloc(ctx, &A.work, n * A.S * A.S * 3)) != DRV_OK) return bail(rc);
+    if ((rc = dev_alloc(ctx, &A.spill, n * (2 * A.S + 1) * 3)) != DRV_OK) return bail(rc);
+    drvk::launch_fill_u32(A.atlas, A.texels, 0u, nullptr);
+  }
+
   // Zero-initialize the state a frame may read before writing it.
   drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->counter), sizeof(Counter) / 4, 0u, nullptr);
   drvk::launch_fill_u32(ctx->cav, ctx->cav_cells, 0u, nullptr);
@@ -390,6 +433,9 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->caches);
   dev_free(ctx->counter);
   dev_free(ctx->cav);
+  dev_free(ctx->spec.atlas);
+  dev_free(ctx->spec.work);
+  dev_free(ctx->spec.spill);
   dev_free(ctx->vox_target);
   dev_free(ctx->vox_mips);
   dev_free(ctx->vox_corners);
@@ -603,6 +649,7 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
   drvk::launch_prepare_dispatch(ctx->counter, static_cast<uint32_t>(ctx->cfg.max_caches), ctx->stream);
   record(ctx, 4);
   ctx->allocated = true;
+  ctx->spec_first = true;
   // A new cache list: visibility and training labels of earlier passes no
   // longer refer to these caches.
   ctx->last_n_blocks = 0;
@@ -686,7 +733,8 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     steps = ctx->step_counter;
   }
   record(ctx, 5);
-  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, ctx->stream);
+  const int specular = ctx->cfg.indirect_specular;
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, specular, ctx->stream);
   record(ctx, 6);
   const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
   if (use_net) {
@@ -709,6 +757,15 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->partial,
                          static_cast<uint32_t>(cap), ctx->cu_count * 8, ctx->stream);
   record(ctx, 9);
+  if (specular) {
+    // renderer.cpp:553-558: LightCachesRSM with the env-map writes, then
+    // PrepareSpecularEnvmaps.
+    drvk::launch_specular(ctx->frame.camera_position, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
+                          shadow ? ctx->visibility : nullptr, static_cast<uint32_t>(cap), interval, ctx->spec,
+                          ctx->spec_first ? 1 : 0, ctx->cu_count * 8, ctx->stream);
+    ctx->spec_first = false;
+    record(ctx, 12);
+  }
   ctx->counted_steps = steps != nullptr;
   ctx->last_n_blocks = n_blocks;
   return check_hip(ctx, hipGetLastError(), "light_caches");
@@ -723,9 +780,32 @@ int drv_apply_caches(drv_ctx* ctx, const drv_gbuffer* gb, float* out_rgb, int32_
   if (!ctx->allocated) return fail(ctx, DRV_ERR_STATE, "apply: no allocation yet");
   record(ctx, 10);
   drvk::launch_apply(ctx->cfg, ctx->vol, ctx->frame, ctx->shc, *gb, ctx->cav, ctx->caches, ctx->counter,
-                     ctx->srgb_lut, out_rgb, accumulate, ctx->stream);
+                     ctx->srgb_lut, ctx->cfg.indirect_specular ? &ctx->spec : nullptr, out_rgb, accumulate,
+                     ctx->stream);
   record(ctx, 11);
   return check_hip(ctx, hipGetLastError(), "apply");
+}
+
+// ---- specular atlas -----------------------------------------------------
+
+int drv_get_specular_atlas(drv_ctx* ctx, int32_t* width, int32_t* caches_per_row, int32_t* levels) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  if (!ctx->cfg.indirect_specular) return fail(ctx, DRV_ERR_STATE, "specular atlas: indirect_specular is off");
+  if (width) *width = static_cast<int32_t>(ctx->spec.width);
+  if (caches_per_row) *caches_per_row = static_cast<int32_t>(ctx->spec.per_row);
+  if (levels) *levels = ctx->spec.levels;
+  return DRV_OK;
+}
+
+int drv_read_specular_atlas(drv_ctx* ctx, int32_t level, uint32_t* host_out) {
+  if (!ctx || !host_out) return DRV_ERR_INVALID_ARG;
+  if (!ctx->cfg.indirect_specular) return fail(ctx, DRV_ERR_STATE, "specular atlas: indirect_specular is off");
+  if (level < 0 || level >= ctx->spec.levels) return fail(ctx, DRV_ERR_INVALID_ARG, "specular atlas: no such level");
+  const uint64_t wl = ctx->spec.width >> level;
+  DRV_HIP(ctx, hipMemcpyAsync(host_out, ctx->spec.atlas + ctx->spec.level_offset[level], wl * wl * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  DRV_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return DRV_OK;
 }
 
 // ---- introspection ------------------------------------------------------
@@ -831,6 +911,7 @@ int drv_render_gbuffer(drv_ctx* ctx, const drv_scene* scene, const float view_pr
   out->albedo = ctx->g_albedo;
   out->width = w;
   out->height = h;
+  out->roughness_metallic = nullptr;  // the scenes carry no material maps
   return check_hip(ctx, hipGetLastError(), "render_gbuffer");
 }
 
@@ -966,7 +1047,7 @@ int drv_bruteforce_rsm(drv_ctx* ctx, const drv_gbuffer* gb, const uint8_t* rough
   }
   // VAL records (position, flux, normal) of the read level; the VPL order
   // does not matter to a plain sum.
-  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, ctx->stream);
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, 0, ctx->stream);
   drvk::launch_bruteforce(ctx->cfg, ctx->frame, *light, *gb, roughness_metallic, ctx->srgb_lut, ctx->vpls, n,
                           diffuse_only, out_rgb, accumulate, ctx->stream);
   return check_hip(ctx, hipGetLastError(), "bruteforce_rsm");
@@ -1215,6 +1296,9 @@ int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out) {
         (views[i]->cfg.indirect_shadow != 0) != (views[0]->cfg.indirect_shadow != 0) ||
         views[i]->stream != views[0]->stream)
       return DRV_ERR_INVALID_ARG;
+    // The batched kernel writes SH only; specular views use drv_light_caches.
+    if (views[i]->cfg.indirect_specular)
+      return fail(views[i], DRV_ERR_STATE, "batch: views with indirect_specular are lit one at a time");
   }
   drv_batch* b = new (std::nothrow) drv_batch();
   if (!b) return DRV_ERR_NO_MEMORY;
@@ -1318,7 +1402,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
     join_voxels(ctx);  // the batch runs on the views' shared stream
-    drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, b->stream);
+    drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, 0, b->stream);
     // The batch leaves no per-pair visibility behind.
     ctx->last_n_blocks = 0;
     ctx->label_caches = nullptr;
@@ -1487,7 +1571,8 @@ int drv_get_stage_times(drv_ctx* ctx, drv_stage_times* out) {
     ctx->times.voxelize_ms = elapsed(ctx, 0, 1);
     ctx->times.voxel_mips_ms = elapsed(ctx, 1, 2);
     ctx->times.allocate_ms = elapsed(ctx, 3, 4);
-    ctx->times.light_caches_ms = elapsed(ctx, 5, 9);
+    ctx->times.light_caches_ms = elapsed(ctx, 5, ctx->cfg.indirect_specular ? 12 : 9);
+    ctx->times.specular_ms = ctx->cfg.indirect_specular ? elapsed(ctx, 9, 12) : 0.0f;
     ctx->times.prepare_ms = elapsed(ctx, 5, 6);
     ctx->times.visibility_ms = elapsed(ctx, 6, 7);
     ctx->times.gather_ms = elapsed(ctx, 7, 8);

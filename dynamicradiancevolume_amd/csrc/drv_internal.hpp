@@ -121,6 +121,22 @@ struct Tuning {
   int steps_per_batch = 4;     // cone-trace steps planned (and loaded) together: 2, 4 or 8
 };
 
+// Per-cache specular env-maps (INDIRECT_SPECULAR): the R11G11B10F atlas of
+// Renderer::AllocateCacheData (renderer.cpp:253-286) - caches row-major in
+// an atlas of `width` texels, `per_row` caches per row, S x S texels each,
+// `levels` = log2(S) + 1 mip levels stored back to back - plus the FP32 work
+// sums of the current frame and the per-light spill of each map.
+constexpr int kSpecMaxLevels = 6;  // S <= 32
+struct SpecAtlas {
+  uint32_t* atlas = nullptr;
+  float* work = nullptr;   // [max_caches][S * S][3]
+  float* spill = nullptr;  // [max_caches][2S + 1][3]
+  uint32_t width = 0, per_row = 0;
+  int S = 0, levels = 0, fill = 0;
+  uint64_t level_offset[kSpecMaxLevels] = {};
+  uint64_t texels = 0;  // all levels
+};
+
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 
@@ -134,8 +150,10 @@ void launch_voxel_blend(int res, const uint8_t* target, uint8_t* volume, float a
 void launch_voxel_mips(const VoxelLayout& layout, uint8_t* mips, hipStream_t s);
 void launch_voxel_corners(const VoxelLayout& layout, const uint8_t* mips, uint64_t* corners, hipStream_t s);
 
+// flux_cut: INDIRECT_SPECULAR skips VALs whose flux sums below 0.001
+// (cacheLightingRSM.comp:240-243); their flux is stored as zero.
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         hipStream_t s);
+                         int flux_cut, hipStream_t s);
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
                        uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid, hipStream_t s);
@@ -192,8 +210,15 @@ void launch_bruteforce(const drv_config& cfg, const drv_frame& frame, const drv_
                        const Vpl* vals, uint32_t n_vals, int diffuse_only, float* out_rgb, int accumulate,
                        hipStream_t s);
 
+// spec: the specular atlas when INDIRECT_SPECULAR is on, else null.
 void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const drv_sh_constants& shc,
                   const drv_gbuffer& gb, const uint32_t* cav, const float* caches, const Counter* counter,
-                  const float* srgb_lut, float* out_rgb, int accumulate, hipStream_t s);
+                  const float* srgb_lut, const SpecAtlas* spec, float* out_rgb, int accumulate, hipStream_t s);
+
+// Specular gather of one light into the work sums (first != 0: overwrite),
+// then quantization, mips and hole filling of every live cache's region.
+void launch_specular(const float* cam, const float* caches, int stride, const Counter* counter, const Vpl* vpls,
+                     uint32_t n_vpl, const float* vis, uint32_t vis_pitch, uint32_t interval, const SpecAtlas& at,
+                     int first, int grid, hipStream_t s);
 
 }  // namespace drvk

@@ -9,8 +9,14 @@
 // workgroup), so the 8 corner cells of neighbouring pixels share CAV lines and
 // cache entries in L2.  Addresses that are unclaimed (CAV 0 -> 0xFFFFFFFF) or
 // past the live count read as zero, as robust buffer access does in GL.
+//
+// With INDIRECT_SPECULAR (k_apply<true>) each of the 8 caches also
+// contributes a trilinear lookup in its env-map of the specular atlas
+// (cacheApply.frag:100-106, 139-154), and the diffuse colour is the
+// metallic-weighted one of ComputeMaterialColors.
 #include "drv_internal.hpp"
 #include "k_common.hpp"
+#include "k_specular.hpp"
 
 namespace drvk {
 namespace {
@@ -28,6 +34,61 @@ struct ApplyArgs {
   const float* srgb_lut;
   float* out;
   int width, height, R, C, stride, sh2, transitions, accumulate;
+  // INDIRECT_SPECULAR
+  const uint8_t* roughness_metallic;  // RG8 or null (roughness 1, metallic 0)
+  const uint32_t* atlas;
+  uint64_t level_offset[kSpecMaxLevels];
+  uint32_t atlas_width, per_row;
+  int S, levels;
+  float per_cache_texcoord;  // SpecularEnvmapPerCacheSize_Texcoord = S / width
+  float cam[3];
+};
+
+// Bilinear tap of atlas level l (clamp to edge), the GL 4.5 section 8.14.2
+// weights in FP32.
+__device__ __forceinline__ void atlas_bilinear(const ApplyArgs& a, int l, float s, float t, float* out) {
+  const int wl = static_cast<int>(a.atlas_width >> l);
+  const float u = s * static_cast<float>(wl) - 0.5f, v = t * static_cast<float>(wl) - 0.5f;
+  const float fu = floorf(u), fv = floorf(v);
+  const int i0 = to_int(fu), j0 = to_int(fv);
+  const float al = u - fu, be = v - fv;
+  const int ia = min(max(i0, 0), wl - 1), ib = min(max(i0 + 1, 0), wl - 1);
+  const int ja = min(max(j0, 0), wl - 1), jb = min(max(j0 + 1, 0), wl - 1);
+  const uint32_t* lv = a.atlas + a.level_offset[l];
+  float t00[3], t10[3], t01[3], t11[3];
+  unpack_r11g11b10(lv[static_cast<uint64_t>(ja) * wl + ia], t00);
+  unpack_r11g11b10(lv[static_cast<uint64_t>(ja) * wl + ib], t10);
+  unpack_r11g11b10(lv[static_cast<uint64_t>(jb) * wl + ia], t01);
+  unpack_r11g11b10(lv[static_cast<uint64_t>(jb) * wl + ib], t11);
+  for (int c = 0; c < 3; ++c)
+    out[c] = (1.0f - al) * (1.0f - be) * t00[c] + al * (1.0f - be) * t10[c] + (1.0f - al) * be * t01[c] +
+             al * be * t11[c];
+}
+
+// textureLod with LINEAR_MIPMAP_LINEAR (the linear-clamp sampler bound at
+// renderer.cpp:1070-1071): level 0 at lod <= 0, otherwise floor(lod) and
+// floor(lod) + 1 blended by frac(lod), both clamped to the last level.
+__device__ __forceinline__ void atlas_trilinear(const ApplyArgs& a, float s, float t, float lod, float* out) {
+  const int q = a.levels - 1;
+  if (lod <= 0.0f) {
+    atlas_bilinear(a, 0, s, t, out);
+  } else if (lod >= static_cast<float>(q)) {
+    atlas_bilinear(a, q, s, t, out);
+  } else {
+    const float d1 = floorf(lod), f = lod - d1;
+    const int l1 = to_int(d1);
+    float t1[3], t2[3];
+    atlas_bilinear(a, l1, s, t, t1);
+    atlas_bilinear(a, l1 + 1, s, t, t2);
+    for (int c = 0; c < 3; ++c) out[c] = (1.0f - f) * t1[c] + f * t2[c];
+  }
+}
+
+// The per-pixel inputs of the specular lookup (cacheApply.frag:139-154).
+struct SpecPixel {
+  float lookup[2];  // specEnvLookupCoord
+  float lod;        // specularEnvmapLod
+  float color[3];   // specularColor
 };
 
 // SampleCacheIrradiance, lightcache.glsl:137-183, on an entry already in
@@ -73,8 +134,9 @@ __device__ __forceinline__ void cache_irradiance(const ApplyArgs& a, const float
 
 // ComputeLightingFromCaches, cacheApply.frag:28-118.  All 8 addresses are
 // fetched first, then all 8 entries, so one pixel keeps 16 loads in flight.
+template <bool kSpec>
 __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c, const float* diffuse, uint32_t live,
-                                         float* out) {
+                                         const SpecPixel& sp, float* out) {
   const drv_cascade& k = a.vol.cascades[c];
   const float acx = (wp.x - k.min[0]) / k.world_voxel_size;
   const float acy = (wp.y - k.min[1]) / k.world_voxel_size;
@@ -97,10 +159,19 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c,
   }
   const int nq = a.sh2 ? 8 : 4;
   float acc[3] = {0.0f, 0.0f, 0.0f};
+  float spec[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     // An address past the live caches (unclaimed cell, overflow) reads zeros.
     const bool ok = addr[i] < live;
+    if (kSpec && ok) {
+      // cacheApply.frag:102-106
+      const float ox = static_cast<float>(addr[i] % a.per_row), oy = static_cast<float>(addr[i] / a.per_row);
+      float smp[3];
+      atlas_trilinear(a, (sp.lookup[0] + ox) * a.per_cache_texcoord, (sp.lookup[1] + oy) * a.per_cache_texcoord,
+                      sp.lod, smp);
+      for (int k = 0; k < 3; ++k) spec[k] = spec[k] + smp[k] * w[i];
+    }
     const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(ok ? addr[i] : 0u) * a.stride);
     float4 q[8];
 #pragma unroll
@@ -114,8 +185,14 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c,
   out[0] = acc[0] * diffuse[0] / kPi;
   out[1] = acc[1] * diffuse[1] / kPi;
   out[2] = acc[2] * diffuse[2] / kPi;
+  if (kSpec) {  // cacheApply.frag:116
+    out[0] = out[0] + spec[0] * sp.color[0];
+    out[1] = out[1] + spec[1] * sp.color[1];
+    out[2] = out[2] + spec[2] * sp.color[2];
+  }
 }
 
+template <bool kSpec>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -131,15 +208,39 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
   const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d);
   const int c = cascade_of(a.vol, a.C, wp);
   const f3 n = unpack_normal16i(a.normal[2 * p], a.normal[2 * p + 1]);
-  const float diffuse[3] = {a.srgb_lut[a.albedo[3 * p]], a.srgb_lut[a.albedo[3 * p + 1]],
-                            a.srgb_lut[a.albedo[3 * p + 2]]};
+  float diffuse[3] = {a.srgb_lut[a.albedo[3 * p]], a.srgb_lut[a.albedo[3 * p + 1]],
+                      a.srgb_lut[a.albedo[3 * p + 2]]};
+  SpecPixel sp{};
+  if (kSpec) {
+    // cacheApply.frag:139-154 with lightingfunctions.glsl:26-44 and
+    // utils.glsl:193-201.
+    float rough = 1.0f, metal = 0.0f;
+    if (a.roughness_metallic) {
+      rough = static_cast<float>(a.roughness_metallic[2 * p]) / 255.0f;
+      metal = static_cast<float>(a.roughness_metallic[2 * p + 1]) / 255.0f;
+    }
+    for (int k = 0; k < 3; ++k) {
+      const float base = diffuse[k];
+      diffuse[k] = base * (1.0f - metal) + 0.02f * metal;
+      sp.color[k] = 0.04f * (1.0f - metal) + base * metal;
+    }
+    const float rsq = rough * rough;
+    const float blinn = 2.0f / (rsq * rsq + 0.0005f);
+    const float fs = static_cast<float>(a.S);
+    sp.lod = fmaxf(0.0f, log2f(fs * fs / (1.0f + blinn)) * 0.5f);
+    const float max_half = 0.5f / (exp2f(-ceilf(sp.lod)) * fs);
+    const LocalFrame fr = local_view_space(a.cam, wp);
+    const f3 cn = to_local(fr, n.x, n.y, n.z);
+    hemi_projection(cn.x, cn.y, cn.z, sp.lookup);
+    for (int k = 0; k < 2; ++k) sp.lookup[k] = fminf(fmaxf(sp.lookup[k], max_half), 1.0f - max_half);
+  }
   float col[3];
-  lighting(a, wp, n, c, diffuse, live, col);
+  lighting<kSpec>(a, wp, n, c, diffuse, live, sp, col);
   if (a.transitions) {  // cacheApply.frag:172-185
     const float t = cascade_transition(a.vol, c, wp);
     if (t > 0.0f && c < a.C - 1) {
       float col2[3];
-      lighting(a, wp, n, c + 1, diffuse, live, col2);
+      lighting<kSpec>(a, wp, n, c + 1, diffuse, live, sp, col2);
       col[0] = col[0] * (1.0f - t) + col2[0] * t;
       col[1] = col[1] * (1.0f - t) + col2[1] * t;
       col[2] = col[2] * (1.0f - t) + col2[2] * t;
@@ -160,7 +261,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
 
 void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const drv_sh_constants& shc,
                   const drv_gbuffer& gb, const uint32_t* cav, const float* caches, const Counter* counter,
-                  const float* srgb_lut, float* out_rgb, int accumulate, hipStream_t s) {
+                  const float* srgb_lut, const SpecAtlas* spec, float* out_rgb, int accumulate, hipStream_t s) {
   ApplyArgs a;
   a.vol = vol;
   for (int i = 0; i < 16; ++i) a.inv_vp[i] = frame.inverse_view_projection[i];
@@ -181,8 +282,27 @@ void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame&
   a.sh2 = cfg.sh_bands >= 2 ? 1 : 0;
   a.transitions = cfg.cav_transition_size > 0.0f ? 1 : 0;
   a.accumulate = accumulate;
+  a.roughness_metallic = gb.roughness_metallic;
+  a.atlas = nullptr;
+  for (int l = 0; l < kSpecMaxLevels; ++l) a.level_offset[l] = 0;
+  a.atlas_width = a.per_row = 0;
+  a.S = a.levels = 0;
+  a.per_cache_texcoord = 0.0f;
+  for (int i = 0; i < 3; ++i) a.cam[i] = frame.camera_position[i];
+  if (spec) {
+    a.atlas = spec->atlas;
+    for (int l = 0; l < kSpecMaxLevels; ++l) a.level_offset[l] = spec->level_offset[l];
+    a.atlas_width = spec->width;
+    a.per_row = spec->per_row;
+    a.S = spec->S;
+    a.levels = spec->levels;
+    a.per_cache_texcoord = static_cast<float>(spec->S) / static_cast<float>(spec->width);
+  }
   const dim3 grid((cfg.width + 15) / 16, (cfg.height + 15) / 16);
-  hipLaunchKernelGGL(k_apply, grid, dim3(256), 0, s, a);
+  if (spec)
+    hipLaunchKernelGGL(k_apply<true>, grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_apply<false>, grid, dim3(256), 0, s, a);
 }
 
 }  // namespace drvk

@@ -51,6 +51,7 @@ struct PrepArgs {
   int R_shadow;
   uint32_t interval;
   int block;  // B = 2^lod texels per block side
+  int flux_cut;
   Vpl* vpls;
   ShadowBlock* blocks;
 };
@@ -95,6 +96,9 @@ __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
   v.fr = half_to_float(a.flux[3 * i]);
   v.fg = half_to_float(a.flux[3 * i + 1]);
   v.fb = half_to_float(a.flux[3 * i + 2]);
+  // cacheLightingRSM.comp:240-243: with INDIRECT_SPECULAR a VAL whose flux
+  // sums below 0.001 is skipped (`continue`); a zero flux adds nothing.
+  if (a.flux_cut && (v.fr + v.fg) + v.fb < 0.001f) v.fr = v.fg = v.fb = 0.0f;
   v.pad0 = 0.0f;
   v.nx = nrm.x;
   v.ny = nrm.y;
@@ -389,7 +393,7 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
 }  // namespace
 
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         hipStream_t s) {
+                         int flux_cut, hipStream_t s) {
   PrepArgs a;
   for (int i = 0; i < 16; ++i) a.inv_lvp[i] = light.inverse_light_view_projection[i];
   for (int i = 0; i < 3; ++i) a.light_pos[i] = light.light_position[i];
@@ -404,6 +408,7 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.R_shadow = rsm.read_resolution >> static_cast<int>(light.indirect_shadow_lod);
   a.interval = static_cast<uint32_t>(light.indirect_shadow_sample_interval);
   a.block = 1 << static_cast<int>(light.indirect_shadow_lod);
+  a.flux_cut = flux_cut;
   a.vpls = vpls;
   a.blocks = blocks;
   const uint32_t n = static_cast<uint32_t>(a.R) * a.R;

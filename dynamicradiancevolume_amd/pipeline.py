@@ -53,6 +53,7 @@ class DeviceFrame:
         self.depth = t(fi.depth)
         self.gnormal = t(fi.gnormal)
         self.galbedo = t(fi.galbedo)
+        self.groughmetal = t(fi.groughmetal) if fi.groughmetal is not None else None
         flux, rn, dls = fi.read_level
         self.flux = t(flux)
         self.rnormal = t(rn)
@@ -62,8 +63,9 @@ class DeviceFrame:
 
     def gbuffer(self) -> _abi.DrvGBuffer:
         w = self.inputs.workload
+        rm = self.groughmetal.data_ptr() if self.groughmetal is not None else None
         return _abi.DrvGBuffer(self.depth.data_ptr(), self.gnormal.data_ptr(), self.galbedo.data_ptr(), w.width,
-                               w.height)
+                               w.height, rm)
 
     def scene(self) -> _abi.DrvScene:
         return _abi.DrvScene(self.positions.data_ptr(), self.albedo.data_ptr(), self.normals.data_ptr(),
@@ -258,6 +260,21 @@ class Renderer:
         self._check(self.lib.drv_read_voxels(self.ctx, level, out.ctypes.data), "drv_read_voxels")
         return out
 
+    # --- specular env-map atlas ----------------------------------------------
+    def specular_atlas_geometry(self) -> tuple[int, int, int]:
+        """(width, caches per row, levels) of the specular atlas."""
+        w, n, lv = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.drv_get_specular_atlas(self.ctx, C.byref(w), C.byref(n), C.byref(lv)),
+                    "drv_get_specular_atlas")
+        return w.value, n.value, lv.value
+
+    def read_specular_atlas(self, level: int) -> np.ndarray:
+        """Packed R11G11B10F texels of one atlas level, [w >> level, w >> level]."""
+        w = self.specular_atlas_geometry()[0] >> level
+        out = np.empty((w, w), np.uint32)
+        self._check(self.lib.drv_read_specular_atlas(self.ctx, level, out.ctypes.data), "drv_read_specular_atlas")
+        return out
+
     def read_visibility(self, n_caches: int, n_blocks: int) -> np.ndarray:
         out = np.empty((n_caches, n_blocks), np.float32)
         self._check(self.lib.drv_read_visibility(self.ctx, out.ctypes.data, out.size), "drv_read_visibility")
@@ -347,7 +364,8 @@ class Batch:
         self.handle = C.c_void_p()
         rc = self.lib.drv_batch_create(arr, len(renderers), C.byref(self.handle))
         if rc != _abi.DRV_OK:
-            raise DrvError(f"drv_batch_create failed ({rc}): views must share SH bands, shadow mode and stream")
+            raise DrvError(f"drv_batch_create failed ({rc}): views must share SH bands, shadow mode and stream, "
+                           "and have indirect specular off")
 
     def close(self):
         if self.handle:

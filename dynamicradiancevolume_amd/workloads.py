@@ -61,6 +61,9 @@ class Workload:
     transition_size: float = 2.0   # renderer.cpp:47
     max_caches: int = 262144
     adaption_factor: float = 1.0   # voxels snap to the target
+    specular: int = 0              # INDIRECT_SPECULAR (renderer.cpp:49: off)
+    specular_size: int = 16        # per-cache env-map side (renderer.cpp:43)
+    hole_fill: int = 0             # hole-fill level (renderer.cpp:44)
 
     def config(self) -> _abi.DrvConfig:
         return _abi.DrvConfig(
@@ -68,7 +71,8 @@ class Workload:
             cav_resolution=self.cav_resolution, cav_cascades=self.cav_cascades,
             voxel_resolution=self.voxel_resolution, sh_bands=self.sh_bands,
             indirect_shadow=self.indirect_shadow, cav_transition_size=self.transition_size,
-            cav_cascade0_world_size=self.cascade0_world_size,
+            cav_cascade0_world_size=self.cascade0_world_size, indirect_specular=self.specular,
+            specular_envmap_size=self.specular_size, specular_hole_fill_level=self.hole_fill,
         )
 
     def with_(self, **kw) -> "Workload":
@@ -136,6 +140,7 @@ class FrameInputs:
     galbedo: np.ndarray        # uint8 [H, W, 3]
     light: _abi.DrvSpotLight
     rsm_levels: list = field(default_factory=list)  # [(flux u16 [r,r,3], normal i16 [r,r,2], dls u16 [r,r,2])]
+    groughmetal: np.ndarray | None = None  # uint8 [H, W, 2]: roughness, metallic (RG8)
 
     @property
     def read_level(self):
@@ -221,4 +226,16 @@ def make_inputs(w: Workload, n_threads: int = 0) -> FrameInputs:
         levels.append((f2, n2, d2))
         r = r2
     return FrameInputs(workload=w, positions=pos, albedo=alb, normals=nrm, bbox=bbox, view_proj=vp,
-                       inv_view_proj=ivp, depth=depth, gnormal=gn, galbedo=ga, light=light, rsm_levels=levels)
+                       inv_view_proj=ivp, depth=depth, gnormal=gn, galbedo=ga, light=light, rsm_levels=levels,
+                       groughmetal=roughness_metallic(w))
+
+
+def roughness_metallic(w: Workload) -> np.ndarray:
+    """RG8 material texture of the view: roughness uniform in [0.1, 1];
+    metallic 0 (60 %), 1 (20 %) or uniform (20 %); drawn per pixel with
+    numpy's PCG64 seeded by the scene seed."""
+    rng = np.random.default_rng(1000 + w.seed)
+    rough = rng.uniform(0.1, 1.0, (w.height, w.width))
+    kind = rng.uniform(0.0, 1.0, (w.height, w.width))
+    metal = np.where(kind < 0.6, 0.0, np.where(kind < 0.8, 1.0, rng.uniform(0.0, 1.0, (w.height, w.width))))
+    return np.stack([np.rint(rough * 255), np.rint(metal * 255)], axis=-1).astype(np.uint8)

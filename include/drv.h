@@ -62,6 +62,14 @@ typedef struct drv_config {
   int32_t indirect_shadow;     /* INDIRECT_SHADOW: cone-traced visibility on/off */
   float cav_transition_size;   /* CAVTransitionZoneSize; > 0 defines ADDRESSVOL_CASCADE_TRANSITIONS */
   float cav_cascade0_world_size; /* world size of cascade 0 (renderer.cpp:1185: 4.0); each further cascade doubles */
+  /* INDIRECT_SPECULAR (SetIndirectSpecular, renderer.hpp:80; off by default,
+   * renderer.cpp:49) with DIRECT_SPECULAR_MAP_WRITE: per-cache hemisphere
+   * env-maps in an R11G11B10F atlas, read by drv_apply_caches. */
+  int32_t indirect_specular;
+  int32_t specular_envmap_size;  /* SetPerCacheSpecularEnvMapSize (renderer.cpp:454-460): power of two
+                                    1..32; 0 selects the reference default 16 */
+  int32_t specular_hole_fill_level; /* SetSpecularEnvMapHoleFillLevel (renderer.hpp:97): clamped to
+                                       log2(size); 0 = no hole fill (default) */
 } drv_config;
 
 /* One CAV cascade, struct CAVCascade (shader/globalubos.glsl:48-60). */
@@ -137,6 +145,10 @@ typedef struct drv_gbuffer {
   const uint8_t* albedo;  /* device, SRGB8, 3 bytes per pixel, sRGB-encoded */
   int32_t width;
   int32_t height;
+  /* device, RG8 (roughness, metallic), texture unit 3 (GBuffer_RoughnessMetalic);
+   * read by drv_apply_caches with indirect_specular only.  NULL reads as
+   * roughness 1, metallic 0. */
+  const uint8_t* roughness_metallic;
 } drv_gbuffer;
 
 /* Reflective shadow map of one light at its read level (texture units 0-2 of
@@ -166,6 +178,7 @@ typedef struct drv_stage_times {
   uint64_t cone_traces;  /* (cache, shadow block) traces of the last light, when counting */
   uint64_t cone_steps;   /* march steps the reference takes for those traces, when counting */
   uint64_t cone_steps_executed; /* steps actually sampled (rays stop once opaque) */
+  float specular_ms;     /* specular env-map gather + mips/hole fill (ProcessSpecularEnvmap), inside light_caches_ms */
 } drv_stage_times;
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -221,6 +234,19 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
  * (the reference's GL_ONE/GL_ONE blend), 0 overwrites covered pixels and
  * writes 0 to discarded ones. */
 int drv_apply_caches(drv_ctx* ctx, const drv_gbuffer* gbuffer, float* out_rgb, int32_t accumulate);
+
+/* ---- indirect specular env-map atlas ----------------------------------- */
+
+/* Geometry of the specular atlas (renderer.cpp:253-286): level-0 width in
+ * texels (2^ceil(log2(ceil(sqrt(max_caches)) * size))), caches per atlas row
+ * (cache i sits at ((i % per_row) * size, (i / per_row) * size)) and mip
+ * levels (log2(size) + 1).  DRV_ERR_STATE when indirect_specular is off. */
+int drv_get_specular_atlas(drv_ctx* ctx, int32_t* width, int32_t* caches_per_row, int32_t* levels);
+/* Host copy of one atlas level: (width >> level)^2 packed texels
+ * (GL_UNSIGNED_INT_10F_11F_11F_REV: red bits 0-10, green 11-21, blue 22-31),
+ * row-major.  Only the regions of the live caches of the last light pass
+ * are defined. */
+int drv_read_specular_atlas(drv_ctx* ctx, int32_t level, uint32_t* host_out);
 
 /* ---- introspection (synchronous) --------------------------------------- */
 

@@ -376,6 +376,10 @@ void gather_one(const drv_config* cfg, const drv_volume* vol, const drv_sh_const
     }
     // cacheLightingRSM.comp:236-277
     const Vpl<T>& v = vpls[m];
+    // cacheLightingRSM.comp:240-243: INDIRECT_SPECULAR skips dim VALs.
+    if (cfg->indirect_specular &&
+        (static_cast<float>(v.flux.x) + static_cast<float>(v.flux.y)) + static_cast<float>(v.flux.z) < 0.001f)
+      continue;
     V3<T> to_val = v.pos - wp;
     const T d2 = dot(to_val, to_val);
     to_val = to_val * (T(1.0) / std::sqrt(d2));
@@ -419,6 +423,41 @@ const int kShOffset[9][3] = {
     {24, 25, 26},  // SH2pos1
     {28, 29, 30},  // SH2pos2
 };
+
+// ---- indirect specular ---------------------------------------------------
+
+int specular_size(const drv_config* cfg) { return cfg->specular_envmap_size == 0 ? 16 : cfg->specular_envmap_size; }
+
+int specular_fill(const drv_config* cfg) {
+  return std::min(cfg->specular_hole_fill_level, log2i(specular_size(cfg)));
+}
+
+// renderer.cpp:253: 2^ceil(log2(ceil(sqrt(max caches)) * S)).
+uint32_t specular_width(const drv_config* cfg) {
+  const double side = std::ceil(std::sqrt(static_cast<double>(cfg->max_caches))) * specular_size(cfg);
+  return static_cast<uint32_t>(std::pow(2.0, std::ceil(std::log2(side))));
+}
+
+size_t specular_level_offset(uint32_t width, int level) {
+  size_t o = 0;
+  for (int l = 0; l < level; ++l) o += static_cast<size_t>(width >> l) * (width >> l);
+  return o;
+}
+
+// ComputeLocalViewSpace, lightcache.glsl:98-106 (columns X, Y, Z).
+void local_view_space(const float* cam, const float* wp, float X[3], float Y[3], float Z[3]) {
+  const V3<float> z = normalize(mk<float>(cam[0] - wp[0], cam[1] - wp[1], cam[2] - wp[2]));
+  const V3<float> x = normalize(mk<float>(z.z, 0.0f, -z.x));
+  const V3<float> y = mk<float>(z.y * x.z - z.z * x.y, z.z * x.x - z.x * x.z, z.x * x.y - z.y * x.x);
+  X[0] = x.x, X[1] = x.y, X[2] = x.z;
+  Y[0] = y.x, Y[1] = y.y, Y[2] = y.z;
+  Z[0] = z.x, Z[1] = z.y, Z[2] = z.z;
+}
+
+// `v * mat3(X, Y, Z)` = (dot(v, X), dot(v, Y), dot(v, Z)).
+V3<float> to_local(const float X[3], const float Y[3], const float Z[3], V3<float> v) {
+  return mk<float>(dot(v, load3<float>(X)), dot(v, load3<float>(Y)), dot(v, load3<float>(Z)));
+}
 
 }  // namespace
 
@@ -1104,9 +1143,290 @@ void orc_direct_lighting(const drv_config* cfg, const drv_frame* frame, const dr
     }
 }
 
+// ---- indirect specular -----------------------------------------------------
+
+uint32_t orc_uf_encode(float x, int32_t m) {
+  const uint32_t max_finite = (30u << m) | ((1u << m) - 1u);
+  if (std::isnan(x)) return (31u << m) | 1u;
+  if (!(x > 0.0f)) return 0u;
+  if (std::isinf(x)) return 31u << m;
+  // Candidates: the two representable neighbours of x, found from the
+  // binade; nearest wins, ties to the even mantissa.
+  int e2;
+  (void)std::frexp(x, &e2);  // x = f * 2^e2, f in [0.5, 1)
+  const int e = std::max(e2 - 1, -14);  // binade exponent (denormals share -14)
+  const double ulp = std::ldexp(1.0, e - m);
+  const double q = static_cast<double>(x) / ulp;  // exact
+  double lo = std::floor(q);
+  const double frac = q - lo;
+  double k = lo;
+  if (frac > 0.5 || (frac == 0.5 && std::fmod(lo, 2.0) != 0.0)) k = lo + 1.0;
+  const double v = k * ulp;  // the rounded value
+  if (v > static_cast<double>(std::ldexp(1.0, 15) * (2.0 - std::ldexp(1.0, -m)))) return max_finite;
+  // Encode v: exponent field from its binade.
+  if (v < std::ldexp(1.0, -14)) return static_cast<uint32_t>(v / std::ldexp(1.0, -14 - m));
+  int ev;
+  const double fv = std::frexp(v, &ev);  // v = fv * 2^ev, fv in [0.5, 1)
+  const uint32_t exp_field = static_cast<uint32_t>(ev - 1 + 15);
+  const uint32_t mant = static_cast<uint32_t>((fv * 2.0 - 1.0) * std::ldexp(1.0, m));
+  return (exp_field << m) | mant;
+}
+
+float orc_uf_decode(uint32_t c, int32_t m) {
+  const uint32_t e = c >> m, f = c & ((1u << m) - 1u);
+  if (e == 0u) return static_cast<float>(std::ldexp(static_cast<double>(f), -14 - m));
+  if (e == 31u) return f ? std::nanf("") : INFINITY;
+  return static_cast<float>(std::ldexp(1.0 + std::ldexp(static_cast<double>(f), -m), static_cast<int>(e) - 15));
+}
+
+uint32_t orc_pack_r11g11b10(const float rgb[3]) {
+  return orc_uf_encode(rgb[0], 6) | (orc_uf_encode(rgb[1], 6) << 11) | (orc_uf_encode(rgb[2], 5) << 22);
+}
+
+void orc_unpack_r11g11b10(uint32_t v, float rgb[3]) {
+  rgb[0] = orc_uf_decode(v & 0x7ffu, 6);
+  rgb[1] = orc_uf_decode((v >> 11) & 0x7ffu, 6);
+  rgb[2] = orc_uf_decode(v >> 22, 5);
+}
+
+void orc_hemi_projection(const float d[3], float out[2]) {
+  // utils.glsl:146-177
+  const float r = std::sqrt(1.0f - d[2]) * 0.5f;
+  float phi = std::atan2(d[1], d[0]) * (4.0f / kPi);
+  if (phi < -1.0f) phi += 8.0f;
+  float px, py;
+  if (phi < 3.0f) {
+    if (phi < 1.0f) {
+      px = r;
+      py = phi * r;
+    } else {
+      px = -(phi - 2.0f) * r;
+      py = r;
+    }
+  } else {
+    if (phi < 5.0f) {
+      px = -r;
+      py = -(phi - 4.0f) * r;
+    } else {
+      px = (phi - 6.0f) * r;
+      py = -r;
+    }
+  }
+  out[0] = px + 0.5f;
+  out[1] = py + 0.5f;
+}
+
+void orc_local_view_space(const float cam[3], const float wp[3], float x[3], float y[3], float z[3]) {
+  local_view_space(cam, wp, x, y, z);
+}
+
+void orc_specular_atlas_geometry(const drv_config* cfg, int32_t* width, int32_t* per_row, int32_t* levels) {
+  const int S = specular_size(cfg);
+  const uint32_t w = specular_width(cfg);
+  if (width) *width = static_cast<int32_t>(w);
+  if (per_row) *per_row = static_cast<int32_t>(w / static_cast<uint32_t>(S));
+  if (levels) *levels = log2i(S) + 1;
+}
+
+void orc_specular_gather(const drv_config* cfg, const drv_frame* frame, const drv_spot_light* light,
+                         const orc_rsm* rsm, const float* pos, int64_t n, const float* visibility, float* maps,
+                         int32_t n_threads) {
+  std::vector<Vpl<float>> vpls;
+  prepare_vpls<float>(light, rsm, vpls);
+  const int S = specular_size(cfg), S1 = S + 1;
+  const uint32_t interval = static_cast<uint32_t>(light->indirect_shadow_sample_interval);
+  const int64_t nb = cfg->indirect_shadow ? static_cast<int64_t>(vpls.size() / interval) : 0;
+  const int nt = resolve_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+  for (int64_t c = 0; c < n; ++c) {
+    float* map = maps + static_cast<size_t>(c) * S1 * S1 * 3;
+    std::fill(map, map + static_cast<size_t>(S1) * S1 * 3, 0.0f);
+    const float* wp = pos + 3 * c;
+    float X[3], Y[3], Z[3];
+    local_view_space(frame->camera_position, wp, X, Y, Z);  // cacheLightingRSM.comp:122-124
+    const V3<float> w = load3<float>(wp);
+    float shadowing = 1.0f;
+    for (uint32_t m = 0; m < vpls.size(); ++m) {
+      if (cfg->indirect_shadow && m % interval == 0)
+        shadowing = visibility ? visibility[static_cast<size_t>(c) * nb + m / interval] : 1.0f;
+      const Vpl<float>& v = vpls[m];
+      // cacheLightingRSM.comp:240-264
+      if ((v.flux.x + v.flux.y) + v.flux.z < 0.001f) continue;
+      V3<float> to_val = v.pos - w;
+      const float d2 = dot(to_val, to_val);
+      to_val = to_val * (1.0f / std::sqrt(d2));
+      float k = clamp01(dot(v.normal, mk<float>(-to_val.x, -to_val.y, -to_val.z)));
+      k = k * shadowing;
+      const V3<float> rad = v.flux * (k / (d2 + v.disc_area));
+      // cacheLightingRSM.comp:281-335
+      const V3<float> l = to_local(X, Y, Z, to_val);
+      const V3<float> h = normalize(mk<float>(l.x, l.y, l.z + 1.0f));
+      const float hv[3] = {h.x, h.y, h.z};
+      float proj[2];
+      orc_hemi_projection(hv, proj);
+      const int tx = std::min(std::max(to_int(proj[0] * static_cast<float>(S) + 0.5f), 0), S);
+      const int ty = std::min(std::max(to_int(proj[1] * static_cast<float>(S) + 0.5f), 0), S);
+      const V3<float> out = rad * clamp01(dot(h, l));
+      float* t = map + (static_cast<size_t>(ty) * S1 + tx) * 3;
+      t[0] += out.x;
+      t[1] += out.y;
+      t[2] += out.z;
+    }
+  }
+}
+
+void orc_specular_finalize(const drv_config* cfg, int64_t n, const float* maps, float* work, int32_t accumulate,
+                           uint32_t* atlas) {
+  const int S = specular_size(cfg), S1 = S + 1, levels = log2i(S) + 1, fill = specular_fill(cfg);
+  const uint32_t W = specular_width(cfg), N = W / static_cast<uint32_t>(S);
+  const float norm = ((static_cast<float>(S) * static_cast<float>(S) - 1.0f) + 8.0f) / (8.0f * kPi);
+  const size_t SS = static_cast<size_t>(S) * S;
+  // Own sums of the light (the in-map texels of each cache's map).
+  for (int64_t c = 0; c < n; ++c) {
+    const float* map = maps + static_cast<size_t>(c) * S1 * S1 * 3;
+    float* wk = work + static_cast<size_t>(c) * SS * 3;
+    for (int y = 0; y < S; ++y)
+      for (int x = 0; x < S; ++x)
+        for (int ch = 0; ch < 3; ++ch) {
+          const float v = map[(static_cast<size_t>(y) * S1 + x) * 3 + ch];
+          float& o = wk[(static_cast<size_t>(y) * S + x) * 3 + ch];
+          o = accumulate ? o + v : v;
+        }
+  }
+  // Texel index S lands in the atlas neighbour (dropped outside the atlas
+  // or on a cache past the live count): left column, upper row, corner -
+  // added after the neighbour's own sum in that order.
+  for (int64_t c = 0; c < n; ++c) {
+    const uint32_t cx = static_cast<uint32_t>(c) % N, cy = static_cast<uint32_t>(c) / N;
+    float* wk = work + static_cast<size_t>(c) * SS * 3;
+    const auto spill = [&](int64_t src, int sx, int sy) {
+      return maps + (static_cast<size_t>(src) * S1 * S1 + static_cast<size_t>(sy) * S1 + sx) * 3;
+    };
+    if (cx > 0)
+      for (int y = 0; y < S; ++y)
+        for (int ch = 0; ch < 3; ++ch) wk[(static_cast<size_t>(y) * S) * 3 + ch] += spill(c - 1, S, y)[ch];
+    if (cy > 0)
+      for (int x = 0; x < S; ++x)
+        for (int ch = 0; ch < 3; ++ch) wk[static_cast<size_t>(x) * 3 + ch] += spill(c - N, x, S)[ch];
+    if (cx > 0 && cy > 0)
+      for (int ch = 0; ch < 3; ++ch) wk[ch] += spill(c - N - 1, S, S)[ch];
+  }
+  // renderer.cpp:994-1045 per cache: level 0, mips, hole fill, store.
+  std::vector<float> lv;
+  std::vector<size_t> lofs(levels);
+  size_t total = 0;
+  for (int l = 0; l < levels; ++l) {
+    lofs[l] = total;
+    total += static_cast<size_t>(S >> l) * (S >> l);
+  }
+  lv.resize(total * 3);
+  const auto quantize = [](float v, int ch) {
+    const int m = ch == 2 ? 5 : 6;
+    return orc_uf_decode(orc_uf_encode(v, m), m);
+  };
+  for (int64_t c = 0; c < n; ++c) {
+    const uint32_t cx = static_cast<uint32_t>(c) % N, cy = static_cast<uint32_t>(c) / N;
+    const float* wk = work + static_cast<size_t>(c) * SS * 3;
+    for (size_t t = 0; t < SS; ++t)
+      for (int ch = 0; ch < 3; ++ch) lv[t * 3 + ch] = quantize(wk[t * 3 + ch] * norm, ch);
+    // specularenvmap_mipmap.frag
+    for (int l = 1; l < levels; ++l) {
+      const int sl = S >> l, sp = S >> (l - 1);
+      const float* src = lv.data() + lofs[l - 1] * 3;
+      float* dst = lv.data() + lofs[l] * 3;
+      for (int y = 0; y < sl; ++y)
+        for (int x = 0; x < sl; ++x) {
+          const size_t i00 = (static_cast<size_t>(2 * y) * sp + 2 * x) * 3, i10 = i00 + 3, i01 = i00 + sp * 3,
+                       i11 = i01 + 3;
+          for (int ch = 0; ch < 3; ++ch)
+            dst[(static_cast<size_t>(y) * sl + x) * 3 + ch] =
+                quantize(((src[i00 + ch] + src[i10 + ch]) + (src[i01 + ch] + src[i11 + ch])) * 0.25f, ch);
+        }
+    }
+    // specularenvmap_fillholes.frag:8-51, levels fill .. 1
+    for (int i = fill; i >= 1; --i) {
+      const int si = S >> i, sd = S >> (i - 1);
+      const float* src = lv.data() + lofs[i] * 3;
+      float* dst = lv.data() + lofs[i - 1] * 3;
+      for (int y = 0; y < si; ++y)
+        for (int x = 0; x < si; ++x) {
+          const float* s = src + (static_cast<size_t>(y) * si + x) * 3;
+          if (s[0] + s[1] + s[2] < 0.0001f) continue;
+          size_t di[4];
+          di[0] = (static_cast<size_t>(2 * y) * sd + 2 * x) * 3;
+          di[1] = di[0] + 3;
+          di[2] = di[0] + static_cast<size_t>(sd) * 3;
+          di[3] = di[2] + 3;
+          float d[4][3];
+          float sum[3] = {0.0f, 0.0f, 0.0f};
+          for (int k = 0; k < 4; ++k) {
+            for (int ch = 0; ch < 3; ++ch) d[k][ch] = dst[di[k] + ch];
+            if (d[k][0] + d[k][1] + d[k][2] == 0.0f)
+              for (int ch = 0; ch < 3; ++ch) d[k][ch] = s[ch];
+            for (int ch = 0; ch < 3; ++ch) sum[ch] += d[k][ch];
+          }
+          float en[3];
+          for (int ch = 0; ch < 3; ++ch) en[ch] = 4.0f * s[ch] / (sum[ch] + 0.00001f);
+          for (int k = 0; k < 4; ++k)
+            for (int ch = 0; ch < 3; ++ch) dst[di[k] + ch] = quantize(d[k][ch] * en[ch], ch);
+        }
+    }
+    for (int l = 0; l < levels; ++l) {
+      const int sl = S >> l;
+      const size_t wl = W >> l;
+      uint32_t* out = atlas + specular_level_offset(W, l);
+      for (int y = 0; y < sl; ++y)
+        for (int x = 0; x < sl; ++x) {
+          const float* v = lv.data() + (lofs[l] + static_cast<size_t>(y) * sl + x) * 3;
+          out[(static_cast<size_t>(cy) * sl + y) * wl + static_cast<size_t>(cx) * sl + x] = orc_pack_r11g11b10(v);
+        }
+    }
+  }
+}
+
+void orc_specular_sample(const drv_config* cfg, const uint32_t* atlas, float s, float t, float lod, float out[3]) {
+  // textureLod with LINEAR_MIPMAP_LINEAR, clamp to edge (GL 4.5 section 8.14)
+  const int S = specular_size(cfg);
+  const uint32_t AW = specular_width(cfg);
+  const int levels = log2i(S) + 1;
+  const auto bilinear = [&](int l, float* o) {
+    const int wl = static_cast<int>(AW >> l);
+    const uint32_t* lvl = atlas + specular_level_offset(AW, l);
+    const float u = s * static_cast<float>(wl) - 0.5f, v = t * static_cast<float>(wl) - 0.5f;
+    const float fu = std::floor(u), fv = std::floor(v);
+    const int i0 = to_int(fu), j0 = to_int(fv);
+    const float al = u - fu, be = v - fv;
+    const auto tex = [&](int i, int j, float* rgb) {
+      i = std::min(std::max(i, 0), wl - 1);
+      j = std::min(std::max(j, 0), wl - 1);
+      orc_unpack_r11g11b10(lvl[static_cast<size_t>(j) * wl + i], rgb);
+    };
+    float a[3], b[3], c[3], d[3];
+    tex(i0, j0, a);
+    tex(i0 + 1, j0, b);
+    tex(i0, j0 + 1, c);
+    tex(i0 + 1, j0 + 1, d);
+    for (int ch = 0; ch < 3; ++ch)
+      o[ch] = (1.0f - al) * (1.0f - be) * a[ch] + al * (1.0f - be) * b[ch] + (1.0f - al) * be * c[ch] +
+              al * be * d[ch];
+  };
+  const int q = levels - 1;
+  if (lod <= 0.0f) {
+    bilinear(0, out);
+  } else if (lod >= static_cast<float>(q)) {
+    bilinear(q, out);
+  } else {
+    const float d1 = std::floor(lod), f = lod - d1;
+    float t1[3], t2[3];
+    bilinear(to_int(d1), t1);
+    bilinear(to_int(d1) + 1, t2);
+    for (int ch = 0; ch < 3; ++ch) out[ch] = (1.0f - f) * t1[ch] + f * t2[ch];
+  }
+}
+
 void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* frame, const drv_sh_constants* shc,
                const float* depth, const int16_t* normal, const uint8_t* albedo, const uint32_t* cav,
-               const float* entries, uint32_t live, float* out, int32_t n_threads) {
+               const float* entries, uint32_t live, const orc_specular* spec, float* out, int32_t n_threads) {
   const int W = cfg->width, H = cfg->height, R = cfg->cav_resolution, C = cfg->cav_cascades;
   const int stride = entry_stride(cfg->sh_bands);
   const bool sh2 = cfg->sh_bands >= 2;
@@ -1138,8 +1458,19 @@ void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* fr
     }
   };
 
+  // Specular atlas: textureLod with the linear-clamp trilinear sampler
+  // (renderer.cpp:28-29, 1070-1071).
+  const bool use_spec = spec && cfg->indirect_specular;
+  const int S = use_spec ? specular_size(cfg) : 0;
+  const uint32_t AW = use_spec ? specular_width(cfg) : 0u;
+  const uint32_t per_row = use_spec ? AW / static_cast<uint32_t>(S) : 1u;
+  const auto trilinear = [&](float s, float t, float lod, float* o) {
+    orc_specular_sample(cfg, spec->atlas, s, t, lod, o);
+  };
+
   // ComputeLightingFromCaches, cacheApply.frag:28-118
-  auto lighting = [&](const float* wp, const float* n, int c, const float* diffuse, float* outc) {
+  auto lighting = [&](const float* wp, const float* n, int c, const float* diffuse, const float* lookup, float lod,
+                      const float* spec_color, float* outc) {
     const drv_cascade& k = vol->cascades[c];
     float ac[3];
     int a0[3];
@@ -1164,17 +1495,27 @@ void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* fr
     static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
                                   {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
     float acc[3] = {0.0f, 0.0f, 0.0f};
+    float sacc[3] = {0.0f, 0.0f, 0.0f};
     for (int i = 0; i < 8; ++i) {
       const int x = a0[0] + off[i][0] + R * c, y = a0[1] + off[i][1], z = a0[2] + off[i][2];
       uint32_t addr = 0;
       if (x >= 0 && y >= 0 && z >= 0 && x < C * R && y < R && z < R)
         addr = cav[(static_cast<size_t>(z) * R + y) * (static_cast<size_t>(C) * R) + x];
       addr -= 1u;
+      if (use_spec && addr < live) {  // cacheApply.frag:102-106 (no env-map past the live caches)
+        const float ox = static_cast<float>(addr % per_row), oy = static_cast<float>(addr / per_row);
+        const float tc = static_cast<float>(S) / static_cast<float>(AW);
+        float smp[3];
+        trilinear((lookup[0] + ox) * tc, (lookup[1] + oy) * tc, lod, smp);
+        for (int ch = 0; ch < 3; ++ch) sacc[ch] = sacc[ch] + smp[ch] * w[i];
+      }
       float irr[3];
       sample_cache(addr, n, irr);
       for (int ch = 0; ch < 3; ++ch) acc[ch] = acc[ch] + irr[ch] * w[i];
     }
     for (int ch = 0; ch < 3; ++ch) outc[ch] = acc[ch] * diffuse[ch] / kPi;
+    if (use_spec)
+      for (int ch = 0; ch < 3; ++ch) outc[ch] = outc[ch] + sacc[ch] * spec_color[ch];
   };
 
   const int nt = resolve_threads(n_threads);
@@ -1193,14 +1534,37 @@ void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* fr
       const int c = cascade_of(vol, C, wp);
       float n[3];
       orc_unpack_normal16i(normal[2 * p], normal[2 * p + 1], n);
-      const float diffuse[3] = {lut[albedo[3 * p]], lut[albedo[3 * p + 1]], lut[albedo[3 * p + 2]]};
+      float diffuse[3] = {lut[albedo[3 * p]], lut[albedo[3 * p + 1]], lut[albedo[3 * p + 2]]};
+      float lookup[2] = {0.0f, 0.0f}, lod = 0.0f, spec_color[3] = {0.0f, 0.0f, 0.0f};
+      if (use_spec) {
+        // cacheApply.frag:139-154, lightingfunctions.glsl:26-44, utils.glsl:193-201
+        const uint8_t* rm = spec->roughness_metallic;
+        const float rough = rm ? static_cast<float>(rm[2 * p]) / 255.0f : 1.0f;
+        const float metal = rm ? static_cast<float>(rm[2 * p + 1]) / 255.0f : 0.0f;
+        for (int ch = 0; ch < 3; ++ch) {
+          const float base = diffuse[ch];
+          diffuse[ch] = base * (1.0f - metal) + 0.02f * metal;
+          spec_color[ch] = 0.04f * (1.0f - metal) + base * metal;
+        }
+        const float rsq = rough * rough;
+        const float blinn = 2.0f / (rsq * rsq + 0.0005f);
+        const float fs = static_cast<float>(S);
+        lod = std::max(0.0f, std::log2(fs * fs / (1.0f + blinn)) * 0.5f);
+        const float max_half = 0.5f / (std::exp2(-std::ceil(lod)) * fs);
+        float X[3], Y[3], Z[3];
+        local_view_space(frame->camera_position, wp, X, Y, Z);
+        const V3<float> cn = to_local(X, Y, Z, load3<float>(n));
+        const float cv[3] = {cn.x, cn.y, cn.z};
+        orc_hemi_projection(cv, lookup);
+        for (int k = 0; k < 2; ++k) lookup[k] = std::min(std::max(lookup[k], max_half), 1.0f - max_half);
+      }
       float col[3];
-      lighting(wp, n, c, diffuse, col);
+      lighting(wp, n, c, diffuse, lookup, lod, spec_color, col);
       if (cfg->cav_transition_size > 0.0f) {  // cacheApply.frag:172-185
         const float t = cascade_transition(vol, c, wp);
         if (t > 0.0f && c < C - 1) {
           float col2[3];
-          lighting(wp, n, c + 1, diffuse, col2);
+          lighting(wp, n, c + 1, diffuse, lookup, lod, spec_color, col2);
           for (int ch = 0; ch < 3; ++ch) col[ch] = col[ch] * (1.0f - t) + col2[ch] * t;
         }
       }

@@ -128,12 +128,59 @@ void orc_mlp_features(const drv_volume* vol, int32_t voxel_res, const uint8_t* v
 /* Extracts the 27 SH coefficients (canonical order above) from an entry. */
 void orc_entry_sh(const float* entry, int32_t sh_bands, float out27[27]);
 
+/* ---- indirect specular (INDIRECT_SPECULAR, DIRECT_SPECULAR_MAP_WRITE) ---- */
+/* GL unsigned 11/10-bit float conversion (GL 4.5 section 2.3.4.3-4): round
+ * to the nearest representable finite value, ties to even; negative -> 0,
+ * finite overflow -> largest finite, +inf -> inf, NaN -> NaN.  m = mantissa
+ * bits (6 or 5).  Written by nearest-neighbour search in double. */
+uint32_t orc_uf_encode(float x, int32_t m);
+float orc_uf_decode(uint32_t code, int32_t m);
+/* GL_UNSIGNED_INT_10F_11F_11F_REV packing: red bits 0-10, green 11-21, blue 22-31. */
+uint32_t orc_pack_r11g11b10(const float rgb[3]);
+void orc_unpack_r11g11b10(uint32_t v, float rgb[3]);
+/* HemisphericalProjection, concentric quad (utils.glsl:102-178). */
+void orc_hemi_projection(const float dir[3], float out[2]);
+/* ComputeLocalViewSpace (lightcache.glsl:98-106): the columns X, Y, Z. */
+void orc_local_view_space(const float cam[3], const float wp[3], float x[3], float y[3], float z[3]);
+/* Atlas of renderer.cpp:253-286 for cfg (specular_envmap_size 0 = 16). */
+void orc_specular_atlas_geometry(const drv_config* cfg, int32_t* width, int32_t* per_row, int32_t* levels);
+/* cacheLightingRSM.comp:87-99, 122-124, 240-335 for caches at pos[3*n]:
+ * each cache's (S+1)^2 RGB map maps[n][(S+1)^2][3] (x fastest) of the light's
+ * reflected radiance before the base normalization, VALs in the shader's
+ * order; texel index S is the column / row that the reference's store puts
+ * into the next cache of the atlas.  visibility[n][n_blocks] (as
+ * orc_gather_f32 writes it) is the shadowing per block; NULL = unshadowed. */
+void orc_specular_gather(const drv_config* cfg, const drv_frame* frame, const drv_spot_light* light,
+                         const orc_rsm* rsm, const float* pos, int64_t n, const float* visibility, float* maps,
+                         int32_t n_threads);
+/* The atlas writes of one light for the n live caches: adds the in-map
+ * texels into the FP32 sums work[n][S*S][3] (accumulate = 0 overwrites, the
+ * frame's first light), routes index-S texels to the left/upper/upper-left
+ * atlas neighbours' edge sums, then per cache: level 0 = R11G11B10F of
+ * sum * BlinnNormalization(S^2 - 1), the mips of specularenvmap_mipmap.frag
+ * and the hole fill of specularenvmap_fillholes.frag
+ * (renderer.cpp:994-1045), packed into atlas (levels back to back). */
+void orc_specular_finalize(const drv_config* cfg, int64_t n, const float* maps, float* work, int32_t accumulate,
+                           uint32_t* atlas);
+
+/* textureLod(CacheSpecularEnvmap, (s, t), lod) on the atlas with the
+ * linear-clamp sampler (LINEAR_MIPMAP_LINEAR, clamp to edge). */
+void orc_specular_sample(const drv_config* cfg, const uint32_t* atlas, float s, float t, float lod, float out[3]);
+
+typedef struct orc_specular {
+  const uint32_t* atlas;             /* as orc_specular_finalize writes it */
+  const uint8_t* roughness_metallic; /* RG8 or NULL (roughness 1, metallic 0) */
+} orc_specular;
+
 /* ---- stage D: apply ----------------------------------------------------- */
 /* cacheApply.frag:28-195 + lightcache.glsl:137-183: out_rgb float[3*W*H];
- * discarded pixels (depth < 1e-5) get 0. */
+ * discarded pixels (depth < 1e-5) get 0.  spec (NULL = none) adds the
+ * INDIRECT_SPECULAR lookup when cfg->indirect_specular is set; caches past
+ * live_count contribute no specular. */
 void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* frame, const drv_sh_constants* shc,
                const float* depth, const int16_t* normal, const uint8_t* albedo, const uint32_t* cav,
-               const float* entries, uint32_t live_count, float* out_rgb, int32_t n_threads);
+               const float* entries, uint32_t live_count, const orc_specular* spec, float* out_rgb,
+               int32_t n_threads);
 
 /* ---- brute-force RSM reference ----------------------------------------- */
 /* bruteforcersm.frag:15-88 with BRDF of lightingfunctions.glsl:46-58 for the

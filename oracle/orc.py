@@ -66,8 +66,23 @@ _FUNCS = {
     "orc_mlp_features": (None, [P(_abi.DrvVolume), C.c_int32, vp, P(C.c_float), P(C.c_float), P(C.c_float),
                                 P(C.c_float)]),
     "orc_apply": (None, [P(_abi.DrvConfig), P(_abi.DrvVolume), P(_abi.DrvFrame), P(_abi.DrvShConstants), vp, vp, vp,
-                         vp, vp, C.c_uint32, vp, C.c_int32]),
+                         vp, vp, C.c_uint32, vp, vp, C.c_int32]),
+    "orc_uf_encode": (C.c_uint32, [C.c_float, C.c_int32]),
+    "orc_uf_decode": (C.c_float, [C.c_uint32, C.c_int32]),
+    "orc_pack_r11g11b10": (C.c_uint32, [P(C.c_float)]),
+    "orc_unpack_r11g11b10": (None, [C.c_uint32, P(C.c_float)]),
+    "orc_hemi_projection": (None, [P(C.c_float), P(C.c_float)]),
+    "orc_local_view_space": (None, [P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float)]),
+    "orc_specular_atlas_geometry": (None, [P(_abi.DrvConfig), P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+    "orc_specular_gather": (None, [P(_abi.DrvConfig), P(_abi.DrvFrame), P(_abi.DrvSpotLight), P(OrcRsm), vp,
+                                   C.c_int64, vp, vp, C.c_int32]),
+    "orc_specular_finalize": (None, [P(_abi.DrvConfig), C.c_int64, vp, vp, C.c_int32, vp]),
+    "orc_specular_sample": (None, [P(_abi.DrvConfig), vp, C.c_float, C.c_float, C.c_float, P(C.c_float)]),
 }
+
+
+class OrcSpecular(C.Structure):
+    _fields_ = [("atlas", vp), ("roughness_metallic", vp)]
 
 _lib = None
 
@@ -239,12 +254,115 @@ def bruteforce_rsm(fi, diffuse_only=True, roughness_metallic=None, pixels=None, 
     return out
 
 
-def apply(cfg, vol, frame, shc, fi, cav: np.ndarray, entries: np.ndarray, live: int, n_threads: int = 0):
+def apply(cfg, vol, frame, shc, fi, cav: np.ndarray, entries: np.ndarray, live: int, n_threads: int = 0,
+          atlas: np.ndarray | None = None, roughness_metallic: np.ndarray | None = None):
+    """Stage D; with `atlas` (specular_finalize's output) and
+    cfg.indirect_specular the specular lookup is added."""
     out = np.zeros((cfg.height, cfg.width, 3), np.float32)
+    spec = None
+    keep = []
+    if atlas is not None:
+        keep = [np.ascontiguousarray(atlas, np.uint32)]
+        if roughness_metallic is not None:
+            keep.append(np.ascontiguousarray(roughness_metallic, np.uint8))
+        spec = OrcSpecular(_ptr(keep[0]), _ptr(keep[1]) if len(keep) > 1 else None)
     lib().orc_apply(C.byref(cfg), C.byref(vol), C.byref(frame), C.byref(shc), _ptr(fi.depth), _ptr(fi.gnormal),
                     _ptr(fi.galbedo), _ptr(np.ascontiguousarray(cav)), _ptr(np.ascontiguousarray(entries)), live,
-                    _ptr(out), n_threads)
+                    C.byref(spec) if spec is not None else None, _ptr(out), n_threads)
+    del keep
     return out
+
+
+# ---- indirect specular ------------------------------------------------------
+
+def uf_encode(x: float, m: int) -> int:
+    return int(lib().orc_uf_encode(float(x), m))
+
+
+def uf_decode(code: int, m: int) -> float:
+    return float(lib().orc_uf_decode(int(code), m))
+
+
+def unpack_r11g11b10(packed: np.ndarray) -> np.ndarray:
+    """Decodes packed texels (any shape) to float32 [..., 3]."""
+    flat = np.ascontiguousarray(packed, np.uint32).ravel()
+    out = np.empty((flat.size, 3), np.float32)
+    # Table decode: every 11- and 10-bit code through the oracle.
+    t11 = np.array([uf_decode(c, 6) for c in range(2048)], np.float32)
+    t10 = np.array([uf_decode(c, 5) for c in range(1024)], np.float32)
+    out[:, 0] = t11[flat & 0x7FF]
+    out[:, 1] = t11[(flat >> 11) & 0x7FF]
+    out[:, 2] = t10[flat >> 22]
+    return out.reshape(packed.shape + (3,))
+
+
+def hemi_projection(d) -> np.ndarray:
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.empty(2, np.float32)
+    lib().orc_hemi_projection(d.ctypes.data_as(P(C.c_float)), out.ctypes.data_as(P(C.c_float)))
+    return out
+
+
+def local_view_space(cam, wp) -> np.ndarray:
+    """Rows X, Y, Z of ComputeLocalViewSpace (its columns)."""
+    c = np.ascontiguousarray(cam, np.float32)
+    w = np.ascontiguousarray(wp, np.float32)
+    o = np.empty((3, 3), np.float32)
+    f = lambda a: a.ctypes.data_as(P(C.c_float))  # noqa: E731
+    lib().orc_local_view_space(f(c), f(w), f(o[0]), f(o[1]), f(o[2]))
+    return o
+
+
+def specular_atlas_geometry(cfg) -> tuple[int, int, int]:
+    w, n, l = C.c_int32(), C.c_int32(), C.c_int32()
+    lib().orc_specular_atlas_geometry(C.byref(cfg), C.byref(w), C.byref(n), C.byref(l))
+    return w.value, n.value, l.value
+
+
+def specular_level_offsets(width: int, levels: int) -> list:
+    offs, o = [], 0
+    for lvl in range(levels):
+        offs.append(o)
+        o += (width >> lvl) ** 2
+    return offs + [o]
+
+
+def specular_sample(cfg, atlas: np.ndarray, s: float, t: float, lod: float) -> np.ndarray:
+    out = np.empty(3, np.float32)
+    lib().orc_specular_sample(C.byref(cfg), _ptr(np.ascontiguousarray(atlas, np.uint32)), s, t, lod,
+                              out.ctypes.data_as(P(C.c_float)))
+    return out
+
+
+def specular_gather(cfg, frame, light, fi, positions: np.ndarray, visibility: np.ndarray | None = None,
+                    n_threads: int = 0) -> np.ndarray:
+    """(S+1)^2 maps of one light for caches at `positions` [n, 3]."""
+    rsm, keep = orc_rsm(fi)
+    pos = np.ascontiguousarray(positions, np.float32)
+    n = len(pos)
+    S = cfg.specular_envmap_size or 16
+    maps = np.zeros((n, (S + 1) ** 2, 3), np.float32)
+    vis = np.ascontiguousarray(visibility, np.float32) if visibility is not None else None
+    lib().orc_specular_gather(C.byref(cfg), C.byref(frame), C.byref(light), C.byref(rsm), _ptr(pos), n,
+                              _ptr(vis) if vis is not None else None, _ptr(maps), n_threads)
+    del keep
+    return maps
+
+
+def specular_finalize(cfg, maps: np.ndarray, work: np.ndarray | None = None, atlas: np.ndarray | None = None):
+    """Atlas writes of one light for len(maps) live caches; `work` (FP32 sums)
+    and `atlas` carry over between the lights of a frame (None = first light)."""
+    n = len(maps)
+    S = cfg.specular_envmap_size or 16
+    width, _, levels = specular_atlas_geometry(cfg)
+    accumulate = work is not None
+    if work is None:
+        work = np.zeros((n, S * S, 3), np.float32)
+    if atlas is None:
+        atlas = np.zeros(specular_level_offsets(width, levels)[-1], np.uint32)
+    lib().orc_specular_finalize(C.byref(cfg), n, _ptr(np.ascontiguousarray(maps, np.float32)), _ptr(work),
+                                1 if accumulate else 0, _ptr(atlas))
+    return work, atlas
 
 
 def run_frame(fi, n_threads: int = 0, gather: bool = True):
