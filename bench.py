@@ -269,6 +269,62 @@ def _inputs_and_reference(fi, df, stream, w):
     }
 
 
+def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int):
+    """The same frame with INDIRECT_SPECULAR (16 x 16 env-map per cache, the
+    reference defaults): frame time, the env-map stage, and the end-to-end
+    error of the unshadowed cache path (diffuse SH + specular env-maps)
+    against the brute-force RSM with the full BRDF."""
+    import numpy as np
+    import torch
+
+    from dynamicradiancevolume_amd.pipeline import Renderer
+
+    ws = w.with_(specular=1)
+    r = Renderer(ws.config(), stream)
+    for _ in range(warmup):
+        r.render_frame(df)
+    ms, _ = _time_frames(r, df, stream, steps)
+    r.set_profiling(1)
+    keys = ("light_caches_ms", "gather_ms", "specular_ms", "apply_ms")
+    acc = {k: 0.0 for k in keys}
+    n_prof = max(3, min(10, steps))
+    for _ in range(n_prof):
+        r.render_frame(df)
+        t = r.stage_times()
+        for k in keys:
+            acc[k] += t[k]
+    r.set_profiling(0)
+    stages = {k: round(v / n_prof, 4) for k, v in acc.items()}
+    _, live = r.cache_count()
+    width, per_row, levels = r.specular_atlas_geometry()
+    r.close()
+
+    ru = Renderer(ws.with_(indirect_shadow=0).config(), stream)
+    ru.render_frame(df)
+    stream.synchronize()
+    cache_rgb = df.out.clone()
+    bf = torch.zeros_like(cache_rgb)
+    torch.cuda.synchronize()
+    ru.bruteforce_rsm(df.gbuffer(), df.rsm(), fi.light, bf, False, roughness_metallic=df.groughmetal)
+    stream.synchronize()
+    lit = torch.from_numpy(fi.depth >= 1e-5).to(bf.device)
+    a, b = cache_rgb[lit].double(), bf[lit].double()
+    err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    ru.close()
+    return {
+        "envmap_size": ws.specular_size,
+        "hole_fill_level": ws.hole_fill,
+        "atlas": {"width": width, "caches_per_row": per_row, "levels": levels},
+        "ms_per_frame": ms,
+        "stage_ms": stages,
+        "envmap_interactions_per_s": live * n_vpl / (stages["specular_ms"] * 1e-3) if stages["specular_ms"] > 0
+        else None,
+        "roughness_metallic": "per pixel, roughness U[0.1, 1], metallic 0 / 1 / U[0, 1] (60/20/20 %)",
+        "unshadowed_cache_path_vs_bruteforce_full_brdf_rel_l2": err,
+        "lit_pixels": int(np.count_nonzero(fi.depth >= 1e-5)),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +334,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-specular", action="store_true")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched workload (0 = skip)")
     ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
     args = ap.parse_args()
@@ -381,6 +438,11 @@ def main() -> int:
         print("[bench] surrogate", file=sys.stderr, flush=True)
         sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold)
 
+    spec = None
+    if not args.no_specular:
+        print("[bench] indirect specular", file=sys.stderr, flush=True)
+        spec = _specular(fi, df, stream, w, args.steps, args.warmup, n_vpl)
+
     print("[bench] device inputs and brute-force reference", file=sys.stderr, flush=True)
     ref = _inputs_and_reference(fi, df, stream, w)
     ref["host_producer_s"] = host_inputs_s
@@ -429,6 +491,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "specular": spec,
         "batch": batch,
         "inputs_and_reference": ref,
     }

@@ -434,6 +434,7 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->counter);
   dev_free(ctx->cav);
   dev_free(ctx->spec.atlas);
+  dev_free(ctx->spec.vpls_lanes);
   dev_free(ctx->spec.work);
   dev_free(ctx->spec.spill);
   dev_free(ctx->vox_target);
@@ -734,7 +735,14 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   }
   record(ctx, 5);
   const int specular = ctx->cfg.indirect_specular;
-  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, specular, ctx->stream);
+  if (specular && n_vpl > ctx->spec.vpls_lanes_cap) {
+    dev_free(ctx->spec.vpls_lanes);
+    ctx->spec.vpls_lanes_cap = 0;
+    if ((rc = dev_alloc(ctx, &ctx->spec.vpls_lanes, n_vpl)) != DRV_OK) return rc;
+    ctx->spec.vpls_lanes_cap = n_vpl;
+  }
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, specular,
+                            specular ? ctx->spec.vpls_lanes : nullptr, ctx->stream);
   record(ctx, 6);
   const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
   if (use_net) {
@@ -760,7 +768,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   if (specular) {
     // renderer.cpp:553-558: LightCachesRSM with the env-map writes, then
     // PrepareSpecularEnvmaps.
-    drvk::launch_specular(ctx->frame.camera_position, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
+    drvk::launch_specular(ctx->frame.camera_position, ctx->caches, stride, ctx->counter, ctx->spec.vpls_lanes, n_vpl,
                           shadow ? ctx->visibility : nullptr, static_cast<uint32_t>(cap), interval, ctx->spec,
                           ctx->spec_first ? 1 : 0, ctx->cu_count * 8, ctx->stream);
     ctx->spec_first = false;
@@ -1047,7 +1055,7 @@ int drv_bruteforce_rsm(drv_ctx* ctx, const drv_gbuffer* gb, const uint8_t* rough
   }
   // VAL records (position, flux, normal) of the read level; the VPL order
   // does not matter to a plain sum.
-  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, 0, ctx->stream);
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, 0, nullptr, ctx->stream);
   drvk::launch_bruteforce(ctx->cfg, ctx->frame, *light, *gb, roughness_metallic, ctx->srgb_lut, ctx->vpls, n,
                           diffuse_only, out_rgb, accumulate, ctx->stream);
   return check_hip(ctx, hipGetLastError(), "bruteforce_rsm");
@@ -1402,7 +1410,8 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
     join_voxels(ctx);  // the batch runs on the views' shared stream
-    drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, 0, b->stream);
+    drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, 0, nullptr,
+                              b->stream);
     // The batch leaves no per-pair visibility behind.
     ctx->last_n_blocks = 0;
     ctx->label_caches = nullptr;

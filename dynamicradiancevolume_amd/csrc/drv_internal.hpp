@@ -129,6 +129,8 @@ struct Tuning {
 constexpr int kSpecMaxLevels = 6;  // S <= 32
 struct SpecAtlas {
   uint32_t* atlas = nullptr;
+  Vpl* vpls_lanes = nullptr;  // lane-interleaved VPL records of the current light
+  size_t vpls_lanes_cap = 0;
   float* work = nullptr;   // [max_caches][S * S][3]
   float* spill = nullptr;  // [max_caches][2S + 1][3]
   uint32_t width = 0, per_row = 0;
@@ -152,8 +154,10 @@ void launch_voxel_corners(const VoxelLayout& layout, const uint8_t* mips, uint64
 
 // flux_cut: INDIRECT_SPECULAR skips VALs whose flux sums below 0.001
 // (cacheLightingRSM.comp:240-243); their flux is stored as zero.
+// vpls_lanes (optional): the same records lane-interleaved for the specular
+// gather, record m at (m % per_lane) * 64 + m / per_lane, per_lane = n / 64.
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         int flux_cut, hipStream_t s);
+                         int flux_cut, Vpl* vpls_lanes, hipStream_t s);
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
                        uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid, hipStream_t s);

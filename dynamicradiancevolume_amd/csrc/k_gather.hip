@@ -53,6 +53,7 @@ struct PrepArgs {
   int block;  // B = 2^lod texels per block side
   int flux_cut;
   Vpl* vpls;
+  Vpl* vpls_lanes;  // optional lane-interleaved copy: [m % per_lane][m / per_lane], per_lane = n / 64
   ShadowBlock* blocks;
 };
 
@@ -105,6 +106,10 @@ __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
   v.nz = nrm.z;
   v.pad1 = 0.0f;
   a.vpls[m] = v;
+  if (a.vpls_lanes) {
+    const uint32_t per_lane = n / 64u;
+    a.vpls_lanes[static_cast<size_t>(m % per_lane) * 64u + m / per_lane] = v;
+  }
   if (a.blocks && (m % a.interval) == 0u) {
     // cacheLightingRSM.comp:171-192: bilinear (d, d^2) at the shadow LOD
     const float ss = (static_cast<float>(x) + a.sampling_offset) / fR;
@@ -393,7 +398,7 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
 }  // namespace
 
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         int flux_cut, hipStream_t s) {
+                         int flux_cut, Vpl* vpls_lanes, hipStream_t s) {
   PrepArgs a;
   for (int i = 0; i < 16; ++i) a.inv_lvp[i] = light.inverse_light_view_projection[i];
   for (int i = 0; i < 3; ++i) a.light_pos[i] = light.light_position[i];
@@ -410,6 +415,7 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.block = 1 << static_cast<int>(light.indirect_shadow_lod);
   a.flux_cut = flux_cut;
   a.vpls = vpls;
+  a.vpls_lanes = vpls_lanes;
   a.blocks = blocks;
   const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
   hipLaunchKernelGGL(k_prepare_vpls, dim3((n + 255) / 256), dim3(256), 0, s, a);

@@ -14,9 +14,14 @@
 //
 // MI355X design.  A wave owns one cache at a time and keeps its map in LDS
 // in FP32 (the reference read-modify-writes the R11G11B10F image once per
-// VAL); lanes take VALs strided over the RSM (lane l: VALs l*n/64 ...), so
-// the 64 texel updates of one step come from 64 distant RSM regions and
-// rarely collide in the LDS atomics.  The light's FP32 sums go to a
+// VAL).  Lane l takes the Morton-consecutive VALs l*n/64 ... (l+1)*n/64 - 1
+// of one RSM block, read from a lane-interleaved copy so every step's loads
+// coalesce.  Neighbouring VALs of a block mostly reflect into the same
+// texel, so a lane sums runs in registers and adds a run into the LDS map
+// only when its texel changes: a quarter of the steps add, and the 64
+// blocks of a step rarely hit the same texel (LDS float atomics to one
+// address serialize, which made per-VAL adds four times slower).  The
+// light's FP32 sums go to a
 // per-cache work buffer that accumulates over the lights of a frame; the
 // finalize kernel (one workgroup per cache) quantizes, builds the mips and
 // fills holes in LDS and writes every level of the cache's atlas region
@@ -47,7 +52,7 @@ struct SpecGatherArgs {
   const float* caches;
   int stride;
   const Counter* counter;
-  const Vpl* vpls;
+  const Vpl* vpls;   // lane-interleaved: lane l, step i reads record i * 64 + l = VPL l * n / 64 + i
   uint32_t n_vpl;
   const float* vis;  // [block][vis_pitch] or null (unshadowed)
   uint32_t vis_pitch;
@@ -58,8 +63,11 @@ struct SpecGatherArgs {
   int first;     // 1: the first light of the cache list overwrites the work sums
 };
 
+// ivec2(p * S + 0.5) clamped to [0, S] (NaN -> 0): the clamp runs on the
+// float, so the conversion is always in range.
 __device__ __forceinline__ int texel_index(float p, int S) {
-  return min(max(to_int(p * static_cast<float>(S) + 0.5f), 0), S);
+  const float fs = static_cast<float>(S);
+  return static_cast<int>(fminf(fmaxf(p * fs + 0.5f, 0.0f), fs));
 }
 
 // Dynamic LDS: 4 maps of (S+1)^2 RGB floats.
@@ -80,10 +88,15 @@ __global__ __launch_bounds__(256) void k_specular_gather(SpecGatherArgs a) {
     const f3 wp = make3(e[0], e[1], e[2]);
     const LocalFrame fr = local_view_space(a.cam, wp);
     const uint32_t m0 = lane * per_lane;
+    float shadowing = 1.0f;
+    int run_tex = -1;
+    float run_r = 0.0f, run_g = 0.0f, run_b = 0.0f;
     for (uint32_t i = 0; i < per_lane; ++i) {
       const uint32_t m = m0 + i;
-      const Vpl v = a.vpls[m];
-      const float shadowing = a.vis ? a.vis[static_cast<size_t>(m / a.interval) * a.vis_pitch + c] : 1.0f;
+      const Vpl v = a.vpls[static_cast<size_t>(i) * 64u + lane];
+      // The block's shadowing changes every `interval` VPLs.
+      if (a.vis && (i == 0u || m % a.interval == 0u))
+        shadowing = a.vis[static_cast<size_t>(m / a.interval) * a.vis_pitch + c];
       // radianceIn, cacheLightingRSM.comp:246-264 (the SH gather's expressions)
       const float tx0 = v.px - wp.x, ty0 = v.py - wp.y, tz0 = v.pz - wp.z;
       const float d2 = __builtin_fmaf(tz0, tz0, __builtin_fmaf(ty0, ty0, tx0 * tx0));
@@ -100,10 +113,35 @@ __global__ __launch_bounds__(256) void k_specular_gather(SpecGatherArgs a) {
       hemi_projection(hx, hy, hz, proj);
       const int ix = texel_index(proj[0], S), iy = texel_index(proj[1], S);
       const float k = f * fminf(1.0f, fmaxf(0.0f, __builtin_fmaf(hz, l.z, __builtin_fmaf(hy, l.y, hx * l.x))));
-      float* t = map + (iy * S1 + ix) * 3;
-      atomicAdd(t, v.fr * k);
-      atomicAdd(t + 1, v.fg * k);
-      atomicAdd(t + 2, v.fb * k);
+      // Run accumulation: consecutive VALs of a lane mostly hit the same
+      // texel, so the lane sums in registers and adds the run into the map
+      // when the texel changes (zero contributions change nothing).
+      const float cr = v.fr * k, cg = v.fg * k, cb = v.fb * k;
+      if (!(cr == 0.0f && cg == 0.0f && cb == 0.0f)) {
+        const int tex = iy * S1 + ix;
+        if (tex != run_tex) {
+          if (run_tex >= 0) {
+            float* t = map + run_tex * 3;
+            atomicAdd(t, run_r);
+            atomicAdd(t + 1, run_g);
+            atomicAdd(t + 2, run_b);
+          }
+          run_tex = tex;
+          run_r = cr;
+          run_g = cg;
+          run_b = cb;
+        } else {
+          run_r += cr;
+          run_g += cg;
+          run_b += cb;
+        }
+      }
+    }
+    if (run_tex >= 0) {
+      float* t = map + run_tex * 3;
+      atomicAdd(t, run_r);
+      atomicAdd(t + 1, run_g);
+      atomicAdd(t + 2, run_b);
     }
     wave_lds_sync();
     float* wk = a.work + static_cast<size_t>(c) * S * S * 3;
