@@ -386,7 +386,7 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
     A.fill = ctx->cfg.specular_hole_fill_level;
     const double side = std::ceil(std::sqrt(static_cast<double>(c.max_caches))) * A.S;
     const double w = std::pow(2.0, std::ceil(std::log2(side)));
-    if (w > 65536.0) return bail(DRV_ERR_INVALID_ARG);
+    if (w > 32768.0) return bail(DRV_ERR_INVALID_ARG);  // texel indices of a level stay 32-bit
     A.width = static_cast<uint32_t>(w);
     A.per_row = A.width / static_cast<uint32_t>(A.S);
     uint64_t off = 0;

@@ -38,57 +38,86 @@ struct ApplyArgs {
   const uint8_t* roughness_metallic;  // RG8 or null (roughness 1, metallic 0)
   const uint32_t* atlas;
   uint64_t level_offset[kSpecMaxLevels];
-  uint32_t atlas_width, per_row;
+  uint32_t atlas_width, per_row, per_row_log2;  // caches per atlas row: a power of two
   int S, levels;
-  float per_cache_texcoord;  // SpecularEnvmapPerCacheSize_Texcoord = S / width
   float cam[3];
 };
 
-// Bilinear tap of atlas level l (clamp to edge), the GL 4.5 section 8.14.2
-// weights in FP32.
-__device__ __forceinline__ void atlas_bilinear(const ApplyArgs& a, int l, float s, float t, float* out) {
-  const int wl = static_cast<int>(a.atlas_width >> l);
-  const float u = s * static_cast<float>(wl) - 0.5f, v = t * static_cast<float>(wl) - 0.5f;
+// One level's bilinear footprint inside a cache's S_l x S_l region: the
+// lookup is clamped half a texel of the coarser level away from the region's
+// border (cacheApply.frag:151-152), so the 2x2 footprint, its weights and
+// its offset within the region are the same for all 8 caches.  The texture
+// coordinate (lookup + cache offset) * S_l - 0.5 is split into the in-region
+// part lookup * S_l - 0.5 (exact in FP32 for lookups >= half a texel) and
+// the integer cache offset, which is the exact value of GL's coordinate.
+struct Tap {
+  int l;          // level
+  int i0, j0;     // footprint origin within the region
+  float w[4];     // GL 4.5 section 8.14.2 weights: (i0,j0), (i0+1,j0), (i0,j0+1), (i0+1,j0+1)
+};
+
+__device__ __forceinline__ Tap make_tap(const ApplyArgs& a, int l, const float* lookup) {
+  Tap t;
+  t.l = l;
+  const float sl = static_cast<float>(a.S >> l);
+  const float u = lookup[0] * sl - 0.5f, v = lookup[1] * sl - 0.5f;
   const float fu = floorf(u), fv = floorf(v);
-  const int i0 = to_int(fu), j0 = to_int(fv);
+  t.i0 = to_int(fu);
+  t.j0 = to_int(fv);
   const float al = u - fu, be = v - fv;
-  const int ia = min(max(i0, 0), wl - 1), ib = min(max(i0 + 1, 0), wl - 1);
-  const int ja = min(max(j0, 0), wl - 1), jb = min(max(j0 + 1, 0), wl - 1);
-  const uint32_t* lv = a.atlas + a.level_offset[l];
+  t.w[0] = (1.0f - al) * (1.0f - be);
+  t.w[1] = al * (1.0f - be);
+  t.w[2] = (1.0f - al) * be;
+  t.w[3] = al * be;
+  return t;
+}
+
+// R11G11B10F texel -> RGB: exponent rebias for normal codes, f * 2^(-14-m)
+// for denormal ones, inf / NaN for exponent 31.
+__device__ __forceinline__ float decode_channel(uint32_t c, int m) {
+  const uint32_t e = c >> m;
+  float v = __builtin_bit_cast(float, (c << (23 - m)) + (112u << 23));
+  if (e == 0u) v = static_cast<float>(c) * __builtin_ldexpf(1.0f, -14 - m);
+  if (e == 31u) v = (c & ((1u << m) - 1u)) ? __builtin_nanf("") : __builtin_inff();
+  return v;
+}
+
+__device__ __forceinline__ void decode_texel(uint32_t x, float* rgb) {
+  rgb[0] = decode_channel(x & 0x7ffu, 6);
+  rgb[1] = decode_channel((x >> 11) & 0x7ffu, 6);
+  rgb[2] = decode_channel(x >> 22, 5);
+}
+
+// Bilinear tap of cache (cx, cy) at tap.l, clamp to the atlas edge.
+__device__ __forceinline__ void tap_cache(const ApplyArgs& a, const Tap& t, uint32_t cx, uint32_t cy, float* out) {
+  const int sl = a.S >> t.l;
+  const int wl = static_cast<int>(a.atlas_width >> t.l);
+  const int x0 = static_cast<int>(cx) * sl + t.i0, y0 = static_cast<int>(cy) * sl + t.j0;
+  const int ia = min(max(x0, 0), wl - 1), ib = min(max(x0 + 1, 0), wl - 1);
+  const int ja = min(max(y0, 0), wl - 1), jb = min(max(y0 + 1, 0), wl - 1);
+  const uint32_t* lv = a.atlas + a.level_offset[t.l];
+  const uint32_t ra = static_cast<uint32_t>(ja) * static_cast<uint32_t>(wl), rb = static_cast<uint32_t>(jb) * static_cast<uint32_t>(wl);
+  const uint32_t q00 = lv[ra + ia], q10 = lv[ra + ib], q01 = lv[rb + ia], q11 = lv[rb + ib];
   float t00[3], t10[3], t01[3], t11[3];
-  unpack_r11g11b10(lv[static_cast<uint64_t>(ja) * wl + ia], t00);
-  unpack_r11g11b10(lv[static_cast<uint64_t>(ja) * wl + ib], t10);
-  unpack_r11g11b10(lv[static_cast<uint64_t>(jb) * wl + ia], t01);
-  unpack_r11g11b10(lv[static_cast<uint64_t>(jb) * wl + ib], t11);
-  for (int c = 0; c < 3; ++c)
-    out[c] = (1.0f - al) * (1.0f - be) * t00[c] + al * (1.0f - be) * t10[c] + (1.0f - al) * be * t01[c] +
-             al * be * t11[c];
+  decode_texel(q00, t00);
+  decode_texel(q10, t10);
+  decode_texel(q01, t01);
+  decode_texel(q11, t11);
+  for (int c = 0; c < 3; ++c) out[c] = t.w[0] * t00[c] + t.w[1] * t10[c] + t.w[2] * t01[c] + t.w[3] * t11[c];
 }
 
-// textureLod with LINEAR_MIPMAP_LINEAR (the linear-clamp sampler bound at
-// renderer.cpp:1070-1071): level 0 at lod <= 0, otherwise floor(lod) and
-// floor(lod) + 1 blended by frac(lod), both clamped to the last level.
-__device__ __forceinline__ void atlas_trilinear(const ApplyArgs& a, float s, float t, float lod, float* out) {
-  const int q = a.levels - 1;
-  if (lod <= 0.0f) {
-    atlas_bilinear(a, 0, s, t, out);
-  } else if (lod >= static_cast<float>(q)) {
-    atlas_bilinear(a, q, s, t, out);
-  } else {
-    const float d1 = floorf(lod), f = lod - d1;
-    const int l1 = to_int(d1);
-    float t1[3], t2[3];
-    atlas_bilinear(a, l1, s, t, t1);
-    atlas_bilinear(a, l1 + 1, s, t, t2);
-    for (int c = 0; c < 3; ++c) out[c] = (1.0f - f) * t1[c] + f * t2[c];
-  }
-}
-
-// The per-pixel inputs of the specular lookup (cacheApply.frag:139-154).
+// The per-pixel inputs of the specular lookup (cacheApply.frag:139-154) and
+// the textureLod level selection of LINEAR_MIPMAP_LINEAR (GL 4.5 section
+// 8.14.3, the sampler bound at renderer.cpp:1070-1071): level 0 at lod <= 0,
+// the last level at lod >= q, otherwise floor(lod) and floor(lod) + 1
+// blended by frac(lod).
 struct SpecPixel {
   float lookup[2];  // specEnvLookupCoord
   float lod;        // specularEnvmapLod
   float color[3];   // specularColor
+  Tap t1, t2;       // the two levels (t2 unused when blend = 0)
+  float blend;
+  bool two;
 };
 
 // SampleCacheIrradiance, lightcache.glsl:137-183, on an entry already in
@@ -166,10 +195,14 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c,
     const bool ok = addr[i] < live;
     if (kSpec && ok) {
       // cacheApply.frag:102-106
-      const float ox = static_cast<float>(addr[i] % a.per_row), oy = static_cast<float>(addr[i] / a.per_row);
+      const uint32_t cx = addr[i] & (a.per_row - 1u), cy = addr[i] >> a.per_row_log2;
       float smp[3];
-      atlas_trilinear(a, (sp.lookup[0] + ox) * a.per_cache_texcoord, (sp.lookup[1] + oy) * a.per_cache_texcoord,
-                      sp.lod, smp);
+      tap_cache(a, sp.t1, cx, cy, smp);
+      if (sp.two) {
+        float s2[3];
+        tap_cache(a, sp.t2, cx, cy, s2);
+        for (int k = 0; k < 3; ++k) smp[k] = (1.0f - sp.blend) * smp[k] + sp.blend * s2[k];
+      }
       for (int k = 0; k < 3; ++k) spec[k] = spec[k] + smp[k] * w[i];
     }
     const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(ok ? addr[i] : 0u) * a.stride);
@@ -233,6 +266,20 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
     const f3 cn = to_local(fr, n.x, n.y, n.z);
     hemi_projection(cn.x, cn.y, cn.z, sp.lookup);
     for (int k = 0; k < 2; ++k) sp.lookup[k] = fminf(fmaxf(sp.lookup[k], max_half), 1.0f - max_half);
+    const int q = a.levels - 1;
+    sp.two = false;
+    sp.blend = 0.0f;
+    if (sp.lod <= 0.0f) {
+      sp.t1 = make_tap(a, 0, sp.lookup);
+    } else if (sp.lod >= static_cast<float>(q)) {
+      sp.t1 = make_tap(a, q, sp.lookup);
+    } else {
+      const float d1 = floorf(sp.lod);
+      sp.blend = sp.lod - d1;
+      sp.t1 = make_tap(a, to_int(d1), sp.lookup);
+      sp.t2 = make_tap(a, to_int(d1) + 1, sp.lookup);
+      sp.two = true;
+    }
   }
   float col[3];
   lighting<kSpec>(a, wp, n, c, diffuse, live, sp, col);
@@ -285,18 +332,17 @@ void launch_apply(const drv_config& cfg, const drv_volume& vol, const drv_frame&
   a.roughness_metallic = gb.roughness_metallic;
   a.atlas = nullptr;
   for (int l = 0; l < kSpecMaxLevels; ++l) a.level_offset[l] = 0;
-  a.atlas_width = a.per_row = 0;
+  a.atlas_width = a.per_row = a.per_row_log2 = 0;
   a.S = a.levels = 0;
-  a.per_cache_texcoord = 0.0f;
   for (int i = 0; i < 3; ++i) a.cam[i] = frame.camera_position[i];
   if (spec) {
     a.atlas = spec->atlas;
     for (int l = 0; l < kSpecMaxLevels; ++l) a.level_offset[l] = spec->level_offset[l];
     a.atlas_width = spec->width;
     a.per_row = spec->per_row;
+    a.per_row_log2 = static_cast<uint32_t>(__builtin_ctz(spec->per_row));
     a.S = spec->S;
     a.levels = spec->levels;
-    a.per_cache_texcoord = static_cast<float>(spec->S) / static_cast<float>(spec->width);
   }
   const dim3 grid((cfg.width + 15) / 16, (cfg.height + 15) / 16);
   if (spec)

@@ -67,7 +67,8 @@ typedef struct drv_config {
    * env-maps in an R11G11B10F atlas, read by drv_apply_caches. */
   int32_t indirect_specular;
   int32_t specular_envmap_size;  /* SetPerCacheSpecularEnvMapSize (renderer.cpp:454-460): power of two
-                                    1..32; 0 selects the reference default 16 */
+                                    1..32; 0 selects the reference default 16.  The atlas side
+                                    (see drv_get_specular_atlas) must stay <= 32768 texels. */
   int32_t specular_hole_fill_level; /* SetSpecularEnvMapHoleFillLevel (renderer.hpp:97): clamped to
                                        log2(size); 0 = no hole fill (default) */
 } drv_config;

@@ -1464,8 +1464,48 @@ void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* fr
   const int S = use_spec ? specular_size(cfg) : 0;
   const uint32_t AW = use_spec ? specular_width(cfg) : 0u;
   const uint32_t per_row = use_spec ? AW / static_cast<uint32_t>(S) : 1u;
-  const auto trilinear = [&](float s, float t, float lod, float* o) {
-    orc_specular_sample(cfg, spec->atlas, s, t, lod, o);
+  const int levels = use_spec ? log2i(S) + 1 : 0;
+  // textureLod on the env-map of cache (cx, cy): the texture coordinate
+  // (lookup + offset) * S_l - 0.5 is evaluated exactly, as the in-region part
+  // lookup * S_l - 0.5 (exact in FP32 for lookups at least half a texel in)
+  // plus the integer offset; the FP32 sum lookup + offset of the GLSL would
+  // round away up to 2^-15 of a cache, below GL's sub-texel precision.
+  const auto bilinear_cache = [&](int l, uint32_t cx, uint32_t cy, const float* lookup, float* o) {
+    const int sl = S >> l;
+    const int wl = static_cast<int>(AW >> l);
+    const uint32_t* lvl = spec->atlas + specular_level_offset(AW, l);
+    const float u = lookup[0] * static_cast<float>(sl) - 0.5f, v = lookup[1] * static_cast<float>(sl) - 0.5f;
+    const float fu = std::floor(u), fv = std::floor(v);
+    const int i0 = static_cast<int>(cx) * sl + to_int(fu), j0 = static_cast<int>(cy) * sl + to_int(fv);
+    const float al = u - fu, be = v - fv;
+    const auto tex = [&](int i, int j, float* rgb) {
+      i = std::min(std::max(i, 0), wl - 1);
+      j = std::min(std::max(j, 0), wl - 1);
+      orc_unpack_r11g11b10(lvl[static_cast<size_t>(j) * wl + i], rgb);
+    };
+    float a[3], b[3], c[3], d[3];
+    tex(i0, j0, a);
+    tex(i0 + 1, j0, b);
+    tex(i0, j0 + 1, c);
+    tex(i0 + 1, j0 + 1, d);
+    for (int ch = 0; ch < 3; ++ch)
+      o[ch] = (1.0f - al) * (1.0f - be) * a[ch] + al * (1.0f - be) * b[ch] + (1.0f - al) * be * c[ch] +
+              al * be * d[ch];
+  };
+  const auto trilinear = [&](uint32_t addr, const float* lookup, float lod, float* o) {
+    const uint32_t cx = addr % per_row, cy = addr / per_row;
+    const int q = levels - 1;
+    if (lod <= 0.0f) {
+      bilinear_cache(0, cx, cy, lookup, o);
+    } else if (lod >= static_cast<float>(q)) {
+      bilinear_cache(q, cx, cy, lookup, o);
+    } else {
+      const float d1 = std::floor(lod), f = lod - d1;
+      float t1[3], t2[3];
+      bilinear_cache(to_int(d1), cx, cy, lookup, t1);
+      bilinear_cache(to_int(d1) + 1, cx, cy, lookup, t2);
+      for (int ch = 0; ch < 3; ++ch) o[ch] = (1.0f - f) * t1[ch] + f * t2[ch];
+    }
   };
 
   // ComputeLightingFromCaches, cacheApply.frag:28-118
@@ -1503,10 +1543,8 @@ void orc_apply(const drv_config* cfg, const drv_volume* vol, const drv_frame* fr
         addr = cav[(static_cast<size_t>(z) * R + y) * (static_cast<size_t>(C) * R) + x];
       addr -= 1u;
       if (use_spec && addr < live) {  // cacheApply.frag:102-106 (no env-map past the live caches)
-        const float ox = static_cast<float>(addr % per_row), oy = static_cast<float>(addr / per_row);
-        const float tc = static_cast<float>(S) / static_cast<float>(AW);
         float smp[3];
-        trilinear((lookup[0] + ox) * tc, (lookup[1] + oy) * tc, lod, smp);
+        trilinear(addr, lookup, lod, smp);
         for (int ch = 0; ch < 3; ++ch) sacc[ch] = sacc[ch] + smp[ch] * w[i];
       }
       float irr[3];
