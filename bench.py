@@ -311,6 +311,13 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int):
     a, b = cache_rgb[lit].double(), bf[lit].double()
     err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
     ru.close()
+    # the same unshadowed frame without the specular term, against the same reference
+    rd = Renderer(w.with_(indirect_shadow=0).config(), stream)
+    rd.render_frame(df)
+    stream.synchronize()
+    d = df.out[lit].double()
+    err_diffuse = float(torch.linalg.norm(d - b) / torch.linalg.norm(b))
+    rd.close()
     return {
         "envmap_size": ws.specular_size,
         "hole_fill_level": ws.hole_fill,
@@ -321,6 +328,7 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int):
         else None,
         "roughness_metallic": "per pixel, roughness U[0.1, 1], metallic 0 / 1 / U[0, 1] (60/20/20 %)",
         "unshadowed_cache_path_vs_bruteforce_full_brdf_rel_l2": err,
+        "unshadowed_diffuse_only_cache_path_vs_bruteforce_full_brdf_rel_l2": err_diffuse,
         "lit_pixels": int(np.count_nonzero(fi.depth >= 1e-5)),
     }
 
