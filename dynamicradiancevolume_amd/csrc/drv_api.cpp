@@ -211,6 +211,7 @@ float elapsed(drv_ctx* ctx, int a, int b) {
 drvk::VoxelView voxel_view(const drv_ctx* ctx) {
   drvk::VoxelView v{};
   v.corners = ctx->vox_corners;
+  v.corner_bytes = static_cast<uint32_t>(ctx->vox.corner_count * sizeof(uint64_t));
   v.res = ctx->vox.res;
   v.levels = ctx->vox.levels;
   for (int l = 0; l < ctx->vox.levels && l < 16; ++l) v.corner_offset[l] = static_cast<uint32_t>(ctx->vox.corner_offset[l]);
@@ -296,7 +297,7 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   const drv_config& c = *config;
   if (c.width <= 0 || c.height <= 0 || c.max_caches <= 0 || c.cav_resolution <= 0 || c.cav_cascades <= 0 ||
       c.cav_cascades > DRV_MAX_CASCADES || !is_pow2(c.voxel_resolution) || c.voxel_resolution < 2 ||
-      c.voxel_resolution > 1024 || (c.sh_bands != 1 && c.sh_bands != 2) || c.cav_cascade0_world_size <= 0.0f)
+      c.voxel_resolution > 512 || (c.sh_bands != 1 && c.sh_bands != 2) || c.cav_cascade0_world_size <= 0.0f)
     return DRV_ERR_INVALID_ARG;
   // Address arithmetic of the CAV is 32-bit (cacheGather.comp:25-28).
   if (static_cast<double>(c.cav_cascades) * c.cav_resolution * c.cav_resolution * c.cav_resolution > 2147483647.0)

@@ -62,6 +62,7 @@ struct VoxelLayout {
 struct VoxelView {
   const uint64_t* corners;
   uint32_t corner_offset[16];
+  uint32_t corner_bytes;  // < 2^32 (voxel resolution <= 512)
   int res;
   int levels;
 };

@@ -82,10 +82,10 @@ struct BatchArgs {
 
 template <bool kSh2, bool kShadow>
 __global__ __launch_bounds__(256) void k_batch_gather(BatchArgs a) {
-  __shared__ uint32_t offs[4][16];  // per wave: corner offsets of the current item's view
+  __shared__ LevelInfo lvs[4][16];  // per wave: voxel level table of the current item's view
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t* off = offs[threadIdx.x >> 6];
-  uint32_t staged = 0xffffffffu;  // view whose offsets are in `off`
+  LevelInfo* lvi = lvs[threadIdx.x >> 6];
+  uint32_t staged = 0xffffffffu;  // view whose table is in `lvi`
   const uint32_t cpw = a.cpw;
   const uint32_t nsub = 64u / cpw;
   const uint32_t sub = lane / cpw, cl = lane % cpw;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(BatchArgs a) {
     const BatchView& bv = a.views[v];
     if (v != staged) {  // wave-local table: store, then read after a wavefront fence
       __builtin_amdgcn_wave_barrier();
-      if (lane < 16u) off[lane] = bv.vox.corner_offset[lane];
+      if (lane < 16u) lvi[lane] = level_info(bv.vox, static_cast<int>(lane));
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -132,7 +132,8 @@ __global__ __launch_bounds__(256) void k_batch_gather(BatchArgs a) {
           t.s = 32;
         }
         bool finished = false;
-        while (!finished) finished = trace_batch(bv.vox, off, t);
+        const __amdgpu_buffer_rsrc_t cr = corner_rsrc(bv.vox);
+        while (!finished) finished = trace_batch(bv.vox, cr, lvi, t);
         shadowing = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
       }
       const uint32_t m0 = b * interval;

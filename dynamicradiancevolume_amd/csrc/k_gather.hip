@@ -194,8 +194,9 @@ __device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint
 // rays from neighbouring caches towards the same VPL block).
 template <int kBatch>
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
-  __shared__ uint32_t off[16];
-  stage_offsets(a.vox, off);
+  __shared__ LevelInfo lvi[16];
+  stage_levels(a.vox, lvi);
+  const __amdgpu_buffer_rsrc_t cr = corner_rsrc(a.vox);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t lane = threadIdx.x & 63u;
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     for (uint32_t b = b0; b < b1; ++b) {
       Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[b]);
       bool finished = false;
-      while (!finished) finished = trace_batch<kBatch>(a.vox, off, t);
+      while (!finished) finished = trace_batch<kBatch>(a.vox, cr, lvi, t);
       a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
       my_traces += 1u;
       my_exec += static_cast<uint32_t>(t.s);
@@ -236,8 +237,9 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
 // the lanes whose trace ended take the next pool entries (ballot + mbcnt
 // ranks), so lanes do not idle behind the longest ray of the wave.
 __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
-  __shared__ uint32_t off[16];
-  stage_offsets(a.vox, off);
+  __shared__ LevelInfo lvi[16];
+  stage_levels(a.vox, lvi);
+  const __amdgpu_buffer_rsrc_t cr = corner_rsrc(a.vox);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t lane = threadIdx.x & 63u;
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
     };
     if (has) start(idx);
     while (true) {
-      const bool ended = trace_batch(a.vox, off, t);
+      const bool ended = trace_batch(a.vox, cr, lvi, t);
       const bool fin = has && ended;
       if (fin) {
         a.vis[static_cast<size_t>(blk) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));

@@ -56,7 +56,7 @@ struct FeatureCtx {
   VoxelView vox;
 };
 
-__device__ __forceinline__ void features_all(const FeatureCtx& fc, const uint32_t* off, f3 wp, const ShadowBlock& b,
+__device__ __forceinline__ void features_all(const FeatureCtx& fc, const LevelInfo* lvi, f3 wp, const ShadowBlock& b,
                                              float* out) {
   const float res = static_cast<float>(fc.vox.res);
   const float vsz = fc.voxel_size * res;
@@ -97,11 +97,12 @@ __device__ __forceinline__ void features_all(const FeatureCtx& fc, const uint32_
       d = 2.0f + span * ((static_cast<float>(k - 8) + 0.5f) * 0.25f);
       lvl = min(3, q);
     }
-    fp[k] = footprint(fc.vox, off, lvl, p0x + tx * d, p0y + ty * d, p0z + tz * d);
+    fp[k] = footprint(lvi, lvl, p0x + tx * d, p0y + ty * d, p0z + tz * d);
   }
   uint64_t c[12];
+  const __amdgpu_buffer_rsrc_t cr = corner_rsrc(fc.vox);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) c[k] = fc.vox.corners[fp[k].index];
+  for (int k = 0; k < 12; ++k) c[k] = corner_load(cr, fp[k].index);
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
 #pragma unroll
@@ -221,8 +222,8 @@ constexpr int kFeatLd = 40;  // bf16 per staged feature row (32 + pad)
 // in the lane that owns that cache, so the store is one coalesced row.
 __global__ __launch_bounds__(256) void k_visibility_mlp(InferArgs a) {
   __shared__ __align__(16) __bf16 stage[4][64 * kFeatLd];
-  __shared__ uint32_t off[16];
-  stage_offsets(a.fc.vox, off);
+  __shared__ LevelInfo lvi[16];
+  stage_levels(a.fc.vox, lvi);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const int lane = static_cast<int>(threadIdx.x & 63u);
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(256) void k_visibility_mlp(InferArgs a) {
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
       float f[32];
-      features_all(a.fc, off, wp, a.blocks[b], f);
+      features_all(a.fc, lvi, wp, a.blocks[b], f);
       bf16x8 row[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -291,8 +292,8 @@ struct FeatArgs {
 };
 
 __global__ __launch_bounds__(256) void k_mlp_features(FeatArgs a) {
-  __shared__ uint32_t off[16];
-  stage_offsets(a.fc.vox, off);
+  __shared__ LevelInfo lvi[16];
+  stage_levels(a.fc.vox, lvi);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const uint32_t cache = a.pairs[2 * i], b = a.pairs[2 * i + 1];
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(256) void k_mlp_features(FeatArgs a) {
   const float* e = a.caches + static_cast<size_t>(cache) * a.stride;
   const f3 wp = make3(e[0], e[1], e[2]);
   float f[32];
-  features_all(a.fc, off, wp, a.blocks[b], f);
+  features_all(a.fc, lvi, wp, a.blocks[b], f);
   float4* xo = reinterpret_cast<float4*>(a.x + static_cast<size_t>(i) * 32);
 #pragma unroll
   for (int k = 0; k < 8; ++k) xo[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);

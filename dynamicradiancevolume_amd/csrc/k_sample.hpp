@@ -7,33 +7,70 @@
 
 namespace drvk {
 
-// Corner-expanded fetch address of point p at mip level l and its trilinear
+// Per-level constants of the corner-expanded copy, staged in LDS by every
+// sampling kernel (a per-lane level index into a kernel-argument table would
+// be a memory load on the address path of every tap): corner offset, corners
+// per row (w + 1) and per slice ((w + 1)^2), w = res >> l as float and 2^-l.
+struct LevelInfo {
+  uint32_t off;
+  uint32_t wp;
+  uint32_t wp2;
+  float w;
+  float scale;
+};
+
+__device__ __forceinline__ LevelInfo level_info(const VoxelView& vx, int l) {
+  LevelInfo li;
+  const int w = vx.res >> l;
+  li.off = vx.corner_offset[l];
+  li.wp = static_cast<uint32_t>(w + 1);
+  li.wp2 = li.wp * li.wp;
+  li.w = static_cast<float>(w);
+  li.scale = __builtin_amdgcn_ldexpf(1.0f, -l);
+  return li;
+}
+
+// Stages the level table in LDS; every thread of the workgroup must call it
+// (it synchronises the workgroup).
+__device__ __forceinline__ void stage_levels(const VoxelView& vx, LevelInfo* lv) {
+  if (threadIdx.x < 16u) lv[threadIdx.x] = level_info(vx, static_cast<int>(threadIdx.x));
+  __syncthreads();
+}
+
+// The corner-expanded copy as a buffer resource: fetches take a 32-bit byte
+// offset (no 64-bit address arithmetic per tap) and are bounds-checked.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t corner_rsrc(const VoxelView& vx) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(vx.corners), static_cast<short>(0),
+                                           static_cast<int>(vx.corner_bytes), 0x00020000);
+}
+
+__device__ __forceinline__ uint64_t corner_load(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(index * 8u), 0, 0);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(v[1])) << 32) | static_cast<uint32_t>(v[0]);
+}
+
+// Corner-expanded fetch index of point p at mip level l and its trilinear
 // weights (GL 4.5 section 8.14.2, clamp to edge).  p is in level-0 texel
-// units; the lower filter corner floor(p * 2^-l - 0.5) is clamped to
-// [-1, w - 1], which selects the same clamped texels.
+// units; the lower filter corner floor(p * 2^-l - 0.5) clamped to [-1, w - 1]
+// selects the same clamped texels, and is computed one up, as
+// floor(p * 2^-l + 0.5) clamped to [0, w] (the corner table's row index).
 struct Footprint {
   uint32_t index;
   float ax, ay, az;
 };
 
-// `off` is the view's corner_offset table staged in LDS (a per-lane level
-// index into the kernel-argument copy would be a memory load per sample).
-__device__ __forceinline__ Footprint footprint(const VoxelView& vx, const uint32_t* off, int l, float px, float py,
-                                               float pz) {
-  const int w = vx.res >> l;
-  const float scale = __builtin_amdgcn_ldexpf(1.0f, -l);
-  const float u = __builtin_fmaf(px, scale, -0.5f);
-  const float v = __builtin_fmaf(py, scale, -0.5f);
-  const float q = __builtin_fmaf(pz, scale, -0.5f);
+__device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float px, float py, float pz) {
+  const LevelInfo li = lv[l];
+  const float u = __builtin_fmaf(px, li.scale, 0.5f);
+  const float v = __builtin_fmaf(py, li.scale, 0.5f);
+  const float q = __builtin_fmaf(pz, li.scale, 0.5f);
   const float fu = floorf(u), fv = floorf(v), fq = floorf(q);
-  const float hi = static_cast<float>(w - 1);
-  // Clamped in float first so the conversion is always in range (NaN -> -1).
-  const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fu, -1.0f, hi) + 1.0f);
-  const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fv, -1.0f, hi) + 1.0f);
-  const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fq, -1.0f, hi) + 1.0f);
-  const uint32_t wp = static_cast<uint32_t>(w + 1);
+  // Clamped in float first so the conversion is always in range (NaN -> 0).
+  const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fu, 0.0f, li.w));
+  const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fv, 0.0f, li.w));
+  const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fq, 0.0f, li.w));
   Footprint f;
-  f.index = off[l] + __umul24(__umul24(k, wp) + j, wp) + i;
+  f.index = li.off + i + __umul24(j, li.wp) + __umul24(k, li.wp2);  // (w + 1)^2 < 2^24 for res <= 512
   f.ax = u - fu;
   f.ay = v - fv;
   f.az = q - fq;
@@ -58,17 +95,10 @@ __device__ __forceinline__ float blend8(uint64_t c, float ax, float ay, float az
 }
 
 // Single-level sample (0..1) of the voxel mips at level l, p in level-0 texel units.
-__device__ __forceinline__ float sample_level(const VoxelView& vx, const uint32_t* off, int l, float px, float py,
-                                              float pz) {
-  const Footprint f = footprint(vx, off, l, px, py, pz);
-  return blend8(vx.corners[f.index], f.ax, f.ay, f.az) * (1.0f / 255.0f);
-}
-
-// Stages the corner_offset table in LDS; every thread of the workgroup must
-// call it (it synchronises the workgroup).
-__device__ __forceinline__ void stage_offsets(const VoxelView& vx, uint32_t* off) {
-  if (threadIdx.x < 16u) off[threadIdx.x] = vx.corner_offset[threadIdx.x];
-  __syncthreads();
+__device__ __forceinline__ float sample_level(__amdgpu_buffer_rsrc_t r, const LevelInfo* lv, int l, float px,
+                                              float py, float pz) {
+  const Footprint f = footprint(lv, l, px, py, pz);
+  return blend8(corner_load(r, f.index), f.ax, f.ay, f.az) * (1.0f / 255.0f);
 }
 
 }  // namespace drvk

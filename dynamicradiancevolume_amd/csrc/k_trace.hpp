@@ -56,7 +56,8 @@ __device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_s
 // batch are planned first and their loads issued together; occlusion is then
 // accumulated in step order.  Returns true when the trace has finished.
 template <int kBatch = kStepBatch>
-__device__ __forceinline__ bool trace_batch(const VoxelView& vx, const uint32_t* off, Trace& t) {
+__device__ __forceinline__ bool trace_batch(const VoxelView& vx, __amdgpu_buffer_rsrc_t cr, const LevelInfo* lvi,
+                                            Trace& t) {
   const int q = vx.levels - 1;
   const float qf = static_cast<float>(q);
   float sx[kBatch], sy[kBatch], sz[kBatch], frac[kBatch];
@@ -82,7 +83,7 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, const uint32_t*
     sx[i] = t.px;
     sy[i] = t.py;
     sz[i] = t.pz;
-    f1[i] = footprint(vx, off, lv[i], t.px, t.py, t.pz);
+    f1[i] = footprint(lvi, lv[i], t.px, t.py, t.pz);
     if (take[i]) {
       if (t.dist >= t.goal) t.done = true;
       else t.step = fmaxf(1.0f, r * t.k);
@@ -90,15 +91,15 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, const uint32_t*
   }
   uint64_t c1[kBatch], c2[kBatch];
 #pragma unroll
-  for (int i = 0; i < kBatch; ++i) c1[i] = take[i] ? vx.corners[f1[i].index] : 0ull;
+  for (int i = 0; i < kBatch; ++i) c1[i] = take[i] ? corner_load(cr, f1[i].index) : 0ull;
   float o[kBatch];
   // The second level is only planned when some lane of the wave blends two.
   if (__any(need2)) {
     Footprint f2[kBatch];
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
-      f2[i] = footprint(vx, off, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
-      c2[i] = frac[i] > 0.0f ? vx.corners[f2[i].index] : 0ull;
+      f2[i] = footprint(lvi, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
+      c2[i] = frac[i] > 0.0f ? corner_load(cr, f2[i].index) : 0ull;
     }
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {

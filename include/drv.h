@@ -57,7 +57,7 @@ typedef struct drv_config {
   int32_t max_caches;          /* capacity of the light cache buffer */
   int32_t cav_resolution;      /* cache address volume cells per axis and cascade */
   int32_t cav_cascades;        /* 1..DRV_MAX_CASCADES */
-  int32_t voxel_resolution;    /* voxel volume cells per axis, power of two >= 2 */
+  int32_t voxel_resolution;    /* voxel volume cells per axis, power of two in [2, 512] */
   int32_t sh_bands;            /* 1 = INDDIFFUSE_VIA_SH1 (64 B/cache), 2 = INDDIFFUSE_VIA_SH2 (128 B/cache) */
   int32_t indirect_shadow;     /* INDIRECT_SHADOW: cone-traced visibility on/off */
   float cav_transition_size;   /* CAVTransitionZoneSize; > 0 defines ADDRESSVOL_CASCADE_TRANSITIONS */
