@@ -110,6 +110,18 @@ __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
     const uint32_t per_lane = n / 64u;
     a.vpls_lanes[static_cast<size_t>(m % per_lane) * 64u + m / per_lane] = v;
   }
+  // Mean normal of a block's VPLs: the B x B texel square of a block is the
+  // `interval` Morton-consecutive VPLs m .. m + interval - 1, i.e. adjacent
+  // lanes of one wave when interval <= 64 (sum by butterfly); larger blocks
+  // are summed by their first lane.
+  float bsx = nrm.x, bsy = nrm.y, bsz = nrm.z;
+  if (a.blocks && a.interval <= 64u) {
+    for (uint32_t o = 1; o < a.interval; o <<= 1) {
+      bsx += __shfl_xor(bsx, static_cast<int>(o));
+      bsy += __shfl_xor(bsy, static_cast<int>(o));
+      bsz += __shfl_xor(bsz, static_cast<int>(o));
+    }
+  }
   if (a.blocks && (m % a.interval) == 0u) {
     // cacheLightingRSM.comp:171-192: bilinear (d, d^2) at the shadow LOD
     const float ss = (static_cast<float>(x) + a.sampling_offset) / fR;
@@ -131,17 +143,19 @@ __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
     const float cand = sqrtf(variance) * 2.0f / dd[0];
     // GLSL max(x, y) = (x < y) ? y : x, so a NaN candidate keeps SuperValWidth.
     const float rho = (a.super_val_width < cand) ? cand : a.super_val_width;
-    // Mean normal of the block's VPLs (the B x B texel square at (x, y)).
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    const int B = a.block;
-    for (int dy = 0; dy < B; ++dy)
-      for (int dx = 0; dx < B; ++dx) {
-        const size_t t = static_cast<size_t>(y + dy) * a.R + (x + dx);
-        const f3 vn = unpack_normal16i(a.normal[2 * t], a.normal[2 * t + 1]);
-        sx += vn.x;
-        sy += vn.y;
-        sz += vn.z;
-      }
+    float sx = bsx, sy = bsy, sz = bsz;
+    if (a.interval > 64u) {
+      sx = sy = sz = 0.0f;
+      const int B = a.block;
+      for (int dy = 0; dy < B; ++dy)
+        for (int dx = 0; dx < B; ++dx) {
+          const size_t t = static_cast<size_t>(y + dy) * a.R + (x + dx);
+          const f3 vn = unpack_normal16i(a.normal[2 * t], a.normal[2 * t + 1]);
+          sx += vn.x;
+          sy += vn.y;
+          sz += vn.z;
+        }
+    }
     const float nl = sqrtf(sx * sx + sy * sy + sz * sz);
     const float inv = nl > 0.0f ? 1.0f / nl : 0.0f;
     a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho, sx * inv, sy * inv, sz * inv, 0.0f};

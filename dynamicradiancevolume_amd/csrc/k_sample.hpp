@@ -11,12 +11,13 @@ namespace drvk {
 // sampling kernel (a per-lane level index into a kernel-argument table would
 // be a memory load on the address path of every tap): corner offset, corners
 // per row (w + 1) and per slice ((w + 1)^2), w = res >> l as float and 2^-l.
-struct LevelInfo {
+struct __attribute__((aligned(32))) LevelInfo {  // 32 B: the table index is a shift
   uint32_t off;
   uint32_t wp;
   uint32_t wp2;
   float w;
   float scale;
+  uint32_t pad[3];
 };
 
 __device__ __forceinline__ LevelInfo level_info(const VoxelView& vx, int l) {
@@ -27,6 +28,7 @@ __device__ __forceinline__ LevelInfo level_info(const VoxelView& vx, int l) {
   li.wp2 = li.wp * li.wp;
   li.w = static_cast<float>(w);
   li.scale = __builtin_amdgcn_ldexpf(1.0f, -l);
+  li.pad[0] = li.pad[1] = li.pad[2] = 0u;
   return li;
 }
 
