@@ -59,7 +59,21 @@ def test_invalid_arguments_are_rejected():
     bad.voxel_resolution = 8
     bad.cav_cascades = 5
     assert lib.drv_create(C.byref(bad), C.byref(ctx)) == _abi.DRV_ERR_INVALID_ARG
+    bad.cav_cascades = 1
+    bad.voxel_resolution = 1024  # the corner-expanded copy needs 32-bit byte offsets
+    assert lib.drv_create(C.byref(bad), C.byref(ctx)) == _abi.DRV_ERR_INVALID_ARG
+    bad.voxel_resolution = 8
+    bad.indirect_specular = 1
+    for size in (3, 64, -2):  # env-map side: a power of two up to 32 (0 = the default 16)
+        bad.specular_envmap_size = size
+        assert lib.drv_create(C.byref(bad), C.byref(ctx)) == _abi.DRV_ERR_INVALID_ARG
+    bad.specular_envmap_size = 16
+    bad.specular_hole_fill_level = -1
+    assert lib.drv_create(C.byref(bad), C.byref(ctx)) == _abi.DRV_ERR_INVALID_ARG
     assert lib.drv_destroy(None) == _abi.DRV_OK
+    # the atlas entry points refuse a null context
+    assert lib.drv_get_specular_atlas(None, None, None, None) == _abi.DRV_ERR_INVALID_ARG
+    assert lib.drv_read_specular_atlas(None, 0, None) == _abi.DRV_ERR_INVALID_ARG
 
 
 def test_prepare_spot_light_matches_oracle_and_is_consistent():
