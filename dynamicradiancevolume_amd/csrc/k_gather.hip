@@ -15,12 +15,14 @@
 //                         the corner-expanded voxel mips (one 8-byte load per
 //                         level per step), written block-major so both writes
 //                         and the gather's reads coalesce;
-//   3. k_gather         - persistent waves pull (64-cache tile, VPL split)
-//                         items; VPL records are wave-uniform, so they stream
-//                         through the scalar cache into SGPRs and every lane
-//                         accumulates 27 SH sums in VGPRs; SH constants are
-//                         folded in at write-out;
-//   4. k_reduce         - sums the per-split partials in a fixed order and
+//   3. k_gather         - persistent workgroups pull (64-cache tile, group
+//                         of 4 VPL splits) items, one split per wave; VPL
+//                         records are wave-uniform, so they stream through
+//                         the scalar cache into SGPRs and every lane
+//                         accumulates 27 SH sums in VGPRs; the four waves'
+//                         sums are combined in LDS and one partial per group
+//                         is written; SH constants are folded in at write-out;
+//   4. k_reduce         - sums the per-group partials in a fixed order and
 //                         adds them into the cache entries (the reference's
 //                         `Entry.SH += accumulated`).
 // Splitting the VPL range fills 256 CUs even when there are only a few
@@ -333,18 +335,21 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
                                                 const float* __restrict__ vis, uint32_t vis_pitch,
                                                 uint32_t interval, float* __restrict__ partial,
                                                 uint32_t partial_pitch) {
+  __shared__ float part[2][27][64];
   const uint32_t live = counter->live;
   const uint32_t splits = counter->splits;
+  const uint32_t groups = (splits + 3u) / 4u;
   const uint32_t tiles = (live + 63u) / 64u;
-  const uint32_t items = tiles * splits;
+  const uint32_t items = tiles * groups;
   const uint32_t chunk = n_vpl / splits;
   const uint32_t lane = threadIdx.x & 63u;
-  // Wave-uniform work index (readfirstlane lets the compiler keep the item
+  // Wave-uniform split index (readfirstlane lets the compiler keep the item
   // loop and the VPL / block addresses in SGPRs).
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t item = wave; item < items; item += n_waves) {
-    const uint32_t tile = item / splits, split = item % splits;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n_out = kSh2 ? 27 : 12;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    const uint32_t tile = item / groups, group = item % groups;
+    const uint32_t split = group * 4u + w;
     const uint32_t cache = tile * 64u + lane;
     const bool valid = cache < live;
     const uint32_t c = valid ? cache : 0u;
@@ -353,20 +358,43 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
     // Raw sums; the SH evaluation factors are applied at write-out.
     ShSums acc;
     sh_zero(acc);
-    const uint32_t m0 = split * chunk, m1 = m0 + chunk;
-    for (uint32_t mb = m0; mb < m1; mb += interval) {
-      float shadowing = 1.0f;
-      if (kShadow) shadowing = vis[static_cast<size_t>(mb / interval) * vis_pitch + c];
+    if (split < splits) {
+      const uint32_t m0 = split * chunk, m1 = m0 + chunk;
+      for (uint32_t mb = m0; mb < m1; mb += interval) {
+        float shadowing = 1.0f;
+        if (kShadow) shadowing = vis[static_cast<size_t>(mb / interval) * vis_pitch + c];
 #pragma unroll 4
-      for (uint32_t m = mb; m < mb + interval; ++m) sh_accumulate<kSh2>(acc, vpls[m], wx, wy, wz, shadowing);
+        for (uint32_t m = mb; m < mb + interval; ++m) sh_accumulate<kSh2>(acc, vpls[m], wx, wy, wz, shadowing);
+      }
     }
-    if (!valid) continue;
-    // Raw sums, coefficient-major: partial[(split * 27 + k) * pitch + cache].
-    float* p = partial + static_cast<size_t>(split) * 27u * partial_pitch + cache;
-    const int n_out = kSh2 ? 27 : 12;
+    // The workgroup's four splits of the tile are combined in LDS in a fixed
+    // order, (s0 + s1) + (s2 + s3), so one partial per group goes to HBM.
+    if (w & 1u) {
 #pragma unroll
-    for (int k = 0; k < 27; ++k)
-      if (k < n_out) p[static_cast<size_t>(k) * partial_pitch] = acc.s[k];
+      for (int k = 0; k < 27; ++k)
+        if (k < n_out) part[w >> 1][k][lane] = acc.s[k];
+    }
+    __syncthreads();
+    if (!(w & 1u)) {
+#pragma unroll
+      for (int k = 0; k < 27; ++k)
+        if (k < n_out) acc.s[k] += part[w >> 1][k][lane];
+    }
+    __syncthreads();
+    if (w == 2u) {
+#pragma unroll
+      for (int k = 0; k < 27; ++k)
+        if (k < n_out) part[0][k][lane] = acc.s[k];
+    }
+    __syncthreads();
+    if (w == 0u && valid) {
+      // Raw sums, coefficient-major: partial[(group * 27 + k) * pitch + cache].
+      float* p = partial + static_cast<size_t>(group) * 27u * partial_pitch + cache;
+#pragma unroll
+      for (int k = 0; k < 27; ++k)
+        if (k < n_out) p[static_cast<size_t>(k) * partial_pitch] = acc.s[k] + part[0][k][lane];
+    }
+    __syncthreads();  // part is rewritten by the next item
   }
 }
 
@@ -381,7 +409,7 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
                                                 int sh2) {
   __shared__ float part[2][4][64];
   const uint32_t live = counter->live;
-  const uint32_t splits = counter->splits;
+  const uint32_t splits = (counter->splits + 3u) / 4u;  // the gather wrote one partial per group of 4 splits
   const uint32_t n_jc = sh2 ? 27u : 12u;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
