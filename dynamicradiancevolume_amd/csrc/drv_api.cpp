@@ -626,8 +626,7 @@ int drv_voxelize_end(drv_ctx* ctx, float adaption_factor) {
   // voxelization.cpp:100: nothing is blended or mipmapped when the floored
   // adaption is zero.
   if (adaption_factor > 0.0f) {
-    drvk::launch_voxel_blend(ctx->vox.res, ctx->vox_target, ctx->vox_mips, adaption_factor, ctx->aux);
-    drvk::launch_voxel_mips(ctx->vox, ctx->vox_mips, ctx->aux);
+    drvk::launch_voxel_blend_mips(ctx->vox, ctx->vox_target, ctx->vox_mips, adaption_factor, ctx->aux);
     drvk::launch_voxel_corners(ctx->vox, ctx->vox_mips, ctx->vox_corners, ctx->aux);
   }
   record(ctx, 2, ctx->aux);

@@ -149,8 +149,10 @@ void launch_prepare_dispatch(Counter* counter, uint32_t capacity, hipStream_t s)
 
 void launch_voxelize(const drv_volume& vol, int res, const float* positions, const uint32_t* indices, uint32_t n_tri,
                      const float* world, uint8_t* target, hipStream_t s);
-void launch_voxel_blend(int res, const uint8_t* target, uint8_t* volume, float adaption, hipStream_t s);
-void launch_voxel_mips(const VoxelLayout& layout, uint8_t* mips, hipStream_t s);
+// Blend and the whole mip chain: one launch for the 16^3 bricks and their
+// levels, one single-workgroup launch for the levels above.
+void launch_voxel_blend_mips(const VoxelLayout& layout, const uint8_t* target, uint8_t* mips, float adaption,
+                             hipStream_t s);
 void launch_voxel_corners(const VoxelLayout& layout, const uint8_t* mips, uint64_t* corners, hipStream_t s);
 
 // flux_cut: INDIRECT_SPECULAR skips VALs whose flux sums below 0.001

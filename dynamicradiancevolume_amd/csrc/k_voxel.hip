@@ -160,32 +160,122 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxelizeArgs a) {
   }
 }
 
-// voxelblend.comp:8-18: old + sign(target - old) * AdaptionFactor, stored as unorm8.
-__global__ void k_voxel_blend(const uint8_t* target, uint8_t* volume, size_t n, float f) {
-  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (; i < n; i += stride) {
-    const float t = static_cast<float>(target[i]) / 255.0f;
-    const float o = static_cast<float>(volume[i]) / 255.0f;
-    const float d = t - o;
-    const float s = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
-    float v = o + s * f;
-    v = fminf(1.0f, fmaxf(0.0f, v));
-    volume[i] = static_cast<uint8_t>(floorf(v * 255.0f + 0.5f));
+// voxelblend.comp:8-18 (old + sign(target - old) * AdaptionFactor, stored as
+// unorm8) and voxelmipmap.comp:8-12 (the linear sample at the centre of each
+// 2x2x2 child block is the 8-texel mean, computed exactly and rounded half
+// up), fused: one workgroup per 16^3 brick of level 0 blends its texels and
+// builds the brick's levels 1..4 in LDS; k_voxel_upper_mips then builds the
+// levels above from the bricks' top texels.
+constexpr int kBrick = 16;
+
+struct MipArgs {
+  const uint8_t* target;
+  uint8_t* mips;
+  uint32_t mip_offset[16];
+  int res, levels;
+  int brick, brick_levels;  // B = min(16, res), log2(B)
+  float f;
+};
+
+// voxelblend.comp:8-18: old + sign(target - old) * AdaptionFactor, unorm8.
+__device__ __forceinline__ uint8_t blend_voxel(uint32_t target, uint32_t old, float f) {
+  const float tv = static_cast<float>(target) / 255.0f;
+  const float o = static_cast<float>(old) / 255.0f;
+  const float d = tv - o;
+  const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+  float v = o + sg * f;
+  v = fminf(1.0f, fmaxf(0.0f, v));
+  return static_cast<uint8_t>(floorf(v * 255.0f + 0.5f));
+}
+
+__device__ __forceinline__ uint8_t mean8(const uint8_t* s, int w, int x, int y, int z) {
+  const uint8_t* s0 = s + (static_cast<size_t>(2 * z) * w + 2 * y) * w + 2 * x;
+  const uint8_t* s1 = s0 + static_cast<size_t>(w) * w;
+  const int sum = s0[0] + s0[1] + s0[w] + s0[w + 1] + s1[0] + s1[1] + s1[w] + s1[w + 1];
+  return static_cast<uint8_t>((sum + 4) >> 3);
+}
+
+// Dynamic LDS: the brick's levels, B^3 + (B/2)^3 + ... bytes.
+__global__ __launch_bounds__(256) void k_voxel_blend_mips(MipArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int B = a.brick, res = a.res;
+  const int nb = res / B;
+  const int bx = static_cast<int>(blockIdx.x) % nb, by = (static_cast<int>(blockIdx.x) / nb) % nb,
+            bz = static_cast<int>(blockIdx.x) / (nb * nb);
+  const int tid = static_cast<int>(threadIdx.x);
+  // Level 0: voxelblend.comp:8-18.  With 16^3 bricks a thread blends one
+  // 16-byte row (one load of target and volume each).
+  if (B == kBrick) {
+    const int y = tid % B, z = tid / B;
+    const size_t g = (static_cast<size_t>(bz * B + z) * res + (by * B + y)) * res + bx * B;
+    const uint4 tr = *reinterpret_cast<const uint4*>(a.target + g);
+    const uint4 orow = *reinterpret_cast<const uint4*>(a.mips + g);
+    const uint32_t tw[4] = {tr.x, tr.y, tr.z, tr.w}, ow[4] = {orow.x, orow.y, orow.z, orow.w};
+    uint32_t nw[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint32_t packed = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        packed |= static_cast<uint32_t>(blend_voxel((tw[k] >> (8 * b)) & 0xffu, (ow[k] >> (8 * b)) & 0xffu, a.f))
+                  << (8 * b);
+      nw[k] = packed;
+    }
+    const uint4 nr = make_uint4(nw[0], nw[1], nw[2], nw[3]);
+    *reinterpret_cast<uint4*>(a.mips + g) = nr;
+    *reinterpret_cast<uint4*>(lds + tid * 16) = nr;
+  } else {
+    for (int t = tid; t < B * B * B; t += 256) {
+      const int x = t % B, y = (t / B) % B, z = t / (B * B);
+      const size_t g = (static_cast<size_t>(bz * B + z) * res + (by * B + y)) * res + (bx * B + x);
+      const uint8_t q = blend_voxel(a.target[g], a.mips[g], a.f);
+      a.mips[g] = q;
+      lds[t] = q;
+    }
+  }
+  __syncthreads();
+  // Levels 1..log2(B) inside the brick: voxelmipmap.comp:8-12.
+  int src = 0, dst = B * B * B;
+  for (int k = 1; k <= a.brick_levels; ++k) {
+    const int sp = B >> (k - 1), sk = B >> k, wk = res >> k;
+    uint8_t* out = a.mips + a.mip_offset[k];
+    for (int t = tid; t < sk * sk * sk; t += 256) {
+      const int x = t % sk, y = (t / sk) % sk, z = t / (sk * sk);
+      const uint8_t q = mean8(lds + src, sp, x, y, z);
+      lds[dst + t] = q;
+      out[(static_cast<size_t>(bz * sk + z) * wk + (by * sk + y)) * wk + (bx * sk + x)] = q;
+    }
+    __syncthreads();
+    src = dst;
+    dst += sk * sk * sk;
   }
 }
 
-// voxelmipmap.comp:8-12: the linear sample at the centre of each 2x2x2 child
-// block is the 8-texel mean; it is computed exactly and rounded half up.
-__global__ void k_voxel_mip(const uint8_t* src, uint8_t* dst, int w2) {
-  const int n = w2 * w2 * w2;
-  const int w = 2 * w2;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int x = i % w2, y = (i / w2) % w2, z = i / (w2 * w2);
-    const uint8_t* s0 = src + (static_cast<size_t>(2 * z) * w + 2 * y) * w + 2 * x;
-    const uint8_t* s1 = s0 + static_cast<size_t>(w) * w;
-    const int sum = s0[0] + s0[1] + s0[w] + s0[w + 1] + s1[0] + s1[1] + s1[w] + s1[w + 1];
-    dst[i] = static_cast<uint8_t>((sum + 4) >> 3);
+// The levels above the brick size (more than one brick per axis), built by
+// one workgroup from level log2(B) after the brick kernel (the kernel
+// boundary makes every brick's writes visible; an in-kernel device-scope
+// fence per brick costs an L2 write-back on this multi-XCD part).
+// Dynamic LDS: (res / B)^3 + (res / 2B)^3 + ... bytes.
+__global__ __launch_bounds__(256) void k_voxel_upper_mips(MipArgs a) {
+  extern __shared__ uint8_t up[];
+  const int nb = a.res / a.brick;
+  const int tid = static_cast<int>(threadIdx.x);
+  const uint8_t* lb = a.mips + a.mip_offset[a.brick_levels];
+  for (int t = tid; t < nb * nb * nb; t += 256) up[t] = lb[t];
+  __syncthreads();
+  int usrc = 0, udst = nb * nb * nb;
+  for (int k = a.brick_levels + 1; k < a.levels; ++k) {
+    const int wp = a.res >> (k - 1), wk = a.res >> k;
+    uint8_t* out = a.mips + a.mip_offset[k];
+    for (int t = tid; t < wk * wk * wk; t += 256) {
+      const int x = t % wk, y = (t / wk) % wk, z = t / (wk * wk);
+      const uint8_t q = mean8(up + usrc, wp, x, y, z);
+      up[udst + t] = q;
+      out[t] = q;
+    }
+    __syncthreads();
+    usrc = udst;
+    udst += wk * wk * wk;
   }
 }
 
@@ -239,19 +329,27 @@ void launch_voxelize(const drv_volume& vol, int res, const float* positions, con
   hipLaunchKernelGGL(k_voxelize, dim3((n_tri + 255) / 256), dim3(256), 0, s, a);
 }
 
-void launch_voxel_blend(int res, const uint8_t* target, uint8_t* volume, float adaption, hipStream_t s) {
-  const size_t n = static_cast<size_t>(res) * res * res;
-  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 4096));
-  hipLaunchKernelGGL(k_voxel_blend, dim3(blocks), dim3(256), 0, s, target, volume, n, adaption);
-}
-
-void launch_voxel_mips(const VoxelLayout& L, uint8_t* mips, hipStream_t s) {
-  for (int l = 0; l + 1 < L.levels; ++l) {
-    const int w2 = L.res >> (l + 1);
-    const int n = w2 * w2 * w2;
-    const unsigned blocks = static_cast<unsigned>(std::min(std::max(1, (n + 255) / 256), 4096));
-    hipLaunchKernelGGL(k_voxel_mip, dim3(blocks), dim3(256), 0, s, mips + L.mip_offset[l], mips + L.mip_offset[l + 1],
-                       w2);
+void launch_voxel_blend_mips(const VoxelLayout& L, const uint8_t* target, uint8_t* mips, float adaption,
+                             hipStream_t s) {
+  MipArgs a;
+  a.target = target;
+  a.mips = mips;
+  for (int l = 0; l < L.levels && l < 16; ++l) a.mip_offset[l] = static_cast<uint32_t>(L.mip_offset[l]);
+  a.res = L.res;
+  a.levels = L.levels;
+  a.brick = std::min(kBrick, L.res);
+  int bl = 0;
+  while ((1 << (bl + 1)) <= a.brick) ++bl;
+  a.brick_levels = bl;
+  a.f = adaption;
+  const int nb = L.res / a.brick;
+  size_t lds = 0;
+  for (int k = 0; k <= bl; ++k) lds += static_cast<size_t>(a.brick >> k) * (a.brick >> k) * (a.brick >> k);
+  hipLaunchKernelGGL(k_voxel_blend_mips, dim3(static_cast<unsigned>(nb * nb * nb)), dim3(256), lds, s, a);
+  if (bl + 1 < L.levels) {
+    size_t upper = 0;
+    for (int w = nb; w >= 1; w /= 2) upper += static_cast<size_t>(w) * w * w;
+    hipLaunchKernelGGL(k_voxel_upper_mips, dim3(1), dim3(256), upper, s, a);
   }
 }
 
