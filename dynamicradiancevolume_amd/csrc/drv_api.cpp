@@ -265,7 +265,7 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     for (int set = 0; set < 2; ++set) {
       float* cs = set == 0 ? set_a : set_b;
       drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->vpls, n_vpl,
-                          set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, spot_splits, grid,
+                          set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, spot_splits, grid, true,
                           ctx->stream);
       drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->partial, sc,
                              ctx->cu_count * 8, ctx->stream);
@@ -644,8 +644,8 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
   if (!ctx->have_frame || !ctx->have_volume) return fail(ctx, DRV_ERR_STATE, "allocate: frame or volume not set");
   record(ctx, 3);
   // renderer.cpp:969-970: clear the counter and the address volume.
-  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->counter), sizeof(Counter) / 4, 0u, ctx->stream);
-  drvk::launch_fill_u32(ctx->cav, ctx->cav_cells, 0u, ctx->stream);
+  drvk::launch_fill_u32_pair(ctx->cav, ctx->cav_cells, reinterpret_cast<uint32_t*>(ctx->counter),
+                             sizeof(Counter) / 4, 0u, ctx->stream);
   drvk::launch_allocate(ctx->cfg, ctx->vol, ctx->frame, gb->depth, ctx->cav, ctx->caches, ctx->counter, ctx->stream);
   drvk::launch_prepare_dispatch(ctx->counter, static_cast<uint32_t>(ctx->cfg.max_caches), ctx->stream);
   record(ctx, 4);
@@ -741,8 +741,11 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     if ((rc = dev_alloc(ctx, &ctx->spec.vpls_lanes, n_vpl)) != DRV_OK) return rc;
     ctx->spec.vpls_lanes_cap = n_vpl;
   }
+  // The gather's split plan rides on the VPL preparation (one launch less).
+  const drvk::GatherPlan plan = drvk::gather_plan(ctx->counter, interval,
+                                                  static_cast<uint32_t>(ctx->tune.gather_max_splits), grid);
   drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, specular,
-                            specular ? ctx->spec.vpls_lanes : nullptr, ctx->stream);
+                            specular ? ctx->spec.vpls_lanes : nullptr, &plan, ctx->stream);
   record(ctx, 6);
   const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
   if (use_net) {
@@ -760,7 +763,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
                       ctx->visibility, static_cast<uint32_t>(cap), interval, shadow ? 1 : 0,
                       ctx->partial, static_cast<uint32_t>(cap), static_cast<uint32_t>(ctx->tune.gather_max_splits),
-                      grid, ctx->stream);
+                      grid, false, ctx->stream);  // planned by the VPL preparation
   record(ctx, 8);
   drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->partial,
                          static_cast<uint32_t>(cap), ctx->cu_count * 8, ctx->stream);
@@ -1055,7 +1058,7 @@ int drv_bruteforce_rsm(drv_ctx* ctx, const drv_gbuffer* gb, const uint8_t* rough
   }
   // VAL records (position, flux, normal) of the read level; the VPL order
   // does not matter to a plain sum.
-  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, 0, nullptr, ctx->stream);
+  drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, nullptr, 0, nullptr, nullptr, ctx->stream);
   drvk::launch_bruteforce(ctx->cfg, ctx->frame, *light, *gb, roughness_metallic, ctx->srgb_lut, ctx->vpls, n,
                           diffuse_only, out_rgb, accumulate, ctx->stream);
   return check_hip(ctx, hipGetLastError(), "bruteforce_rsm");
@@ -1411,7 +1414,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
     drv_ctx* ctx = b->views[v];
     join_voxels(ctx);  // the batch runs on the views' shared stream
     drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, 0, nullptr,
-                              b->stream);
+                              nullptr, b->stream);
     // The batch leaves no per-pair visibility behind.
     ctx->last_n_blocks = 0;
     ctx->label_caches = nullptr;

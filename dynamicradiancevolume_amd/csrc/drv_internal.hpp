@@ -141,6 +141,7 @@ struct SpecAtlas {
 };
 
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+void launch_fill_u32_pair(uint32_t* p, size_t n, uint32_t* q, size_t m, uint32_t v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 
 void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const float* depth,
@@ -155,19 +156,31 @@ void launch_voxel_blend_mips(const VoxelLayout& layout, const uint8_t* target, u
                              hipStream_t s);
 void launch_voxel_corners(const VoxelLayout& layout, const uint8_t* mips, uint64_t* corners, hipStream_t s);
 
+// Split plan of the gather for a cache counter: enough (tile, split) items
+// for the persistent grid (4 waves per workgroup), each split a whole number
+// of quanta (shadow blocks, at least 64 VPLs).
+struct GatherPlan {
+  Counter* counter;
+  uint32_t quantum, max_splits, target_items;
+};
+inline GatherPlan gather_plan(Counter* counter, uint32_t interval, uint32_t max_splits, int grid) {
+  return GatherPlan{counter, interval > 64u ? interval : 64u, max_splits, static_cast<uint32_t>(grid) * 4u};
+}
+
 // flux_cut: INDIRECT_SPECULAR skips VALs whose flux sums below 0.001
 // (cacheLightingRSM.comp:240-243); their flux is stored as zero.
 // vpls_lanes (optional): the same records lane-interleaved for the specular
 // gather, record m at (m % per_lane) * 64 + m / per_lane, per_lane = n / 64.
+// plan (optional): the gather's split plan, written by the first thread.
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         int flux_cut, Vpl* vpls_lanes, hipStream_t s);
+                         int flux_cut, Vpl* vpls_lanes, const GatherPlan* plan, hipStream_t s);
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
                        uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid, hipStream_t s);
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,
                    const Counter* counter, const Vpl* vpls, uint32_t n_vpl, const float* visibility,
                    uint32_t vis_pitch, uint32_t sample_interval, int indirect_shadow, float* partial,
-                   uint32_t partial_pitch, uint32_t max_splits, int grid, hipStream_t s);
+                   uint32_t partial_pitch, uint32_t max_splits, int grid, bool plan, hipStream_t s);
 void launch_reduce_sh(const drv_sh_constants& shc, int sh_bands, float* caches, int stride, const Counter* counter,
                       const float* partial, uint32_t partial_pitch, int grid, hipStream_t s);
 

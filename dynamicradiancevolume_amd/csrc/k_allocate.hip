@@ -26,6 +26,14 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// Two buffers cleared by one launch (the allocation's counter and CAV).
+__global__ void k_fill_u32_pair(uint32_t* p, size_t n, uint32_t* q, size_t m, uint32_t v) {
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t j = i; j < n; j += stride) p[j] = v;
+  for (size_t j = i; j < m; j += stride) q[j] = v;
+}
+
 __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
   size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -192,6 +200,12 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   if (n == 0) return;
   const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_fill_u32_pair(uint32_t* p, size_t n, uint32_t* q, size_t m, uint32_t v, hipStream_t s) {
+  const size_t blocks = std::min<size_t>((std::max(n, m) + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill_u32_pair, dim3(static_cast<unsigned>(std::max<size_t>(blocks, 1))), dim3(256), 0, s, p, n,
+                     q, m, v);
 }
 
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s) {

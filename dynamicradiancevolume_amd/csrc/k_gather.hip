@@ -56,6 +56,9 @@ struct PrepArgs {
   int flux_cut;
   Vpl* vpls;
   Vpl* vpls_lanes;  // optional lane-interleaved copy: [m % per_lane][m / per_lane], per_lane = n / 64
+  // Optional split plan of the gather (k_plan_gather), done by the first thread.
+  Counter* plan;
+  uint32_t quantum, max_splits, target_items;
   ShadowBlock* blocks;
 };
 
@@ -77,9 +80,20 @@ __device__ __forceinline__ float rg16f_texel(const uint16_t* img, int w, int i, 
   return half_to_float(img[2 * (static_cast<size_t>(j) * w + i) + comp]);
 }
 
+// Split planning of the gather: enough (tile, split) items for the grid,
+// each split a whole number of quanta (shadow blocks, at least 64 VPLs).
+__device__ __forceinline__ void plan_splits(Counter* ctr, uint32_t n_vpl, uint32_t quantum, uint32_t max_splits,
+                                            uint32_t target_items) {
+  const uint32_t tiles64 = (ctr->live + 63u) / 64u;
+  uint32_t s = 1u;
+  while (s < max_splits && tiles64 * s < target_items && (n_vpl / (2u * s)) >= quantum) s *= 2u;
+  ctr->splits = s;
+}
+
 __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
   const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == 0u && a.plan) plan_splits(a.plan, n, a.quantum, a.max_splits, a.target_items);
   if (m >= n) return;
   uint32_t x, y;
   morton_decode(m, x, y);
@@ -323,10 +337,7 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
 
 __global__ void k_plan_gather(Counter* ctr, uint32_t n_vpl, uint32_t quantum, uint32_t max_splits,
                               uint32_t target_items) {
-  const uint32_t tiles64 = (ctr->live + 63u) / 64u;
-  uint32_t s = 1u;
-  while (s < max_splits && tiles64 * s < target_items && (n_vpl / (2u * s)) >= quantum) s *= 2u;
-  ctr->splits = s;
+  plan_splits(ctr, n_vpl, quantum, max_splits, target_items);
 }
 
 template <bool kSh2, bool kShadow>
@@ -442,7 +453,7 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
 }  // namespace
 
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         int flux_cut, Vpl* vpls_lanes, hipStream_t s) {
+                         int flux_cut, Vpl* vpls_lanes, const GatherPlan* plan, hipStream_t s) {
   PrepArgs a;
   for (int i = 0; i < 16; ++i) a.inv_lvp[i] = light.inverse_light_view_projection[i];
   for (int i = 0; i < 3; ++i) a.light_pos[i] = light.light_position[i];
@@ -460,6 +471,10 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.flux_cut = flux_cut;
   a.vpls = vpls;
   a.vpls_lanes = vpls_lanes;
+  a.plan = plan ? plan->counter : nullptr;
+  a.quantum = plan ? plan->quantum : 0u;
+  a.max_splits = plan ? plan->max_splits : 0u;
+  a.target_items = plan ? plan->target_items : 0u;
   a.blocks = blocks;
   const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
   hipLaunchKernelGGL(k_prepare_vpls, dim3((n + 255) / 256), dim3(256), 0, s, a);
@@ -496,13 +511,13 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,
                    const Counter* counter, const Vpl* vpls, uint32_t n_vpl, const float* visibility,
                    uint32_t vis_pitch, uint32_t interval, int indirect_shadow, float* partial,
-                   uint32_t partial_pitch, uint32_t max_splits, int grid, hipStream_t s) {
+                   uint32_t partial_pitch, uint32_t max_splits, int grid, bool plan, hipStream_t s) {
   (void)shc;
-  // Split planning: enough (tile, split) items for ~16 waves per CU, each
-  // split a whole number of shadow blocks.
-  const uint32_t quantum = std::max<uint32_t>(interval, 64u);
-  hipLaunchKernelGGL(k_plan_gather, dim3(1), dim3(1), 0, s, const_cast<Counter*>(counter), n_vpl, quantum, max_splits,
-                     static_cast<uint32_t>(grid) * 4u);
+  if (plan) {
+    const GatherPlan p = gather_plan(const_cast<Counter*>(counter), interval, max_splits, grid);
+    hipLaunchKernelGGL(k_plan_gather, dim3(1), dim3(1), 0, s, p.counter, n_vpl, p.quantum, p.max_splits,
+                       p.target_items);
+  }
   const bool sh2 = sh_bands >= 2;
   if (sh2 && indirect_shadow)
     hipLaunchKernelGGL((k_gather<true, true>), dim3(grid), dim3(256), 0, s, caches, stride, counter, vpls, n_vpl,
