@@ -38,11 +38,12 @@ struct Vpl {
 };
 
 // One shadow block (cacheLightingRSM.comp:171-192): averaged VAL position and
-// the cone aperture distToSphereRad, plus the mean VPL normal of the block
-// (an input of the surrogate visibility network only).
+// the cone aperture distToSphereRad, the march's step factor
+// k = 2 / (1 - distToSphereRad) (cacheLightingRSM.comp:227), plus the mean
+// VPL normal of the block (an input of the surrogate visibility network only).
 struct ShadowBlock {
   float x, y, z, rho;
-  float nx, ny, nz, pad;
+  float nx, ny, nz, k;
 };
 
 // Voxel volume layout: per mip level l (w = res >> l) the R8 texels, x
@@ -119,7 +120,7 @@ struct Tuning {
   int gather_max_splits = 128; // VPL splits of the gather
   int waves_per_cu = 64;       // persistent grid size
   int refill_min_lanes = 16;   // lane pool: idle lanes that trigger a refill
-  int steps_per_batch = 4;     // cone-trace steps planned (and loaded) together: 2, 4 or 8
+  int steps_per_batch = 2;     // cone-trace steps planned (and loaded) together: 1..4 or 8
 };
 
 // Per-cache specular env-maps (INDIRECT_SPECULAR): the R11G11B10F atlas of

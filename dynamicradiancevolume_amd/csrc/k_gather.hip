@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
     }
     const float nl = sqrtf(sx * sx + sy * sy + sz * sz);
     const float inv = nl > 0.0f ? 1.0f / nl : 0.0f;
-    a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho, sx * inv, sy * inv, sz * inv, 0.0f};
+    a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho, sx * inv, sy * inv, sz * inv, 2.0f / (1.0f - rho)};
   }
 }
 
@@ -219,9 +219,9 @@ __device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint
   }
 }
 
-// A wave owns 64 caches x blocks_per_item shadow blocks and traces one
-// block at a time for all its lanes (one uniform block record, coherent
-// rays from neighbouring caches towards the same VPL block).
+// A wave owns 64 caches x blocks_per_item shadow blocks; each lane traces
+// its cache's blocks in order (rays from neighbouring caches towards nearby
+// VPL blocks stay coherent).
 template <int kBatch>
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   __shared__ LevelInfo lvi[16];
@@ -248,14 +248,23 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
     const uint32_t b0 = group * bpi;
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
-    for (uint32_t b = b0; b < b1; ++b) {
-      Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[b]);
-      bool finished = false;
-      while (!finished) finished = trace_batch<kBatch>(a.vox, cr, lvi, t);
-      a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
-      my_traces += 1u;
-      my_exec += static_cast<uint32_t>(t.s);
-      if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
+    // Each lane walks its blocks on its own: when its ray ends it starts the
+    // next block at the next batch (the record was fetched a trace ahead),
+    // so lanes idle only once per item instead of behind the longest ray of
+    // every block.
+    uint32_t b = b0;
+    Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[b]);
+    ShadowBlock nb = a.blocks[min(b + 1u, b1 - 1u)];
+    while (true) {
+      if (trace_batch<kBatch>(a.vox, cr, lvi, t)) {
+        a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
+        my_traces += 1u;
+        my_exec += static_cast<uint32_t>(t.s);
+        if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
+        if (++b >= b1) break;
+        t = trace_setup(a.vox, a.voxel_size, wp, vpos, nb);
+        nb = a.blocks[min(b + 1u, b1 - 1u)];
+      }
     }
   }
   if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
@@ -265,14 +274,19 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
 // traces form a pool (block-major, so lanes that refill together mostly
 // share a block), every lane holds one trace, and after each batch of steps
 // the lanes whose trace ended take the next pool entries (ballot + mbcnt
-// ranks), so lanes do not idle behind the longest ray of the wave.
+// ranks) once at least refill_min lanes are idle, so lanes do not idle
+// behind the longest ray of the wave.  The tile's cache positions are staged
+// in LDS, so a refill costs no global load.
+template <int kBatch>
 __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
   __shared__ LevelInfo lvi[16];
+  __shared__ float4 cpos[4][64][2];  // per wave: the tile's cache positions (world, voxel units)
   stage_levels(a.vox, lvi);
   const __amdgpu_buffer_rsrc_t cr = corner_rsrc(a.vox);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t w = threadIdx.x >> 6;
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t bpi = items_blocks(tiles, a.n_blocks, a.blocks_per_item, n_waves);
@@ -286,49 +300,54 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
     const uint32_t n_valid = min(64u, live - tile * 64u);
     const uint32_t b0 = group * bpi;
     const uint32_t pool = n_valid * (min(a.n_blocks, b0 + bpi) - b0);
-    uint32_t next = 64u;  // wave-uniform: first pool entry not yet taken
-    uint32_t idx = lane;
-    bool has = idx < pool;
-    uint32_t cache = 0u, blk = 0u;
+    if (lane < n_valid) {
+      const float* e = a.caches + static_cast<size_t>(tile * 64u + lane) * a.stride;
+      const f3 wp = make3(e[0], e[1], e[2]);
+      // cacheLightingRSM.comp:103-104
+      cpos[w][lane][0] = make_float4(wp.x, wp.y, wp.z, 0.0f);
+      cpos[w][lane][1] = make_float4((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz, 0.0f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t next = 0u;  // wave-uniform: first entry not yet taken
+    bool has = false;
+    uint32_t cl = 0u, blk = 0u;
     Trace t{};  // an idle lane: finished, takes no samples
     t.done = true;
     t.s = 32;
-    auto start = [&](uint32_t i) {
-      const uint32_t bl = i / n_valid;
-      cache = tile * 64u + (i - bl * n_valid);
-      blk = b0 + bl;
-      const float* e = a.caches + static_cast<size_t>(cache) * a.stride;
-      const f3 wp = make3(e[0], e[1], e[2]);
-      // cacheLightingRSM.comp:103-104
-      const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
-      t = trace_setup(a.vox, a.voxel_size, wp, vpos, a.blocks[blk]);
-    };
-    if (has) start(idx);
     while (true) {
-      const bool ended = trace_batch(a.vox, cr, lvi, t);
-      const bool fin = has && ended;
-      if (fin) {
-        a.vis[static_cast<size_t>(blk) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
+      // Idle lanes take the next entries (block-major: concurrent rays share
+      // a block and stay coherent).
+      const uint64_t idle = __ballot(!has);
+      const uint32_t n_idle = static_cast<uint32_t>(__builtin_popcountll(idle));
+      if (next < pool && n_idle != 0u && (n_idle >= a.refill_min || n_idle == 64u)) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(idle >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(idle), 0u));
+        if (!has && next + rank < pool) {
+          const uint32_t i = next + rank;
+          const uint32_t bl = n_valid == 64u ? (i >> 6) : i / n_valid;
+          cl = i - bl * n_valid;
+          blk = b0 + bl;
+          const float4 p0 = cpos[w][cl][0], p1 = cpos[w][cl][1];
+          t = trace_setup(a.vox, a.voxel_size, make3(p0.x, p0.y, p0.z), make3(p1.x, p1.y, p1.z), a.blocks[blk]);
+          has = true;
+        }
+        next += n_idle;
+      }
+      if (!__any(has)) break;
+      const bool ended = trace_batch<kBatch>(a.vox, cr, lvi, t);
+      if (has && ended) {
+        a.vis[static_cast<size_t>(blk) * a.pitch + tile * 64u + cl] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
         my_traces += 1u;
         my_exec += static_cast<uint32_t>(t.s);
         if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
         has = false;
+        t.done = true;
+        t.s = 32;
       }
-      if (!__any(has) && next >= pool) break;
-      const uint64_t want = __ballot(!has);
-      // Refill once enough lanes are idle (or all are), so the setup of new
-      // traces is shared by several lanes.
-      const uint32_t idle = static_cast<uint32_t>(__builtin_popcountll(want));
-      if (next < pool && idle < a.refill_min && idle < 64u) continue;
-      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(want >> 32),
-                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(want), 0u));
-      if (!has && next + rank < pool) {
-        idx = next + rank;
-        has = true;
-        start(idx);
-      }
-      next += static_cast<uint32_t>(__builtin_popcountll(want));
     }
+    __builtin_amdgcn_wave_barrier();  // cpos is restaged by the next item
   }
   if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
 }
@@ -498,14 +517,24 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
   a.steps = step_counter;
   a.blocks_per_item = static_cast<uint32_t>(tune.blocks_per_item);
   a.refill_min = static_cast<uint32_t>(tune.refill_min_lanes);
-  if (tune.visibility_kernel == 1)
-    hipLaunchKernelGGL(k_visibility_pool, dim3(grid), dim3(256), 0, s, a);
-  else if (tune.steps_per_batch == 2)
-    hipLaunchKernelGGL(k_visibility<2>, dim3(grid), dim3(256), 0, s, a);
-  else if (tune.steps_per_batch == 8)
-    hipLaunchKernelGGL(k_visibility<8>, dim3(grid), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_visibility<4>, dim3(grid), dim3(256), 0, s, a);
+  const dim3 g(grid), b(256);
+  if (tune.visibility_kernel == 1) {
+    switch (tune.steps_per_batch) {
+      case 1: hipLaunchKernelGGL(k_visibility_pool<1>, g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(k_visibility_pool<3>, g, b, 0, s, a); break;
+      case 4: hipLaunchKernelGGL(k_visibility_pool<4>, g, b, 0, s, a); break;
+      case 8: hipLaunchKernelGGL(k_visibility_pool<8>, g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL(k_visibility_pool<2>, g, b, 0, s, a); break;
+    }
+  } else {
+    switch (tune.steps_per_batch) {
+      case 1: hipLaunchKernelGGL(k_visibility<1>, g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(k_visibility<3>, g, b, 0, s, a); break;
+      case 4: hipLaunchKernelGGL(k_visibility<4>, g, b, 0, s, a); break;
+      case 8: hipLaunchKernelGGL(k_visibility<8>, g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL(k_visibility<2>, g, b, 0, s, a); break;
+    }
+  }
 }
 
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,

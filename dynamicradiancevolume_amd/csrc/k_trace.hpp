@@ -12,7 +12,7 @@ namespace drvk {
 // Below this transmittance a ray counts as fully occluded: the remaining
 // samples can lower the returned visibility by at most this much.
 constexpr float kOpaque = 1.0f / 1048576.0f;
-constexpr int kStepBatch = 4;
+constexpr int kStepBatch = 2;  // steps planned together (a lane starts its next ray at the next batch)
 
 // Per-lane state of one cone trace (cacheLightingRSM.comp:195-210).
 struct Trace {
@@ -24,14 +24,18 @@ struct Trace {
   bool done;         // dist reached goal (the reference's break)
 };
 
+// The direction is scaled by one reciprocal of the distance and the goal by
+// the reciprocal of the voxel size (uniform, hoisted by the compiler): they
+// differ from the divisions of the GLSL by at most an ulp.
 __device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_size, f3 wp, f3 vpos,
                                              const ShadowBlock& b) {
   const float fres = static_cast<float>(vox.res);
   float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
   const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
-  tx = tx / ld;
-  ty = ty / ld;
-  tz = tz / ld;
+  const float inv = 1.0f / ld;
+  tx = tx * inv;
+  ty = ty * inv;
+  tz = tz * inv;
   Trace t;
   // dirInVoxel = toAverageVal / volumeSize; positions are kept scaled by res.
   t.dx = tx;
@@ -42,9 +46,9 @@ __device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_s
   t.pz = vpos.z * fres + tz * 2.0f;
   t.dist = 0.0f;
   t.step = 1.0f;
-  t.goal = ld / voxel_size - 2.0f;
+  t.goal = ld * (1.0f / voxel_size) - 2.0f;
   t.rho = b.rho;
-  t.k = 2.0f / (1.0f - b.rho);
+  t.k = b.k;
   t.occ = 0.0f;
   t.s = 0;
   t.done = false;
