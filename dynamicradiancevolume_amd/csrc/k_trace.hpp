@@ -24,15 +24,18 @@ struct Trace {
   bool done;         // dist reached goal (the reference's break)
 };
 
-// The direction is scaled by one reciprocal of the distance and the goal by
-// the reciprocal of the voxel size (uniform, hoisted by the compiler): they
-// differ from the divisions of the GLSL by at most an ulp.
+// The direction and the distance come from one hardware reciprocal square
+// root (d / |d| = d * rsq, |d| = |d|^2 * rsq) and the goal is scaled by the
+// reciprocal of the voxel size (uniform, hoisted by the compiler): a few
+// ulp from the GLSL's normalize and divisions, far below the visibility
+// tolerance, and cheap enough to set up a ray in every batch.
 __device__ __forceinline__ Trace trace_setup(const VoxelView& vox, float voxel_size, f3 wp, f3 vpos,
                                              const ShadowBlock& b) {
   const float fres = static_cast<float>(vox.res);
   float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
-  const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
-  const float inv = 1.0f / ld;
+  const float d2 = tx * tx + ty * ty + tz * tz;
+  const float inv = __builtin_amdgcn_rsqf(d2);
+  const float ld = d2 * inv;
   tx = tx * inv;
   ty = ty * inv;
   tz = tz * inv;
