@@ -38,6 +38,8 @@ struct drv_ctx {
 
   float* caches = nullptr;
   Counter* counter = nullptr;
+  float4* phantoms = nullptr;        // allocation: positions of out-of-volume appends [max_caches]
+  uint32_t* alloc_counts = nullptr;  // allocation: claimed cells per chunk [kAllocChunks]
   uint32_t* cav = nullptr;
   size_t cav_cells = 0;
   uint8_t* vox_target = nullptr;
@@ -346,6 +348,8 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   const size_t stride = static_cast<size_t>(drvk::entry_stride(c.sh_bands));
   if ((rc = dev_alloc(ctx, &ctx->caches, stride * c.max_caches)) != DRV_OK) return bail(rc);
   if ((rc = dev_alloc(ctx, &ctx->counter, 1)) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->phantoms, static_cast<size_t>(c.max_caches))) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->alloc_counts, drvk::kAllocChunks)) != DRV_OK) return bail(rc);
   // SetCAVCascades: R32UI atlas of (C * R) x R x R (renderer.cpp:1179).
   ctx->cav_cells = static_cast<size_t>(c.cav_cascades) * c.cav_resolution * c.cav_resolution * c.cav_resolution;
   if ((rc = dev_alloc(ctx, &ctx->cav, ctx->cav_cells)) != DRV_OK) return bail(rc);
@@ -433,6 +437,8 @@ int drv_destroy(drv_ctx* ctx) {
   if (ctx->vox_done_ev) (void)hipEventDestroy(ctx->vox_done_ev);
   dev_free(ctx->caches);
   dev_free(ctx->counter);
+  dev_free(ctx->phantoms);
+  dev_free(ctx->alloc_counts);
   dev_free(ctx->cav);
   dev_free(ctx->spec.atlas);
   dev_free(ctx->spec.vpls_lanes);
@@ -646,8 +652,8 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
   // renderer.cpp:969-970: clear the counter and the address volume.
   drvk::launch_fill_u32_pair(ctx->cav, ctx->cav_cells, reinterpret_cast<uint32_t*>(ctx->counter),
                              sizeof(Counter) / 4, 0u, ctx->stream);
-  drvk::launch_allocate(ctx->cfg, ctx->vol, ctx->frame, gb->depth, ctx->cav, ctx->caches, ctx->counter, ctx->stream);
-  drvk::launch_prepare_dispatch(ctx->counter, static_cast<uint32_t>(ctx->cfg.max_caches), ctx->stream);
+  drvk::launch_allocate(ctx->cfg, ctx->vol, ctx->frame, gb->depth, ctx->cav, ctx->caches, ctx->counter,
+                        ctx->phantoms, ctx->alloc_counts, ctx->stream);
   record(ctx, 4);
   ctx->allocated = true;
   ctx->spec_first = true;

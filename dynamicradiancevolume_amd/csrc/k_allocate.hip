@@ -1,17 +1,19 @@
 // Stage A: G-buffer-driven cache allocation into the clipmap address volume.
 //
 // Restates shader/cacheGather.comp:93-164 (per-pixel cell, 16x16 tile
-// dedupe with the leader rule, CAS-locked claim of the 8 corner cells) and
+// dedupe with the leader rule, claim of the 8 corner cells) and
 // shader/cachePrepareLighting.comp:8-14, driven as in
 // Renderer::AllocateCaches (rendering/renderer.cpp:951-992).
 //
-// MI355X design: one 256-thread workgroup (4 wave64) per 16x16 pixel tile,
-// the two tile lists in LDS, and a wave-aggregated append: per corner the
-// lanes that won their CAS are counted with one ballot, one lane reserves the
-// whole run of cache indices with a single atomicAdd, and each lane takes its
-// slot by a prefix popcount.  The set of claimed cells and every cache
-// position is independent of the append order, which is the only thing the
-// reference leaves nondeterministic.
+// MI355X design: the reference's per-corner CAS lock and append form a
+// chain of dependent atomics on a few hot cells.  Here a claim is a plain
+// store of a mark (pass 1: one 256-thread workgroup per 16x16 pixel tile,
+// the two tile lists in LDS), and the claimed cells are numbered afterwards
+// by a two-launch count / assign over the address volume in a fixed cell
+// order (pass 2).  The set of claimed cells, every cache position and the
+// demanded count are the reference's; the append order, which the reference
+// leaves to its atomics, becomes deterministic for the claimed cells and
+// groups spatial neighbours into the same gather tile.
 #include <algorithm>
 
 #include "drv_internal.hpp"
@@ -45,14 +47,22 @@ __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
   for (size_t j = n16 * 16 + i; j < n; j += stride) p[j] = v;
 }
 
+// Marks pass 1 leaves in a claimed cell until pass 2 gives it its cache
+// index.  A cell of slab c + 1 at x = 0 is also the x = R corner of cascade c
+// (the reference's corner offsets run into the next slab of the atlas);
+// which cascade's position it gets depends on whose claim lands, in the
+// reference (CAS order) as here (store order).
+constexpr uint32_t kClaimed = 0xFFFFFFFFu;      // claimed by a corner of the cell's own cascade
+constexpr uint32_t kClaimedLeft = 0xFFFFFFFEu;  // claimed by the x = R corner of the cascade before
+
 struct AllocArgs {
   drv_volume vol;
   float inv_vp[16];
   const float* depth;
   uint32_t* cav;
-  float* caches;
   Counter* counter;
-  int width, height, R, C, stride;
+  float4* phantoms;  // positions of appended corners outside the CAV, [capacity]
+  int width, height, R, C;
   uint32_t capacity;
   int transitions;
 };
@@ -69,12 +79,24 @@ __device__ __forceinline__ int cache_1d_coord(const drv_volume& vol, int R, f3 w
   return gx + gy * R + gz * R * R + c * R * R * R;
 }
 
-// AllocateCaches, cacheGather.comp:32-91, called by every lane of the wave
-// (inactive lanes pass active = false) so the appends can be aggregated.
-// A lane issues its 8 corner CASes back to back and waits once; then one
-// ballot per corner numbers the winners (corner-major, lane-minor) and a
-// single atomicAdd per wave reserves the whole run of cache indices.
-__device__ void allocate_corners(const AllocArgs& a, bool active, int coord, int c) {
+// Entry.Position of corner (lx, ly, lz) of cascade c, cacheGather.comp:75-77.
+__device__ __forceinline__ float4 corner_position(const drv_volume& vol, int c, int lx, int ly, int lz) {
+  const drv_cascade& k = vol.cascades[c];
+  return make_float4(static_cast<float>(lx) * k.world_voxel_size + k.min[0],
+                     static_cast<float>(ly) * k.world_voxel_size + k.min[1],
+                     static_cast<float>(lz) * k.world_voxel_size + k.min[2], 0.0f);
+}
+
+// AllocateCaches, cacheGather.comp:32-91, first half: the claims.  The
+// reference locks each of the 8 corner cells with a CAS and the winner
+// appends; the set of claimed cells and each cell's position do not depend
+// on which claim wins, so a claim here is a plain store of a mark (no round
+// trip) and the appends of claimed cells happen in pass 2.  A lock on a cell
+// outside the image "succeeds" (image atomics outside the image return 0)
+// and appends a cache no cell points to: those appends are made here, one
+// atomicAdd per wave, into the phantom list.  Called by every lane of the
+// wave (inactive lanes pass active = false).
+__device__ void mark_corners(const AllocArgs& a, bool active, int coord, int c) {
   const int R = a.R, R2 = R * R, R3 = R2 * R;
   int bx = 0, by = 0, bz = 0;
   if (active) {
@@ -83,58 +105,34 @@ __device__ void allocate_corners(const AllocArgs& a, bool active, int coord, int
     by = (coord - bz * R2) / R;
     bx = coord % R;
   }
-  uint32_t cell[8];
-  uint32_t old[8];
-  uint32_t inside_mask = 0;
+  uint32_t outside = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     // offsets (0,0,0),(0,1,0),(0,0,1),(0,1,1),(1,0,0),(1,1,0),(1,0,1),(1,1,1)
     const int lx = bx + (i >> 2), ly = by + (i & 1), lz = bz + ((i >> 1) & 1);
     const int gx = lx + c * R;
-    const bool inside = active && gx >= 0 && gx < a.C * R && ly >= 0 && ly < R && lz >= 0 && lz < R;
-    cell[i] = inside ? static_cast<uint32_t>((lz * R + ly) * (a.C * R) + gx) : 0u;
-    old[i] = 1u;
-    if (inside) {
-      inside_mask |= 1u << i;
-      old[i] = atomicCAS(&a.cav[cell[i]], 0u, 0xFFFFFFFFu);
-    }
+    const bool inside = gx < a.C * R && ly < R && lz < R;  // the offsets are >= 0
+    if (active && inside) a.cav[(lz * R + ly) * (a.C * R) + gx] = lx == R ? kClaimedLeft : kClaimed;
+    if (active && !inside) outside |= 1u << i;
   }
-  // A lock on a cell outside the image "succeeds" (image atomics outside the
-  // image return 0) and appends a cache that no cell points to.
-  uint32_t won = 0;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const bool inside = (inside_mask >> i) & 1u;
-    if (active && (!inside || old[i] == 0u)) won |= 1u << i;
-  }
-  const unsigned lane = __lane_id();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t masks[8];
   uint32_t total = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    masks[i] = __ballot((won >> i) & 1u);
+    masks[i] = __ballot((outside >> i) & 1u);
     total += static_cast<uint32_t>(__popcll(masks[i]));
   }
   if (total == 0) return;
+  const unsigned lane = __lane_id();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(&a.counter->total, total);
+  if (lane == 0) base = atomicAdd(&a.counter->phantoms, total);
   base = __shfl(base, 0);
-  const drv_cascade& k = a.vol.cascades[c < 0 ? 0 : c];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    if ((won >> i) & 1u) {
-      const uint32_t idx = base + static_cast<uint32_t>(__popcll(masks[i] & lt_mask));
-      if (idx < a.capacity) {
-        const int lx = bx + (i >> 2), ly = by + (i & 1), lz = bz + ((i >> 1) & 1);
-        const float px = static_cast<float>(lx) * k.world_voxel_size + k.min[0];
-        const float py = static_cast<float>(ly) * k.world_voxel_size + k.min[1];
-        const float pz = static_cast<float>(lz) * k.world_voxel_size + k.min[2];
-        float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
-        e[0] = make_float4(px, py, pz, 0.0f);
-        for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      }
-      if ((inside_mask >> i) & 1u) a.cav[cell[i]] = idx + 1u;  // the cell stays locked for others
+    if ((outside >> i) & 1u) {
+      const uint32_t p = base + static_cast<uint32_t>(__popcll(masks[i] & lt_mask));
+      if (p < a.capacity) a.phantoms[p] = corner_position(a.vol, c, bx + (i >> 2), by + (i & 1), bz + ((i >> 1) & 1));
     }
     base += static_cast<uint32_t>(__popcll(masks[i]));
   }
@@ -177,21 +175,175 @@ __global__ __launch_bounds__(256) void k_allocate(AllocArgs a) {
                        (list2[vy * 16 + vx] != own2)) ||
                       (vx == 15 && vy == 15)) &&
                      own2 != -1;
-  allocate_corners(a, lead, own, c);
-  if (a.transitions) allocate_corners(a, lead2, own2, c + 1);
+  mark_corners(a, lead, own, c);
+  if (a.transitions) mark_corners(a, lead2, own2, c + 1);
 }
 
-// cachePrepareLighting.comp:8-14 plus the live count and cache tiling used by
-// the gather (capacity clamp = robust buffer access).
-__global__ void k_prepare_dispatch(Counter* ctr, uint32_t capacity) {
-  const uint32_t total = ctr->total;
-  ctr->groups_x = (total + 63u) / 64u;
-  ctr->groups_y = 1u;
-  ctr->groups_z = 1u;
-  const uint32_t live = min(total, capacity);
-  ctr->live = live;
-  ctr->tiles = (live + kGatherTile - 1) / kGatherTile;
-  ctr->splits = 1u;
+// ---- pass 2: cache indices in cell order -------------------------------
+//
+// Claimed cells are numbered in cascade-major order of 4x4x4 bricks (cells
+// of one brick are consecutive), so the caches a gather tile holds are
+// neighbours in space; the phantoms follow them.  Each workgroup numbers a
+// fixed chunk of that order: one launch counts the claims per chunk, the
+// next adds the counts of the chunks before (at most kAllocChunks of them)
+// and writes the indices, the entries and the dispatch counter.
+
+struct AssignArgs {
+  drv_volume vol;
+  uint32_t* cav;
+  float* caches;
+  Counter* counter;
+  const float4* phantoms;
+  uint32_t* counts;  // [gridDim.x] claims per chunk
+  int R, C, stride;
+  uint32_t n_cells, chunk, capacity;
+};
+
+// Logical cell L (cascade c, brick, cell in brick) -> cascade and local
+// coordinates; bricks need R % 4 == 0, else rows of x.
+struct CellCoord {
+  int c, x, y, z;
+};
+
+// Every third bit of v, packed (the inverse of a 3D Morton interleave).
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030C30C3u;
+  v = (v ^ (v >> 4)) & 0x0300F00Fu;
+  v = (v ^ (v >> 8)) & 0x030000FFu;
+  v = (v ^ (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
+__device__ __forceinline__ CellCoord cell_coord(const AssignArgs& a, uint32_t L) {
+  const uint32_t R = static_cast<uint32_t>(a.R), R3 = R * R * R;
+  const uint32_t c = L / R3, r = L - c * R3;
+  uint32_t x, y, z;
+  if ((R & 3u) == 0u) {
+    const uint32_t nb = R >> 2, b = r >> 6, w = r & 63u;
+    uint32_t bx, by, bz;
+    if ((nb & (nb - 1u)) == 0u) {
+      bx = compact3(b);
+      by = compact3(b >> 1);
+      bz = compact3(b >> 2);
+    } else {
+      bz = b / (nb * nb);
+      const uint32_t rem = b - bz * nb * nb;
+      by = rem / nb;
+      bx = rem - by * nb;
+    }
+    x = (bx << 2) | (w & 3u);
+    y = (by << 2) | ((w >> 2) & 3u);
+    z = (bz << 2) | (w >> 4);
+  } else {
+    z = r / (R * R);
+    const uint32_t q = r - z * R * R;
+    y = q / R;
+    x = q - y * R;
+  }
+  return CellCoord{static_cast<int>(c), static_cast<int>(x), static_cast<int>(y), static_cast<int>(z)};
+}
+
+__device__ __forceinline__ uint32_t cell_flat(const AssignArgs& a, const CellCoord& k) {
+  return static_cast<uint32_t>((k.z * a.R + k.y) * (a.C * a.R) + k.c * a.R + k.x);
+}
+
+__device__ __forceinline__ bool is_claim(uint32_t v) { return v >= kClaimedLeft; }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_allocate_count(AssignArgs a) {
+  __shared__ uint32_t part[4];
+  const uint32_t begin = blockIdx.x * a.chunk;
+  const uint32_t end = min(a.n_cells, begin + a.chunk);
+  uint32_t n = 0;
+  for (uint32_t L = begin + threadIdx.x; L < end; L += 256u) n += is_claim(a.cav[cell_flat(a, cell_coord(a, L))]);
+  n = wave_sum(n);
+  if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) a.counts[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+__global__ __launch_bounds__(256) void k_allocate_assign(AssignArgs a) {
+  __shared__ uint32_t part[2][4];
+  __shared__ uint32_t wcnt[2][4];
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  // Claims before this chunk and in all chunks.
+  uint32_t before = 0, all = 0;
+  for (uint32_t i = t; i < gridDim.x; i += 256u) {
+    const uint32_t v = a.counts[i];
+    all += v;
+    before += i < blockIdx.x ? v : 0u;
+  }
+  before = wave_sum(before);
+  all = wave_sum(all);
+  if (lane == 0) {
+    part[0][w] = before;
+    part[1][w] = all;
+  }
+  __syncthreads();
+  uint32_t base = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
+  const uint32_t n_claimed = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
+  const uint32_t n_phantoms = a.counter->phantoms;
+  const uint32_t total = n_claimed + n_phantoms;
+  if (blockIdx.x == 0 && t == 0) {
+    // cachePrepareLighting.comp:8-14 plus the live count and cache tiling used
+    // by the gather (capacity clamp = robust buffer access).
+    Counter* ctr = a.counter;
+    ctr->total = total;
+    ctr->groups_x = (total + 63u) / 64u;
+    ctr->groups_y = 1u;
+    ctr->groups_z = 1u;
+    const uint32_t live = min(total, a.capacity);
+    ctr->live = live;
+    ctr->tiles = (live + kGatherTile - 1) / kGatherTile;
+    ctr->splits = 1u;
+  }
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t begin = blockIdx.x * a.chunk;
+  const uint32_t end = min(a.n_cells, begin + a.chunk);
+  uint32_t step = 0;
+  for (uint32_t L0 = begin; L0 < end; L0 += 256u, step ^= 1u) {
+    const uint32_t L = L0 + t;
+    CellCoord k{};
+    uint32_t flat = 0, v = 0;
+    if (L < end) {
+      k = cell_coord(a, L);
+      flat = cell_flat(a, k);
+      v = a.cav[flat];
+    }
+    const bool claim = L < end && is_claim(v);
+    const uint64_t m = __ballot(claim);
+    if (lane == 0) wcnt[step][w] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();  // wcnt[step] is rewritten two steps on, after the next barrier
+    uint32_t off = base;
+    for (uint32_t j = 0; j < w; ++j) off += wcnt[step][j];
+    if (claim) {
+      const uint32_t idx = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+      if (idx < a.capacity) {
+        // Entry.Position from the claiming cascade, cacheGather.comp:75-77.
+        const bool left = v == kClaimedLeft;
+        const float4 p = corner_position(a.vol, left ? k.c - 1 : k.c, left ? a.R : k.x, k.y, k.z);
+        float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
+        e[0] = p;
+        for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      a.cav[flat] = idx + 1u;  // cacheGather.comp:79
+    }
+    base += (wcnt[step][0] + wcnt[step][1]) + (wcnt[step][2] + wcnt[step][3]);
+  }
+  // The phantoms, after the claimed cells.
+  for (uint32_t p = blockIdx.x * 256u + t; p < n_phantoms; p += gridDim.x * 256u) {
+    const uint32_t idx = n_claimed + p;
+    if (idx >= a.capacity) break;
+    float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
+    e[0] = a.phantoms[p];
+    for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
 }  // namespace
@@ -215,27 +367,43 @@ void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s) {
 }
 
 void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const float* depth,
-                     uint32_t* cav, float* caches, Counter* counter, hipStream_t s) {
+                     uint32_t* cav, float* caches, Counter* counter, float4* phantoms, uint32_t* counts,
+                     hipStream_t s) {
   AllocArgs a;
   a.vol = vol;
   for (int i = 0; i < 16; ++i) a.inv_vp[i] = frame.inverse_view_projection[i];
   a.depth = depth;
   a.cav = cav;
-  a.caches = caches;
   a.counter = counter;
+  a.phantoms = phantoms;
   a.width = cfg.width;
   a.height = cfg.height;
   a.R = cfg.cav_resolution;
   a.C = cfg.cav_cascades;
-  a.stride = entry_stride(cfg.sh_bands);
   a.capacity = static_cast<uint32_t>(cfg.max_caches);
   a.transitions = cfg.cav_transition_size > 0.0f ? 1 : 0;
   const dim3 grid((cfg.width + 15) / 16, (cfg.height + 15) / 16);
   hipLaunchKernelGGL(k_allocate, grid, dim3(256), 0, s, a);
+
+  AssignArgs b;
+  b.vol = vol;
+  b.cav = cav;
+  b.caches = caches;
+  b.counter = counter;
+  b.phantoms = phantoms;
+  b.counts = counts;
+  b.R = cfg.cav_resolution;
+  b.C = cfg.cav_cascades;
+  b.stride = entry_stride(cfg.sh_bands);
+  b.n_cells = static_cast<uint32_t>(static_cast<uint64_t>(cfg.cav_cascades) * cfg.cav_resolution *
+                                    cfg.cav_resolution * cfg.cav_resolution);
+  b.capacity = static_cast<uint32_t>(cfg.max_caches);
+  const uint32_t chunks = std::min<uint32_t>((b.n_cells + 1023u) / 1024u, static_cast<uint32_t>(kAllocChunks));
+  b.chunk = ((b.n_cells + chunks - 1u) / chunks + 255u) & ~255u;
+  const uint32_t groups = (b.n_cells + b.chunk - 1u) / b.chunk;
+  hipLaunchKernelGGL(k_allocate_count, dim3(groups), dim3(256), 0, s, b);
+  hipLaunchKernelGGL(k_allocate_assign, dim3(groups), dim3(256), 0, s, b);
 }
 
-void launch_prepare_dispatch(Counter* counter, uint32_t capacity, hipStream_t s) {
-  hipLaunchKernelGGL(k_prepare_dispatch, dim3(1), dim3(1), 0, s, counter, capacity);
-}
 
 }  // namespace drvk

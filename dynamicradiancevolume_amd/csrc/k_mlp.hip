@@ -729,7 +729,7 @@ __global__ void k_spot_select(const Counter* counter, uint32_t k, const float* c
       spot_ctr[c].live = n_spot;
       spot_ctr[c].tiles = (n_spot + 63u) / 64u;
       spot_ctr[c].splits = 1u;
-      spot_ctr[c].pad = 0u;
+      spot_ctr[c].phantoms = 0u;
     }
   }
   if (i >= n_spot) return;
