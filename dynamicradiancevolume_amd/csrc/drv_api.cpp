@@ -39,7 +39,8 @@ struct drv_ctx {
   float* caches = nullptr;
   Counter* counter = nullptr;
   float4* phantoms = nullptr;        // allocation: positions of out-of-volume appends [max_caches]
-  uint32_t* alloc_counts = nullptr;  // allocation: claimed cells per chunk [kAllocChunks]
+  uint32_t* alloc_counts = nullptr;  // allocation: claimed cells per chunk, then of the last call [kAllocChunks + 1]
+  uint32_t* alloc_claimed = nullptr; // allocation: CAV index of each claimed cell [cav_cells]
   uint32_t* cav = nullptr;
   size_t cav_cells = 0;
   uint8_t* vox_target = nullptr;
@@ -349,10 +350,11 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   if ((rc = dev_alloc(ctx, &ctx->caches, stride * c.max_caches)) != DRV_OK) return bail(rc);
   if ((rc = dev_alloc(ctx, &ctx->counter, 1)) != DRV_OK) return bail(rc);
   if ((rc = dev_alloc(ctx, &ctx->phantoms, static_cast<size_t>(c.max_caches))) != DRV_OK) return bail(rc);
-  if ((rc = dev_alloc(ctx, &ctx->alloc_counts, drvk::kAllocChunks)) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->alloc_counts, drvk::kAllocChunks + 1)) != DRV_OK) return bail(rc);
   // SetCAVCascades: R32UI atlas of (C * R) x R x R (renderer.cpp:1179).
   ctx->cav_cells = static_cast<size_t>(c.cav_cascades) * c.cav_resolution * c.cav_resolution * c.cav_resolution;
   if ((rc = dev_alloc(ctx, &ctx->cav, ctx->cav_cells)) != DRV_OK) return bail(rc);
+  if ((rc = dev_alloc(ctx, &ctx->alloc_claimed, ctx->cav_cells)) != DRV_OK) return bail(rc);
 
   // Voxelization::SetResolution (voxelization.cpp:56-60): R8 target plus an
   // R8 volume with its full mip chain, plus the corner-expanded copy.
@@ -411,6 +413,7 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   // Zero-initialize the state a frame may read before writing it.
   drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->counter), sizeof(Counter) / 4, 0u, nullptr);
   drvk::launch_fill_u32(ctx->cav, ctx->cav_cells, 0u, nullptr);
+  drvk::launch_fill_u32(ctx->alloc_counts, drvk::kAllocChunks + 1, 0u, nullptr);
   drvk::launch_fill_u8(ctx->vox_target, res3, 0, nullptr);
   drvk::launch_fill_u8(ctx->vox_mips, L.mip_bytes, 0, nullptr);
   drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->vox_corners), L.corner_count * 2, 0u, nullptr);
@@ -439,6 +442,7 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->counter);
   dev_free(ctx->phantoms);
   dev_free(ctx->alloc_counts);
+  dev_free(ctx->alloc_claimed);
   dev_free(ctx->cav);
   dev_free(ctx->spec.atlas);
   dev_free(ctx->spec.vpls_lanes);
@@ -649,11 +653,10 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
     return fail(ctx, DRV_ERR_INVALID_ARG, "allocate: G-buffer size differs from the configured backbuffer");
   if (!ctx->have_frame || !ctx->have_volume) return fail(ctx, DRV_ERR_STATE, "allocate: frame or volume not set");
   record(ctx, 3);
-  // renderer.cpp:969-970: clear the counter and the address volume.
-  drvk::launch_fill_u32_pair(ctx->cav, ctx->cav_cells, reinterpret_cast<uint32_t*>(ctx->counter),
-                             sizeof(Counter) / 4, 0u, ctx->stream);
+  // renderer.cpp:969-970 (the clear of the counter and the address volume)
+  // and cacheGather.comp, cachePrepareLighting.comp.
   drvk::launch_allocate(ctx->cfg, ctx->vol, ctx->frame, gb->depth, ctx->cav, ctx->caches, ctx->counter,
-                        ctx->phantoms, ctx->alloc_counts, ctx->stream);
+                        ctx->phantoms, ctx->alloc_counts, ctx->alloc_claimed, ctx->stream);
   record(ctx, 4);
   ctx->allocated = true;
   ctx->spec_first = true;

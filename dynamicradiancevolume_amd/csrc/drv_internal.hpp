@@ -142,16 +142,17 @@ struct SpecAtlas {
 };
 
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
-void launch_fill_u32_pair(uint32_t* p, size_t n, uint32_t* q, size_t m, uint32_t v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 
-// Pass 1 (claims, phantom appends) and pass 2 (count, assign: indices,
-// entries, dispatch counter); `counts` holds kAllocChunks words, `phantoms`
+// Clear of the previous claims, pass 1 (claims, phantom appends) and pass 2
+// (count, assign: indices, entries, dispatch counter).  `counts` holds
+// kAllocChunks + 1 words (the last: claimed cells of the previous call,
+// zero with an all-zero CAV), `claimed` one word per CAV cell, `phantoms`
 // max_caches positions.
 constexpr int kAllocChunks = 1024;
 void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const float* depth,
                      uint32_t* cav, float* caches, Counter* counter, float4* phantoms, uint32_t* counts,
-                     hipStream_t s);
+                     uint32_t* claimed, hipStream_t s);
 
 void launch_voxelize(const drv_volume& vol, int res, const float* positions, const uint32_t* indices, uint32_t n_tri,
                      const float* world, uint8_t* target, hipStream_t s);

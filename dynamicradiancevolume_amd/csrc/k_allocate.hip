@@ -28,14 +28,6 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-// Two buffers cleared by one launch (the allocation's counter and CAV).
-__global__ void k_fill_u32_pair(uint32_t* p, size_t n, uint32_t* q, size_t m, uint32_t v) {
-  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (size_t j = i; j < n; j += stride) p[j] = v;
-  for (size_t j = i; j < m; j += stride) q[j] = v;
-}
-
 __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
   size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -194,7 +186,8 @@ struct AssignArgs {
   float* caches;
   Counter* counter;
   const float4* phantoms;
-  uint32_t* counts;  // [gridDim.x] claims per chunk
+  uint32_t* counts;  // [gridDim.x] claims per chunk; [kAllocChunks] claimed cells of the last allocation
+  uint32_t* claimed;  // flat CAV index of every claimed cell, by cache index
   int R, C, stride;
   uint32_t n_cells, chunk, capacity;
 };
@@ -256,12 +249,35 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+// renderer.cpp:969-970 clears the whole address volume before each
+// allocation; only the cells the previous allocation claimed are non-zero
+// (the volume is cleared once at creation), so only those are cleared, with
+// the counter.
+__global__ __launch_bounds__(256) void k_allocate_clear(uint32_t* cav, const uint32_t* claimed,
+                                                       const uint32_t* counts, Counter* counter) {
+  const uint32_t n = counts[kAllocChunks];
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) cav[claimed[i]] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(Counter) / 4) reinterpret_cast<uint32_t*>(counter)[threadIdx.x] = 0u;
+}
+
+// Steps of 256 cells a thread loads ahead in the count / assign scans.
+constexpr int kScanAhead = 4;
+
 __global__ __launch_bounds__(256) void k_allocate_count(AssignArgs a) {
   __shared__ uint32_t part[4];
   const uint32_t begin = blockIdx.x * a.chunk;
   const uint32_t end = min(a.n_cells, begin + a.chunk);
   uint32_t n = 0;
-  for (uint32_t L = begin + threadIdx.x; L < end; L += 256u) n += is_claim(a.cav[cell_flat(a, cell_coord(a, L))]);
+  for (uint32_t L0 = begin + threadIdx.x; L0 < end; L0 += 256u * kScanAhead) {
+    uint32_t v[kScanAhead];
+#pragma unroll
+    for (int u = 0; u < kScanAhead; ++u) {
+      const uint32_t L = L0 + 256u * u;
+      v[u] = L < end ? a.cav[cell_flat(a, cell_coord(a, L))] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kScanAhead; ++u) n += is_claim(v[u]);
+  }
   n = wave_sum(n);
   if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = n;
   __syncthreads();
@@ -302,39 +318,50 @@ __global__ __launch_bounds__(256) void k_allocate_assign(AssignArgs a) {
     ctr->live = live;
     ctr->tiles = (live + kGatherTile - 1) / kGatherTile;
     ctr->splits = 1u;
+    a.counts[kAllocChunks] = n_claimed;  // the cells the next allocation clears
   }
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t begin = blockIdx.x * a.chunk;
   const uint32_t end = min(a.n_cells, begin + a.chunk);
   uint32_t step = 0;
-  for (uint32_t L0 = begin; L0 < end; L0 += 256u, step ^= 1u) {
-    const uint32_t L = L0 + t;
-    CellCoord k{};
-    uint32_t flat = 0, v = 0;
-    if (L < end) {
-      k = cell_coord(a, L);
-      flat = cell_flat(a, k);
-      v = a.cav[flat];
-    }
-    const bool claim = L < end && is_claim(v);
-    const uint64_t m = __ballot(claim);
-    if (lane == 0) wcnt[step][w] = static_cast<uint32_t>(__popcll(m));
-    __syncthreads();  // wcnt[step] is rewritten two steps on, after the next barrier
-    uint32_t off = base;
-    for (uint32_t j = 0; j < w; ++j) off += wcnt[step][j];
-    if (claim) {
-      const uint32_t idx = off + static_cast<uint32_t>(__popcll(m & lt_mask));
-      if (idx < a.capacity) {
-        // Entry.Position from the claiming cascade, cacheGather.comp:75-77.
-        const bool left = v == kClaimedLeft;
-        const float4 p = corner_position(a.vol, left ? k.c - 1 : k.c, left ? a.R : k.x, k.y, k.z);
-        float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
-        e[0] = p;
-        for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (uint32_t L0 = begin; L0 < end; L0 += 256u * kScanAhead) {
+    CellCoord k[kScanAhead];
+    uint32_t flat[kScanAhead], v[kScanAhead];
+#pragma unroll
+    for (int u = 0; u < kScanAhead; ++u) {
+      const uint32_t L = L0 + 256u * u + t;
+      k[u] = CellCoord{};
+      flat[u] = 0u;
+      v[u] = 0u;
+      if (L < end) {
+        k[u] = cell_coord(a, L);
+        flat[u] = cell_flat(a, k[u]);
+        v[u] = a.cav[flat[u]];
       }
-      a.cav[flat] = idx + 1u;  // cacheGather.comp:79
     }
-    base += (wcnt[step][0] + wcnt[step][1]) + (wcnt[step][2] + wcnt[step][3]);
+#pragma unroll
+    for (int u = 0; u < kScanAhead; ++u, step ^= 1u) {
+      const bool claim = is_claim(v[u]);  // 0 past the end
+      const uint64_t m = __ballot(claim);
+      if (lane == 0) wcnt[step][w] = static_cast<uint32_t>(__popcll(m));
+      __syncthreads();  // wcnt[step] is rewritten two steps on, after the next barrier
+      uint32_t off = base;
+      for (uint32_t j = 0; j < w; ++j) off += wcnt[step][j];
+      if (claim) {
+        const uint32_t idx = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+        if (idx < a.capacity) {
+          // Entry.Position from the claiming cascade, cacheGather.comp:75-77.
+          const bool left = v[u] == kClaimedLeft;
+          const float4 p = corner_position(a.vol, left ? k[u].c - 1 : k[u].c, left ? a.R : k[u].x, k[u].y, k[u].z);
+          float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
+          e[0] = p;
+          for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        a.cav[flat[u]] = idx + 1u;  // cacheGather.comp:79
+        a.claimed[idx] = flat[u];
+      }
+      base += (wcnt[step][0] + wcnt[step][1]) + (wcnt[step][2] + wcnt[step][3]);
+    }
   }
   // The phantoms, after the claimed cells.
   for (uint32_t p = blockIdx.x * 256u + t; p < n_phantoms; p += gridDim.x * 256u) {
@@ -354,12 +381,6 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p, n, v);
 }
 
-void launch_fill_u32_pair(uint32_t* p, size_t n, uint32_t* q, size_t m, uint32_t v, hipStream_t s) {
-  const size_t blocks = std::min<size_t>((std::max(n, m) + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_fill_u32_pair, dim3(static_cast<unsigned>(std::max<size_t>(blocks, 1))), dim3(256), 0, s, p, n,
-                     q, m, v);
-}
-
 void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s) {
   if (n == 0) return;
   const size_t blocks = std::min<size_t>((n / 16 + 255) / 256 + 1, 8192);
@@ -368,7 +389,8 @@ void launch_fill_u8(uint8_t* p, size_t n, uint8_t v, hipStream_t s) {
 
 void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_frame& frame, const float* depth,
                      uint32_t* cav, float* caches, Counter* counter, float4* phantoms, uint32_t* counts,
-                     hipStream_t s) {
+                     uint32_t* claimed, hipStream_t s) {
+  hipLaunchKernelGGL(k_allocate_clear, dim3(64), dim3(256), 0, s, cav, claimed, counts, counter);
   AllocArgs a;
   a.vol = vol;
   for (int i = 0; i < 16; ++i) a.inv_vp[i] = frame.inverse_view_projection[i];
@@ -392,6 +414,7 @@ void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_fra
   b.counter = counter;
   b.phantoms = phantoms;
   b.counts = counts;
+  b.claimed = claimed;
   b.R = cfg.cav_resolution;
   b.C = cfg.cav_cascades;
   b.stride = entry_stride(cfg.sh_bands);

@@ -161,11 +161,19 @@ __device__ __forceinline__ void cache_irradiance(const ApplyArgs& a, const float
   irr[2] = fmaxf(v2, 0.0f);
 }
 
-// ComputeLightingFromCaches, cacheApply.frag:28-118.  All 8 addresses are
-// fetched first, then all 8 entries, so one pixel keeps 16 loads in flight.
+// ComputeLightingFromCaches, cacheApply.frag:28-118.  The lanes of a wave
+// (16x4 pixels) mostly share their 8 caches (2.1 distinct addresses per
+// corner on a 1080p frame), and 8 x 7 per-lane 16-byte loads keep the
+// texture address and data units busy.  So per corner the wave's first two
+// distinct entries are fetched once - lanes 0-27 and 32-59 load one float
+// each, the 8 corners' loads in flight together - and staged in LDS, where
+// their lanes read them; lanes with a third or later address load their own
+// entry.  Per lane the arithmetic and its order are those of one fetch per
+// pixel.  Every lane of the wave must call this (the staging is
+// cooperative); `active` = false lanes add nothing.
 template <bool kSpec>
-__device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c, const float* diffuse, uint32_t live,
-                                         const SpecPixel& sp, float* out) {
+__device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp, f3 n, int c, const float* diffuse,
+                                         uint32_t live, const SpecPixel& sp, float* stage, float* out) {
   const drv_cascade& k = a.vol.cascades[c];
   const float acx = (wp.x - k.min[0]) / k.world_voxel_size;
   const float acy = (wp.y - k.min[1]) / k.world_voxel_size;
@@ -183,38 +191,82 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c,
     const int x = ax + (i & 1) + R * c, y = ay + ((i >> 1) & 1), z = az + (i >> 2);
     // texelFetch outside the CAV reads 0 -> address 0xFFFFFFFF
     uint32_t v = 0u;
-    if (x >= 0 && y >= 0 && z >= 0 && x < CR && y < R && z < R) v = a.cav[(static_cast<uint32_t>(z) * R + y) * CR + x];
+    if (active && x >= 0 && y >= 0 && z >= 0 && x < CR && y < R && z < R)
+      v = a.cav[(static_cast<uint32_t>(z) * R + y) * CR + x];
     addr[i] = v - 1u;
   }
   const int nq = a.sh2 ? 8 : 4;
   float acc[3] = {0.0f, 0.0f, 0.0f};
   float spec[3] = {0.0f, 0.0f, 0.0f};
+  if (kSpec) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (addr[i] < live) {
+        // cacheApply.frag:102-106
+        const uint32_t cx = addr[i] & (a.per_row - 1u), cy = addr[i] >> a.per_row_log2;
+        float smp[3];
+        tap_cache(a, sp.t1, cx, cy, smp);
+        if (sp.two) {
+          float s2[3];
+          tap_cache(a, sp.t2, cx, cy, s2);
+          for (int k = 0; k < 3; ++k) smp[k] = (1.0f - sp.blend) * smp[k] + sp.blend * s2[k];
+        }
+        for (int k = 0; k < 3; ++k) spec[k] = spec[k] + smp[k] * w[i];
+      }
+    }
+  }
+  // An address past the live caches (unclaimed cell, overflow) adds zero.
+  const uint32_t lane = __lane_id();
+  const uint32_t sub = lane & 31u;
+  const uint32_t nf = static_cast<uint32_t>(4 * (nq - 1));  // entry floats 4 .. 4 + nf - 1 are read
+  uint32_t from[8];  // 0: first address, 1: second, 2: own load, 3: none
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    // An address past the live caches (unclaimed cell, overflow) reads zeros.
     const bool ok = addr[i] < live;
-    if (kSpec && ok) {
-      // cacheApply.frag:102-106
-      const uint32_t cx = addr[i] & (a.per_row - 1u), cy = addr[i] >> a.per_row_log2;
-      float smp[3];
-      tap_cache(a, sp.t1, cx, cy, smp);
-      if (sp.two) {
-        float s2[3];
-        tap_cache(a, sp.t2, cx, cy, s2);
-        for (int k = 0; k < 3; ++k) smp[k] = (1.0f - sp.blend) * smp[k] + sp.blend * s2[k];
-      }
-      for (int k = 0; k < 3; ++k) spec[k] = spec[k] + smp[k] * w[i];
+    uint64_t pend = __ballot(ok);
+    uint32_t ua = 0u, ub = 0u;
+    bool in_a = false, in_b = false, have = false;
+    if (pend != 0ull) {
+      ua = static_cast<uint32_t>(
+          __builtin_amdgcn_readlane(static_cast<int>(addr[i]), static_cast<int>(__builtin_ctzll(pend))));
+      in_a = ok && addr[i] == ua;
+      pend &= ~__ballot(in_a);
+      have = lane < 32u;
     }
-    const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(ok ? addr[i] : 0u) * a.stride);
-    float4 q[8];
+    if (pend != 0ull) {
+      ub = static_cast<uint32_t>(
+          __builtin_amdgcn_readlane(static_cast<int>(addr[i]), static_cast<int>(__builtin_ctzll(pend))));
+      in_b = ok && !in_a && addr[i] == ub;
+      have = true;
+    }
+    from[i] = in_a ? 0u : in_b ? 1u : ok ? 2u : 3u;
+    const uint32_t u = lane < 32u ? ua : ub;
+    const bool load = have && sub < nf;
+    stage[i * 64 + lane] = load ? a.caches[static_cast<size_t>(u) * a.stride + 4u + sub] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int j = 1; j < 8; ++j) q[j] = (ok && j < nq) ? e4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int i = 0; i < 8; ++i) {
+    if (from[i] == 3u) continue;
+    float4 q[8];
+    if (from[i] == 2u) {
+      const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(addr[i]) * a.stride);
+#pragma unroll
+      for (int j = 1; j < 8; ++j) q[j] = j < nq ? e4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      const float4* st = reinterpret_cast<const float4*>(stage + i * 64 + 32 * from[i]);
+#pragma unroll
+      for (int j = 1; j < 8; ++j) q[j] = j < nq ? st[j - 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     float irr[3];
     cache_irradiance(a, q, n, irr);
     acc[0] = acc[0] + irr[0] * w[i];
     acc[1] = acc[1] + irr[1] * w[i];
     acc[2] = acc[2] + irr[2] * w[i];
   }
+  __builtin_amdgcn_wave_barrier();  // the stage is rewritten by the next call
   out[0] = acc[0] * diffuse[0] / kPi;
   out[1] = acc[1] * diffuse[1] / kPi;
   out[2] = acc[2] * diffuse[2] / kPi;
@@ -225,33 +277,41 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, f3 wp, f3 n, int c,
   }
 }
 
+// The pixel's G-buffer texels are all loaded before the discard test and
+// the sRGB table sits in LDS, so a wave waits for one G-buffer round trip.
 template <bool kSpec>
 __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
+  __shared__ float lut[256];
+  __shared__ __align__(16) float stage[4][8 * 64];  // per wave: the shared entries of each corner
+  lut[threadIdx.x] = a.srgb_lut[threadIdx.x];
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= a.width || y >= a.height) return;
-  const size_t p = static_cast<size_t>(y) * a.width + x;
-  float* o = a.out + 3 * p;
+  const bool inside = x < a.width && y < a.height;
+  const size_t p = inside ? static_cast<size_t>(y) * a.width + x : 0;
   const float d = a.depth[p];
-  if (d < 0.00001f) {  // cacheApply.frag:127-131: discard
-    if (!a.accumulate) o[0] = o[1] = o[2] = 0.0f;
-    return;
+  const uint32_t nrm = reinterpret_cast<const uint32_t*>(a.normal)[p];  // RG16I texel
+  const uint32_t al0 = a.albedo[3 * p], al1 = a.albedo[3 * p + 1], al2 = a.albedo[3 * p + 2];
+  uint32_t rm0 = 255u, rm1 = 0u;
+  if (kSpec && a.roughness_metallic) {
+    rm0 = a.roughness_metallic[2 * p];
+    rm1 = a.roughness_metallic[2 * p + 1];
   }
+  __syncthreads();
+  // cacheApply.frag:127-131: discard.  Discarded and outside lanes still run
+  // the (cooperative) cache fetches, inactive.
+  const bool valid = inside && !(d < 0.00001f);
   const uint32_t live = a.counter->live;
   const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d);
   const int c = cascade_of(a.vol, a.C, wp);
-  const f3 n = unpack_normal16i(a.normal[2 * p], a.normal[2 * p + 1]);
-  float diffuse[3] = {a.srgb_lut[a.albedo[3 * p]], a.srgb_lut[a.albedo[3 * p + 1]],
-                      a.srgb_lut[a.albedo[3 * p + 2]]};
+  const f3 n = unpack_normal16i(static_cast<int16_t>(nrm & 0xFFFFu), static_cast<int16_t>(nrm >> 16));
+  float diffuse[3] = {lut[al0], lut[al1], lut[al2]};
   SpecPixel sp{};
   if (kSpec) {
     // cacheApply.frag:139-154 with lightingfunctions.glsl:26-44 and
-    // utils.glsl:193-201.
-    float rough = 1.0f, metal = 0.0f;
-    if (a.roughness_metallic) {
-      rough = static_cast<float>(a.roughness_metallic[2 * p]) / 255.0f;
-      metal = static_cast<float>(a.roughness_metallic[2 * p + 1]) / 255.0f;
-    }
+    // utils.glsl:193-201 (no roughness / metallic texture: roughness 1,
+    // metallic 0).
+    const float rough = static_cast<float>(rm0) / 255.0f;
+    const float metal = static_cast<float>(rm1) / 255.0f;
     for (int k = 0; k < 3; ++k) {
       const float base = diffuse[k];
       diffuse[k] = base * (1.0f - metal) + 0.02f * metal;
@@ -281,17 +341,27 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
       sp.two = true;
     }
   }
+  float* st = stage[threadIdx.x >> 6];
   float col[3];
-  lighting<kSpec>(a, wp, n, c, diffuse, live, sp, col);
+  lighting<kSpec>(a, valid, wp, n, c, diffuse, live, sp, st, col);
   if (a.transitions) {  // cacheApply.frag:172-185
-    const float t = cascade_transition(a.vol, c, wp);
-    if (t > 0.0f && c < a.C - 1) {
+    const float t = valid ? cascade_transition(a.vol, c, wp) : 0.0f;
+    const bool two = t > 0.0f && c < a.C - 1;
+    if (__ballot(two) != 0ull) {
       float col2[3];
-      lighting<kSpec>(a, wp, n, c + 1, diffuse, live, sp, col2);
-      col[0] = col[0] * (1.0f - t) + col2[0] * t;
-      col[1] = col[1] * (1.0f - t) + col2[1] * t;
-      col[2] = col[2] * (1.0f - t) + col2[2] * t;
+      lighting<kSpec>(a, two, wp, n, two ? c + 1 : c, diffuse, live, sp, st, col2);
+      if (two) {
+        col[0] = col[0] * (1.0f - t) + col2[0] * t;
+        col[1] = col[1] * (1.0f - t) + col2[1] * t;
+        col[2] = col[2] * (1.0f - t) + col2[2] * t;
+      }
     }
+  }
+  if (!inside) return;
+  float* o = a.out + 3 * p;
+  if (!valid) {
+    if (!a.accumulate) o[0] = o[1] = o[2] = 0.0f;
+    return;
   }
   if (a.accumulate) {
     o[0] += col[0];

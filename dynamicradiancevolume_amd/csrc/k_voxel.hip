@@ -284,29 +284,48 @@ struct CornerArgs {
   uint64_t* corners;
   uint32_t mip_offset[16];
   uint32_t corner_offset[17];
+  uint32_t quad_offset[17];  // threads (four corners of a row each) before level l
   int res;
   int levels;
 };
 
-__global__ void k_voxel_corners(CornerArgs a) {
-  const uint32_t total = a.corner_offset[a.levels];
+// One thread per four consecutive corners of a row: their 2x2x2 footprints
+// share 5 texels in each of 4 rows, which are loaded once and all before the
+// first use (the kernel is bound by load latency, not bytes).
+__global__ __launch_bounds__(256) void k_voxel_corners(CornerArgs a) {
+  const uint32_t total = a.quad_offset[a.levels];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
     int l = 0;
-    while (l + 1 < a.levels && i >= a.corner_offset[l + 1]) ++l;
-    const int w = a.res >> l, wp = w + 1;
-    const uint32_t j = i - a.corner_offset[l];
-    const int cx = static_cast<int>(j % wp) - 1, cy = static_cast<int>((j / wp) % wp) - 1,
-              cz = static_cast<int>(j / (wp * wp)) - 1;
+    while (l + 1 < a.levels && i >= a.quad_offset[l + 1]) ++l;
+    const int w = a.res >> l;
+    const uint32_t wp = static_cast<uint32_t>(w) + 1u, qpr = (wp + 3u) / 4u;
+    const uint32_t j = i - a.quad_offset[l];
+    const uint32_t row = j / qpr, qx = j - row * qpr;  // row = (cz + 1) * wp + (cy + 1)
+    const uint32_t rz = row / wp, ry = row - rz * wp;
+    const int cx0 = static_cast<int>(4u * qx) - 1, cy = static_cast<int>(ry) - 1, cz = static_cast<int>(rz) - 1;
     const uint8_t* lvl = a.mips + a.mip_offset[l];
-    uint64_t packed = 0;
+    uint32_t t[4][5];  // [y bit | z bit << 1][x - cx0]
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int x = min(w - 1, max(0, cx + (k & 1)));
-      const int y = min(w - 1, max(0, cy + ((k >> 1) & 1)));
-      const int z = min(w - 1, max(0, cz + (k >> 2)));
-      packed |= static_cast<uint64_t>(lvl[(static_cast<size_t>(z) * w + y) * w + x]) << (8 * k);
+    for (int r = 0; r < 4; ++r) {
+      const int y = min(w - 1, max(0, cy + (r & 1)));
+      const int z = min(w - 1, max(0, cz + (r >> 1)));
+      const uint8_t* src = lvl + (static_cast<size_t>(z) * w + y) * w;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) t[r][c] = src[min(w - 1, max(0, cx0 + c))];
     }
-    a.corners[i] = packed;
+    uint64_t* dst = a.corners + a.corner_offset[l] + static_cast<size_t>(row) * wp;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cx = cx0 + k;
+      if (cx > w - 1) break;
+      // byte b: x offset b & 1, y offset (b >> 1) & 1, z offset b >> 2
+      uint32_t lo = 0, hi = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) lo |= t[b >> 1][k + (b & 1)] << (8 * b);
+#pragma unroll
+      for (int b = 4; b < 8; ++b) hi |= t[b >> 1][k + (b & 1)] << (8 * (b - 4));
+      dst[cx + 1] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+    }
   }
 }
 
@@ -364,7 +383,14 @@ void launch_voxel_corners(const VoxelLayout& L, const uint8_t* mips, uint64_t* c
     a.corner_offset[l] = static_cast<uint32_t>(L.corner_offset[l]);
   }
   a.corner_offset[L.levels] = static_cast<uint32_t>(L.corner_count);
-  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((L.corner_count + 255) / 256, 8192));
+  uint32_t quads = 0;
+  for (int l = 0; l < L.levels; ++l) {
+    a.quad_offset[l] = quads;
+    const uint32_t wp = static_cast<uint32_t>(L.res >> l) + 1u;
+    quads += wp * wp * ((wp + 3u) / 4u);
+  }
+  a.quad_offset[L.levels] = quads;
+  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((quads + 255) / 256, 8192));
   hipLaunchKernelGGL(k_voxel_corners, dim3(blocks), dim3(256), 0, s, a);
 }
 
