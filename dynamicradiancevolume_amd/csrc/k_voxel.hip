@@ -135,15 +135,16 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxelizeArgs a) {
   const int py0 = max(0, to_int(floorf(bb1 - 0.5f)));
   const int px1 = min(res - 1, to_int(ceilf(bb2 - 0.5f)));
   const int py1 = min(res - 1, to_int(ceilf(bb3 - 0.5f)));
+  // res is a power of two, so x / res == x * h exactly.
   for (int py = py0; py <= py1; ++py) {
     const float fy = static_cast<float>(py) + 0.5f;
     if (fy < bb1 || fy > bb3) continue;
-    const float ny = fy / fres * 2.0f - 1.0f;
+    const float ny = fy * h * 2.0f - 1.0f;
     for (int px = px0; px <= px1; ++px) {
       const float fx = static_cast<float>(px) + 0.5f;
       // voxelize.frag:26-30
       if (fx < bb0 || fx > bb2) continue;
-      const float nx = fx / fres * 2.0f - 1.0f;
+      const float nx = fx * h * 2.0f - 1.0f;
       const bool in = (pl[0][0] * nx + pl[0][1] * ny + pl[0][2] <= 0.0f) &&
                       (pl[1][0] * nx + pl[1][1] * ny + pl[1][2] <= 0.0f) &&
                       (pl[2][0] * nx + pl[2][1] * ny + pl[2][2] <= 0.0f);

@@ -143,6 +143,36 @@ def test_allocation_overflow_counts_and_clamps():
     assert rel_l2(df.out.cpu().numpy(), ref) < RGB_TOL
 
 
+def test_allocation_reuses_address_volume_across_frames():
+    """A renderer that allocates frame A, then frame B (a G-buffer with its
+    upper half cleared and its lower half shifted), then A again holds the
+    same claimed cells, positions, phantoms and counts as fresh renderers on
+    B and on A: no claim of an earlier frame survives into the next."""
+    from dataclasses import replace
+
+    orc = _orc()
+    fi = frame_inputs("cornell_256")
+    cfg = fi.workload.config()
+    h, w = cfg.height, cfg.width
+    depth_b = fi.depth.reshape(h, w).copy()
+    depth_b[: h // 2] = 0.0
+    depth_b[h // 2:] = np.roll(depth_b[h // 2:], 7, axis=1)
+    fi_b = replace(fi, depth=depth_b.reshape(fi.depth.shape))
+    announce("allocate A, B, A on one renderer")
+    r = _renderer(fi)
+    r.set_frame(fi.frame())
+    r.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    counts = []
+    for f in (fi, fi_b, fi):
+        df = _device(f)
+        r.allocate_caches(df.gbuffer())
+        torch.cuda.synchronize()
+        _assert_alloc_equal(r, f, cfg, orc)
+        counts.append(r.cache_count()[0])
+    assert counts[1] < counts[0] == counts[2]  # B claims a different, smaller set
+    r.close()
+
+
 def test_allocation_empty_gbuffer():
     fi = frame_inputs("cornell_256")
     df = _device(fi)
