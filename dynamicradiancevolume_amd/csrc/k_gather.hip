@@ -200,7 +200,10 @@ struct VisArgs {
 // too few items to occupy the grid (a small spot-check set).
 __device__ __forceinline__ uint32_t items_blocks(uint32_t tiles, uint32_t n_blocks, uint32_t want,
                                                  uint32_t n_waves) {
-  return tiles * ((n_blocks + want - 1) / want) >= 2u * n_waves ? want : 1u;
+  // The largest count up to `want` that still leaves two items per wave.
+  uint32_t bpi = want;
+  while (bpi > 1u && tiles * ((n_blocks + bpi - 1) / bpi) < 2u * n_waves) --bpi;
+  return bpi;
 }
 
 __device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint32_t my_traces, uint32_t my_steps,
@@ -221,7 +224,12 @@ __device__ __forceinline__ void count_steps(uint64_t* steps, uint32_t lane, uint
 
 // A wave owns 64 caches x blocks_per_item shadow blocks; each lane traces
 // its cache's blocks in order (rays from neighbouring caches towards nearby
-// VPL blocks stay coherent).
+// VPL blocks stay coherent).  Items are numbered block-group-major (tile
+// fastest) and dealt to the persistent waves round robin, so a wave's items
+// reach different VPL blocks: rays to some blocks are much longer than to
+// others, and a wave whose items all shared one block group (as with the
+// tile-major numbering whenever the group count divides the wave count) left
+// a tail of slow waves.
 template <int kBatch>
 __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   __shared__ LevelInfo lvi[16];
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   const float vsz = a.voxel_size * fres;
   uint32_t my_traces = 0, my_steps = 0, my_exec = 0;
   for (uint32_t item = wave; item < items; item += n_waves) {
-    const uint32_t tile = item / groups, group = item % groups;
+    const uint32_t group = item / tiles, tile = item - group * tiles;
     const uint32_t cache = tile * 64u + lane;
     if (cache >= live) continue;
     const float* e = a.caches + static_cast<size_t>(cache) * a.stride;
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
   const float vsz = a.voxel_size * fres;
   uint32_t my_traces = 0, my_steps = 0, my_exec = 0;
   for (uint32_t item = wave; item < items; item += n_waves) {
-    const uint32_t tile = item / groups, group = item % groups;
+    const uint32_t group = item / tiles, tile = item - group * tiles;
     const uint32_t n_valid = min(64u, live - tile * 64u);
     const uint32_t b0 = group * bpi;
     const uint32_t pool = n_valid * (min(a.n_blocks, b0 + bpi) - b0);
