@@ -120,7 +120,7 @@ struct Tuning {
   int gather_max_splits = 128; // VPL splits of the gather
   int waves_per_cu = 64;       // persistent grid size
   int refill_min_lanes = 16;   // lane pool: idle lanes that trigger a refill
-  int steps_per_batch = 2;     // cone-trace steps planned (and loaded) together: 1..4 or 8
+  int steps_per_batch = 1;     // cone-trace steps planned (and loaded) together: 1..4 or 8
 };
 
 // Per-cache specular env-maps (INDIRECT_SPECULAR): the R11G11B10F atlas of

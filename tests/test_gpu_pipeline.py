@@ -350,6 +350,7 @@ def test_full_frame_matches_oracle(name):
     dict(visibility_kernel=1, blocks_per_item=32, gather_max_splits=128, waves_per_cu=64),
     dict(visibility_kernel=0, blocks_per_item=3, gather_max_splits=1, waves_per_cu=4),
     dict(visibility_kernel=0, steps_per_batch=4),
+    dict(visibility_kernel=0, steps_per_batch=2),
     dict(visibility_kernel=0, steps_per_batch=1, blocks_per_item=7),
     dict(visibility_kernel=1, steps_per_batch=3, refill_min_lanes=1),
     dict(visibility_kernel=0, steps_per_batch=8, blocks_per_item=5),
@@ -378,7 +379,7 @@ def test_tuning_variants_agree(tuning):
         r.close()
     (e0, v0), (e1, v1) = res
     np.testing.assert_array_equal(e0[:, :3], e1[:, :3])
-    if tuning.get("steps_per_batch", 2) == 2:
+    if tuning.get("steps_per_batch", 1) == 1:
         np.testing.assert_array_equal(v0, v1)
     else:
         # A ray stops at the first batch boundary after its transmittance
