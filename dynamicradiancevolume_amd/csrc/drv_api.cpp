@@ -718,7 +718,10 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
   int rc = prepare_light(ctx, rsm, light, &n_vpl, &interval, &n_blocks);
   if (rc != DRV_OK) return rc;
-  join_voxels(ctx);
+  // The VPL preparation reads no voxels, so it runs while a voxelization on
+  // the side stream finishes; the cone trace joins it.  With stage timing on
+  // the join comes first, so no stage's time includes the wait.
+  if (ctx->profiling) join_voxels(ctx);
   const bool shadow = ctx->cfg.indirect_shadow != 0;
   const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
   if (shadow && n_blocks > ctx->vis_blocks_cap) {
@@ -756,6 +759,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, specular,
                             specular ? ctx->spec.vpls_lanes : nullptr, &plan, ctx->stream);
   record(ctx, 6);
+  join_voxels(ctx);
   const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
   if (use_net) {
     if ((rc = surrogate_visibility(ctx, n_blocks, n_vpl, interval, grid)) != DRV_OK) return rc;

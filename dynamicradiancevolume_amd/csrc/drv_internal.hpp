@@ -156,11 +156,12 @@ void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_fra
 
 void launch_voxelize(const drv_volume& vol, int res, const float* positions, const uint32_t* indices, uint32_t n_tri,
                      const float* world, uint8_t* target, hipStream_t s);
-// Blend and the whole mip chain: one launch for the 16^3 bricks and their
-// levels, one single-workgroup launch for the levels above.
+// Blend and the levels inside each 16^3 brick (one launch); then the levels
+// above the brick size and the corner-expanded copy of every level (one
+// launch, its first workgroup building the upper levels).
 void launch_voxel_blend_mips(const VoxelLayout& layout, const uint8_t* target, uint8_t* mips, float adaption,
                              hipStream_t s);
-void launch_voxel_corners(const VoxelLayout& layout, const uint8_t* mips, uint64_t* corners, hipStream_t s);
+void launch_voxel_corners(const VoxelLayout& L, uint8_t* mips, uint64_t* corners, hipStream_t s);
 
 // Split plan of the gather for a cache counter: enough (tile, split) items
 // for the persistent grid (4 waves per workgroup), each split a whole number

@@ -165,8 +165,8 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxelizeArgs a) {
 // unorm8) and voxelmipmap.comp:8-12 (the linear sample at the centre of each
 // 2x2x2 child block is the 8-texel mean, computed exactly and rounded half
 // up), fused: one workgroup per 16^3 brick of level 0 blends its texels and
-// builds the brick's levels 1..4 in LDS; k_voxel_upper_mips then builds the
-// levels above from the bricks' top texels.
+// builds the brick's levels 1..4 in LDS; the first workgroup of
+// k_voxel_corners then builds the levels above from the bricks' top texels.
 constexpr int kBrick = 16;
 
 struct MipArgs {
@@ -252,81 +252,91 @@ __global__ __launch_bounds__(256) void k_voxel_blend_mips(MipArgs a) {
   }
 }
 
-// The levels above the brick size (more than one brick per axis), built by
-// one workgroup from level log2(B) after the brick kernel (the kernel
-// boundary makes every brick's writes visible; an in-kernel device-scope
-// fence per brick costs an L2 write-back on this multi-XCD part).
-// Dynamic LDS: (res / B)^3 + (res / 2B)^3 + ... bytes.
-__global__ __launch_bounds__(256) void k_voxel_upper_mips(MipArgs a) {
-  extern __shared__ uint8_t up[];
-  const int nb = a.res / a.brick;
-  const int tid = static_cast<int>(threadIdx.x);
-  const uint8_t* lb = a.mips + a.mip_offset[a.brick_levels];
-  for (int t = tid; t < nb * nb * nb; t += 256) up[t] = lb[t];
-  __syncthreads();
-  int usrc = 0, udst = nb * nb * nb;
-  for (int k = a.brick_levels + 1; k < a.levels; ++k) {
-    const int wp = a.res >> (k - 1), wk = a.res >> k;
-    uint8_t* out = a.mips + a.mip_offset[k];
-    for (int t = tid; t < wk * wk * wk; t += 256) {
-      const int x = t % wk, y = (t / wk) % wk, z = t / (wk * wk);
-      const uint8_t q = mean8(up + usrc, wp, x, y, z);
-      up[udst + t] = q;
-      out[t] = q;
-    }
-    __syncthreads();
-    usrc = udst;
-    udst += wk * wk * wk;
-  }
-}
-
 struct CornerArgs {
-  const uint8_t* mips;
+  uint8_t* mips;
   uint64_t* corners;
   uint32_t mip_offset[16];
   uint32_t corner_offset[17];
   uint32_t quad_offset[17];  // threads (four corners of a row each) before level l
   int res;
   int levels;
+  int brick, brick_levels;   // levels 0..brick_levels come from the brick kernel
 };
 
-// One thread per four consecutive corners of a row: their 2x2x2 footprints
-// share 5 texels in each of 4 rows, which are loaded once and all before the
-// first use (the kernel is bound by load latency, not bytes).
+// Four consecutive corners of one row of level l (quad q of the level): their
+// 2x2x2 footprints share 5 texels in each of 4 rows, which are loaded once
+// and all before the first use (the work is bound by load latency).
+__device__ __forceinline__ void corner_quad(const CornerArgs& a, const uint8_t* lvl, int l, uint32_t q) {
+  const int w = a.res >> l;
+  const uint32_t wp = static_cast<uint32_t>(w) + 1u, qpr = (wp + 3u) / 4u;
+  const uint32_t row = q / qpr, qx = q - row * qpr;  // row = (cz + 1) * wp + (cy + 1)
+  const uint32_t rz = row / wp, ry = row - rz * wp;
+  const int cx0 = static_cast<int>(4u * qx) - 1, cy = static_cast<int>(ry) - 1, cz = static_cast<int>(rz) - 1;
+  uint32_t t[4][5];  // [y bit | z bit << 1][x - cx0]
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int y = min(w - 1, max(0, cy + (r & 1)));
+    const int z = min(w - 1, max(0, cz + (r >> 1)));
+    const uint8_t* src = lvl + (static_cast<size_t>(z) * w + y) * w;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) t[r][c] = src[min(w - 1, max(0, cx0 + c))];
+  }
+  uint64_t* dst = a.corners + a.corner_offset[l] + static_cast<size_t>(row) * wp;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int cx = cx0 + k;
+    if (cx > w - 1) break;
+    // byte b: x offset b & 1, y offset (b >> 1) & 1, z offset b >> 2
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) lo |= t[b >> 1][k + (b & 1)] << (8 * b);
+#pragma unroll
+    for (int b = 4; b < 8; ++b) hi |= t[b >> 1][k + (b & 1)] << (8 * (b - 4));
+    dst[cx + 1] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+  }
+}
+
+// The corner-expanded copy of every level, and the levels above the brick
+// size (more than one brick per axis).  Workgroup 0 builds those upper
+// levels from level brick_levels in LDS (voxelmipmap.comp:8-12), stores
+// them and expands their corners from LDS; the other workgroups expand the
+// levels the brick kernel wrote.  (The kernel boundary after the brick
+// kernel makes every brick's writes visible; an in-kernel device-scope fence
+// per brick costs an L2 write-back on this multi-XCD part.)
+// Dynamic LDS: (res / B)^3 + (res / 2B)^3 + ... bytes when upper levels exist.
 __global__ __launch_bounds__(256) void k_voxel_corners(CornerArgs a) {
-  const uint32_t total = a.quad_offset[a.levels];
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+  extern __shared__ uint8_t up[];
+  const int top = a.brick_levels + 1;  // first upper level
+  if (blockIdx.x == 0) {
+    if (top >= a.levels) return;
+    const int nb = a.res / a.brick;
+    const int tid = static_cast<int>(threadIdx.x);
+    const uint8_t* lb = a.mips + a.mip_offset[a.brick_levels];
+    for (int t = tid; t < nb * nb * nb; t += 256) up[t] = lb[t];
+    __syncthreads();
+    int usrc = 0, udst = nb * nb * nb;
+    for (int k = top; k < a.levels; ++k) {
+      const int wp = a.res >> (k - 1), wk = a.res >> k;
+      uint8_t* out = a.mips + a.mip_offset[k];
+      for (int t = tid; t < wk * wk * wk; t += 256) {
+        const int x = t % wk, y = (t / wk) % wk, z = t / (wk * wk);
+        const uint8_t q = mean8(up + usrc, wp, x, y, z);
+        up[udst + t] = q;
+        out[t] = q;
+      }
+      __syncthreads();
+      const uint32_t n = a.quad_offset[k + 1] - a.quad_offset[k];
+      for (uint32_t q = tid; q < n; q += 256u) corner_quad(a, up + udst, k, q);
+      usrc = udst;
+      udst += wk * wk * wk;
+    }
+    return;
+  }
+  const uint32_t total = a.quad_offset[min(top, a.levels)];
+  for (uint32_t i = (blockIdx.x - 1u) * blockDim.x + threadIdx.x; i < total; i += (gridDim.x - 1u) * blockDim.x) {
     int l = 0;
     while (l + 1 < a.levels && i >= a.quad_offset[l + 1]) ++l;
-    const int w = a.res >> l;
-    const uint32_t wp = static_cast<uint32_t>(w) + 1u, qpr = (wp + 3u) / 4u;
-    const uint32_t j = i - a.quad_offset[l];
-    const uint32_t row = j / qpr, qx = j - row * qpr;  // row = (cz + 1) * wp + (cy + 1)
-    const uint32_t rz = row / wp, ry = row - rz * wp;
-    const int cx0 = static_cast<int>(4u * qx) - 1, cy = static_cast<int>(ry) - 1, cz = static_cast<int>(rz) - 1;
-    const uint8_t* lvl = a.mips + a.mip_offset[l];
-    uint32_t t[4][5];  // [y bit | z bit << 1][x - cx0]
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int y = min(w - 1, max(0, cy + (r & 1)));
-      const int z = min(w - 1, max(0, cz + (r >> 1)));
-      const uint8_t* src = lvl + (static_cast<size_t>(z) * w + y) * w;
-#pragma unroll
-      for (int c = 0; c < 5; ++c) t[r][c] = src[min(w - 1, max(0, cx0 + c))];
-    }
-    uint64_t* dst = a.corners + a.corner_offset[l] + static_cast<size_t>(row) * wp;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int cx = cx0 + k;
-      if (cx > w - 1) break;
-      // byte b: x offset b & 1, y offset (b >> 1) & 1, z offset b >> 2
-      uint32_t lo = 0, hi = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) lo |= t[b >> 1][k + (b & 1)] << (8 * b);
-#pragma unroll
-      for (int b = 4; b < 8; ++b) hi |= t[b >> 1][k + (b & 1)] << (8 * (b - 4));
-      dst[cx + 1] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
-    }
+    corner_quad(a, a.mips + a.mip_offset[l], l, i - a.quad_offset[l]);
   }
 }
 
@@ -366,19 +376,18 @@ void launch_voxel_blend_mips(const VoxelLayout& L, const uint8_t* target, uint8_
   size_t lds = 0;
   for (int k = 0; k <= bl; ++k) lds += static_cast<size_t>(a.brick >> k) * (a.brick >> k) * (a.brick >> k);
   hipLaunchKernelGGL(k_voxel_blend_mips, dim3(static_cast<unsigned>(nb * nb * nb)), dim3(256), lds, s, a);
-  if (bl + 1 < L.levels) {
-    size_t upper = 0;
-    for (int w = nb; w >= 1; w /= 2) upper += static_cast<size_t>(w) * w * w;
-    hipLaunchKernelGGL(k_voxel_upper_mips, dim3(1), dim3(256), upper, s, a);
-  }
 }
 
-void launch_voxel_corners(const VoxelLayout& L, const uint8_t* mips, uint64_t* corners, hipStream_t s) {
+void launch_voxel_corners(const VoxelLayout& L, uint8_t* mips, uint64_t* corners, hipStream_t s) {
   CornerArgs a;
   a.mips = mips;
   a.corners = corners;
   a.res = L.res;
   a.levels = L.levels;
+  a.brick = std::min(kBrick, L.res);
+  int bl = 0;
+  while ((1 << (bl + 1)) <= a.brick) ++bl;
+  a.brick_levels = bl;
   for (int l = 0; l < L.levels; ++l) {
     a.mip_offset[l] = static_cast<uint32_t>(L.mip_offset[l]);
     a.corner_offset[l] = static_cast<uint32_t>(L.corner_offset[l]);
@@ -391,8 +400,13 @@ void launch_voxel_corners(const VoxelLayout& L, const uint8_t* mips, uint64_t* c
     quads += wp * wp * ((wp + 3u) / 4u);
   }
   a.quad_offset[L.levels] = quads;
-  const unsigned blocks = static_cast<unsigned>(std::min<size_t>((quads + 255) / 256, 8192));
-  hipLaunchKernelGGL(k_voxel_corners, dim3(blocks), dim3(256), 0, s, a);
+  const uint32_t brick_quads = a.quad_offset[std::min(bl + 1, L.levels)];
+  size_t upper = 0;
+  if (bl + 1 < L.levels)
+    for (int w = L.res / a.brick; w >= 1; w /= 2) upper += static_cast<size_t>(w) * w * w;
+  // Workgroup 0 for the upper levels, the rest for the brick levels.
+  const unsigned blocks = 1u + static_cast<unsigned>(std::min<size_t>((brick_quads + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_voxel_corners, dim3(blocks), dim3(256), upper, s, a);
 }
 
 }  // namespace drvk
