@@ -1358,9 +1358,10 @@ const char* drv_batch_last_error(const drv_batch* b) { return b ? b->err.c_str()
 
 int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t waves_per_cu) {
   if (!b) return DRV_ERR_INVALID_ARG;
-  if ((cpw != 64 && cpw != 32 && cpw != 16) || !is_pow2(vpl_tile) || vpl_tile < 16 || waves_per_cu < 4 ||
+  if ((cpw != 64 && cpw != 32 && cpw != 16 && cpw != 0) || !is_pow2(vpl_tile) || vpl_tile < 16 || waves_per_cu < 4 ||
       waves_per_cu > 64)
-    return bfail(b, DRV_ERR_INVALID_ARG, "batch_set_tuning: caches per wave 64/32/16, VPL tile a power of two >= 16");
+    return bfail(b, DRV_ERR_INVALID_ARG,
+                 "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes), VPL tile a power of two >= 16");
   b->cpw = static_cast<uint32_t>(cpw);
   b->vpl_tile = static_cast<uint32_t>(vpl_tile);
   b->waves_per_cu = waves_per_cu;
@@ -1378,6 +1379,16 @@ int drv_batch_get_tuning(const drv_batch* b, int32_t* cpw, int32_t* vpl_tile, in
 int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights) {
   if (!b || !rsms || !lights) return DRV_ERR_INVALID_ARG;
   const size_t n = b->views.size();
+  if (b->cpw == 0) {
+    // Per-view passes: each view's own cone trace, SH gather and reduction,
+    // back to back on the shared stream (views large enough to fill the GPU
+    // on their own gain nothing from the fused queue).
+    for (size_t v = 0; v < n; ++v) {
+      const int rc = drv_light_caches(b->views[v], &rsms[v], &lights[v]);
+      if (rc != DRV_OK) return bfail(b, rc, "batch view " + std::to_string(v) + ": " + b->views[v]->err);
+    }
+    return DRV_OK;
+  }
   const bool capturing = is_capturing(b->stream);
   size_t partial_need = 1;
   std::vector<drvk::BatchView> hv(n);
@@ -1473,9 +1484,10 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   const uint32_t keep_cpw = b->cpw, keep_tile = b->vpl_tile;
   float best = 1e30f;
   uint32_t best_cpw = keep_cpw, best_tile = keep_tile;
-  const uint32_t cpws[3] = {64, 32, 16};
+  const uint32_t cpws[4] = {0, 64, 32, 16};
   for (uint32_t cpw : cpws) {
-    for (uint32_t tile = 256; tile <= max_vpl; tile *= 2) {
+    // Per-view passes (cpw 0) have no VPL tile: one candidate.
+    for (uint32_t tile = cpw == 0 ? max_vpl : 256; tile <= max_vpl; tile *= 2) {
       b->cpw = cpw;
       b->vpl_tile = tile;
       std::vector<float> t;

@@ -423,7 +423,11 @@ const char* drv_batch_last_error(const drv_batch* batch);
 /* Work decomposition: caches per wavefront (64, 32 or 16; lane groups of a
  * wave split the shadow blocks of a cache when < 64), VPL tile length (VPLs
  * per work item, a power of two >= the shadow block size; a view with more
- * VPLs is split into partial sums), persistent grid size in waves per CU. */
+ * VPLs is split into partial sums), persistent grid size in waves per CU.
+ * Caches per wavefront 0 selects per-view passes instead of the fused queue:
+ * each view's own cone trace, SH gather and reduction (drv_light_caches)
+ * back to back on the shared stream, the better choice for views that fill
+ * the GPU on their own. */
 int drv_batch_set_tuning(drv_batch* batch, int32_t caches_per_wave, int32_t vpl_tile, int32_t waves_per_cu);
 int drv_batch_get_tuning(const drv_batch* batch, int32_t* caches_per_wave, int32_t* vpl_tile, int32_t* waves_per_cu);
 /* rsms[i], lights[i] belong to view i. */

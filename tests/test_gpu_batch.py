@@ -2,8 +2,9 @@
 
 * drv_batch_light_caches over uneven views equals each view's own
   drv_light_caches (the single-view path, itself pinned against the oracle)
-  up to the order of the VPL sums, for every work decomposition; one view is
-  also checked against the FP32 oracle directly;
+  up to the order of the VPL sums, for every work decomposition, and bit for
+  bit with per-view passes (caches per wave 0); one view is also checked
+  against the FP32 oracle directly;
 * the autotuner returns a valid decomposition and leaves the entries as it
   found them;
 * a captured frame (allocate, batched light, apply for every view) replays
@@ -97,6 +98,25 @@ def test_batch_matches_single_views(sh_bands):
     b.close()
 
 
+def test_batch_per_view_passes_match_single_views():
+    """Per-view passes (caches per wave 0) run the single-view kernels: the
+    SH of every view are bit-identical to lighting the views one by one."""
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce(f"batch of {len(frames)} views as per-view passes")
+    ref = _single_view_sh(frames, 2)
+    rs, dfs = _group(frames, 2)
+    b = Batch(rs)
+    b.set_tuning(0, 256)
+    b.render_frame(dfs)
+    for (p0, s0), r in zip(ref, rs):
+        p1, s1 = _by_position(r)
+        np.testing.assert_array_equal(p0, p1)
+        np.testing.assert_array_equal(s1, s0)
+    b.close()
+
+
 def test_batch_view_matches_oracle():
     from dynamicradiancevolume_amd.pipeline import Batch
     from oracle import orc
@@ -119,7 +139,7 @@ def test_batch_view_matches_oracle():
     assert rel_l2(s1, entries_sh(e[order], 1)) <= 1e-4
 
 
-@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536)])
+@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536), (0, 256)])
 def test_batch_decompositions_agree(cpw, tile):
     from dynamicradiancevolume_amd.pipeline import Batch
 
@@ -149,7 +169,7 @@ def test_batch_autotune_restores_entries():
     b.render_frame(dfs)
     before = [r.read_caches(r.cache_count()[1]) for r in rs]
     cpw, tile, ms = b.autotune(dfs, reps=1)
-    assert cpw in (64, 32, 16) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
+    assert cpw in (64, 32, 16, 0) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
     assert b.tuning()[:2] == (cpw, tile)
     for r, e in zip(rs, before):
         np.testing.assert_array_equal(r.read_caches(len(e)), e)
