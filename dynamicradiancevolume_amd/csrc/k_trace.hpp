@@ -12,7 +12,9 @@ namespace drvk {
 // Below this transmittance a ray counts as fully occluded: the remaining
 // samples can lower the returned visibility by at most this much.
 constexpr float kOpaque = 1.0f / 1048576.0f;
-constexpr int kStepBatch = 2;  // steps planned together (a lane starts its next ray at the next batch)
+// Steps planned together by the fused batch kernel; the single-view cone
+// trace takes its count from the launch tuning (steps_per_batch).
+constexpr int kStepBatch = 2;
 
 // Per-lane state of one cone trace (cacheLightingRSM.comp:195-210).
 struct Trace {

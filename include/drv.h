@@ -394,7 +394,7 @@ int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t 
  * the reference (its dispatch is fixed at 64 caches per workgroup,
  * shader/cacheLightingRSM.comp:7). */
 typedef struct drv_tuning {
-  int32_t visibility_kernel; /* 0: lane per cache, one shadow block at a time (default); 1: refilling lane pool */
+  int32_t visibility_kernel; /* 0: lane per cache, walking its item's blocks (default); 1: refilling lane pool */
   int32_t blocks_per_item;   /* shadow blocks per visibility work item, 1..64 (default 1); the kernel
                                 uses the largest count up to this that leaves two items per wave */
   int32_t gather_max_splits; /* largest VPL split count of the gather, power of two 1..128 (default 128) */
