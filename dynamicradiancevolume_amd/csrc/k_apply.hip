@@ -72,14 +72,15 @@ __device__ __forceinline__ Tap make_tap(const ApplyArgs& a, int l, const float* 
   return t;
 }
 
-// R11G11B10F texel -> RGB: exponent rebias for normal codes, f * 2^(-14-m)
-// for denormal ones, inf / NaN for exponent 31.
+// R11G11B10F texel -> RGB.  The code shifted into FP32 position reads as
+// 2^(e-127) (1 + f) for exponents 1..30 and as the FP32 denormal f 2^-126
+// for exponent 0; one multiply by 2^112 rebiases both exactly to GL's
+// 2^(e-15) (1 + f) and f 2^-14 (FP32 denormals are preserved in these
+// kernels).  Exponent 31 is inf (f = 0) or NaN.
 __device__ __forceinline__ float decode_channel(uint32_t c, int m) {
-  const uint32_t e = c >> m;
-  float v = __builtin_bit_cast(float, (c << (23 - m)) + (112u << 23));
-  if (e == 0u) v = static_cast<float>(c) * __builtin_ldexpf(1.0f, -14 - m);
-  if (e == 31u) v = (c & ((1u << m) - 1u)) ? __builtin_nanf("") : __builtin_inff();
-  return v;
+  const uint32_t bits = c << (23 - m);
+  const float v = __builtin_bit_cast(float, bits) * 0x1p112f;
+  return (c >> m) == 31u ? __builtin_bit_cast(float, 0x7F800000u | bits) : v;
 }
 
 __device__ __forceinline__ void decode_texel(uint32_t x, float* rgb) {
