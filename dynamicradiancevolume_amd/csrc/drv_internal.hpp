@@ -26,9 +26,7 @@ struct Counter {
   uint32_t phantoms;                      // appends of corners outside the address volume
 };
 
-constexpr int kGatherTile = 256;   // caches per gather work item (one workgroup)
-constexpr int kVplChunk = 64;      // VPLs staged in LDS at a time
-constexpr int kVisTile = 256;      // caches per visibility work item
+constexpr int kGatherTile = 64;    // caches per gather / cone-trace tile (one per lane)
 
 // One VPL ("LightInfo", cacheLightingRSM.comp:34-40) packed for LDS staging.
 struct Vpl {

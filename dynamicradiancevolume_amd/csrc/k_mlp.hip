@@ -727,7 +727,7 @@ __global__ void k_spot_select(const Counter* counter, uint32_t k, const float* c
       spot_ctr[c].groups_y = spot_ctr[c].groups_z = 1u;
       spot_ctr[c].total = n_spot;
       spot_ctr[c].live = n_spot;
-      spot_ctr[c].tiles = (n_spot + 63u) / 64u;
+      spot_ctr[c].tiles = (n_spot + kGatherTile - 1) / kGatherTile;
       spot_ctr[c].splits = 1u;
       spot_ctr[c].phantoms = 0u;
     }
