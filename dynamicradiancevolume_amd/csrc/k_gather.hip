@@ -398,9 +398,10 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
     sh_zero(acc);
     if (split < splits) {
       const uint32_t m0 = split * chunk, m1 = m0 + chunk;
-      for (uint32_t mb = m0; mb < m1; mb += interval) {
+      uint32_t blk = m0 / interval;  // the split starts on a block boundary
+      for (uint32_t mb = m0; mb < m1; mb += interval, ++blk) {
         float shadowing = 1.0f;
-        if (kShadow) shadowing = vis[static_cast<size_t>(mb / interval) * vis_pitch + c];
+        if (kShadow) shadowing = vis[static_cast<size_t>(blk) * vis_pitch + c];
 #pragma unroll 4
         for (uint32_t m = mb; m < mb + interval; ++m) sh_accumulate<kSh2>(acc, vpls[m], wx, wy, wz, shadowing);
       }

@@ -56,7 +56,8 @@ struct SpecGatherArgs {
   uint32_t n_vpl;
   const float* vis;  // [block][vis_pitch] or null (unshadowed)
   uint32_t vis_pitch;
-  uint32_t interval;
+  uint32_t interval;        // VPLs per shadow block, a power of two
+  uint32_t interval_log2;
   int S;
   float* work;   // [cache][S * S][3]
   float* spill;  // [cache][2S + 1][3]: column x = S (y = 0..S-1), row y = S (x = 0..S-1), corner
@@ -95,8 +96,8 @@ __global__ __launch_bounds__(256) void k_specular_gather(SpecGatherArgs a) {
       const uint32_t m = m0 + i;
       const Vpl v = a.vpls[static_cast<size_t>(i) * 64u + lane];
       // The block's shadowing changes every `interval` VPLs.
-      if (a.vis && (i == 0u || m % a.interval == 0u))
-        shadowing = a.vis[static_cast<size_t>(m / a.interval) * a.vis_pitch + c];
+      if (a.vis && (i == 0u || (m & (a.interval - 1u)) == 0u))
+        shadowing = a.vis[static_cast<size_t>(m >> a.interval_log2) * a.vis_pitch + c];
       // radianceIn, cacheLightingRSM.comp:246-264 (the SH gather's expressions)
       const float tx0 = v.px - wp.x, ty0 = v.py - wp.y, tz0 = v.pz - wp.z;
       const float d2 = __builtin_fmaf(tz0, tz0, __builtin_fmaf(ty0, ty0, tx0 * tx0));
@@ -299,6 +300,7 @@ void launch_specular(const float* cam, const float* caches, int stride, const Co
   g.vis = vis;
   g.vis_pitch = vis_pitch;
   g.interval = interval;
+  g.interval_log2 = static_cast<uint32_t>(__builtin_ctz(interval));
   g.S = at.S;
   g.work = at.work;
   g.spill = at.spill;
