@@ -121,6 +121,22 @@ struct SpecPixel {
   bool two;
 };
 
+constexpr float kInvPi = 1.0f / kPi;
+
+// UnpackNormal16I (utils.glsl:43-60) for shading: the angle in revolutions
+// (a / 65536) through the hardware sine and cosine, normalized with one
+// reciprocal square root.  Within ~1e-6 of the exactly rounded unpack, far
+// below the apply's tolerance; the exact unpack stays where normals are
+// repacked bit-exactly (the RSM mip chain).
+__device__ __forceinline__ f3 shading_normal(int16_t a, int16_t b) {
+  const float rev = static_cast<float>(a) * (1.0f / 65536.0f);
+  const float py = static_cast<float>(b) * (1.0f / 32768.0f);
+  const float sin_phi = __builtin_sqrtf(fmaxf(0.0f, 1.0f - py * py));
+  const float x = __builtin_amdgcn_cosf(rev) * sin_phi, y = __builtin_amdgcn_sinf(rev) * sin_phi, z = py;
+  const float inv = __builtin_amdgcn_rsqf(__builtin_fmaf(z, z, __builtin_fmaf(y, y, x * x)));
+  return make3(x * inv, y * inv, z * inv);
+}
+
 // SampleCacheIrradiance, lightcache.glsl:137-183, on an entry already in
 // registers (q[k] = float4 k of the std430 LightCacheEntry).
 __device__ __forceinline__ void cache_irradiance(const ApplyArgs& a, const float4* q, f3 n, float* irr) {
@@ -268,9 +284,9 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp,
     acc[2] = acc[2] + irr[2] * w[i];
   }
   __builtin_amdgcn_wave_barrier();  // the stage is rewritten by the next call
-  out[0] = acc[0] * diffuse[0] / kPi;
-  out[1] = acc[1] * diffuse[1] / kPi;
-  out[2] = acc[2] * diffuse[2] / kPi;
+  out[0] = acc[0] * diffuse[0] * kInvPi;
+  out[1] = acc[1] * diffuse[1] * kInvPi;
+  out[2] = acc[2] * diffuse[2] * kInvPi;
   if (kSpec) {  // cacheApply.frag:116
     out[0] = out[0] + spec[0] * sp.color[0];
     out[1] = out[1] + spec[1] * sp.color[1];
@@ -304,7 +320,7 @@ __global__ __launch_bounds__(256) void k_apply(ApplyArgs a) {
   const uint32_t live = a.counter->live;
   const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d);
   const int c = cascade_of(a.vol, a.C, wp);
-  const f3 n = unpack_normal16i(static_cast<int16_t>(nrm & 0xFFFFu), static_cast<int16_t>(nrm >> 16));
+  const f3 n = shading_normal(static_cast<int16_t>(nrm & 0xFFFFu), static_cast<int16_t>(nrm >> 16));
   float diffuse[3] = {lut[al0], lut[al1], lut[al2]};
   SpecPixel sp{};
   if (kSpec) {
