@@ -1285,6 +1285,7 @@ struct drv_batch {
   uint32_t cpw = 64;
   uint32_t vpl_tile = 4096;
   int waves_per_cu = 64;
+  bool specular = false;  // some view keeps env-maps: per-view passes only
   drvk::BatchView* d_views = nullptr;
   std::vector<drvk::BatchView> h_views;  // as last uploaded
   drvk::BatchQueue* d_queue = nullptr;
@@ -1320,9 +1321,6 @@ int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out) {
         (views[i]->cfg.indirect_shadow != 0) != (views[0]->cfg.indirect_shadow != 0) ||
         views[i]->stream != views[0]->stream)
       return DRV_ERR_INVALID_ARG;
-    // The batched kernel writes SH only; specular views use drv_light_caches.
-    if (views[i]->cfg.indirect_specular)
-      return fail(views[i], DRV_ERR_STATE, "batch: views with indirect_specular are lit one at a time");
   }
   drv_batch* b = new (std::nothrow) drv_batch();
   if (!b) return DRV_ERR_NO_MEMORY;
@@ -1332,6 +1330,10 @@ int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out) {
   b->stride = drvk::entry_stride(b->sh_bands);
   b->shadow = views[0]->cfg.indirect_shadow != 0 ? 1 : 0;
   b->cu_count = views[0]->cu_count;
+  // The fused kernel writes SH only; a batch with env-map views lights every
+  // view with its own passes.
+  for (int i = 0; i < n_views; ++i) b->specular = b->specular || views[i]->cfg.indirect_specular != 0;
+  if (b->specular) b->cpw = 0;
   b->h_views.resize(n_views);
   std::memset(b->h_views.data(), 0, sizeof(drvk::BatchView) * n_views);
   if (hipMalloc(reinterpret_cast<void**>(&b->d_views), sizeof(drvk::BatchView) * n_views) != hipSuccess ||
@@ -1362,6 +1364,8 @@ int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t wa
       waves_per_cu > 64)
     return bfail(b, DRV_ERR_INVALID_ARG,
                  "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes), VPL tile a power of two >= 16");
+  if (b->specular && cpw != 0)
+    return bfail(b, DRV_ERR_INVALID_ARG, "batch_set_tuning: views with indirect_specular need per-view passes (0)");
   b->cpw = static_cast<uint32_t>(cpw);
   b->vpl_tile = static_cast<uint32_t>(vpl_tile);
   b->waves_per_cu = waves_per_cu;
@@ -1486,6 +1490,7 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   uint32_t best_cpw = keep_cpw, best_tile = keep_tile;
   const uint32_t cpws[4] = {0, 64, 32, 16};
   for (uint32_t cpw : cpws) {
+    if (b->specular && cpw != 0) continue;  // env-map views: per-view passes only
     // Per-view passes (cpw 0) have no VPL tile: one candidate.
     for (uint32_t tile = cpw == 0 ? max_vpl : 256; tile <= max_vpl; tile *= 2) {
       b->cpw = cpw;

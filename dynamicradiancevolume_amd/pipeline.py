@@ -354,8 +354,9 @@ class Renderer:
 
 
 class Batch:
-    """Several views lit by one persistent work-queue kernel
-    (drv_batch_light_caches), each view a Renderer on the same stream."""
+    """Several views lit by one persistent work-queue kernel or by their own
+    passes back to back (drv_batch_light_caches), each view a Renderer on the
+    same stream."""
 
     def __init__(self, renderers: list):
         self.lib = _abi.load_drv()
@@ -364,8 +365,7 @@ class Batch:
         self.handle = C.c_void_p()
         rc = self.lib.drv_batch_create(arr, len(renderers), C.byref(self.handle))
         if rc != _abi.DRV_OK:
-            raise DrvError(f"drv_batch_create failed ({rc}): views must share SH bands, shadow mode and stream, "
-                           "and have indirect specular off")
+            raise DrvError(f"drv_batch_create failed ({rc}): views must share SH bands, shadow mode and stream")
 
     def close(self):
         if self.handle:

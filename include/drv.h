@@ -413,7 +413,9 @@ int drv_get_tuning(const drv_ctx* ctx, drv_tuning* out);
  * work-queue kernel instead of a pass per view: drv_batch_light_caches is
  * drv_light_caches for every view at once (same results up to the order of
  * the VPL sums).  All contexts must share sh_bands, indirect_shadow and their
- * HIP stream.  No counterpart in the reference (one view per frame,
+ * HIP stream.  A batch with a view that has indirect_specular lights every
+ * view with its own passes (caches per wave 0 below; the fused kernel writes
+ * SH only).  No counterpart in the reference (one view per frame,
  * rendering/renderer.cpp:539-594). */
 typedef struct drv_batch drv_batch;
 
