@@ -1572,7 +1572,7 @@ int drv_set_tuning(drv_ctx* ctx, const drv_tuning* t) {
   if (!ctx || !t) return DRV_ERR_INVALID_ARG;
   if ((t->visibility_kernel != 0 && t->visibility_kernel != 1) || t->blocks_per_item < 1 || t->blocks_per_item > 64 ||
       t->gather_max_splits < 1 || t->gather_max_splits > 128 || !is_pow2(t->gather_max_splits) ||
-      t->waves_per_cu < 4 || t->waves_per_cu > 64 || t->refill_min_lanes < 1 || t->refill_min_lanes > 64 ||
+      t->waves_per_cu < 4 || t->waves_per_cu > 1024 || t->refill_min_lanes < 1 || t->refill_min_lanes > 64 ||
       ((t->steps_per_batch < 1 || t->steps_per_batch > 4) && t->steps_per_batch != 8))
     return fail(ctx, DRV_ERR_INVALID_ARG, "set_tuning: value out of range");
   ctx->tune.visibility_kernel = t->visibility_kernel;

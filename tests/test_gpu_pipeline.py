@@ -345,6 +345,8 @@ def test_full_frame_matches_oracle(name):
 @pytest.mark.parametrize("tuning", [
     dict(visibility_kernel=0, blocks_per_item=8),
     dict(visibility_kernel=0, blocks_per_item=2),
+    dict(visibility_kernel=0, waves_per_cu=64),
+    dict(visibility_kernel=0, waves_per_cu=1024),
     dict(visibility_kernel=1, blocks_per_item=8),
     dict(visibility_kernel=1, blocks_per_item=1, waves_per_cu=8),
     dict(visibility_kernel=1, blocks_per_item=32, gather_max_splits=128, waves_per_cu=64),
