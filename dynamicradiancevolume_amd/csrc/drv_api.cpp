@@ -221,8 +221,13 @@ drvk::VoxelView voxel_view(const drv_ctx* ctx) {
   return v;
 }
 
-// Persistent kernels: waves_per_cu waves per CU in 256-thread workgroups.
+// The light pass grid: waves_per_cu waves per CU in 256-thread workgroups.
 int persistent_grid(const drv_ctx* ctx) { return std::max(1, ctx->cu_count * ctx->tune.waves_per_cu / 4); }
+
+// The network's kernels keep their weights in registers, loaded once per
+// workgroup: a persistent grid of 64 waves per CU, whatever the light pass
+// grid is.
+int network_grid(const drv_ctx* ctx) { return std::max(1, ctx->cu_count * 64 / 4); }
 
 template <typename T>
 int ensure(drv_ctx* ctx, T** p, size_t* cap, size_t need) {
@@ -261,7 +266,7 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     drvk::launch_visibility(ctx->vol, vox, set_a, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, vis_a, sc, nullptr,
                             ctx->tune, grid, ctx->stream);
     drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, set_b, stride, &ctx->spot_ctr[1], ctx->blocks,
-                                n_blocks, vis_b, sc, grid, ctx->stream);
+                                n_blocks, vis_b, sc, network_grid(ctx), ctx->stream);
     // The split partials of a small set fit many more splits than the
     // main list's (the partial buffer holds partial_splits x capacity).
     const uint32_t spot_splits = static_cast<uint32_t>(std::min<size_t>(128, ctx->partial_splits * cap / spot_cap));
@@ -283,7 +288,7 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     ctx->label_caches = nullptr;
   }
   drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, ctx->caches, stride, ctx->counter, ctx->blocks,
-                              n_blocks, ctx->visibility, static_cast<uint32_t>(cap), grid, ctx->stream);
+                              n_blocks, ctx->visibility, static_cast<uint32_t>(cap), network_grid(ctx), ctx->stream);
   if (spot)  // gate counter: live = 0 unless the spot check failed
     drvk::launch_visibility(ctx->vol, vox, ctx->caches, stride, &ctx->spot_ctr[2], ctx->blocks, n_blocks,
                             ctx->visibility, static_cast<uint32_t>(cap), nullptr, ctx->tune, grid, ctx->stream);
@@ -1221,7 +1226,7 @@ int drv_mlp_forward(drv_ctx* ctx, const float* x_dev, uint32_t n, float* p_dev) 
   if (!ctx || (n && (!x_dev || !p_dev))) return DRV_ERR_INVALID_ARG;
   int rc;
   if ((rc = require_net(ctx)) != DRV_OK) return rc;
-  drvk::launch_mlp_forward(ctx->mlp_pack, ctx->mlp_params, x_dev, n, p_dev, persistent_grid(ctx), ctx->stream);
+  drvk::launch_mlp_forward(ctx->mlp_pack, ctx->mlp_params, x_dev, n, p_dev, network_grid(ctx), ctx->stream);
   return check_hip(ctx, hipGetLastError(), "mlp_forward");
 }
 
