@@ -66,16 +66,19 @@ __device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float
   const float u = __builtin_fmaf(px, li.scale, 0.5f);
   const float v = __builtin_fmaf(py, li.scale, 0.5f);
   const float q = __builtin_fmaf(pz, li.scale, 0.5f);
-  const float fu = floorf(u), fv = floorf(v), fq = floorf(q);
-  // Clamped in float first so the conversion is always in range (NaN -> 0).
-  const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fu, 0.0f, li.w));
-  const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fv, 0.0f, li.w));
-  const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fq, 0.0f, li.w));
+  // Clamped in float first so the conversion is always in range (NaN -> 0);
+  // w is an integer, so truncating the clamped value is floor(u) clamped.
+  const uint32_t i = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(u, 0.0f, li.w));
+  const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(v, 0.0f, li.w));
+  const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(q, 0.0f, li.w));
   Footprint f;
   f.index = li.off + i + __umul24(j, li.wp) + __umul24(k, li.wp2);  // (w + 1)^2 < 2^24 for res <= 512
-  f.ax = u - fu;
-  f.ay = v - fv;
-  f.az = q - fq;
+  // u - floor(u) (v_fract_f32 caps it below 1, which only happens for
+  // u in (-2^-24, 0), where both texels of the footprint are the clamped
+  // edge texel and the weight has no effect).
+  f.ax = __builtin_amdgcn_fractf(u);
+  f.ay = __builtin_amdgcn_fractf(v);
+  f.az = __builtin_amdgcn_fractf(q);
   return f;
 }
 

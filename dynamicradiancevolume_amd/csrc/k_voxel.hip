@@ -131,19 +131,21 @@ __global__ __launch_bounds__(256) void k_voxelize(VoxelizeArgs a) {
   const float gx = (dz1 * ey2 - dz2 * ey1) / area;
   const float gy = (dz2 * ex1 - dz1 * ex2) / area;
   const float max_change = sqrtf(gx * gx + gy * gy) * 1.414f;
-  const int px0 = max(0, to_int(floorf(bb0 - 0.5f)));
-  const int py0 = max(0, to_int(floorf(bb1 - 0.5f)));
-  const int px1 = min(res - 1, to_int(ceilf(bb2 - 0.5f)));
-  const int py1 = min(res - 1, to_int(ceilf(bb3 - 0.5f)));
+  // voxelize.frag:26-30 keeps the pixel centres c = p + 0.5 with
+  // bb_lo <= c <= bb_hi, i.e. ceil(bb_lo - 0.5) <= p <= floor(bb_hi - 0.5).
+  // bb - 0.5 is exact for bb >= 0.25 (Sterbenz up to 1, on bb's grid above);
+  // a bound below 0.5 only ever yields p <= 0 (lower) or p < 0 (upper), as
+  // the exact value does, so the ranges hold exactly the passing centres.
+  const int px0 = max(0, to_int(ceilf(bb0 - 0.5f)));
+  const int py0 = max(0, to_int(ceilf(bb1 - 0.5f)));
+  const int px1 = min(res - 1, to_int(floorf(bb2 - 0.5f)));
+  const int py1 = min(res - 1, to_int(floorf(bb3 - 0.5f)));
   // res is a power of two, so x / res == x * h exactly.
   for (int py = py0; py <= py1; ++py) {
     const float fy = static_cast<float>(py) + 0.5f;
-    if (fy < bb1 || fy > bb3) continue;
     const float ny = fy * h * 2.0f - 1.0f;
     for (int px = px0; px <= px1; ++px) {
       const float fx = static_cast<float>(px) + 0.5f;
-      // voxelize.frag:26-30
-      if (fx < bb0 || fx > bb2) continue;
       const float nx = fx * h * 2.0f - 1.0f;
       const bool in = (pl[0][0] * nx + pl[0][1] * ny + pl[0][2] <= 0.0f) &&
                       (pl[1][0] * nx + pl[1][1] * ny + pl[1][2] <= 0.0f) &&
