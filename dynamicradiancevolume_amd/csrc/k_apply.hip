@@ -142,36 +142,38 @@ __device__ __forceinline__ f3 shading_normal(int16_t a, int16_t b) {
 __device__ __forceinline__ void cache_irradiance(const ApplyArgs& a, const float4* q, f3 n, float* irr) {
   // {SH1neg1, SH00_r}, {SH10, SH00_g}, {SH1pos1, SH00_b}
   const float CL0 = a.shc.cos_lobe0, CL1 = a.shc.cos_lobe1;
+  // Terms accumulate in the reference's order, each as one fused multiply-add.
+  const float ny = -(CL1 * n.y), nz = CL1 * n.z, nx = -(CL1 * n.x);
   float v0 = q[1].w * CL0, v1 = q[2].w * CL0, v2 = q[3].w * CL0;
-  v0 = v0 - q[1].x * (CL1 * n.y);
-  v1 = v1 - q[1].y * (CL1 * n.y);
-  v2 = v2 - q[1].z * (CL1 * n.y);
-  v0 = v0 + q[2].x * (CL1 * n.z);
-  v1 = v1 + q[2].y * (CL1 * n.z);
-  v2 = v2 + q[2].z * (CL1 * n.z);
-  v0 = v0 - q[3].x * (CL1 * n.x);
-  v1 = v1 - q[3].y * (CL1 * n.x);
-  v2 = v2 - q[3].z * (CL1 * n.x);
+  v0 = __builtin_fmaf(q[1].x, ny, v0);
+  v1 = __builtin_fmaf(q[1].y, ny, v1);
+  v2 = __builtin_fmaf(q[1].z, ny, v2);
+  v0 = __builtin_fmaf(q[2].x, nz, v0);
+  v1 = __builtin_fmaf(q[2].y, nz, v1);
+  v2 = __builtin_fmaf(q[2].z, nz, v2);
+  v0 = __builtin_fmaf(q[3].x, nx, v0);
+  v1 = __builtin_fmaf(q[3].y, nx, v1);
+  v2 = __builtin_fmaf(q[3].z, nx, v2);
   if (a.sh2) {
     // {SH2neg2, SH20_r}, {SH2neg1, SH20_g}, {SH2pos1, SH20_b}, {SH2pos2, -}
     const float CL2 = a.shc.cos_lobe2n2_p1_n1, CL20 = a.shc.cos_lobe20, CL22 = a.shc.cos_lobe2p2;
-    const float fxy = CL2 * n.x * n.y, fyz = CL2 * n.y * n.z, fxz = CL2 * n.x * n.z;
+    const float fxy = -(CL2 * n.x * n.y), fyz = CL2 * n.y * n.z, fxz = CL2 * n.x * n.z;
     const float f20 = CL20 * (n.z * n.z * 3.0f - 1.0f), f22 = CL22 * (n.x * n.x - n.y * n.y);
-    v0 = v0 - q[4].x * fxy;
-    v1 = v1 - q[4].y * fxy;
-    v2 = v2 - q[4].z * fxy;
-    v0 = v0 + q[5].x * fyz;
-    v1 = v1 + q[5].y * fyz;
-    v2 = v2 + q[5].z * fyz;
-    v0 = v0 + q[4].w * f20;
-    v1 = v1 + q[5].w * f20;
-    v2 = v2 + q[6].w * f20;
-    v0 = v0 + q[6].x * fxz;
-    v1 = v1 + q[6].y * fxz;
-    v2 = v2 + q[6].z * fxz;
-    v0 = v0 + q[7].x * f22;
-    v1 = v1 + q[7].y * f22;
-    v2 = v2 + q[7].z * f22;
+    v0 = __builtin_fmaf(q[4].x, fxy, v0);
+    v1 = __builtin_fmaf(q[4].y, fxy, v1);
+    v2 = __builtin_fmaf(q[4].z, fxy, v2);
+    v0 = __builtin_fmaf(q[5].x, fyz, v0);
+    v1 = __builtin_fmaf(q[5].y, fyz, v1);
+    v2 = __builtin_fmaf(q[5].z, fyz, v2);
+    v0 = __builtin_fmaf(q[4].w, f20, v0);
+    v1 = __builtin_fmaf(q[5].w, f20, v1);
+    v2 = __builtin_fmaf(q[6].w, f20, v2);
+    v0 = __builtin_fmaf(q[6].x, fxz, v0);
+    v1 = __builtin_fmaf(q[6].y, fxz, v1);
+    v2 = __builtin_fmaf(q[6].z, fxz, v2);
+    v0 = __builtin_fmaf(q[7].x, f22, v0);
+    v1 = __builtin_fmaf(q[7].y, f22, v1);
+    v2 = __builtin_fmaf(q[7].z, f22, v2);
   }
   irr[0] = fmaxf(v0, 0.0f);
   irr[1] = fmaxf(v1, 0.0f);
@@ -228,7 +230,7 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp,
           tap_cache(a, sp.t2, cx, cy, s2);
           for (int k = 0; k < 3; ++k) smp[k] = (1.0f - sp.blend) * smp[k] + sp.blend * s2[k];
         }
-        for (int k = 0; k < 3; ++k) spec[k] = spec[k] + smp[k] * w[i];
+        for (int k = 0; k < 3; ++k) spec[k] = __builtin_fmaf(smp[k], w[i], spec[k]);
       }
     }
   }
@@ -267,21 +269,30 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp,
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     if (from[i] == 3u) continue;
+    // q[4..7] (the L2 bands) are loaded and read only with sh2.
     float4 q[8];
     if (from[i] == 2u) {
       const float4* e4 = reinterpret_cast<const float4*>(a.caches + static_cast<size_t>(addr[i]) * a.stride);
 #pragma unroll
-      for (int j = 1; j < 8; ++j) q[j] = j < nq ? e4[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int j = 1; j < 4; ++j) q[j] = e4[j];
+      if (a.sh2) {
+#pragma unroll
+        for (int j = 4; j < 8; ++j) q[j] = e4[j];
+      }
     } else {
       const float4* st = reinterpret_cast<const float4*>(stage + i * 64 + 32 * from[i]);
 #pragma unroll
-      for (int j = 1; j < 8; ++j) q[j] = j < nq ? st[j - 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int j = 1; j < 4; ++j) q[j] = st[j - 1];
+      if (a.sh2) {
+#pragma unroll
+        for (int j = 4; j < 8; ++j) q[j] = st[j - 1];
+      }
     }
     float irr[3];
     cache_irradiance(a, q, n, irr);
-    acc[0] = acc[0] + irr[0] * w[i];
-    acc[1] = acc[1] + irr[1] * w[i];
-    acc[2] = acc[2] + irr[2] * w[i];
+    acc[0] = __builtin_fmaf(irr[0], w[i], acc[0]);
+    acc[1] = __builtin_fmaf(irr[1], w[i], acc[1]);
+    acc[2] = __builtin_fmaf(irr[2], w[i], acc[2]);
   }
   __builtin_amdgcn_wave_barrier();  // the stage is rewritten by the next call
   out[0] = acc[0] * diffuse[0] * kInvPi;
