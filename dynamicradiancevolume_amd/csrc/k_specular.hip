@@ -68,7 +68,7 @@ struct SpecGatherArgs {
 // float, so the conversion is always in range.
 __device__ __forceinline__ int texel_index(float p, int S) {
   const float fs = static_cast<float>(S);
-  return static_cast<int>(fminf(fmaxf(p * fs + 0.5f, 0.0f), fs));
+  return static_cast<int>(fminf(fmaxf(__builtin_fmaf(p, fs, 0.5f), 0.0f), fs));
 }
 
 // Dynamic LDS: 4 maps of (S+1)^2 RGB floats.

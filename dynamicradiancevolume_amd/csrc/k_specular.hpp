@@ -47,12 +47,36 @@ __device__ __forceinline__ void unpack_r11g11b10(uint32_t v, float* rgb) {
   rgb[2] = uf_decode(v >> 22, 5);
 }
 
+// atan2(y, x) for the hemisphere projection, by octant reduction: the
+// quotient t = min(|x|, |y|) / max(|x|, |y|) in [0, 1] (hardware reciprocal),
+// atan(t) = t P(t^2) with P the degree-7 minimax fit of atan(t) / t on
+// [0, 1] (relative error 1e-7; a few ulp in FP32), then the octant, the
+// sign of x and the sign of y restore the angle.  atan2(0, 0) = 0.
+__device__ __forceinline__ float octant_atan2(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+  const float s = t * t;
+  float p = -0.004693402908742428f;
+  p = __builtin_fmaf(p, s, 0.02425288036465645f);
+  p = __builtin_fmaf(p, s, -0.05948710814118385f);
+  p = __builtin_fmaf(p, s, 0.09914347529411316f);
+  p = __builtin_fmaf(p, s, -0.14019502699375153f);
+  p = __builtin_fmaf(p, s, 0.19969728589057922f);
+  p = __builtin_fmaf(p, s, -0.33331990242004395f);
+  p = __builtin_fmaf(p, s, 0.9999998807907104f);
+  float a = t * p;
+  a = ay > ax ? 1.5707963267948966f - a : a;
+  a = x < 0.0f ? 3.141592653589793f - a : a;
+  return y < 0.0f ? -a : a;
+}
+
 // HemisphericalProjection with HEMIPROJECTION_LAMBERT_CONCENTRICQUAD
 // (shader/utils.glsl:102-178, the "better perf" form): z+ hemisphere ->
-// [0, 1]^2, equal-area.
+// [0, 1]^2, equal-area.  The square root is the hardware one (1 ulp).
 __device__ __forceinline__ void hemi_projection(float dx, float dy, float dz, float* out) {
-  const float r = sqrtf(1.0f - dz) * 0.5f;
-  float phi = atan2f(dy, dx) * (4.0f / kPi);
+  const float r = __builtin_amdgcn_sqrtf(1.0f - dz) * 0.5f;
+  float phi = octant_atan2(dy, dx) * (4.0f / kPi);
   if (phi < -1.0f) phi += 8.0f;
   float px, py;
   if (phi < 3.0f) {
@@ -97,8 +121,9 @@ __device__ __forceinline__ LocalFrame local_view_space(const float* cam, f3 wp) 
 }
 
 __device__ __forceinline__ f3 to_local(const LocalFrame& f, float vx, float vy, float vz) {
-  return make3(vx * f.x.x + vy * f.x.y + vz * f.x.z, vx * f.y.x + vy * f.y.y + vz * f.y.z,
-               vx * f.z.x + vy * f.z.y + vz * f.z.z);
+  return make3(__builtin_fmaf(vz, f.x.z, __builtin_fmaf(vy, f.x.y, vx * f.x.x)),
+               __builtin_fmaf(vz, f.y.z, __builtin_fmaf(vy, f.y.y, vx * f.y.x)),
+               __builtin_fmaf(vz, f.z.z, __builtin_fmaf(vy, f.z.y, vx * f.z.x)));
 }
 
 // BlinnNormalization(GetBlinnPhongExponentForHemispherePixel(S))
