@@ -9,6 +9,12 @@ region covers the whole frame.
 
 value = cache.VPL interactions per second of whole frames
         (live caches x VPLs / frame time); ms_per_step = ms per frame.
+
+Clock settle: the GPU raises its clocks over the first ~20 ms of
+back-to-back work (boxes_1080p frames go from 0.52 to 0.47 ms over ~35
+frames, and again after an idle second).  Each timed section therefore runs
+its frame untimed for --settle-s seconds of wall time before its warm-up
+steps, so the timed steps measure a renderer that draws continuously.
 """
 from __future__ import annotations
 
@@ -82,6 +88,25 @@ def _traffic(kernel: str):
     return k["hbm_bytes"] if k else None
 
 
+SETTLE_S = 0.15
+
+
+def _settle(fn, stream, seconds: float) -> int:
+    """Runs `fn` back to back, untimed, for `seconds` of wall time (the GPU
+    clock ramp); returns the number of calls."""
+    t0 = time.perf_counter()
+    n = 0
+    while seconds > 0:
+        fn()
+        n += 1
+        if n % 8 == 0:
+            stream.synchronize()
+            if time.perf_counter() - t0 >= seconds:
+                break
+    stream.synchronize()
+    return n
+
+
 def _time_frames(r, df, stream, steps: int, finetune_every: int = 0):
     """ms per frame over `steps` whole frames (HIP events on the library's
     stream), optionally with an online fine-tuning step every N frames."""
@@ -113,7 +138,7 @@ def _sh_by_position(r, bands: int):
     return dict(zip(keys, entries_sh(e, bands).astype(np.float64)))
 
 
-def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, threshold: float):
+def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, threshold: float, settle_s: float):
     """The same frames with the surrogate visibility network (committed
     weights, 1 % spot check, fine-tuning every 16 frames)."""
     import numpy as np
@@ -128,6 +153,7 @@ def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, thre
     for label, thr in (("no_fallback", 1e30), ("fallback", threshold)):
         r.mlp_set_params(S.load_weights())
         r.set_surrogate(True, spot_fraction=0.01, fallback_threshold=thr)
+        _settle(lambda: r.render_frame(df), stream, settle_s)
         for _ in range(warmup):
             r.render_frame(df)
         torch.cuda.synchronize()
@@ -269,7 +295,7 @@ def _inputs_and_reference(fi, df, stream, w):
     }
 
 
-def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int):
+def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int, settle_s: float):
     """The same frame with INDIRECT_SPECULAR (16 x 16 env-map per cache, the
     reference defaults): frame time, the env-map stage, and the end-to-end
     error of the unshadowed cache path (diffuse SH + specular env-maps)
@@ -281,6 +307,7 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int):
 
     ws = w.with_(specular=1)
     r = Renderer(ws.config(), stream)
+    _settle(lambda: r.render_frame(df), stream, settle_s)
     for _ in range(warmup):
         r.render_frame(df)
     ms, _ = _time_frames(r, df, stream, steps)
@@ -338,6 +365,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--settle-s", type=float, default=SETTLE_S,
+                    help="seconds of untimed frames before each section's warm-up (GPU clock ramp)")
     ap.add_argument("--workload", default="boxes_1080p")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -367,7 +396,8 @@ def main() -> int:
     torch.cuda.synchronize()
     n_vpl = fi.read_resolution ** 2
 
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
+    settled = _settle(lambda: r.render_frame(df), stream, args.settle_s)
+    print(f"[bench] clock settle {settled} frames, warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         r.render_frame(df)
     torch.cuda.synchronize()
@@ -444,12 +474,12 @@ def main() -> int:
     sur = None
     if not args.no_surrogate:
         print("[bench] surrogate", file=sys.stderr, flush=True)
-        sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold)
+        sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold, args.settle_s)
 
     spec = None
     if not args.no_specular:
         print("[bench] indirect specular", file=sys.stderr, flush=True)
-        spec = _specular(fi, df, stream, w, args.steps, args.warmup, n_vpl)
+        spec = _specular(fi, df, stream, w, args.steps, args.warmup, n_vpl, args.settle_s)
 
     print("[bench] device inputs and brute-force reference", file=sys.stderr, flush=True)
     ref = _inputs_and_reference(fi, df, stream, w)
@@ -495,6 +525,7 @@ def main() -> int:
             "stage_ms": stages,
             "path": "plain cone-traced",
             "graph_replay_ms_per_frame": graph_ms,
+            "clock_settle_s": args.settle_s,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
