@@ -72,21 +72,19 @@ __device__ __forceinline__ Tap make_tap(const ApplyArgs& a, int l, const float* 
   return t;
 }
 
-// R11G11B10F texel -> RGB.  The code shifted into FP32 position reads as
-// 2^(e-127) (1 + f) for exponents 1..30 and as the FP32 denormal f 2^-126
-// for exponent 0; one multiply by 2^112 rebiases both exactly to GL's
-// 2^(e-15) (1 + f) and f 2^-14 (FP32 denormals are preserved in these
-// kernels).  Exponent 31 is inf (f = 0) or NaN.
-__device__ __forceinline__ float decode_channel(uint32_t c, int m) {
-  const uint32_t bits = c << (23 - m);
-  const float v = __builtin_bit_cast(float, bits) * 0x1p112f;
-  return (c >> m) == 31u ? __builtin_bit_cast(float, 0x7F800000u | bits) : v;
+// R11G11B10F texel -> RGB.  GL's unsigned 11- and 10-bit floats have the
+// 5-bit exponent and the bias (15) of IEEE half precision, so a channel's
+// bits shifted to the top of a half's mantissa are exactly that half
+// (normals, denormals, inf and NaN alike), and the hardware half -> float
+// conversion decodes it: a shift, a mask and a conversion per channel.
+__device__ __forceinline__ float half_bits(uint32_t h) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h)));
 }
 
 __device__ __forceinline__ void decode_texel(uint32_t x, float* rgb) {
-  rgb[0] = decode_channel(x & 0x7ffu, 6);
-  rgb[1] = decode_channel((x >> 11) & 0x7ffu, 6);
-  rgb[2] = decode_channel(x >> 22, 5);
+  rgb[0] = half_bits((x << 4) & 0x7FF0u);   // bits 0-10: 6-bit mantissa, 5-bit exponent
+  rgb[1] = half_bits((x >> 7) & 0x7FF0u);   // bits 11-21
+  rgb[2] = half_bits((x >> 17) & 0x7FE0u);  // bits 22-31: 5-bit mantissa
 }
 
 // Bilinear tap of cache (cx, cy) at tap.l, clamp to the atlas edge.
@@ -104,7 +102,8 @@ __device__ __forceinline__ void tap_cache(const ApplyArgs& a, const Tap& t, uint
   decode_texel(q10, t10);
   decode_texel(q01, t01);
   decode_texel(q11, t11);
-  for (int c = 0; c < 3; ++c) out[c] = t.w[0] * t00[c] + t.w[1] * t10[c] + t.w[2] * t01[c] + t.w[3] * t11[c];
+  for (int c = 0; c < 3; ++c)
+    out[c] = __builtin_fmaf(t.w[3], t11[c], __builtin_fmaf(t.w[2], t01[c], __builtin_fmaf(t.w[1], t10[c], t.w[0] * t00[c])));
 }
 
 // The per-pixel inputs of the specular lookup (cacheApply.frag:139-154) and
@@ -228,7 +227,7 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp,
         if (sp.two) {
           float s2[3];
           tap_cache(a, sp.t2, cx, cy, s2);
-          for (int k = 0; k < 3; ++k) smp[k] = (1.0f - sp.blend) * smp[k] + sp.blend * s2[k];
+          for (int k = 0; k < 3; ++k) smp[k] = __builtin_fmaf(sp.blend, s2[k], (1.0f - sp.blend) * smp[k]);
         }
         for (int k = 0; k < 3; ++k) spec[k] = __builtin_fmaf(smp[k], w[i], spec[k]);
       }
