@@ -95,7 +95,7 @@ struct drv_ctx {
   uint32_t label_pitch = 0;
 
   drvk::Tuning tune;
-  size_t partial_splits = 0;  // splits the partial buffer holds
+  size_t partial_groups = 0;  // groups of 4 VPL splits the partial buffer holds (x 27 x capacity)
 
   // Indirect specular: atlas, work sums, spill; spec_first is set by the
   // allocation pass so that the frame's first light overwrites the sums.
@@ -268,8 +268,10 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, set_b, stride, &ctx->spot_ctr[1], ctx->blocks,
                                 n_blocks, vis_b, sc, network_grid(ctx), ctx->stream);
     // The split partials of a small set fit many more splits than the
-    // main list's (the partial buffer holds partial_splits x capacity).
-    const uint32_t spot_splits = static_cast<uint32_t>(std::min<size_t>(128, ctx->partial_splits * cap / spot_cap));
+    // main list's (the gather writes one partial per group of 4 splits, and
+    // the buffer holds partial_groups x capacity of them).
+    const uint32_t spot_splits =
+        static_cast<uint32_t>(std::min<size_t>(drvk::kMaxSplits, 4 * ctx->partial_groups * cap / spot_cap));
     for (int set = 0; set < 2; ++set) {
       float* cs = set == 0 ? set_a : set_b;
       drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->vpls, n_vpl,
@@ -734,12 +736,13 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     if ((rc = dev_alloc(ctx, &ctx->visibility, cap * n_blocks)) != DRV_OK) return rc;
     ctx->vis_blocks_cap = n_blocks;
   }
-  const size_t splits = static_cast<size_t>(ctx->tune.gather_max_splits);
-  if (ctx->partial_splits < splits) {
+  // One partial of 27 sums per cache and group of 4 splits.
+  const size_t groups = (static_cast<size_t>(ctx->tune.gather_max_splits) + 3) / 4;
+  if (ctx->partial_groups < groups) {
     dev_free(ctx->partial);
-    ctx->partial_splits = 0;
-    if ((rc = dev_alloc(ctx, &ctx->partial, cap * 27 * splits)) != DRV_OK) return rc;
-    ctx->partial_splits = splits;
+    ctx->partial_groups = 0;
+    if ((rc = dev_alloc(ctx, &ctx->partial, cap * 27 * groups)) != DRV_OK) return rc;
+    ctx->partial_groups = groups;
   }
 
   const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
@@ -1576,7 +1579,7 @@ int drv_graph_launch(drv_ctx* ctx) {
 int drv_set_tuning(drv_ctx* ctx, const drv_tuning* t) {
   if (!ctx || !t) return DRV_ERR_INVALID_ARG;
   if ((t->visibility_kernel != 0 && t->visibility_kernel != 1) || t->blocks_per_item < 1 || t->blocks_per_item > 64 ||
-      t->gather_max_splits < 1 || t->gather_max_splits > 128 || !is_pow2(t->gather_max_splits) ||
+      t->gather_max_splits < 1 || t->gather_max_splits > static_cast<int32_t>(drvk::kMaxSplits) || !is_pow2(t->gather_max_splits) ||
       t->waves_per_cu < 4 || t->waves_per_cu > 1024 || t->refill_min_lanes < 1 || t->refill_min_lanes > 64 ||
       ((t->steps_per_batch < 1 || t->steps_per_batch > 4) && t->steps_per_batch != 8))
     return fail(ctx, DRV_ERR_INVALID_ARG, "set_tuning: value out of range");

@@ -27,6 +27,7 @@ struct Counter {
 };
 
 constexpr int kGatherTile = 64;    // caches per gather / cone-trace tile (one per lane)
+constexpr uint32_t kMaxSplits = 256;  // VPL splits of the SH gather (each at least 64 VPLs)
 
 // One VPL ("LightInfo", cacheLightingRSM.comp:34-40) packed for LDS staging.
 struct Vpl {
@@ -115,7 +116,7 @@ struct BatchQueue {
 struct Tuning {
   int visibility_kernel = 0;   // 0: lane per cache, one block at a time; 1: refilling lane pool
   int blocks_per_item = 1;     // shadow blocks per visibility work item (an upper bound)
-  int gather_max_splits = 128; // VPL splits of the gather
+  int gather_max_splits = 256; // VPL splits of the gather
   int waves_per_cu = 256;      // launch grid of the light pass kernels, in waves per CU
   int refill_min_lanes = 16;   // lane pool: idle lanes that trigger a refill
   int steps_per_batch = 1;     // cone-trace steps planned (and loaded) together: 1..4 or 8

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(BatchArgs a) {
         bool finished = false;
         const __amdgpu_buffer_rsrc_t cr = corner_rsrc(bv.vox);
         while (!finished) finished = trace_batch(bv.vox, cr, lvi, t);
-        shadowing = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
+        shadowing = trace_visibility(t);
       }
       const uint32_t m0 = b * interval;
 #pragma unroll 4

@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
     ShadowBlock nb = a.blocks[min(b + 1u, b1 - 1u)];
     while (true) {
       if (trace_batch<kBatch>(a.vox, cr, lvi, t)) {
-        a.vis[static_cast<size_t>(b) * a.pitch + cache] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
+        a.vis[static_cast<size_t>(b) * a.pitch + cache] = trace_visibility(t);
         my_traces += 1u;
         my_exec += static_cast<uint32_t>(t.s);
         if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
       if (!__any(has)) break;
       const bool ended = trace_batch<kBatch>(a.vox, cr, lvi, t);
       if (has && ended) {
-        a.vis[static_cast<size_t>(blk) * a.pitch + tile * 64u + cl] = fminf(1.0f, fmaxf(0.0f, 1.0f - t.occ));
+        a.vis[static_cast<size_t>(blk) * a.pitch + tile * 64u + cl] = trace_visibility(t);
         my_traces += 1u;
         my_exec += static_cast<uint32_t>(t.s);
         if (a.steps) my_steps += static_cast<uint32_t>(reference_steps(t));
@@ -399,9 +399,17 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ caches
     if (split < splits) {
       const uint32_t m0 = split * chunk, m1 = m0 + chunk;
       uint32_t blk = m0 / interval;  // the split starts on a block boundary
+      // The next block's shadowing is loaded a block ahead, so the skip test
+      // below does not wait for it.
+      float next = kShadow ? vis[static_cast<size_t>(blk) * vis_pitch + c] : 1.0f;
       for (uint32_t mb = m0; mb < m1; mb += interval, ++blk) {
-        float shadowing = 1.0f;
-        if (kShadow) shadowing = vis[static_cast<size_t>(blk) * vis_pitch + c];
+        const float shadowing = next;
+        if (kShadow) {
+          if (mb + interval < m1) next = vis[static_cast<size_t>(blk + 1u) * vis_pitch + c];
+          // A block that no lane of the wave sees would add only exact
+          // zeros (finite flux and distances): its VPLs are skipped.
+          if (__ballot(valid && shadowing != 0.0f) == 0ull) continue;
+        }
 #pragma unroll 4
         for (uint32_t m = mb; m < mb + interval; ++m) sh_accumulate<kSh2>(acc, vpls[m], wx, wy, wz, shadowing);
       }

@@ -131,6 +131,17 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, __amdgpu_buffer
   return t.done || t.s >= 32 || (1.0f - t.occ) <= kOpaque;
 }
 
+// The visibility a finished trace returns, saturate(1 - occlusion)
+// (cacheLightingRSM.comp:229-231).  A trace stopped at transmittance
+// <= kOpaque returns exactly 0: later samples only lower the reference's
+// value, so it lies in [0, kOpaque] and 0 is as close to it as the value at
+// the stop.  Exact zeros let the SH gather skip a shadow block that no lane
+// of a wave sees.
+__device__ __forceinline__ float trace_visibility(const Trace& t) {
+  const float v = 1.0f - t.occ;
+  return v <= kOpaque ? 0.0f : fminf(1.0f, fmaxf(0.0f, v));
+}
+
 // Steps the reference still takes after a trace stopped early (geometry only).
 __device__ __forceinline__ int reference_steps(Trace t) {
   int s = t.s;

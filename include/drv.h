@@ -397,7 +397,7 @@ typedef struct drv_tuning {
   int32_t visibility_kernel; /* 0: lane per cache, walking its item's blocks (default); 1: refilling lane pool */
   int32_t blocks_per_item;   /* shadow blocks per visibility work item, 1..64 (default 1); the kernel
                                 uses the largest count up to this that leaves two items per wave */
-  int32_t gather_max_splits; /* largest VPL split count of the gather, power of two 1..128 (default 128) */
+  int32_t gather_max_splits; /* largest VPL split count of the gather, power of two 1..256 (default 256) */
   int32_t waves_per_cu;      /* launch grid of the light pass kernels, 4..1024 waves per CU (default 256:
                                 more waves than fit at once, so the hardware deals out the last items) */
   int32_t refill_min_lanes;  /* lane pool: idle lanes of a wave that trigger a refill, 1..64 (default 16) */

@@ -267,6 +267,10 @@ def _gather_case(name, n_sub=None, **overrides):
         nb = vis.shape[1]
         gvis = r.read_visibility(live, nb)[ids]
         assert np.abs(gvis - vis).max() < VIS_TOL
+        # A ray stopped at transmittance <= 2^-20 returns exactly 0; the
+        # oracle marches on, and its visibility there is at most 2^-20.
+        zero = gvis == 0.0
+        assert zero.any() and (vis[zero] <= 2.0 ** -20).all()
     return r, df, fi, cfg, vol, live
 
 
