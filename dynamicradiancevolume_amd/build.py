@@ -50,9 +50,15 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
 
 
+# Per-file flags.  k_mlp.hip: matrix-core results go to ordinary VGPRs (not
+# accumulation registers), which saves one register read per result value in
+# the inference kernel and keeps it within 128 registers (4 waves per SIMD).
+FILE_FLAGS = {"k_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile_hip(src: str) -> Path:
     obj = BUILD / (Path(src).stem + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+    _run([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src, []), "-c", str(CSRC / src), "-o", str(obj)])
     return obj
 
 

@@ -56,17 +56,38 @@ struct FeatureCtx {
   VoxelView vox;
 };
 
-__device__ __forceinline__ void features_all(const FeatureCtx& fc, const LevelInfo* lvi, f3 wp, const ShadowBlock& b,
-                                             float* out) {
+// The features that depend on the cache alone (computed once per cache, not
+// once per pair): position in the volume and unit direction to the camera.
+struct CacheFeatures {
+  f3 wp;
+  float vpx, vpy, vpz;
+  float cx, cy, cz;
+};
+
+__device__ __forceinline__ CacheFeatures cache_features(const FeatureCtx& fc, f3 wp) {
   const float res = static_cast<float>(fc.vox.res);
   const float vsz = fc.voxel_size * res;
-  const float vpx = (wp.x - fc.vmin[0]) / vsz, vpy = (wp.y - fc.vmin[1]) / vsz, vpz = (wp.z - fc.vmin[2]) / vsz;
+  CacheFeatures k;
+  k.wp = wp;
+  k.vpx = (wp.x - fc.vmin[0]) / vsz;
+  k.vpy = (wp.y - fc.vmin[1]) / vsz;
+  k.vpz = (wp.z - fc.vmin[2]) / vsz;
   float cx = fc.cam[0] - wp.x, cy = fc.cam[1] - wp.y, cz = fc.cam[2] - wp.z;
   const float cl = sqrtf(cx * cx + cy * cy + cz * cz);
   const float ci = cl > 0.0f ? 1.0f / cl : 0.0f;
-  cx *= ci;
-  cy *= ci;
-  cz *= ci;
+  k.cx = cx * ci;
+  k.cy = cy * ci;
+  k.cz = cz * ci;
+  return k;
+}
+
+__device__ __forceinline__ void features_pair(const FeatureCtx& fc, const LevelInfo* lvi, const CacheFeatures& k,
+                                              const ShadowBlock& b, float* out) {
+  const float res = static_cast<float>(fc.vox.res);
+  const float vsz = fc.voxel_size * res;
+  const f3 wp = k.wp;
+  const float vpx = k.vpx, vpy = k.vpy, vpz = k.vpz;
+  const float cx = k.cx, cy = k.cy, cz = k.cz;
   float tx = b.x - wp.x, ty = b.y - wp.y, tz = b.z - wp.z;
   const float ld = sqrtf(tx * tx + ty * ty + tz * tz);
   const float li = ld > 0.0f ? 1.0f / ld : 0.0f;
@@ -115,26 +136,35 @@ __device__ __forceinline__ void features_all(const FeatureCtx& fc, const LevelIn
   }
 }
 
-// ---- register-resident inference ------------------------------------------
+__device__ __forceinline__ void features_all(const FeatureCtx& fc, const LevelInfo* lvi, f3 wp, const ShadowBlock& b,
+                                             float* out) {
+  features_pair(fc, lvi, cache_features(fc, wp), b, out);
+}
 
-// Packed A operands (one bf16x8 per lane): layer 1 [t][lane], layers 2 and 3
-// [t][kb][lane].
-struct InferPack {
-  bf16x8 a1[4];
-  bf16x8 a2[4][2];
-  bf16x8 a3[4][2];
+// ---- inference with the network in LDS --------------------------------------
+
+// The network as a workgroup holds it in LDS: the packed A operands in the
+// layout of the global pack (layer 1 [t][lane], layers 2 and 3 [t][kb][lane],
+// one bf16x8 per lane, so a wave's operand read is one 1 KiB row without bank
+// conflicts), then the biases and the output weights.  Keeping the 80
+// operand registers out of the VGPR file lets a SIMD hold several waves, whose
+// voxel taps then overlap each other's matrix work.
+struct NetLds {
+  bf16x8 a[1280];
+  float b1[64], b2[64], b3[64], w4[64];
+  float b4;
 };
 
-__device__ __forceinline__ void load_pack(const bf16x8* pack, int lane, InferPack& w) {
-#pragma unroll
-  for (int t = 0; t < 4; ++t) w.a1[t] = pack[t * 64 + lane];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-      w.a2[t][kb] = pack[256 + (t * 2 + kb) * 64 + lane];
-      w.a3[t][kb] = pack[768 + (t * 2 + kb) * 64 + lane];
-    }
+// Whole workgroup: copies the network into LDS (the caller synchronizes).
+__device__ __forceinline__ void stage_net(const bf16x8* pack, const float* params, NetLds& n) {
+  for (uint32_t i = threadIdx.x; i < 1280u; i += blockDim.x) n.a[i] = pack[i];
+  for (uint32_t i = threadIdx.x; i < 64u; i += blockDim.x) {
+    n.b1[i] = params[kMlpB1 + i];
+    n.b2[i] = params[kMlpB2 + i];
+    n.b3[i] = params[kMlpB3 + i];
+    n.w4[i] = params[kMlpW4 + i];
+  }
+  if (threadIdx.x == 0) n.b4 = params[kMlpB4];
 }
 
 // Bias (or w4) values of the units a lane's D registers hold: 16t + 4g + r.
@@ -143,59 +173,108 @@ __device__ __forceinline__ f32x4 unit4(const float* v, int t, int g) {
   return f32x4{q.x, q.y, q.z, q.w};
 }
 
-// relu(acc) of tiles (2kb, 2kb + 1) -> bf16 slots (kb, e) of the next layer.
-__device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
-  bf16x8 r;
-  r[0] = to_bf16(fmaxf(lo[0], 0.0f));
-  r[1] = to_bf16(fmaxf(lo[1], 0.0f));
-  r[2] = to_bf16(fmaxf(lo[2], 0.0f));
-  r[3] = to_bf16(fmaxf(lo[3], 0.0f));
-  r[4] = to_bf16(fmaxf(hi[0], 0.0f));
-  r[5] = to_bf16(fmaxf(hi[1], 0.0f));
-  r[6] = to_bf16(fmaxf(hi[2], 0.0f));
-  r[7] = to_bf16(fmaxf(hi[3], 0.0f));
-  return r;
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// relu of two values, rounded to bf16 and packed (a in the low half).  The
+// ReLU acts on the rounded values as 16-bit integers: a negative bf16 (sign
+// bit set) is a negative integer and goes to +0, a non-negative one is kept,
+// which is relu-then-round for every input (one packed integer max instead of
+// two quieting float maxes per value).
+__device__ __forceinline__ uint32_t relu2(float a, float b) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  const bf16x2 v = __builtin_convertvector((f32x2{a, b}), bf16x2);
+  const s16x2 r = __builtin_elementwise_max(__builtin_bit_cast(s16x2, v), s16x2{0, 0});
+  return __builtin_bit_cast(uint32_t, r);
 }
 
-// Forward pass of 16 samples whose 8 feature slots per lane are in x;
-// returns the visibility of sample lane % 16 in every lane.
-__device__ __forceinline__ float mlp_forward16(const InferPack& w, const float* params, bf16x8 x, int g) {
-  const float* b1 = params + kMlpB1;
-  const float* b2 = params + kMlpB2;
-  const float* b3 = params + kMlpB3;
-  const float* w4 = params + kMlpW4;
-  f32x4 h1[4];
+// relu(acc) of tiles (2kb, 2kb + 1) -> bf16 slots (kb, e) of the next layer.
+__device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
+  const u32x4 w = {relu2(lo[0], lo[1]), relu2(lo[2], lo[3]), relu2(hi[0], hi[1]), relu2(hi[2], hi[3])};
+  return __builtin_bit_cast(bf16x8, w);
+}
+
+// The two bf16 halves of a packed word as floats.
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+
+constexpr int kChains = 4;  // 16-sample chains per forward call (64 samples)
+
+// Forward pass of four 16-sample chains; chain c's 8 feature slots per lane
+// are in x[c].  Each weight operand is read from LDS once and used by all
+// four chains.  Returns in p[c] the visibility of sample 16c + lane % 16, in
+// every lane group.
+__device__ __forceinline__ void mlp_forward64(const NetLds& n, const bf16x8 (&x)[kChains], int lane, int g,
+                                              float (&p)[kChains]) {
+  bf16x8 x2[kChains][2];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) h1[t] = mfma(w.a1[t], x, unit4(b1, t, g));
-  bf16x8 x2[2] = {relu_pack(h1[0], h1[1]), relu_pack(h1[2], h1[3])};
-  f32x4 h2[4];
+  for (int kb = 0; kb < 2; ++kb) {
+    f32x4 h[2][kChains];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    h2[t] = unit4(b2, t, g);
+    for (int u = 0; u < 2; ++u) {
+      const int t = 2 * kb + u;
+      const bf16x8 w = n.a[t * 64 + lane];
+      const f32x4 bias = unit4(n.b1, t, g);
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) h2[t] = mfma(w.a2[t][kb], x2[kb], h2[t]);
+      for (int c = 0; c < kChains; ++c) h[u][c] = mfma(w, x[c], bias);
+    }
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) x2[c][kb] = relu_pack(h[0][c], h[1][c]);
   }
-  bf16x8 x3[2] = {relu_pack(h2[0], h2[1]), relu_pack(h2[2], h2[3])};
-  f32x4 h3[4];
+  bf16x8 x3[kChains][2];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    h3[t] = unit4(b3, t, g);
+  for (int kb = 0; kb < 2; ++kb) {
+    f32x4 h[2][kChains];
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb) h3[t] = mfma(w.a3[t][kb], x3[kb], h3[t]);
+    for (int u = 0; u < 2; ++u) {
+      const int t = 2 * kb + u;
+      const f32x4 bias = unit4(n.b2, t, g);
+#pragma unroll
+      for (int c = 0; c < kChains; ++c) h[u][c] = bias;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const bf16x8 w = n.a[256 + (t * 2 + k) * 64 + lane];
+#pragma unroll
+        for (int c = 0; c < kChains; ++c) h[u][c] = mfma(w, x2[c][k], h[u][c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) x3[c][kb] = relu_pack(h[0][c], h[1][c]);
   }
   // Output unit: h3 rounded to bf16 like every hidden activation, dot with
   // the FP32 w4, summed over the four lane groups of the sample.
-  float z = 0.0f;
+  float z[kChains] = {};
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const f32x4 wv = unit4(w4, t, g);
+    const f32x4 bias = unit4(n.b3, t, g);
+    f32x4 h[kChains];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) z = __builtin_fmaf(wv[r], from_bf16(to_bf16(fmaxf(h3[t][r], 0.0f))), z);
+    for (int c = 0; c < kChains; ++c) h[c] = bias;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const bf16x8 w = n.a[768 + (t * 2 + k) * 64 + lane];
+#pragma unroll
+      for (int c = 0; c < kChains; ++c) h[c] = mfma(w, x3[c][k], h[c]);
+    }
+    const f32x4 wv = unit4(n.w4, t, g);
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) {
+      const uint32_t w01 = relu2(h[c][0], h[c][1]), w23 = relu2(h[c][2], h[c][3]);
+      z[c] = __builtin_fmaf(wv[0], bf16_lo(w01), z[c]);
+      z[c] = __builtin_fmaf(wv[1], bf16_hi(w01), z[c]);
+      z[c] = __builtin_fmaf(wv[2], bf16_lo(w23), z[c]);
+      z[c] = __builtin_fmaf(wv[3], bf16_hi(w23), z[c]);
+    }
   }
-  z += __shfl_xor(z, 16);
-  z += __shfl_xor(z, 32);
-  z += params[kMlpB4];
-  return 1.0f / (1.0f + __expf(-z));
+#pragma unroll
+  for (int c = 0; c < kChains; ++c) {
+    float s = z[c];
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    s += n.b4;
+    p[c] = 1.0f / (1.0f + __expf(-s));
+  }
 }
 
 struct InferArgs {
@@ -212,45 +291,52 @@ struct InferArgs {
 };
 
 constexpr uint32_t kMlpBlocksPerItem = 8;
-constexpr int kFeatLd = 40;  // bf16 per staged feature row (32 + pad)
+// Staged feature rows: 32 bf16 (four 16-byte chunks) per row, chunk q of
+// row r stored at chunk position q ^ ((r >> 2) & 3), so that the 16 rows a
+// chain's lanes read (and the 16 lanes of a row-store) fall on distinct banks.
+__device__ __forceinline__ int stage_chunk(int row, int q) { return row * 32 + 8 * (q ^ ((row >> 2) & 3)); }
+constexpr int kInferWaves = 4;  // waves per inference workgroup
 
-// Surrogate visibility for every (live cache, shadow block): persistent waves
-// over (64-cache tile, 8-block group) items.  Per block, each lane builds the
-// features of its own cache (the block record is wave-uniform) and stages
-// them in LDS as bf16; four 16-sample MFMA chains then evaluate the network,
-// and chain c leaves the result of sample 16c + j in lane (j, g = c), i.e.
-// in the lane that owns that cache, so the store is one coalesced row.
-__global__ __launch_bounds__(256) void k_visibility_mlp(InferArgs a) {
-  __shared__ __align__(16) __bf16 stage[4][64 * kFeatLd];
+// Surrogate visibility for every (live cache, shadow block): waves over
+// (64-cache tile, 8-block group) items, one workgroup's waves sharing one LDS
+// copy of the network.  Per block, each lane builds the features of its own
+// cache (the block record is wave-uniform) and stages them in LDS as bf16;
+// the four 16-sample MFMA chains then evaluate the network together, and
+// chain c's result for sample 16c + j is taken by lane (j, g = c), the lane
+// that owns that cache, so the store is one coalesced row.
+__global__ __launch_bounds__(64 * kInferWaves) void k_visibility_mlp(InferArgs a) {
+  __shared__ NetLds net;
+  __shared__ __align__(16) __bf16 stage[kInferWaves][64 * 32];
   __shared__ LevelInfo lvi[16];
-  stage_levels(a.fc.vox, lvi);
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
-  const int lane = static_cast<int>(threadIdx.x & 63u);
-  const int j = lane & 15, g = lane >> 4;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t n_waves = gridDim.x * kInferWaves;
   // Few caches (a spot-check set): one block per item, so more waves work.
   const uint32_t bpi = tiles * ((a.n_blocks + kMlpBlocksPerItem - 1) / kMlpBlocksPerItem) >= 2u * n_waves
                            ? kMlpBlocksPerItem
                            : 1u;
   const uint32_t groups = (a.n_blocks + bpi - 1) / bpi;
   const uint32_t items = tiles * groups;
+  if (blockIdx.x * kInferWaves >= items) return;  // workgroup-uniform: no wave of it has an item
+  stage_levels(a.fc.vox, lvi);
+  stage_net(a.pack, a.params, net);
+  __syncthreads();
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  const int j = lane & 15, g = lane >> 4;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kInferWaves + (threadIdx.x >> 6));
   if (wave >= items) return;
   __bf16* st = stage[threadIdx.x >> 6];
-  InferPack w;
-  load_pack(a.pack, lane, w);
   for (uint32_t item = wave; item < items; item += n_waves) {
     const uint32_t tile = item / groups, group = item % groups;
     const uint32_t cache = tile * 64u + static_cast<uint32_t>(lane);
     const bool valid = cache < live;
     const float* e = a.caches + static_cast<size_t>(valid ? cache : 0u) * a.stride;
-    const f3 wp = make3(e[0], e[1], e[2]);
+    const CacheFeatures cf = cache_features(a.fc, make3(e[0], e[1], e[2]));
     const uint32_t b0 = group * bpi;
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
       float f[32];
-      features_all(a.fc, lvi, wp, a.blocks[b], f);
+      features_pair(a.fc, lvi, cf, a.blocks[b], f);
       bf16x8 row[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -259,17 +345,16 @@ __global__ __launch_bounds__(256) void k_visibility_mlp(InferArgs a) {
       // the wave's own rows: all lanes store, then read other lanes' rows
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) *reinterpret_cast<bf16x8*>(st + lane * kFeatLd + 8 * q) = row[q];
+      for (int q = 0; q < 4; ++q) *reinterpret_cast<bf16x8*>(st + stage_chunk(lane, q)) = row[q];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      float mine = 0.0f;
+      bf16x8 x[kChains];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const bf16x8 x = *reinterpret_cast<const bf16x8*>(st + (16 * c + j) * kFeatLd + 8 * g);
-        const float p = mlp_forward16(w, a.params, x, g);
-        mine = (g == c) ? p : mine;
-      }
+      for (int c = 0; c < kChains; ++c) x[c] = *reinterpret_cast<const bf16x8*>(st + stage_chunk(16 * c + j, g));
+      float p[kChains];
+      mlp_forward64(net, x, lane, g, p);
+      const float mine = g == 0 ? p[0] : g == 1 ? p[1] : g == 2 ? p[2] : p[3];
       if (valid) a.vis[static_cast<size_t>(b) * a.pitch + cache] = mine;
     }
   }
@@ -336,25 +421,33 @@ __global__ void k_mlp_sample_pairs(const Counter* counter, uint32_t n_blocks, ui
   }
 }
 
-// Batch inference on given features (the same register-resident path).
-__global__ __launch_bounds__(256) void k_mlp_forward(const bf16x8* pack, const float* params, const float* x,
-                                                     uint32_t n, float* p_out) {
+// Batch inference on given features (the same LDS-resident forward pass,
+// 64 samples per wave pass).
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_forward(const bf16x8* pack, const float* params,
+                                                                  const float* x, uint32_t n, float* p_out) {
+  __shared__ NetLds net;
+  const uint32_t n_waves = gridDim.x * kInferWaves;
+  const uint32_t groups = (n + 63u) / 64u;
+  if (blockIdx.x * kInferWaves >= groups) return;  // workgroup-uniform
+  stage_net(pack, params, net);
+  __syncthreads();
   const int lane = static_cast<int>(threadIdx.x & 63u);
   const int j = lane & 15, g = lane >> 4;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t chains = (n + 15u) / 16u;
-  if (wave >= chains) return;
-  InferPack w;
-  load_pack(pack, lane, w);
-  for (uint32_t c = wave; c < chains; c += n_waves) {
-    const uint32_t s = c * 16u + static_cast<uint32_t>(j);
-    const bool valid = s < n;
-    bf16x8 xv;
+  const uint32_t wave = blockIdx.x * kInferWaves + (threadIdx.x >> 6);
+  for (uint32_t q = wave; q < groups; q += n_waves) {
+    bf16x8 xv[kChains];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) xv[k] = to_bf16(valid ? x[static_cast<size_t>(s) * 32 + 8 * g + k] : 0.0f);
-    const float p = mlp_forward16(w, params, xv, g);
-    if (g == 0 && valid) p_out[s] = p;
+    for (int c = 0; c < kChains; ++c) {
+      const uint32_t s = q * 64u + 16u * c + static_cast<uint32_t>(j);
+      const bool valid = s < n;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) xv[c][k] = to_bf16(valid ? x[static_cast<size_t>(s) * 32 + 8 * g + k] : 0.0f);
+    }
+    float p[kChains];
+    mlp_forward64(net, xv, lane, g, p);
+    const uint32_t s = q * 64u + 16u * g + static_cast<uint32_t>(j);
+    const float mine = g == 0 ? p[0] : g == 1 ? p[1] : g == 2 ? p[2] : p[3];
+    if (s < n) p_out[s] = mine;
   }
 }
 
@@ -842,7 +935,7 @@ void launch_visibility_mlp(const FeatureArgsHost& fh, const void* pack, const fl
   a.n_blocks = n_blocks;
   a.vis = vis;
   a.pitch = pitch;
-  hipLaunchKernelGGL(k_visibility_mlp, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_visibility_mlp, dim3(grid), dim3(64 * kInferWaves), 0, s, a);
 }
 
 void launch_mlp_sample_pairs(const Counter* counter, uint32_t n_blocks, uint32_t n, uint32_t seed, uint32_t* pairs,
@@ -874,7 +967,7 @@ void launch_mlp_features(const FeatureArgsHost& fh, const float* caches, int str
 void launch_mlp_forward(const void* pack, const float* params, const float* x, uint32_t n, float* p, int grid,
                         hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(256), 0, s, static_cast<const bf16x8*>(pack), params, x, n, p);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64 * kInferWaves), 0, s, static_cast<const bf16x8*>(pack), params, x, n, p);
 }
 
 void launch_mlp_batch_indices(uint32_t n, uint32_t batch, uint32_t seed, uint32_t* idx, bool all, hipStream_t s) {

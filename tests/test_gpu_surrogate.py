@@ -190,6 +190,35 @@ def test_surrogate_fallback_is_bit_exact(cornell):
     np.testing.assert_array_equal(both[:, 4:], 2.0 * traced[:, 4:])
 
 
+def test_surrogate_visibility_is_network_of_features(cornell):
+    """The fused inference kernel (features built in registers, network in
+    LDS, four chains per weight read) gives, for every (cache, block) pair,
+    what the feature kernel followed by the batch forward kernel gives."""
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    announce("surrogate visibility == forward(features) (cornell)")
+    r, df = _traced_frame(cornell)
+    _, live = r.cache_count()
+    params = init_params(4)
+    params[2048:2112] = np.random.default_rng(9).normal(0, 0.2, 64)
+    r.mlp_set_params(params)
+    r.set_surrogate(True, spot_fraction=0.0, fallback_threshold=1e9)
+    _light_again(r, df)
+    n_blocks = cornell.shadow_dls.shape[0] * cornell.shadow_dls.shape[1]  # one block per shadow-level texel
+    vis = r.read_visibility(live, n_blocks)
+    rng = np.random.default_rng(12)
+    n = 20000
+    pairs = np.stack([rng.integers(0, live, n), rng.integers(0, n_blocks, n)], 1).astype(np.int32)
+    xd = torch.empty((n, 32), device=DEV)
+    r.mlp_features(torch.from_numpy(pairs).to(DEV), xd)
+    pd = torch.full((n,), -1.0, device=DEV)
+    r.mlp_forward(xd, pd)
+    p = pd.cpu().numpy()
+    got = vis[pairs[:, 0], pairs[:, 1]]
+    assert np.all(np.isfinite(got)) and (got > 0).all() and (got < 1).all()
+    np.testing.assert_allclose(got, p, rtol=0, atol=1e-6)
+
+
 def test_surrogate_spot_error_matches_host(cornell):
     from dynamicradiancevolume_amd.surrogate import init_params
 
