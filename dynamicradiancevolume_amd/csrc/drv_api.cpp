@@ -24,6 +24,8 @@ struct drv_ctx {
   // back into `stream`, so it overlaps stage A (allocation).
   hipStream_t aux = nullptr;
   hipEvent_t fork_ev = nullptr, vox_done_ev = nullptr;
+  // The surrogate's spot check also runs on `aux`, beside the network pass.
+  hipEvent_t spot_fork_ev = nullptr, spot_done_ev = nullptr;
   bool vox_pending = false;
   int device = 0;
   int cu_count = 256;
@@ -241,7 +243,9 @@ int ensure(drv_ctx* ctx, T** p, size_t* cap, size_t need) {
 
 // Visibility of every (live cache, shadow block) from the network, checked on
 // a spot set of caches against the cone trace; the full cone trace runs
-// (device-gated) when the spot check fails.
+// (device-gated) when the spot check fails.  The spot check (a few latency-
+// bound launches over ~1 % of the caches) runs on the side stream while the
+// network pass over every cache fills the GPU; the gated trace waits for it.
 int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32_t interval, int grid) {
   const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
   const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
@@ -262,11 +266,14 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     float* set_b = ctx->spot_caches + spot_cap * stride;
     float* vis_a = ctx->spot_vis;
     float* vis_b = ctx->spot_vis + spot_cap * n_blocks;
-    drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, ctx->spot_caches, sc, ctx->spot_ctr, ctx->stream);
+    hipStream_t ss = ctx->aux;
+    DRV_HIP(ctx, hipEventRecord(ctx->spot_fork_ev, ctx->stream));
+    DRV_HIP(ctx, hipStreamWaitEvent(ss, ctx->spot_fork_ev, 0));
+    drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, ctx->spot_caches, sc, ctx->spot_ctr, ss);
     drvk::launch_visibility(ctx->vol, vox, set_a, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, vis_a, sc, nullptr,
-                            ctx->tune, grid, ctx->stream);
+                            ctx->tune, grid, ss);
     drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, set_b, stride, &ctx->spot_ctr[1], ctx->blocks,
-                                n_blocks, vis_b, sc, network_grid(ctx), ctx->stream);
+                                n_blocks, vis_b, sc, network_grid(ctx), ss);
     // The split partials of a small set fit many more splits than the
     // main list's (the gather writes one partial per group of 4 splits, and
     // the buffer holds partial_groups x capacity of them).
@@ -276,12 +283,13 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
       float* cs = set == 0 ? set_a : set_b;
       drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->vpls, n_vpl,
                           set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, spot_splits, grid, true,
-                          ctx->stream);
+                          ss);
       drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->partial, sc,
-                             ctx->cu_count * 8, ctx->stream);
+                             ctx->cu_count * 8, ss);
     }
     drvk::launch_spot_error(ctx->spot_caches, sc, stride, ctx->cfg.sh_bands, ctx->spot_ctr, ctx->sur.fallback_threshold,
-                            ctx->counter, ctx->spot_stats, &ctx->spot_ctr[2], ctx->stream);
+                            ctx->counter, ctx->spot_stats, &ctx->spot_ctr[2], ss);
+    DRV_HIP(ctx, hipEventRecord(ctx->spot_done_ev, ss));
     ctx->label_caches = set_a;
     ctx->label_counter = &ctx->spot_ctr[0];
     ctx->label_vis = vis_a;
@@ -291,9 +299,11 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
   }
   drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, ctx->caches, stride, ctx->counter, ctx->blocks,
                               n_blocks, ctx->visibility, static_cast<uint32_t>(cap), network_grid(ctx), ctx->stream);
-  if (spot)  // gate counter: live = 0 unless the spot check failed
+  if (spot) {  // gate counter: live = 0 unless the spot check failed
+    DRV_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->spot_done_ev, 0));
     drvk::launch_visibility(ctx->vol, vox, ctx->caches, stride, &ctx->spot_ctr[2], ctx->blocks, n_blocks,
                             ctx->visibility, static_cast<uint32_t>(cap), nullptr, ctx->tune, grid, ctx->stream);
+  }
   return DRV_OK;
 }
 
@@ -428,7 +438,9 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
     if (hipEventCreate(&e) != hipSuccess) return bail(DRV_ERR_HIP);
   if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->vox_done_ev, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&ctx->vox_done_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->spot_fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->spot_done_ev, hipEventDisableTiming) != hipSuccess)
     return bail(DRV_ERR_HIP);
   if (hipDeviceSynchronize() != hipSuccess) return bail(DRV_ERR_HIP);
   *out_ctx = ctx;
@@ -445,6 +457,8 @@ int drv_destroy(drv_ctx* ctx) {
   }
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->vox_done_ev) (void)hipEventDestroy(ctx->vox_done_ev);
+  if (ctx->spot_fork_ev) (void)hipEventDestroy(ctx->spot_fork_ev);
+  if (ctx->spot_done_ev) (void)hipEventDestroy(ctx->spot_done_ev);
   dev_free(ctx->caches);
   dev_free(ctx->counter);
   dev_free(ctx->phantoms);
