@@ -311,10 +311,11 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_visibility_mlp(InferArgs a
   const uint32_t live = a.counter->live;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t n_waves = gridDim.x * kInferWaves;
-  // Few caches (a spot-check set): one block per item, so more waves work.
-  const uint32_t bpi = tiles * ((a.n_blocks + kMlpBlocksPerItem - 1) / kMlpBlocksPerItem) >= 2u * n_waves
-                           ? kMlpBlocksPerItem
-                           : 1u;
+  // Blocks per item: the largest of 8, 4, 2 that still gives every wave
+  // two items or more (the cache record and its cache-only features are
+  // loaded once per item); a few caches (a spot-check set) get one.
+  uint32_t bpi = kMlpBlocksPerItem;
+  while (bpi > 1u && tiles * ((a.n_blocks + bpi - 1) / bpi) < 2u * n_waves) bpi >>= 1;
   const uint32_t groups = (a.n_blocks + bpi - 1) / bpi;
   const uint32_t items = tiles * groups;
   if (blockIdx.x * kInferWaves >= items) return;  // workgroup-uniform: no wave of it has an item
