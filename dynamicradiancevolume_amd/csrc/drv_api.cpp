@@ -258,47 +258,45 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     const size_t spot_cap = (cap + k - 1) / k;
     if ((rc = ensure(ctx, &ctx->spot_caches, &ctx->spot_caches_size, 2 * spot_cap * stride)) != DRV_OK) return rc;
     if ((rc = ensure(ctx, &ctx->spot_vis, &ctx->spot_vis_size, 2 * spot_cap * n_blocks)) != DRV_OK) return rc;
-    if (!ctx->spot_ctr && (rc = dev_alloc(ctx, &ctx->spot_ctr, 3)) != DRV_OK) return rc;
+    if (!ctx->spot_ctr && (rc = dev_alloc(ctx, &ctx->spot_ctr, 4)) != DRV_OK) return rc;
     if (!ctx->spot_stats && (rc = dev_alloc(ctx, &ctx->spot_stats, 4)) != DRV_OK) return rc;
     ctx->spot_cap = spot_cap;
     const uint32_t sc = static_cast<uint32_t>(spot_cap);
-    float* set_a = ctx->spot_caches;
-    float* set_b = ctx->spot_caches + spot_cap * stride;
-    float* vis_a = ctx->spot_vis;
-    float* vis_b = ctx->spot_vis + spot_cap * n_blocks;
+    // One list of both sets: set a (traced) at entries [0, n), set b (network)
+    // at [n, 2n), n = spot_ctr[0].live on the device; one visibility array of
+    // pitch 2 sc, so one SH gather and one reduction light both sets.
+    const uint32_t pitch2 = 2u * sc;
+    float* spot_list = ctx->spot_caches;
     hipStream_t ss = ctx->aux;
     DRV_HIP(ctx, hipEventRecord(ctx->spot_fork_ev, ctx->stream));
     DRV_HIP(ctx, hipStreamWaitEvent(ss, ctx->spot_fork_ev, 0));
-    drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, ctx->spot_caches, sc, ctx->spot_ctr, ss);
-    drvk::launch_visibility(ctx->vol, vox, set_a, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, vis_a, sc, nullptr,
-                            ctx->tune, grid, ss);
-    drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, set_b, stride, &ctx->spot_ctr[1], ctx->blocks,
-                                n_blocks, vis_b, sc, network_grid(ctx), ss);
-    // The split partials of a small set fit many more splits than the
+    drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, spot_list, sc, ctx->spot_ctr, ss);
+    drvk::launch_visibility(ctx->vol, vox, spot_list, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, ctx->spot_vis,
+                            pitch2, nullptr, ctx->tune, grid, ss);
+    drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, spot_list, stride, &ctx->spot_ctr[1],
+                                &ctx->spot_ctr[0], ctx->blocks, n_blocks, ctx->spot_vis, pitch2, network_grid(ctx), ss);
+    // The split partials of a small list fit many more splits than the
     // main list's (the gather writes one partial per group of 4 splits, and
     // the buffer holds partial_groups x capacity of them).
     const uint32_t spot_splits =
-        static_cast<uint32_t>(std::min<size_t>(drvk::kMaxSplits, 4 * ctx->partial_groups * cap / spot_cap));
-    for (int set = 0; set < 2; ++set) {
-      float* cs = set == 0 ? set_a : set_b;
-      drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->vpls, n_vpl,
-                          set == 0 ? vis_a : vis_b, sc, interval, 1, ctx->partial, sc, spot_splits, grid, true,
-                          ss);
-      drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, cs, stride, &ctx->spot_ctr[set], ctx->partial, sc,
-                             ctx->cu_count * 8, ss);
-    }
-    drvk::launch_spot_error(ctx->spot_caches, sc, stride, ctx->cfg.sh_bands, ctx->spot_ctr, ctx->sur.fallback_threshold,
+        static_cast<uint32_t>(std::min<size_t>(drvk::kMaxSplits, 4 * ctx->partial_groups * cap / (2 * spot_cap)));
+    drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, spot_list, stride, &ctx->spot_ctr[3], ctx->vpls, n_vpl,
+                        ctx->spot_vis, pitch2, interval, 1, ctx->partial, pitch2, spot_splits, grid, true, ss);
+    drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, spot_list, stride, &ctx->spot_ctr[3], ctx->partial, pitch2,
+                           ctx->cu_count * 8, ss);
+    drvk::launch_spot_error(spot_list, sc, stride, ctx->cfg.sh_bands, ctx->spot_ctr, ctx->sur.fallback_threshold,
                             ctx->counter, ctx->spot_stats, &ctx->spot_ctr[2], ss);
     DRV_HIP(ctx, hipEventRecord(ctx->spot_done_ev, ss));
-    ctx->label_caches = set_a;
+    ctx->label_caches = spot_list;
     ctx->label_counter = &ctx->spot_ctr[0];
-    ctx->label_vis = vis_a;
-    ctx->label_pitch = sc;
+    ctx->label_vis = ctx->spot_vis;
+    ctx->label_pitch = pitch2;
   } else {
     ctx->label_caches = nullptr;
   }
-  drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, ctx->caches, stride, ctx->counter, ctx->blocks,
-                              n_blocks, ctx->visibility, static_cast<uint32_t>(cap), network_grid(ctx), ctx->stream);
+  drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, ctx->caches, stride, ctx->counter, nullptr,
+                              ctx->blocks, n_blocks, ctx->visibility, static_cast<uint32_t>(cap), network_grid(ctx),
+                              ctx->stream);
   if (spot) {  // gate counter: live = 0 unless the spot check failed
     DRV_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->spot_done_ev, 0));
     drvk::launch_visibility(ctx->vol, vox, ctx->caches, stride, &ctx->spot_ctr[2], ctx->blocks, n_blocks,

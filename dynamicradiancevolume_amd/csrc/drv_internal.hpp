@@ -194,8 +194,8 @@ FeatureArgsHost make_feature_args(const drv_volume& vol, const float* cam, const
 size_t mlp_train_lds_bytes();
 void launch_mlp_pack(const float* params, void* pack, void* train_w, hipStream_t s);
 void launch_visibility_mlp(const FeatureArgsHost& fh, const void* pack, const float* params, const float* caches,
-                           int stride, const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks,
-                           float* vis, uint32_t pitch, int grid, hipStream_t s);
+                           int stride, const Counter* counter, const Counter* first, const ShadowBlock* blocks,
+                           uint32_t n_blocks, float* vis, uint32_t pitch, int grid, hipStream_t s);
 void launch_mlp_sample_pairs(const Counter* counter, uint32_t n_blocks, uint32_t n, uint32_t seed, uint32_t* pairs,
                              uint32_t* n_out, hipStream_t s);
 void launch_mlp_features(const FeatureArgsHost& fh, const float* caches, int stride, uint32_t n_caches,

@@ -284,6 +284,7 @@ struct InferArgs {
   const float* caches;
   int stride;
   const Counter* counter;
+  const Counter* first;  // optional: the list's caches start at entry first->live
   const ShadowBlock* blocks;
   uint32_t n_blocks;
   float* vis;
@@ -309,6 +310,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_visibility_mlp(InferArgs a
   __shared__ __align__(16) __bf16 stage[kInferWaves][64 * 32];
   __shared__ LevelInfo lvi[16];
   const uint32_t live = a.counter->live;
+  const uint32_t first = a.first ? a.first->live : 0u;
   const uint32_t tiles = (live + 63u) / 64u;
   const uint32_t n_waves = gridDim.x * kInferWaves;
   // Blocks per item: the largest of 8, 4, 2 that still gives every wave
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_visibility_mlp(InferArgs a
     const uint32_t tile = item / groups, group = item % groups;
     const uint32_t cache = tile * 64u + static_cast<uint32_t>(lane);
     const bool valid = cache < live;
-    const float* e = a.caches + static_cast<size_t>(valid ? cache : 0u) * a.stride;
+    const float* e = a.caches + static_cast<size_t>(first + (valid ? cache : 0u)) * a.stride;
     const CacheFeatures cf = cache_features(a.fc, make3(e[0], e[1], e[2]));
     const uint32_t b0 = group * bpi;
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_visibility_mlp(InferArgs a
       float p[kChains];
       mlp_forward64(net, x, lane, g, p);
       const float mine = g == 0 ? p[0] : g == 1 ? p[1] : g == 2 ? p[2] : p[3];
-      if (valid) a.vis[static_cast<size_t>(b) * a.pitch + cache] = mine;
+      if (valid) a.vis[static_cast<size_t>(b) * a.pitch + first + cache] = mine;
     }
   }
 }
@@ -816,12 +818,16 @@ __global__ void k_spot_select(const Counter* counter, uint32_t k, const float* c
   const uint32_t n_spot = min(spot_cap, (live + k - 1u) / k);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
-    for (int c = 0; c < 2; ++c) {
-      spot_ctr[c].groups_x = (n_spot + 63u) / 64u;
+    // [0], [1]: set a (traced) and set b (network), n_spot each; [3]: both
+    // sets as one list of 2 n_spot (set b follows set a); [2] is the gate.
+    for (int c = 0; c < 4; ++c) {
+      if (c == 2) continue;
+      const uint32_t n = c == 3 ? 2u * n_spot : n_spot;
+      spot_ctr[c].groups_x = (n + 63u) / 64u;
       spot_ctr[c].groups_y = spot_ctr[c].groups_z = 1u;
-      spot_ctr[c].total = n_spot;
-      spot_ctr[c].live = n_spot;
-      spot_ctr[c].tiles = (n_spot + kGatherTile - 1) / kGatherTile;
+      spot_ctr[c].total = n;
+      spot_ctr[c].live = n;
+      spot_ctr[c].tiles = (n + kGatherTile - 1) / kGatherTile;
       spot_ctr[c].splits = 1u;
       spot_ctr[c].phantoms = 0u;
     }
@@ -829,7 +835,7 @@ __global__ void k_spot_select(const Counter* counter, uint32_t k, const float* c
   if (i >= n_spot) return;
   const float* src = caches + static_cast<size_t>(i) * k * stride;
   float* a = spot + static_cast<size_t>(i) * stride;
-  float* b = spot + (static_cast<size_t>(spot_cap) + i) * stride;
+  float* b = spot + (static_cast<size_t>(n_spot) + i) * stride;
   for (int f = 0; f < stride; ++f) {
     const float v = f < 3 ? src[f] : 0.0f;
     a[f] = v;
@@ -851,7 +857,7 @@ __global__ __launch_bounds__(256) void k_spot_error(const float* spot, uint32_t 
   for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
     const uint32_t c = t / fields, f = 4u + t % fields;
     const float ref = spot[static_cast<size_t>(c) * stride + f];
-    const float net = spot[(static_cast<size_t>(spot_cap) + c) * stride + f];
+    const float net = spot[(static_cast<size_t>(n_spot) + c) * stride + f];
     const float d = net - ref;
     num = __builtin_fmaf(d, d, num);
     den = __builtin_fmaf(ref, ref, den);
@@ -923,8 +929,8 @@ void launch_mlp_pack(const float* params, void* pack, void* train_w, hipStream_t
 }
 
 void launch_visibility_mlp(const FeatureArgsHost& fh, const void* pack, const float* params, const float* caches,
-                           int stride, const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks,
-                           float* vis, uint32_t pitch, int grid, hipStream_t s) {
+                           int stride, const Counter* counter, const Counter* first, const ShadowBlock* blocks,
+                           uint32_t n_blocks, float* vis, uint32_t pitch, int grid, hipStream_t s) {
   InferArgs a;
   a.fc = to_ctx(fh);
   a.pack = static_cast<const bf16x8*>(pack);
@@ -932,6 +938,7 @@ void launch_visibility_mlp(const FeatureArgsHost& fh, const void* pack, const fl
   a.caches = caches;
   a.stride = stride;
   a.counter = counter;
+  a.first = first;
   a.blocks = blocks;
   a.n_blocks = n_blocks;
   a.vis = vis;
