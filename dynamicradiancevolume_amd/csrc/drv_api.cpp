@@ -14,6 +14,7 @@
 #include "../../include/drv.h"
 #include "drv_internal.hpp"
 #include "mat4.hpp"
+#include "split_plan.hpp"
 
 using drvk::Counter;
 
@@ -97,7 +98,9 @@ struct drv_ctx {
   uint32_t label_pitch = 0;
 
   drvk::Tuning tune;
-  size_t partial_groups = 0;  // groups of 4 VPL splits the partial buffer holds (x 27 x capacity)
+  // The split partial buffer: 27 rows of `partial_cols` floats (a gather
+  // with s splits at pitch p needs ceil(s / 4) x p columns).
+  size_t partial_cols = 0;
 
   // Indirect specular: atlas, work sums, spill; spec_first is set by the
   // allocation pass so that the frame's first light overwrites the sums.
@@ -241,6 +244,21 @@ int ensure(drv_ctx* ctx, T** p, size_t* cap, size_t need) {
   return rc;
 }
 
+// Every k-th cache is spot-checked, k = round(1 / spot fraction) clamped to
+// [1, capacity] (a fraction below 1 / capacity checks one cache).
+uint32_t spot_divisor(const drv_ctx* ctx) {
+  const double cap = static_cast<double>(ctx->cfg.max_caches);
+  const double k = std::round(1.0 / static_cast<double>(ctx->sur.spot_fraction));
+  return static_cast<uint32_t>(std::min(cap, std::max(1.0, k)));
+}
+
+// Caches per spot set (set a and set b each hold this many entries).
+size_t spot_set_capacity(const drv_ctx* ctx) {
+  const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
+  const size_t k = spot_divisor(ctx);
+  return (cap + k - 1) / k;
+}
+
 // Visibility of every (live cache, shadow block) from the network, checked on
 // a spot set of caches against the cone trace; the full cone trace runs
 // (device-gated) when the spot check fails.  The spot check (a few latency-
@@ -254,8 +272,8 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
   const bool spot = ctx->sur.spot_fraction > 0.0f;
   int rc;
   if (spot) {
-    const uint32_t k = std::max<uint32_t>(1u, static_cast<uint32_t>(std::lround(1.0 / ctx->sur.spot_fraction)));
-    const size_t spot_cap = (cap + k - 1) / k;
+    const uint32_t k = spot_divisor(ctx);
+    const size_t spot_cap = spot_set_capacity(ctx);
     if ((rc = ensure(ctx, &ctx->spot_caches, &ctx->spot_caches_size, 2 * spot_cap * stride)) != DRV_OK) return rc;
     if ((rc = ensure(ctx, &ctx->spot_vis, &ctx->spot_vis_size, 2 * spot_cap * n_blocks)) != DRV_OK) return rc;
     if (!ctx->spot_ctr && (rc = dev_alloc(ctx, &ctx->spot_ctr, 4)) != DRV_OK) return rc;
@@ -275,11 +293,11 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
                             pitch2, nullptr, ctx->tune, grid, ss);
     drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, spot_list, stride, &ctx->spot_ctr[1],
                                 &ctx->spot_ctr[0], ctx->blocks, n_blocks, ctx->spot_vis, pitch2, network_grid(ctx), ss);
-    // The split partials of a small list fit many more splits than the
-    // main list's (the gather writes one partial per group of 4 splits, and
-    // the buffer holds partial_groups x capacity of them).
-    const uint32_t spot_splits =
-        static_cast<uint32_t>(std::min<size_t>(drvk::kMaxSplits, 4 * ctx->partial_groups * cap / (2 * spot_cap)));
+    // A small list fits more splits into the partial buffer than the main
+    // list: the most (a power of two) whose ceil(s / 4) x pitch2 columns fit.
+    // drv_light_caches sized the buffer to hold at least one group.
+    const uint32_t spot_splits = drvk::max_fitting_splits(ctx->partial_cols, pitch2, drvk::kMaxSplits);
+    if (spot_splits == 0u) return fail(ctx, DRV_ERR_STATE, "light_caches: spot-check partials do not fit");
     drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, spot_list, stride, &ctx->spot_ctr[3], ctx->vpls, n_vpl,
                         ctx->spot_vis, pitch2, interval, 1, ctx->partial, pitch2, spot_splits, grid, true, ss);
     drvk::launch_reduce_sh(ctx->shc, ctx->cfg.sh_bands, spot_list, stride, &ctx->spot_ctr[3], ctx->partial, pitch2,
@@ -748,13 +766,18 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     if ((rc = dev_alloc(ctx, &ctx->visibility, cap * n_blocks)) != DRV_OK) return rc;
     ctx->vis_blocks_cap = n_blocks;
   }
-  // One partial of 27 sums per cache and group of 4 splits.
-  const size_t groups = (static_cast<size_t>(ctx->tune.gather_max_splits) + 3) / 4;
-  if (ctx->partial_groups < groups) {
+  // One partial of 27 sums per cache and group of 4 splits; with the
+  // surrogate's spot check the partials of its list (pitch 2 x set capacity)
+  // must fit at least one group too.
+  const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
+  size_t cols = drvk::gather_partial_columns(static_cast<uint32_t>(ctx->tune.gather_max_splits),
+                                             static_cast<uint32_t>(cap));
+  if (use_net && ctx->sur.spot_fraction > 0.0f) cols = std::max(cols, 2 * spot_set_capacity(ctx));
+  if (ctx->partial_cols < cols) {
     dev_free(ctx->partial);
-    ctx->partial_groups = 0;
-    if ((rc = dev_alloc(ctx, &ctx->partial, cap * 27 * groups)) != DRV_OK) return rc;
-    ctx->partial_groups = groups;
+    ctx->partial_cols = 0;
+    if ((rc = dev_alloc(ctx, &ctx->partial, cols * 27)) != DRV_OK) return rc;
+    ctx->partial_cols = cols;
   }
 
   const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
@@ -780,7 +803,6 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
                             specular ? ctx->spec.vpls_lanes : nullptr, &plan, ctx->stream);
   record(ctx, 6);
   join_voxels(ctx);
-  const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
   if (use_net) {
     if ((rc = surrogate_visibility(ctx, n_blocks, n_vpl, interval, grid)) != DRV_OK) return rc;
     steps = nullptr;
@@ -1323,6 +1345,14 @@ int bfail(drv_batch* b, int code, const std::string& msg) {
   return code;
 }
 
+// The fused queue cone-traces every pair; views that predict visibility
+// with the surrogate need their own passes (caches per wave 0).
+bool any_surrogate(const drv_batch* b) {
+  for (const drv_ctx* v : b->views)
+    if (v->sur.enabled && v->mlp_ready && v->cfg.indirect_shadow) return true;
+  return false;
+}
+
 bool is_capturing(hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
@@ -1386,6 +1416,8 @@ int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t wa
                  "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes), VPL tile a power of two >= 16");
   if (b->specular && cpw != 0)
     return bfail(b, DRV_ERR_INVALID_ARG, "batch_set_tuning: views with indirect_specular need per-view passes (0)");
+  if (any_surrogate(b) && cpw != 0)
+    return bfail(b, DRV_ERR_STATE, "batch_set_tuning: views with the surrogate enabled need per-view passes (0)");
   b->cpw = static_cast<uint32_t>(cpw);
   b->vpl_tile = static_cast<uint32_t>(vpl_tile);
   b->waves_per_cu = waves_per_cu;
@@ -1413,6 +1445,10 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
     }
     return DRV_OK;
   }
+  if (any_surrogate(b))
+    return bfail(b, DRV_ERR_STATE,
+                 "batch_light_caches: a view has the surrogate enabled; the fused queue only cone-traces "
+                 "(select caches per wave 0)");
   const bool capturing = is_capturing(b->stream);
   size_t partial_need = 1;
   std::vector<drvk::BatchView> hv(n);
@@ -1509,8 +1545,9 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   float best = 1e30f;
   uint32_t best_cpw = keep_cpw, best_tile = keep_tile;
   const uint32_t cpws[4] = {0, 64, 32, 16};
+  const bool per_view_only = b->specular || any_surrogate(b);
   for (uint32_t cpw : cpws) {
-    if (b->specular && cpw != 0) continue;  // env-map views: per-view passes only
+    if (per_view_only && cpw != 0) continue;  // env-map or surrogate views: per-view passes only
     // Per-view passes (cpw 0) have no VPL tile: one candidate.
     for (uint32_t tile = cpw == 0 ? max_vpl : 256; tile <= max_vpl; tile *= 2) {
       b->cpw = cpw;

@@ -35,6 +35,7 @@
 #include "k_sample.hpp"
 #include "k_sh.hpp"
 #include "k_trace.hpp"
+#include "split_plan.hpp"
 
 namespace drvk {
 namespace {
@@ -84,10 +85,7 @@ __device__ __forceinline__ float rg16f_texel(const uint16_t* img, int w, int i, 
 // each split a whole number of quanta (shadow blocks, at least 64 VPLs).
 __device__ __forceinline__ void plan_splits(Counter* ctr, uint32_t n_vpl, uint32_t quantum, uint32_t max_splits,
                                             uint32_t target_items) {
-  const uint32_t tiles64 = (ctr->live + 63u) / 64u;
-  uint32_t s = 1u;
-  while (s < max_splits && tiles64 * s < target_items && (n_vpl / (2u * s)) >= quantum) s *= 2u;
-  ctr->splits = s;
+  ctr->splits = plan_split_count(ctr->live, n_vpl, quantum, max_splits, target_items);
 }
 
 __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {

@@ -203,3 +203,36 @@ def test_graph_replay_matches_eager():
                 np.testing.assert_array_equal(s1, s0)
                 assert torch.equal(df.out, img0), f"replay {rep}"
     b.close()
+
+
+def test_fused_queue_refuses_surrogate_views():
+    """The fused queue cone-traces every pair: a batch with a view that has
+    the surrogate enabled must use per-view passes (caches per wave 0), so
+    the tuning and the light pass refuse the fused mode and the autotuner
+    keeps per-view passes."""
+    from dynamicradiancevolume_amd.pipeline import Batch, DrvError
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    announce("batch: fused queue refused with a surrogate view")
+    frames = _frames(["cornell_256", "cornell_wide"])
+    stream = torch.cuda.Stream()
+    rs, dfs = _group(frames, 1, stream)
+    b = Batch(rs)
+    b.set_tuning(64, 1024)
+    with torch.cuda.stream(stream):
+        b.render_frame(dfs)          # fused, no surrogate yet
+    rs[1].mlp_set_params(init_params(2))
+    rs[1].set_surrogate(True, spot_fraction=0.05, fallback_threshold=1e9)
+    with pytest.raises(DrvError):
+        b.light_caches(dfs)           # still set to the fused mode
+    with pytest.raises(DrvError):
+        b.set_tuning(32, 1024)
+    b.set_tuning(0, 1024)
+    with torch.cuda.stream(stream):
+        b.light_caches(dfs)
+        cpw, _, _ = b.autotune(dfs, reps=1)
+    assert cpw == 0 and b.tuning()[0] == 0
+    stream.synchronize()
+    b.close()
+    for r in rs:
+        r.close()

@@ -260,3 +260,45 @@ def test_surrogate_collect_and_finetune_on_spot_set(cornell):
     loss = r.mlp_finetune(4096, 1024, 5, 1e-3, 7)
     assert np.isfinite(loss) and loss > 0
     assert not np.array_equal(r.mlp_params(), before)
+
+
+def test_surrogate_spot_check_with_one_gather_split(cornell):
+    """Regression: a 5 % spot check with gather_max_splits 1 set before the
+    first frame (the partial buffer then holds one group of splits at the
+    capacity pitch, fewer columns than the spot list's largest split count
+    needs).  The spot gather's split cap now fits the buffer; the result
+    equals the default-tuned renderer's within the summation-order bound."""
+    from dynamicradiancevolume_amd.surrogate import init_params
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame
+
+    announce("surrogate spot check 5 % with gather_max_splits 1 (cornell)")
+    bands = cornell.workload.sh_bands
+    params = init_params(21)
+    out = []
+    for splits in (1, 256):
+        r = _renderer(cornell)
+        r.set_tuning(gather_max_splits=splits)
+        r.mlp_set_params(params)
+        r.set_surrogate(True, spot_fraction=0.05, fallback_threshold=1e9)
+        df = DeviceFrame(cornell)
+        r.render_frame(df)
+        _, live = r.cache_count()
+        st = r.surrogate_stats()
+        assert not st["fell_back"] and st["spot_caches"] == -(-live // 20)
+        out.append((_sh(r, live, bands).astype(np.float64), st["spot_error"]))
+        r.close()
+    assert rel_l2(out[0][0], out[1][0]) <= 1e-5
+    assert out[0][1] == pytest.approx(out[1][1], rel=1e-3, abs=1e-6)
+
+
+def test_surrogate_tiny_spot_fraction_checks_one_cache(cornell):
+    """A spot fraction below 1 / capacity spot-checks one cache (the divisor
+    is clamped to the capacity), not every cache."""
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    announce("surrogate spot fraction 1e-12 -> one cache")
+    r, df = _traced_frame(cornell)
+    r.mlp_set_params(init_params(5))
+    r.set_surrogate(True, spot_fraction=1e-12, fallback_threshold=1e9)
+    _light_again(r, df)
+    assert r.surrogate_stats()["spot_caches"] == 1
