@@ -856,6 +856,27 @@ float orc_cone_march_f32(const drv_volume* vol, int32_t res, const uint8_t* mips
   return s;
 }
 
+void orc_prepare_vpls(const drv_spot_light* light, const orc_rsm* rsm, float* out) {
+  std::vector<Vpl<float>> vpls;
+  prepare_vpls<float>(light, rsm, vpls);
+  for (size_t m = 0; m < vpls.size(); ++m) {
+    const Vpl<float>& v = vpls[m];
+    float* o = out + 12 * m;
+    o[0] = v.pos.x;
+    o[1] = v.pos.y;
+    o[2] = v.pos.z;
+    o[3] = v.disc_area;
+    o[4] = v.flux.x;
+    o[5] = v.flux.y;
+    o[6] = v.flux.z;
+    o[7] = 0.0f;
+    o[8] = v.normal.x;
+    o[9] = v.normal.y;
+    o[10] = v.normal.z;
+    o[11] = 0.0f;
+  }
+}
+
 void orc_shadow_blocks(const drv_spot_light* light, const orc_rsm* rsm, float* out) {
   std::vector<Block<float>> blocks;
   prepare_blocks<float>(light, rsm, blocks);

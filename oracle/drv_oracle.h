@@ -114,6 +114,10 @@ float orc_cone_trace_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t
  * (cacheLightingRSM.comp:195-230), FP32. */
 float orc_cone_march_f32(const drv_volume* vol, int32_t voxel_res, const uint8_t* voxel_mips, const float wp[3],
                          const float avg_pos[3], float rho, int32_t* steps_out);
+/* The LightInfo records of one light (cacheLightingRSM.comp:138-158), in
+ * Morton order: out[12 * n_vpl] = position x, y, z, DiscArea, flux r, g, b,
+ * 0, normal x, y, z, 0 (the layout the gather sums over). */
+void orc_prepare_vpls(const drv_spot_light* light, const orc_rsm* rsm, float* out);
 /* ---- surrogate visibility features (no counterpart in the reference) ---- */
 /* Shadow blocks of one light: per block the averaged VAL position and cone
  * aperture of cacheLightingRSM.comp:169-192, plus the normalized mean VPL

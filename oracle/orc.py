@@ -59,6 +59,7 @@ _FUNCS = {
                                        P(C.c_int32)]),
     "orc_entry_sh": (None, [P(C.c_float), C.c_int32, P(C.c_float)]),
     "orc_shadow_blocks": (None, [P(_abi.DrvSpotLight), P(OrcRsm), vp]),
+    "orc_prepare_vpls": (None, [P(_abi.DrvSpotLight), P(OrcRsm), vp]),
     "orc_direct_lighting": (None, [P(_abi.DrvConfig), P(_abi.DrvFrame), P(_abi.DrvSpotLight), vp, vp, vp, vp, vp,
                                    C.c_int32, C.c_float, C.c_float, vp, C.c_int32]),
     "orc_bruteforce_rsm": (None, [P(_abi.DrvConfig), P(_abi.DrvFrame), P(_abi.DrvSpotLight), P(OrcRsm), vp, vp, vp,
@@ -199,6 +200,14 @@ def gather_f64(cfg, vol, shc, light, fi, mips: np.ndarray, positions: np.ndarray
                          _ptr(vis) if vis is not None else None, n_threads)
     del keep
     return sh, vis
+
+
+def prepare_vpls(light, rsm: OrcRsm) -> np.ndarray:
+    """[n_vpl][12] LightInfo records in Morton order: position, DiscArea, flux, 0, normal, 0."""
+    n = int(rsm.read_resolution) ** 2
+    out = np.zeros((n, 12), np.float32)
+    lib().orc_prepare_vpls(C.byref(light), C.byref(rsm), _ptr(out))
+    return out
 
 
 def shadow_blocks(light, fi) -> np.ndarray:
