@@ -184,6 +184,7 @@ DRV_FUNCTIONS = {
     "drv_voxelize_end": (C.c_int, [vp, C.c_float]),
     "drv_allocate_caches": (C.c_int, [vp, P(DrvGBuffer)]),
     "drv_light_caches": (C.c_int, [vp, P(DrvRsm), P(DrvSpotLight)]),
+    "drv_light_caches_noalloc": (C.c_int, [vp, P(DrvRsm), P(DrvSpotLight)]),
     "drv_apply_caches": (C.c_int, [vp, P(DrvGBuffer), vp, C.c_int32]),
     "drv_get_specular_atlas": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
     "drv_read_specular_atlas": (C.c_int, [vp, C.c_int32, vp]),

@@ -24,7 +24,7 @@ BUILD = PKG / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DRV_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["k_allocate.hip", "k_voxel.hip", "k_gather.hip", "k_apply.hip", "k_mlp.hip", "k_batch.hip", "k_raster.hip", "k_bruteforce.hip", "k_direct.hip", "k_specular.hip", "drv_api.cpp"]
+HIP_SOURCES = ["k_allocate.hip", "k_voxel.hip", "k_gather.hip", "k_apply.hip", "k_mlp.hip", "k_batch.hip", "k_raster.hip", "k_bruteforce.hip", "k_direct.hip", "k_specular.hip", "k_static.hip", "drv_api.cpp"]
 
 # -ffp-contract=off: the integer decisions (cell, cascade, voxel coverage) must
 # round exactly as the oracle does; hot loops use explicit fmaf.  Packed FP32

@@ -359,6 +359,9 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
     return code;
   };
   if (hipGetDevice(&ctx->device) != hipSuccess) return bail(DRV_ERR_HIP);
+  // Device address of the allocation-free light pass's scratch array on this
+  // device (resolved once per device, with it current).
+  if (drvk::static_scratch_init(ctx->device) != hipSuccess) return bail(DRV_ERR_HIP);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->cu_count = prop.multiProcessorCount;
@@ -711,10 +714,9 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gb) {
 namespace {
 
 // Validation of one light's RSM (cacheLightingRSM.comp:80-163 needs a square
-// power-of-two read resolution; the 64-VPL chunking needs R >= 8) and the
-// VPL / shadow block buffers it needs.
-int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light, uint32_t* n_vpl_out,
-                  uint32_t* interval_out, uint32_t* n_blocks_out) {
+// power-of-two read resolution; the 64-VPL chunking needs R >= 8).
+int validate_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light, uint32_t* n_vpl_out,
+                   uint32_t* interval_out, uint32_t* n_blocks_out) {
   if (!ctx || !rsm || !light || !rsm->flux || !rsm->normal || !rsm->depth_lin_sq) return DRV_ERR_INVALID_ARG;
   if (!ctx->allocated) return fail(ctx, DRV_ERR_STATE, "light_caches: no allocation yet");
   const int R = rsm->read_resolution;
@@ -730,7 +732,19 @@ int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light,
       return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: inconsistent indirect shadow LOD");
   }
   const uint32_t n_blocks = shadow ? n_vpl / interval : 0u;
-  int rc;
+  *n_vpl_out = n_vpl;
+  *interval_out = shadow ? interval : 64u;
+  *n_blocks_out = n_blocks;
+  return DRV_OK;
+}
+
+// validate_light plus the VPL / shadow block buffers the light needs.
+int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light, uint32_t* n_vpl_out,
+                  uint32_t* interval_out, uint32_t* n_blocks_out) {
+  int rc = validate_light(ctx, rsm, light, n_vpl_out, interval_out, n_blocks_out);
+  if (rc != DRV_OK) return rc;
+  const uint32_t n_vpl = *n_vpl_out, n_blocks = *n_blocks_out;
+  const bool shadow = ctx->cfg.indirect_shadow != 0;
   if (n_vpl > ctx->vpl_cap) {
     dev_free(ctx->vpls);
     if ((rc = dev_alloc(ctx, &ctx->vpls, n_vpl)) != DRV_OK) return rc;
@@ -741,9 +755,6 @@ int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light,
     if ((rc = dev_alloc(ctx, &ctx->blocks, n_blocks)) != DRV_OK) return rc;
     ctx->block_cap = n_blocks;
   }
-  *n_vpl_out = n_vpl;
-  *interval_out = shadow ? interval : 64u;
-  *n_blocks_out = n_blocks;
   return DRV_OK;
 }
 
@@ -835,6 +846,46 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   ctx->counted_steps = steps != nullptr;
   ctx->last_n_blocks = n_blocks;
   return check_hip(ctx, hipGetLastError(), "light_caches");
+}
+
+// drv_light_caches without any device allocation: the fused work-queue kernel
+// with its records, descriptor and partials in the library's static scratch
+// array of this device (see k_static.hip).
+int drv_light_caches_noalloc(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light) {
+  uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
+  int rc = validate_light(ctx, rsm, light, &n_vpl, &interval, &n_blocks);
+  if (rc != DRV_OK) return rc;
+  if (n_vpl > drvk::kStaticMaxVpls)
+    return fail(ctx, DRV_ERR_INVALID_ARG,
+                "light_caches_noalloc: " + std::to_string(n_vpl) + " VPLs exceed the static scratch limit of " +
+                    std::to_string(drvk::kStaticMaxVpls) + " (RSM read resolution <= 256)");
+  if (ctx->cfg.indirect_specular)
+    return fail(ctx, DRV_ERR_STATE, "light_caches_noalloc: indirect specular needs drv_light_caches");
+  if (ctx->sur.enabled && ctx->mlp_ready && ctx->cfg.indirect_shadow)
+    return fail(ctx, DRV_ERR_STATE, "light_caches_noalloc: the surrogate needs drv_light_caches (this path cone-traces)");
+  int cur = -1;
+  DRV_HIP(ctx, hipGetDevice(&cur));
+  if (cur != ctx->device) return fail(ctx, DRV_ERR_STATE, "light_caches_noalloc: the context's device is not current");
+  unsigned char* base = drvk::static_scratch(ctx->device);
+  if (!base) return fail(ctx, DRV_ERR_STATE, "light_caches_noalloc: static scratch not resolved on this device");
+  const bool shadow = ctx->cfg.indirect_shadow != 0;
+  drvk::BatchView v;
+  std::memset(&v, 0, sizeof(v));
+  v.caches = ctx->caches;
+  v.counter = ctx->counter;
+  v.vox = voxel_view(ctx);
+  for (int i = 0; i < 3; ++i) v.vmin[i] = ctx->vol.volume_world_min[i];
+  v.voxel_size = ctx->vol.voxel_size_in_world;
+  v.n_vpl = n_vpl;
+  v.interval = interval;
+  join_voxels(ctx);
+  drvk::launch_light_static(base, *rsm, *light, v, drvk::entry_stride(ctx->cfg.sh_bands), ctx->cfg.sh_bands,
+                            shadow ? 1 : 0, ctx->shc, std::max(1, ctx->cu_count * 64 / 4), ctx->stream);
+  // No per-pair visibility or training labels are left behind.
+  ctx->last_n_blocks = 0;
+  ctx->label_caches = nullptr;
+  ctx->counted_steps = false;
+  return check_hip(ctx, hipGetLastError(), "light_caches_noalloc");
 }
 
 // ---- stage D ----------------------------------------------------------

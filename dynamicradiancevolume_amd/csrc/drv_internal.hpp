@@ -214,6 +214,19 @@ void launch_batch(BatchView* views, uint32_t n_views, BatchQueue* q, int stride,
                   uint32_t cpw, uint32_t vpl_tile, float* partial, size_t partial_floats,
                   const drv_sh_constants& shc, int grid, hipStream_t s);
 
+// Allocation-free light pass: a file-scope device array of
+// kStaticScratchBytes (VPL and shadow-block records of at most
+// kStaticMaxVpls VPLs, the queue descriptor and split partials).
+// static_scratch_init (once per device, with that device current) resolves
+// its device address; static_scratch returns it (null before init).
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+constexpr uint32_t kStaticMaxVpls = 65536;
+hipError_t static_scratch_init(int device);
+unsigned char* static_scratch(int device);
+void launch_light_static(unsigned char* base, const drv_rsm& rsm, const drv_spot_light& light, BatchView view,
+                         int stride, int sh_bands, int shadow, const drv_sh_constants& shc, int grid,
+                         hipStream_t s);
+
 void launch_spot_select(const Counter* counter, uint32_t k, const float* caches, int stride, float* spot,
                         uint32_t spot_cap, Counter* spot_ctr, hipStream_t s);
 void launch_spot_error(const float* spot, uint32_t spot_cap, int stride, int sh_bands, const Counter* spot_ctr,

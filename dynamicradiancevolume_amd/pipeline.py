@@ -147,6 +147,12 @@ class Renderer:
     def light_caches_rsm(self, rsm: _abi.DrvRsm, light: _abi.DrvSpotLight):
         self._check(self.lib.drv_light_caches(self.ctx, C.byref(rsm), C.byref(light)), "drv_light_caches")
 
+    def light_caches_noalloc(self, rsm: _abi.DrvRsm, light: _abi.DrvSpotLight):
+        """drv_light_caches_noalloc: the light pass with no device allocation
+        (static scratch of the library; at most 65536 VPLs)."""
+        self._check(self.lib.drv_light_caches_noalloc(self.ctx, C.byref(rsm), C.byref(light)),
+                    "drv_light_caches_noalloc")
+
     def apply_caches(self, gbuffer: _abi.DrvGBuffer, out: torch.Tensor, accumulate: bool = False):
         self._check(self.lib.drv_apply_caches(self.ctx, C.byref(gbuffer), out.data_ptr(), int(accumulate)),
                     "drv_apply_caches")

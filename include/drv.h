@@ -230,6 +230,19 @@ int drv_allocate_caches(drv_ctx* ctx, const drv_gbuffer* gbuffer);
  * shader/cacheLightingRSM.comp.  Results accumulate (+=) over calls. */
 int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light);
 
+/* drv_light_caches without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It lights the caches with the fused work-queue kernel (each (cache, shadow
+ * block) pair cone-traced inside the VPL sum, as drv_batch_light_caches does)
+ * and keeps the VPL and shadow-block records, its queue and its split partials
+ * in one static device array of 32 MiB per GPU, held from the first context on
+ * that GPU until the process ends.  Limits: at most 65536 VPLs (RSM read
+ * resolution <= 256; DRV_ERR_INVALID_ARG beyond); no indirect specular and no
+ * surrogate (DRV_ERR_STATE).  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int drv_light_caches_noalloc(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light);
+
 /* Renderer::ApplyCaches (renderer.cpp:1047-1079): shader/cacheApply.frag.
  * out_rgb: device float[3 * width * height]; accumulate != 0 adds to it
  * (the reference's GL_ONE/GL_ONE blend), 0 overwrites covered pixels and

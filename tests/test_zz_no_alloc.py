@@ -1,0 +1,161 @@
+"""drv_light_caches_noalloc: the light pass with no device allocation.
+
+* two calls in a row through the C ABI, the second with a smaller input
+  (a 1600x900 frame with 16384 VPLs, then the 256x256 Cornell frame with 4096
+  VPLs), each equal to drv_light_caches on an identical context within the
+  summation-order bound (same terms, fused kernel vs split kernels), and the
+  applied image within relative L2 1e-4;
+* a call on a fresh context leaves the device's free memory unchanged, while
+  drv_light_caches on a fresh context visibly allocates (the control);
+* inputs beyond the static scratch (more than 65536 VPLs) are refused with
+  DRV_ERR_INVALID_ARG, and surrogate / specular contexts with DRV_ERR_STATE.
+"""
+import ctypes as C
+import gc
+
+import numpy as np
+import pytest
+import torch
+
+from canon import rel_l2
+from conftest import announce, frame_inputs
+
+pytestmark = pytest.mark.gpu
+
+SH_TOL = 1e-5     # same terms summed in another order
+RGB_TOL = 1e-4
+
+
+def _by_position(r):
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    _, live = r.cache_count()
+    e = r.read_caches(live)
+    order = np.lexsort(e[:, :3].view(np.uint32).T)
+    return e[order, :3].copy(), entries_sh(e[order], r.cfg.sh_bands).astype(np.float64)
+
+
+def _frame(r, df, noalloc):
+    fi = df.inputs
+    w = fi.workload
+    r.set_frame(fi.frame())
+    r.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+    r.voxelize_scene(df.positions, len(fi.positions), adaption_factor=w.adaption_factor)
+    r.allocate_caches(df.gbuffer())
+    if noalloc:
+        r.light_caches_noalloc(df.rsm(), fi.light)
+    else:
+        r.light_caches_rsm(df.rsm(), fi.light)
+    r.apply_caches(df.gbuffer(), df.out)
+
+
+_big = []
+
+
+def _inputs():
+    from dynamicradiancevolume_amd import workloads as W
+
+    if not _big:
+        _big.append(W.make_inputs(W.BOXES_1080P.with_(name="boxes_900p", seed=6, width=1600, height=900,
+                                                      max_caches=65536)))
+    big = _big[0]
+    small = frame_inputs("cornell_256")
+    assert big.read_resolution ** 2 == 16384 and small.read_resolution ** 2 == 4096
+    assert W.BOXES_1080P.sh_bands == 2
+    return big, small
+
+
+def test_noalloc_matches_light_caches_on_two_calls():
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame, Renderer
+
+    for fi in _inputs():   # the second call has the smaller input
+        announce(f"noalloc vs light_caches: {fi.workload.name}")
+        ref = Renderer(fi.workload.config())
+        dref = DeviceFrame(fi)
+        _frame(ref, dref, noalloc=False)
+        got = Renderer(fi.workload.config())
+        dgot = DeviceFrame(fi)
+        _frame(got, dgot, noalloc=True)
+        torch.cuda.synchronize()
+        pr, sr = _by_position(ref)
+        pg, sg = _by_position(got)
+        assert np.array_equal(pr.view(np.uint32), pg.view(np.uint32))
+        assert np.abs(sr).sum() > 0
+        assert rel_l2(sg, sr) <= SH_TOL, rel_l2(sg, sr)
+        assert rel_l2(dgot.out.cpu().numpy(), dref.out.cpu().numpy()) <= RGB_TOL
+        ref.close()
+        got.close()
+
+
+def test_noalloc_allocates_nothing():
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame, Renderer
+
+    fi = frame_inputs("cornell_256")
+    announce("noalloc: device free memory unchanged on a fresh context")
+    # Load every code object the path launches before measuring.
+    warm = Renderer(fi.workload.config())
+    _frame(warm, DeviceFrame(fi), noalloc=True)
+    torch.cuda.synchronize()
+    warm.close()
+
+    def free_around(noalloc):
+        r = Renderer(fi.workload.config())
+        df = DeviceFrame(fi)
+        fi_ = df.inputs
+        w = fi_.workload
+        r.set_frame(fi_.frame())
+        r.update_volume(w.eye, fi_.bbox[:3], fi_.bbox[3:])
+        r.voxelize_scene(df.positions, len(fi_.positions), adaption_factor=w.adaption_factor)
+        r.allocate_caches(df.gbuffer())
+        torch.cuda.synchronize()
+        gc.collect()
+        before = torch.cuda.mem_get_info()[0]
+        if noalloc:
+            r.light_caches_noalloc(df.rsm(), fi_.light)
+        else:
+            r.light_caches_rsm(df.rsm(), fi_.light)
+        torch.cuda.synchronize()
+        after = torch.cuda.mem_get_info()[0]
+        _, live = r.cache_count()
+        assert live > 0
+        r.close()
+        return before, after
+
+    b, a = free_around(noalloc=True)
+    assert a >= b, (b, a)
+    # control: the allocating entry point grows its buffers on a fresh context
+    b, a = free_around(noalloc=False)
+    assert a < b - (32 << 20), (b, a)
+
+
+def test_noalloc_refuses_what_it_cannot_do():
+    from dynamicradiancevolume_amd import _abi
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame, DrvError, Renderer
+    from dynamicradiancevolume_amd.surrogate import init_params
+
+    fi = frame_inputs("cornell_256")
+    announce("noalloc: refusals")
+    r = Renderer(fi.workload.config())
+    df = DeviceFrame(fi)
+    _frame(r, df, noalloc=True)
+    # 512^2 VPLs exceed the scratch; the host refuses before any launch (the
+    # pointers are never read)
+    big = _abi.DrvSpotLight.from_buffer_copy(fi.light)
+    big.rsm_read_resolution = 512
+    rsm = df.rsm()
+    rsm.read_resolution = 512
+    rc = r.lib.drv_light_caches_noalloc(r.ctx, C.byref(rsm), C.byref(big))
+    assert rc == _abi.DRV_ERR_INVALID_ARG
+    assert b"65536" in r.lib.drv_last_error(r.ctx)
+    r.mlp_set_params(init_params(1))
+    r.set_surrogate(True, spot_fraction=0.05, fallback_threshold=1e9)
+    with pytest.raises(DrvError):
+        r.light_caches_noalloc(df.rsm(), fi.light)
+    r.close()
+    spec = Renderer(fi.workload.with_(specular=1).config())
+    spec.set_frame(fi.frame())
+    spec.update_volume(fi.workload.eye, fi.bbox[:3], fi.bbox[3:])
+    spec.allocate_caches(df.gbuffer())
+    with pytest.raises(DrvError):
+        spec.light_caches_noalloc(df.rsm(), fi.light)
+    spec.close()
