@@ -1,6 +1,6 @@
 """Named, seeded workloads and their pipeline inputs.
 
-Two single-view configurations of the indirect-lighting path:
+Single-view configurations of the indirect-lighting path:
 
 ``cornell_256``   Cornell box (5 walls, open front, 2 boxes; albedos U[0.2, 0.9]),
                   256x256 G-buffer, spot light with a 64x64 RSM read at LOD 0
@@ -9,6 +9,7 @@ Two single-view configurations of the indirect-lighting path:
 ``boxes_1080p``   2000 random boxes and ramps (~1M triangles) on a ground plane,
                   1920x1080 G-buffer, 256x256 RSM read at LOD 1 (16384 VPLs),
                   CAV 4 x 64^3, 128^3 voxels, SH L2, shadow LOD 2.
+``boxes_1080p_dense``  the same frame with CAV 4 x 240^3 (62,920 live caches).
 
 Inputs are produced by ``libdrv_inputs.so`` in the reference's texture
 formats (see ``csrc/inputs/drv_inputs.h``).  Everything is seeded.
@@ -96,7 +97,15 @@ BOXES_1080P = Workload(
     max_caches=262144,
 )
 
-WORKLOADS = {w.name: w for w in (CORNELL_256, BOXES_1080P)}
+# The boxes_1080p frame at the cache scale its configuration quotes (~60k
+# live caches).  With 64^3 cells per cascade a cell covers at least ~39
+# pixels across at 1080p for every surface inside its cascade (the cascade
+# reaches size/2 from the camera, a cell is size/64), so the 4 x 64^3 clipmap
+# holds only ~4-8k distinct surface caches in any 1080p view; 240^3 cells per
+# cascade give 62,920 live caches on this frame.
+BOXES_1080P_DENSE = BOXES_1080P.with_(name="boxes_1080p_dense", cav_resolution=240)
+
+WORKLOADS = {w.name: w for w in (CORNELL_256, BOXES_1080P, BOXES_1080P_DENSE)}
 
 # Batched views: resolutions of 16:9 frames from 720p to 4K, and RSM reads of
 # 4096 or 16384 VPLs (square power-of-two RSM reads, which the Morton order of

@@ -40,3 +40,8 @@ def cornell():
 @pytest.fixture(scope="session")
 def boxes():
     return frame_inputs("boxes_1080p")
+
+
+@pytest.fixture(scope="session")
+def boxes_dense():
+    return frame_inputs("boxes_1080p_dense")

@@ -56,7 +56,7 @@ def _assert_alloc_equal(r, fi, cfg, orc, capacity=None):
     return vol, count, cav, entries, g_entries, g_cav
 
 
-@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p"])
+@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p", "boxes_1080p_dense"])
 def test_volume_ubo_matches_oracle(name):
     orc = _orc()
     fi = frame_inputs(name)
@@ -69,7 +69,7 @@ def test_volume_ubo_matches_oracle(name):
     assert bytes(v) == bytes(o)
 
 
-@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p"])
+@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p", "boxes_1080p_dense"])
 def test_allocation_bit_exact(name):
     orc = _orc()
     fi = frame_inputs(name)
@@ -268,9 +268,10 @@ def _gather_case(name, n_sub=None, **overrides):
         gvis = r.read_visibility(live, nb)[ids]
         assert np.abs(gvis - vis).max() < VIS_TOL
         # A ray stopped at transmittance <= 2^-20 returns exactly 0; the
-        # oracle marches on, and its visibility there is at most 2^-20.
+        # oracle marches on, and its visibility there is at most 2^-20 up to
+        # the FP32 rounding of 1 - occlusion near 1 (a few ulp of 1 per step).
         zero = gvis == 0.0
-        assert zero.any() and (vis[zero] <= 2.0 ** -20).all()
+        assert zero.any() and (vis[zero] <= 2.0 ** -20 + 2e-6).all(), float(vis[zero].max())
     return r, df, fi, cfg, vol, live
 
 
@@ -294,6 +295,13 @@ def test_gather_boxes_subset():
     _gather_case("boxes_1080p", n_sub=384)
 
 
+def test_gather_boxes_dense_subset():
+    """The ~60k-cache frame: a spread subset of its caches against the oracle."""
+    announce("gather boxes_1080p_dense subset")
+    r, df, fi, cfg, vol, live = _gather_case("boxes_1080p_dense", n_sub=512)
+    assert live > 50000
+
+
 def test_gather_two_lights_accumulate():
     """LightCachesRSM runs once per light and accumulates (renderer.cpp:924-932)."""
     orc = _orc()
@@ -307,7 +315,7 @@ def test_gather_two_lights_accumulate():
     assert rel_l2(entries_sh(twice, 1), 2.0 * entries_sh(once, 1)) < SH_TOL
 
 
-@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p"])
+@pytest.mark.parametrize("name", ["cornell_256", "boxes_1080p", "boxes_1080p_dense"])
 def test_full_frame_matches_oracle(name):
     """Voxelize -> allocate -> gather -> apply on the GPU against the whole
     oracle frame (cells matched canonically, SH by cell, RGB per pixel)."""
