@@ -42,20 +42,43 @@ FLOP_CONE_STEP = 60               # per march step
 FINETUNE_SAMPLES, FINETUNE_BATCH, FINETUNE_STEPS, FINETUNE_LR = 8192, 2048, 4, 1e-4
 
 
+def _host_cpu() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "os_cpu_count": os.cpu_count(),
+            "sched_affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def _cpu_baseline(fi, r, n_threads: int, budget_s: float):
     """The oracle's FP32 gather (VPL sum + cone tracing) on a bounded sample of
-    the same frame's caches, timed on the host cores."""
+    the same frame's caches, timed on the host cores.  The oracle's other
+    stages of the frame (voxelization, allocation, apply) are timed once
+    beside it, so a whole-frame CPU time can be stated too."""
     import numpy as np
 
     from oracle import orc
 
     w = fi.workload
     cfg = w.config()
+    t0 = time.perf_counter()
     vol = orc.update_volume(cfg, w.eye, fi.bbox[:3], fi.bbox[3:])
     _, mips = orc.voxelize(w.voxel_resolution, vol, fi.positions, adaption_factor=w.adaption_factor)
+    t_vox = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    count, cav, o_entries = orc.allocate(cfg, vol, fi.frame(), fi.depth)
+    t_alloc = time.perf_counter() - t0
     _, live = r.cache_count()
     entries = r.read_caches(live)
     shc = orc.sh_constants()
+    t0 = time.perf_counter()
+    orc.apply(cfg, vol, fi.frame(), shc, fi, cav, o_entries, min(count, cfg.max_caches), n_threads=n_threads)
+    t_apply = time.perf_counter() - t0
     n_vpl = fi.read_resolution ** 2
     # Calibrate on a few caches, then size the sample to the time budget.
     probe = np.linspace(0, live - 1, min(live, n_threads * 2)).astype(np.uint32)
@@ -67,14 +90,70 @@ def _cpu_baseline(fi, r, n_threads: int, budget_s: float):
     t0 = time.perf_counter()
     orc.gather_f32(cfg, vol, shc, fi.light, fi, mips, entries.copy(), ids=ids, n_threads=n_threads)
     dt = time.perf_counter() - t0
+    gather_full_s = dt * live / n
+    frame_s = t_vox + t_alloc + gather_full_s + t_apply
     return {
         "value": n * n_vpl / dt,
         "unit": "cache.VPL interactions/s",
         "cores": n_threads,
         "kind": "port",
+        "stage": "gather only (VPL sum + cone-traced shadows); the GPU value is whole frames",
         "sample": f"{n} of {live} caches x {n_vpl} VPLs of one {w.name} frame (SH L{w.sh_bands}, cone-traced "
-                  f"shadows), oracle FP32 gather, {dt:.1f} s",
+                  f"shadows), oracle FP32 gather, {dt:.1f} s on {n_threads} threads",
+        "host": dict(_host_cpu(), threads_used=n_threads,
+                     threads_note="16 threads: the host CPU share of one GPU on this machine"),
+        "whole_frame": {"voxelize_s": round(t_vox, 3), "allocate_s": round(t_alloc, 3),
+                        "gather_s_extrapolated": round(gather_full_s, 3), "apply_s": round(t_apply, 3),
+                        "frame_s": round(frame_s, 3),
+                        "interactions_per_s": live * n_vpl / frame_s},
     }
+
+
+# SH-error statements against the FP64 oracle: the GPU results are captured
+# during the run (cache positions and SH of a seeded subset, the device's
+# voxel mips), and the oracle evaluates them on the host in the CPU leg at
+# the end, as a checker.
+_FP64_PENDING = []
+
+
+def _fp64_capture(r, fi, n_sub: int, seed: int) -> dict:
+    """Captures a seeded subset of the renderer's current caches (one light's
+    gather) for the FP64 statement; returns the placeholder the CPU leg fills
+    with the SH relative L2 against the FP64 oracle at the same positions,
+    voxels (read back from the device, bit-exact with the oracle's, checked by
+    the tests) and light."""
+    import numpy as np
+
+    from dynamicradiancevolume_amd.pipeline import entries_sh
+
+    w = fi.workload
+    _, live = r.cache_count()
+    e = r.read_caches(live)
+    rng = np.random.default_rng(seed)
+    ids = np.sort(rng.choice(live, size=min(n_sub, live), replace=False))
+    res = w.voxel_resolution
+    mips = np.concatenate([r.read_voxels(lv).reshape(-1) for lv in range(int(np.log2(res)) + 1)])
+    cfg = type(r.cfg).from_buffer_copy(r.cfg)
+    placeholder = {"caches": int(len(ids)), "seed": seed, "sh_rel_l2_vs_fp64": None}
+    _FP64_PENDING.append((placeholder, dict(cfg=cfg, vol=r.volume(), fi=fi, mips=mips, pos=e[ids, :3].copy(),
+                                            sh=entries_sh(e[ids], cfg.sh_bands).astype(np.float64))))
+    return placeholder
+
+
+def _fp64_leg(n_threads: int) -> float:
+    """Evaluates every captured subset with the oracle's FP64 gather."""
+    import numpy as np
+
+    from oracle import orc
+
+    t0 = time.perf_counter()
+    shc = orc.sh_constants()
+    for placeholder, c in _FP64_PENDING:
+        sh64, _ = orc.gather_f64(c["cfg"], c["vol"], shc, c["fi"].light, c["fi"], c["mips"], c["pos"],
+                                 n_threads=n_threads)
+        placeholder["sh_rel_l2_vs_fp64"] = float(np.linalg.norm(c["sh"] - sh64) / np.linalg.norm(sh64))
+    _FP64_PENDING.clear()
+    return time.perf_counter() - t0
 
 
 def _traffic(kernel: str):
@@ -138,7 +217,8 @@ def _sh_by_position(r, bands: int):
     return dict(zip(keys, entries_sh(e, bands).astype(np.float64)))
 
 
-def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, threshold: float, settle_s: float):
+def _surrogate(r, df, fi, stream, w, steps: int, warmup: int, traced_ms: float, threshold: float, settle_s: float,
+               n_threads: int):
     """The same frames with the surrogate visibility network (committed
     weights, 1 % spot check, fine-tuning every 16 frames)."""
     import numpy as np
@@ -170,31 +250,38 @@ def _surrogate(r, df, stream, w, steps: int, warmup: int, traced_ms: float, thre
     a = np.stack([sur[k] for k in keys])
     b = np.stack([ref[k] for k in keys])
     out["sh_rel_l2_vs_traced"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["fp64"] = _fp64_capture(r, fi, 512, 7)
     r.set_surrogate(False)
     return out
 
 
-def _batch(n_views: int, steps: int, stream):
+def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, fp64_caches: int = 64,
+           with_surrogate: bool = True):
     """The batched-views workload: `n_views` seeded views of the boxes family
     (720p to 4K, 4096 or 16384 VPLs, own scene each) rendered (a) view after
     view through the single-view path, (b) as one batch (autotuned work
-    decomposition), (c) as a replayed HIP graph of the batched frame."""
+    decomposition), (c) as a replayed HIP graph of the batched frame; then
+    (d) the same batch with surrogate visibility (one network for every view,
+    1 % spot check and fallback per view, fine-tuning every 16 frames).  The
+    SH error of every view against the FP64 oracle is stated for (b) and (d)
+    on a seeded subset of its caches."""
+    import numpy as np
     import torch
 
+    from dynamicradiancevolume_amd import surrogate as S
     from dynamicradiancevolume_amd import workloads as W
     from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, FrameGraph, Renderer
 
     views = W.batch_views(n_views, seed=0)
     frames = []
     for i, w in enumerate(views):
-        frames.append(W.make_inputs(w, n_threads=min(16, os.cpu_count() or 1)))
+        frames.append(W.make_inputs(w, n_threads=n_threads))
         if (i + 1) % 8 == 0:
             print(f"[bench] batch inputs {i + 1}/{n_views}", file=sys.stderr, flush=True)
     rs = [Renderer(w.config(), stream) for w in views]
     for r in rs:
         r.set_tuning(gather_max_splits=32)
     dfs = [DeviceFrame(fi) for fi in frames]
-    del frames
     torch.cuda.synchronize()
 
     def timed(fn, k):
@@ -208,6 +295,9 @@ def _batch(n_views: int, steps: int, stream):
         e1.synchronize()
         return e0.elapsed_time(e1) / k
 
+    def per_view_fp64():
+        return [_fp64_capture(r, fi, fp64_caches, 100 + i) for i, (r, fi) in enumerate(zip(rs, frames))]
+
     k = max(2, steps // 4)
     seq_ms = timed(lambda: [r.render_frame(df) for r, df in zip(rs, dfs)], k)
     lives = [r.cache_count()[1] for r in rs]
@@ -216,6 +306,7 @@ def _batch(n_views: int, steps: int, stream):
     b.render_frame(dfs)
     cpw, tile, tune_ms = b.autotune(dfs, reps=2)
     batch_ms = timed(lambda: b.render_frame(dfs), k)
+    traced_err = per_view_fp64()
     g = FrameGraph(rs[0])
     with torch.cuda.stream(stream):
         b.render_frame(dfs)
@@ -239,11 +330,43 @@ def _batch(n_views: int, steps: int, stream):
         "interactions_per_s_graph": interactions / (graph_ms * 1e-3),
         "autotuned": {"caches_per_wave": cpw, "vpl_tile": tile, "light_pass_ms": tune_ms},
         "frames_timed": k,
+        "sh_rel_l2_vs_fp64_per_view": traced_err,
+        "fp64_caches_per_view": fp64_caches,
     }
+    sur = None
+    if with_surrogate and S.WEIGHTS.exists():
+        print(f"[bench] batch of {n_views} with surrogate visibility", file=sys.stderr, flush=True)
+        sur = {"weights": S.WEIGHTS.name, "spot_fraction": 0.01, "finetune_every": 16, "views": n_views}
+        n_frames = 16
+        for label, thr in (("no_fallback", 1e30), ("fallback", threshold)):
+            b.set_surrogate(True, S.load_weights(), spot_fraction=0.01, fallback_threshold=thr)
+            b.render_frame(dfs)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for i in range(n_frames):
+                b.render_frame(dfs)
+                if (i + 1) % 16 == 0:
+                    with torch.cuda.stream(stream):
+                        b.mlp_finetune(8192, 2048, 4, 1e-4, 2000 + i)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / n_frames
+            st = [r.surrogate_stats() for r in rs]
+            sur[label] = {"ms_per_batch_frame": ms, "speedup_vs_batched_traced": batch_ms / ms, "threshold": thr,
+                          "frames_timed": n_frames,
+                          "views_fell_back_last_frame": int(sum(x["fell_back"] for x in st)),
+                          "spot_error_last_frame": [round(x["spot_error"], 5) for x in st]}
+        # one more frame without fallback: its per-view SH error against FP64
+        b.set_surrogate(True, S.load_weights(), spot_fraction=0.01, fallback_threshold=1e30)
+        b.render_frame(dfs)
+        stream.synchronize()
+        sur["sh_rel_l2_vs_fp64_per_view"] = per_view_fp64()
+        b.set_surrogate(False)
     b.close()
     for r in rs:
         r.close()
-    return out
+    return out, sur
 
 
 def _inputs_and_reference(fi, df, stream, w):
@@ -360,6 +483,72 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int, settle_s: 
     }
 
 
+def _frame_section(w, stream, steps: int, warmup: int, settle_s: float, n_threads: int):
+    """Whole frames of another single-view workload: ms/frame, interactions/s,
+    stage times, the k_visibility roofline and the FP64 SH error."""
+    import torch
+
+    from dynamicradiancevolume_amd import workloads as W
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame, Renderer
+
+    fi = W.make_inputs(w, n_threads=n_threads)
+    df = DeviceFrame(fi)
+    r = Renderer(w.config(), stream)
+    _settle(lambda: r.render_frame(df), stream, settle_s)
+    for _ in range(warmup):
+        r.render_frame(df)
+    torch.cuda.synchronize()
+    ms, _ = _time_frames(r, df, stream, steps)
+    _, live = r.cache_count()
+    n_vpl = fi.read_resolution ** 2
+    stages, traces, steps_, steps_exec = _stage_profile(r, df, steps)
+    vis_flop = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_
+    vis_exec = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_exec
+    vms = stages["visibility_ms"]
+    gms = stages["gather_ms"]
+    out = {
+        "workload": w.name,
+        "cav": f"{w.cav_cascades}x{w.cav_resolution}^3",
+        "live_caches": live,
+        "interactions_per_frame": live * n_vpl,
+        "ms_per_frame": ms,
+        "interactions_per_s": live * n_vpl / (ms * 1e-3),
+        "stage_ms": stages,
+        "cone_traces": traces,
+        "cone_steps": steps_,
+        "cone_steps_executed": steps_exec,
+        "k_visibility_tflops": vis_flop / (vms * 1e-3) / 1e12 if vms > 0 else None,
+        "k_visibility_frac": vis_flop / (vms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS if vms > 0 else None,
+        "k_visibility_frac_executed": vis_exec / (vms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS if vms > 0 else None,
+        "k_gather_frac": FLOP_SH[w.sh_bands] * live * n_vpl / (gms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS
+        if gms > 0 else None,
+        "fp64": _fp64_capture(r, fi, 512, 11),
+    }
+    r.close()
+    return out
+
+
+def _stage_profile(r, df, steps: int):
+    """Per-stage ms (HIP events inside the library, averaged over a few
+    frames) and the march-step counts of one more frame with counting on."""
+    r.set_profiling(1)
+    keys = ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
+            "gather_ms", "reduce_ms", "apply_ms")
+    acc = {k: 0.0 for k in keys}
+    n_prof = max(3, min(10, steps))
+    for _ in range(n_prof):
+        r.render_frame(df)
+        t = r.stage_times()
+        for k in keys:
+            acc[k] += t[k]
+    stages = {k: round(v / n_prof, 4) for k, v in acc.items()}
+    r.set_profiling(2)
+    r.render_frame(df)
+    t = r.stage_times()
+    r.set_profiling(0)
+    return stages, t["cone_traces"], t["cone_steps"], t["cone_steps_executed"]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +562,8 @@ def main() -> int:
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-specular", action="store_true")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched workload (0 = skip)")
+    ap.add_argument("--dense", default="boxes_1080p_dense",
+                    help="second single-view workload at the ~60k-cache scale ('' = skip)")
     ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -430,36 +621,28 @@ def main() -> int:
     # a profiled pass of the same frames; the march-step counts come from one
     # more frame with step counting on (which slows the trace, so it is not
     # timed).
-    r.set_profiling(1)
-    keys = ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
-            "gather_ms", "reduce_ms", "apply_ms")
-    acc = {k: 0.0 for k in keys}
-    n_prof = max(3, min(10, args.steps))
-    for _ in range(n_prof):
-        r.render_frame(df)
-        t = r.stage_times()
-        for k in keys:
-            acc[k] += t[k]
-    stages = {k: round(v / n_prof, 4) for k, v in acc.items()}
-    r.set_profiling(2)
-    r.render_frame(df)
-    t = r.stage_times()
-    traces, steps_, steps_exec = t["cone_traces"], t["cone_steps"], t["cone_steps_executed"]
-    r.set_profiling(0)
+    stages, traces, steps_, steps_exec = _stage_profile(r, df, args.steps)
+    n_threads = min(16, os.cpu_count() or 1)
+    plain_fp64 = _fp64_capture(r, fi, 512, 7)
 
     vis_flop = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_
+    vis_flop_exec = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_exec
     gather_flop = FLOP_SH[w.sh_bands] * interactions
     gather_name = f"k_gather<{'true' if w.sh_bands >= 2 else 'false'}, {'true' if w.indirect_shadow else 'false'}>"
     kernels = {
-        "k_visibility": (stages["visibility_ms"], vis_flop),
-        gather_name: (stages["gather_ms"], gather_flop),
+        "k_visibility": (stages["visibility_ms"], vis_flop, vis_flop_exec),
+        gather_name: (stages["gather_ms"], gather_flop, gather_flop),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
-    dms, dflop = kernels[dom]
+    dms, dflop, dflop_exec = kernels[dom]
     achieved = dflop / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
+    achieved_exec = dflop_exec / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
     roofline = {
-        "bound": "mfma",
-        "pipe": "fp32 vector ALU (compute-bound; the FP32 peak is the roofline)",
+        "bound": "valu",
+        "pipe": "fp32 vector ALU (VALU-issue-bound; the FP32 vector peak is the roofline)",
+        "frac_executed_steps": round(achieved_exec / PEAK_FP32_TFLOPS, 4),
+        "frac_note": "frac counts 60 FLOP for every march step the reference takes; frac_executed_steps counts "
+                     "only the steps sampled before a ray turns opaque",
         "kernel": dom,
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
@@ -474,7 +657,13 @@ def main() -> int:
     sur = None
     if not args.no_surrogate:
         print("[bench] surrogate", file=sys.stderr, flush=True)
-        sur = _surrogate(r, df, stream, w, args.steps, args.warmup, ms, args.sh_threshold, args.settle_s)
+        sur = _surrogate(r, df, fi, stream, w, args.steps, args.warmup, ms, args.sh_threshold, args.settle_s,
+                         n_threads)
+
+    dense = None
+    if args.dense:
+        print("[bench] dense address volume (~60k caches)", file=sys.stderr, flush=True)
+        dense = _frame_section(W.WORKLOADS[args.dense], stream, args.steps, args.warmup, args.settle_s, n_threads)
 
     spec = None
     if not args.no_specular:
@@ -485,15 +674,26 @@ def main() -> int:
     ref = _inputs_and_reference(fi, df, stream, w)
     ref["host_producer_s"] = host_inputs_s
 
-    batch = None
+    batch = batch_sur = None
     if args.batch_views > 0:
         print(f"[bench] batch of {args.batch_views} views", file=sys.stderr, flush=True)
-        batch = _batch(args.batch_views, args.steps, stream)
+        batch, batch_sur = _batch(args.batch_views, args.steps, stream, n_threads, args.sh_threshold,
+                                  with_surrogate=not args.no_surrogate)
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = _cpu_baseline(fi, r, n_threads=min(16, os.cpu_count() or 1), budget_s=args.cpu_budget)
+        print("[bench] cpu baseline and FP64 error statements (oracle on the host)", file=sys.stderr, flush=True)
+        r.render_frame(df)
+        stream.synchronize()
+        cpu = _cpu_baseline(fi, r, n_threads=n_threads, budget_s=args.cpu_budget)
+        cpu["fp64_statements_s"] = round(_fp64_leg(n_threads), 2)
+        for sec in (batch, batch_sur):
+            if sec and "sh_rel_l2_vs_fp64_per_view" in sec:
+                errs = [p["sh_rel_l2_vs_fp64"] for p in sec["sh_rel_l2_vs_fp64_per_view"]]
+                sec["sh_rel_l2_vs_fp64_per_view"] = [round(e, 7) for e in errs]
+                sec["sh_rel_l2_vs_fp64_max"] = max(errs)
+                sec["sh_rel_l2_vs_fp64_median"] = sorted(errs)[len(errs) // 2]
+    _FP64_PENDING.clear()
 
     line = {
         "metric": "cache.VPL interactions/s (whole frame, 1080p, 16k VPLs, cone-traced)",
@@ -524,14 +724,17 @@ def main() -> int:
             "cone_steps_executed": steps_exec,
             "stage_ms": stages,
             "path": "plain cone-traced",
+            "fp64": plain_fp64,
             "graph_replay_ms_per_frame": graph_ms,
             "clock_settle_s": args.settle_s,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "dense": dense,
         "specular": spec,
         "batch": batch,
+        "batch_surrogate": batch_sur,
         "inputs_and_reference": ref,
     }
     print(json.dumps(line))

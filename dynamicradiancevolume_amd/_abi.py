@@ -222,6 +222,9 @@ DRV_FUNCTIONS = {
     "drv_batch_set_tuning": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
     "drv_batch_get_tuning": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
     "drv_batch_light_caches": (C.c_int, [vp, vp, vp]),
+    "drv_batch_set_surrogate": (C.c_int, [vp, P(C.c_float), P(DrvSurrogate)]),
+    "drv_batch_mlp_finetune": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                                         P(C.c_float)]),
     "drv_batch_autotune": (C.c_int, [vp, vp, vp, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_float)]),
     "drv_graph_begin": (C.c_int, [vp]),
     "drv_graph_end": (C.c_int, [vp]),

@@ -1384,6 +1384,10 @@ struct drv_batch {
   drvk::BatchQueue* d_queue = nullptr;
   float* partial = nullptr;
   size_t partial_size = 0;  // floats
+  // Fine-tuning samples collected from every view's spot set.
+  float* ft_x = nullptr;
+  float* ft_y = nullptr;
+  size_t ft_x_cap = 0, ft_y_cap = 0;
   std::string err;
 };
 
@@ -1453,6 +1457,8 @@ int drv_batch_destroy(drv_batch* b) {
   dev_free(b->d_views);
   dev_free(b->d_queue);
   dev_free(b->partial);
+  dev_free(b->ft_x);
+  dev_free(b->ft_y);
   delete b;
   return DRV_OK;
 }
@@ -1636,6 +1642,67 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   if (cpw_out) *cpw_out = static_cast<int32_t>(b->cpw);
   if (tile_out) *tile_out = static_cast<int32_t>(b->vpl_tile);
   if (ms_out) *ms_out = best;
+  return DRV_OK;
+}
+
+int drv_batch_set_surrogate(drv_batch* b, const float* host_params, const drv_surrogate* settings) {
+  if (!b || !settings) return DRV_ERR_INVALID_ARG;
+  for (size_t v = 0; v < b->views.size(); ++v) {
+    drv_ctx* ctx = b->views[v];
+    int rc;
+    if (host_params && (rc = drv_mlp_set_params(ctx, host_params, static_cast<uint32_t>(drvk::kMlpParams))) != DRV_OK)
+      return bfail(b, rc, "batch_set_surrogate: view " + std::to_string(v) + ": " + ctx->err);
+    if ((rc = drv_set_surrogate(ctx, settings)) != DRV_OK)
+      return bfail(b, rc, "batch_set_surrogate: view " + std::to_string(v) + ": " + ctx->err);
+  }
+  // The fused queue only cone-traces: views with the network run their own
+  // passes (each with its spot check and device-gated fallback).
+  if (settings->enabled) b->cpw = 0;
+  return DRV_OK;
+}
+
+int drv_batch_mlp_finetune(drv_batch* b, uint32_t n_samples, uint32_t batch, uint32_t steps, float lr, uint32_t seed,
+                           float* mean_loss_host) {
+  if (!b || n_samples == 0 || batch == 0 || !(lr > 0.0f)) return DRV_ERR_INVALID_ARG;
+  const size_t n_views = b->views.size();
+  drv_ctx* owner = b->views[0];
+  int rc;
+  if ((rc = require_net(owner)) != DRV_OK) return bfail(b, rc, "batch_mlp_finetune: view 0: " + owner->err);
+  if ((rc = ensure(owner, &b->ft_x, &b->ft_x_cap, static_cast<size_t>(n_samples) * 32)) != DRV_OK ||
+      (rc = ensure(owner, &b->ft_y, &b->ft_y_cap, static_cast<size_t>(n_samples))) != DRV_OK)
+    return bfail(b, rc, "batch_mlp_finetune: sample buffers do not fit");
+  // Fresh cone-traced samples from every view that has labels (its spot set
+  // when the network lit it, else its whole cache list), an equal share each.
+  std::vector<size_t> labelled;
+  for (size_t v = 0; v < n_views; ++v)
+    if (b->views[v]->label_caches && b->views[v]->last_n_blocks > 0) labelled.push_back(v);
+  if (labelled.empty()) return bfail(b, DRV_ERR_STATE, "batch_mlp_finetune: no view has cone-traced labels");
+  uint32_t got = 0;
+  for (size_t i = 0; i < labelled.size(); ++i) {
+    const uint32_t share = n_samples / static_cast<uint32_t>(labelled.size()) +
+                           (i < n_samples % labelled.size() ? 1u : 0u);
+    if (share == 0) continue;
+    drv_ctx* ctx = b->views[labelled[i]];
+    if ((rc = drv_mlp_collect(ctx, share, seed * 131u + static_cast<uint32_t>(labelled[i]), b->ft_x + size_t(got) * 32,
+                              b->ft_y + got)) != DRV_OK)
+      return bfail(b, rc, "batch_mlp_finetune: view " + std::to_string(labelled[i]) + ": " + ctx->err);
+    got += share;
+  }
+  if ((rc = drv_mlp_train(owner, b->ft_x, b->ft_y, got, std::min(batch, got), steps, lr, seed ^ 0x5bd1e995u,
+                          mean_loss_host)) != DRV_OK)
+    return bfail(b, rc, "batch_mlp_finetune: " + owner->err);
+  // One network for the batch: view 0's trained parameters go to every view.
+  for (size_t v = 1; v < n_views; ++v) {
+    drv_ctx* ctx = b->views[v];
+    if ((rc = mlp_alloc(ctx)) != DRV_OK) return bfail(b, rc, "batch_mlp_finetune: view " + std::to_string(v));
+    if (hipMemcpyAsync(ctx->mlp_params, owner->mlp_params, sizeof(float) * drvk::kMlpParams, hipMemcpyDeviceToDevice,
+                       b->stream) != hipSuccess)
+      return bfail(b, DRV_ERR_HIP, "batch_mlp_finetune: parameter copy failed");
+    drvk::launch_mlp_pack(ctx->mlp_params, ctx->mlp_pack, ctx->mlp_train_w, b->stream);
+    ctx->mlp_ready = true;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, DRV_ERR_HIP, std::string("batch_mlp_finetune: ") + hipGetErrorString(e));
   return DRV_OK;
 }
 

@@ -415,6 +415,24 @@ class Batch:
                                                 C.byref(ms)), "drv_batch_autotune")
         return cpw.value, tile.value, ms.value
 
+    def set_surrogate(self, enabled: bool, params: np.ndarray | None = None, spot_fraction: float = 0.01,
+                      fallback_threshold: float = 0.05):
+        """drv_batch_set_surrogate: one network for every view (params, when
+        given), each view with its own spot check and fallback."""
+        s = _abi.DrvSurrogate(int(enabled), spot_fraction, fallback_threshold)
+        ptr = None
+        if params is not None:
+            p = np.ascontiguousarray(params, np.float32)
+            assert p.shape == (_abi.DRV_MLP_PARAMS,)
+            ptr = p.ctypes.data_as(C.POINTER(C.c_float))
+        self._check(self.lib.drv_batch_set_surrogate(self.handle, ptr, C.byref(s)), "drv_batch_set_surrogate")
+
+    def mlp_finetune(self, n_samples: int, batch: int, steps: int, lr: float, seed: int) -> float:
+        loss = C.c_float()
+        self._check(self.lib.drv_batch_mlp_finetune(self.handle, n_samples, batch, steps, lr, seed, C.byref(loss)),
+                    "drv_batch_mlp_finetune")
+        return loss.value
+
     def render_frame(self, frames: list, voxelize: bool = True):
         """Every view's Renderer::Draw, with the light pass batched."""
         for r, df in zip(self.renderers, frames):

@@ -454,6 +454,20 @@ int drv_batch_light_caches(drv_batch* batch, const drv_rsm* rsms, const drv_spot
 int drv_batch_autotune(drv_batch* batch, const drv_rsm* rsms, const drv_spot_light* lights, int32_t reps,
                        int32_t* caches_per_wave, int32_t* vpl_tile, float* ms);
 
+/* Surrogate visibility for every view of a batch: host_params (optional,
+ * DRV_MLP_PARAMS floats) become every view's network, settings every view's
+ * drv_set_surrogate.  With the network enabled the batch lights its views
+ * with their own passes (caches per wave 0): each view runs the network, its
+ * own spot check and its own device-gated fallback (drv_get_surrogate_stats
+ * of the view reports them). */
+int drv_batch_set_surrogate(drv_batch* batch, const float* host_params, const drv_surrogate* settings);
+/* Online fine-tuning of the batch's network: n_samples fresh cone-traced
+ * samples shared out over the views that have labels from the last light
+ * pass (their spot sets when the network lit them), `steps` Adam steps on
+ * view 0's network, whose parameters then go to every view. */
+int drv_batch_mlp_finetune(drv_batch* batch, uint32_t n_samples, uint32_t batch_size, uint32_t steps, float lr,
+                           uint32_t seed, float* mean_loss_host);
+
 /* ---- HIP graph of a frame ----------------------------------------------- */
 
 /* Captures everything the context's stream receives between begin and end
