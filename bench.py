@@ -369,6 +369,57 @@ def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, f
     return out, sur
 
 
+def _batch_small(n_views: int, steps: int, stream, n_threads: int):
+    """Many small views (320x180, 4096 VPLs each, own scene each): the light
+    pass of the whole batch view by view (per-view passes) against the split
+    queue and the fused queue, with the rest of each view's frame (voxelize,
+    allocate, apply) identical in all three."""
+    import torch
+
+    from dynamicradiancevolume_amd import workloads as W
+    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, Renderer
+
+    views = W.batch_views(n_views, seed=1, resolutions=W.SMALL_RESOLUTIONS, rsm=W.SMALL_RSM)
+    frames = [W.make_inputs(w.with_(max_caches=16384), n_threads=n_threads) for w in views]
+    rs = [Renderer(fi.workload.config(), stream) for fi in frames]
+    for r in rs:
+        r.set_tuning(gather_max_splits=32)
+    dfs = [DeviceFrame(fi) for fi in frames]
+    b = Batch(rs)
+    b.render_frame(dfs)
+    torch.cuda.synchronize()
+
+    def timed(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    k = max(4, steps)
+    out = {"views": n_views, "resolution": "320x180", "vpls_per_view": 4096}
+    lives = [r.cache_count()[1] for r in rs]
+    out["live_caches"] = [min(lives), max(lives), sum(lives)]
+    for label, cpw in (("per_view_passes", 0), ("split_queue", -1), ("fused_queue", 64)):
+        b.set_tuning(cpw, 1024)
+        out[label + "_light_ms"] = timed(lambda: b.light_caches(dfs), k)
+        out[label + "_frame_ms"] = timed(lambda: b.render_frame(dfs), k)
+    out["split_vs_per_view_light"] = out["per_view_passes_light_ms"] / out["split_queue_light_ms"]
+    cpw, tile, ms = b.autotune(dfs, reps=2)
+    out["autotuned"] = {"caches_per_wave": cpw, "vpl_tile": tile, "light_pass_ms": ms}
+    print(f"[bench] {n_views} small views: light pass per-view {out['per_view_passes_light_ms']:.2f} ms, "
+          f"split queue {out['split_queue_light_ms']:.2f} ms, fused {out['fused_queue_light_ms']:.2f} ms",
+          file=sys.stderr, flush=True)
+    b.close()
+    for r in rs:
+        r.close()
+    return out
+
+
 def _inputs_and_reference(fi, df, stream, w):
     """The gather's inputs rendered on the device (G-buffer, RSM + mips) and
     the brute-force RSM reference (unshadowed, diffuse) of the same frame,
@@ -562,6 +613,7 @@ def main() -> int:
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-specular", action="store_true")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched workload (0 = skip)")
+    ap.add_argument("--small-views", type=int, default=64, help="views of the small-view batch (0 = skip)")
     ap.add_argument("--dense", default="boxes_1080p_dense",
                     help="second single-view workload at the ~60k-cache scale ('' = skip)")
     ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
@@ -680,6 +732,11 @@ def main() -> int:
         batch, batch_sur = _batch(args.batch_views, args.steps, stream, n_threads, args.sh_threshold,
                                   with_surrogate=not args.no_surrogate)
 
+    batch_small = None
+    if args.small_views > 0:
+        print(f"[bench] batch of {args.small_views} small views", file=sys.stderr, flush=True)
+        batch_small = _batch_small(args.small_views, args.steps, stream, n_threads)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline and FP64 error statements (oracle on the host)", file=sys.stderr, flush=True)
@@ -735,6 +792,7 @@ def main() -> int:
         "specular": spec,
         "batch": batch,
         "batch_surrogate": batch_sur,
+        "batch_small": batch_small,
         "inputs_and_reference": ref,
     }
     print(json.dumps(line))

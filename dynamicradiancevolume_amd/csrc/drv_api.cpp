@@ -738,6 +738,31 @@ int validate_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light
   return DRV_OK;
 }
 
+// The visibility buffer (n_blocks rows of capacity floats) and the split
+// partials: one partial of 27 sums per cache and group of 4 splits; with the
+// surrogate's spot check the partials of its list (pitch 2 x set capacity)
+// must fit at least one group too.
+int ensure_light_buffers(drv_ctx* ctx, uint32_t n_blocks, bool use_net) {
+  const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
+  int rc;
+  if (ctx->cfg.indirect_shadow && n_blocks > ctx->vis_blocks_cap) {
+    dev_free(ctx->visibility);
+    ctx->vis_blocks_cap = 0;
+    if ((rc = dev_alloc(ctx, &ctx->visibility, cap * n_blocks)) != DRV_OK) return rc;
+    ctx->vis_blocks_cap = n_blocks;
+  }
+  size_t cols = drvk::gather_partial_columns(static_cast<uint32_t>(ctx->tune.gather_max_splits),
+                                             static_cast<uint32_t>(cap));
+  if (use_net && ctx->sur.spot_fraction > 0.0f) cols = std::max(cols, 2 * spot_set_capacity(ctx));
+  if (ctx->partial_cols < cols) {
+    dev_free(ctx->partial);
+    ctx->partial_cols = 0;
+    if ((rc = dev_alloc(ctx, &ctx->partial, cols * 27)) != DRV_OK) return rc;
+    ctx->partial_cols = cols;
+  }
+  return DRV_OK;
+}
+
 // validate_light plus the VPL / shadow block buffers the light needs.
 int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light, uint32_t* n_vpl_out,
                   uint32_t* interval_out, uint32_t* n_blocks_out) {
@@ -772,24 +797,8 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   if (ctx->profiling) join_voxels(ctx);
   const bool shadow = ctx->cfg.indirect_shadow != 0;
   const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
-  if (shadow && n_blocks > ctx->vis_blocks_cap) {
-    dev_free(ctx->visibility);
-    if ((rc = dev_alloc(ctx, &ctx->visibility, cap * n_blocks)) != DRV_OK) return rc;
-    ctx->vis_blocks_cap = n_blocks;
-  }
-  // One partial of 27 sums per cache and group of 4 splits; with the
-  // surrogate's spot check the partials of its list (pitch 2 x set capacity)
-  // must fit at least one group too.
   const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
-  size_t cols = drvk::gather_partial_columns(static_cast<uint32_t>(ctx->tune.gather_max_splits),
-                                             static_cast<uint32_t>(cap));
-  if (use_net && ctx->sur.spot_fraction > 0.0f) cols = std::max(cols, 2 * spot_set_capacity(ctx));
-  if (ctx->partial_cols < cols) {
-    dev_free(ctx->partial);
-    ctx->partial_cols = 0;
-    if ((rc = dev_alloc(ctx, &ctx->partial, cols * 27)) != DRV_OK) return rc;
-    ctx->partial_cols = cols;
-  }
+  if ((rc = ensure_light_buffers(ctx, n_blocks, use_net)) != DRV_OK) return rc;
 
   const int stride = drvk::entry_stride(ctx->cfg.sh_bands);
   const int grid = persistent_grid(ctx);
@@ -1375,7 +1384,7 @@ struct drv_batch {
   int sh_bands = 1;
   int shadow = 0;
   int cu_count = 256;
-  uint32_t cpw = 64;
+  int32_t cpw = 64;  // caches per wave of the fused queue; 0 per-view passes; DRV_BATCH_SPLIT_QUEUE
   uint32_t vpl_tile = 4096;
   int waves_per_cu = 64;
   bool specular = false;  // some view keeps env-maps: per-view passes only
@@ -1384,6 +1393,11 @@ struct drv_batch {
   drvk::BatchQueue* d_queue = nullptr;
   float* partial = nullptr;
   size_t partial_size = 0;  // floats
+  // Split queue: device view descriptors (and their last upload) and the
+  // item totals the plan kernel writes.
+  void* d_split = nullptr;
+  std::vector<uint8_t> h_split;
+  drvk::SplitTotals* d_totals = nullptr;
   // Fine-tuning samples collected from every view's spot set.
   float* ft_x = nullptr;
   float* ft_y = nullptr;
@@ -1411,6 +1425,89 @@ bool any_surrogate(const drv_batch* b) {
 bool is_capturing(hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+// The split queue: every view's buffers as its own light pass sizes them,
+// the device descriptors (uploaded when they change), then one launch per
+// stage over the items of all views.  Visibility and training labels stay
+// behind as after drv_light_caches.
+int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights) {
+  const size_t n = b->views.size();
+  const bool capturing = is_capturing(b->stream);
+  std::vector<drvk::SplitViewHost> hv(n);
+  uint32_t max_vpl = 64;
+  for (size_t v = 0; v < n; ++v) {
+    drv_ctx* ctx = b->views[v];
+    uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
+    const size_t vpl_had = ctx->vpl_cap, blk_had = ctx->block_cap, vis_had = ctx->vis_blocks_cap,
+                 cols_had = ctx->partial_cols;
+    int rc = prepare_light(ctx, &rsms[v], &lights[v], &n_vpl, &interval, &n_blocks);
+    if (rc == DRV_OK) rc = ensure_light_buffers(ctx, n_blocks, false);
+    if (rc != DRV_OK) return bfail(b, rc, "batch view " + std::to_string(v) + ": " + ctx->err);
+    if (capturing && (ctx->vpl_cap != vpl_had || ctx->block_cap != blk_had || ctx->vis_blocks_cap != vis_had ||
+                      ctx->partial_cols != cols_had))
+      return bfail(b, DRV_ERR_STATE, "batch_light_caches: buffers grew during graph capture");
+    const uint32_t cap = static_cast<uint32_t>(ctx->cfg.max_caches);
+    drvk::SplitViewHost& d = hv[v];
+    std::memset(&d, 0, sizeof(d));
+    d.caches = ctx->caches;
+    d.counter = ctx->counter;
+    d.vpls = ctx->vpls;
+    d.blocks = b->shadow ? ctx->blocks : nullptr;
+    d.vis = b->shadow ? ctx->visibility : nullptr;
+    d.vis_pitch = cap;
+    d.partial = ctx->partial;
+    d.partial_pitch = cap;
+    d.max_splits = std::max<uint32_t>(1u, drvk::max_fitting_splits(ctx->partial_cols, cap,
+                                                                   static_cast<uint32_t>(ctx->tune.gather_max_splits)));
+    d.vox = voxel_view(ctx);
+    for (int i = 0; i < 3; ++i) d.vmin[i] = ctx->vol.volume_world_min[i];
+    d.voxel_size = ctx->vol.voxel_size_in_world;
+    d.n_vpl = n_vpl;
+    d.interval = interval;
+    d.n_blocks = n_blocks;
+    d.rsm = rsms[v];
+    d.light = lights[v];
+    max_vpl = std::max(max_vpl, n_vpl);
+  }
+  const size_t bytes = drvk::split_desc_bytes() * n;
+  std::vector<uint8_t> img(bytes);
+  drvk::build_split_descs(hv.data(), static_cast<uint32_t>(n), img.data());
+  if (!b->d_split || !b->d_totals || img != b->h_split) {
+    if (capturing) return bfail(b, DRV_ERR_STATE, "batch_light_caches: view descriptors changed during graph capture");
+    if (!b->d_split && hipMalloc(&b->d_split, bytes) != hipSuccess)
+      return bfail(b, DRV_ERR_NO_MEMORY, "batch_light_caches: descriptors do not fit");
+    if (!b->d_totals && hipMalloc(reinterpret_cast<void**>(&b->d_totals), sizeof(drvk::SplitTotals)) != hipSuccess)
+      return bfail(b, DRV_ERR_NO_MEMORY, "batch_light_caches: descriptors do not fit");
+    // Ordered on the batch stream before this frame's kernels read them;
+    // earlier frames' kernels that read the old descriptors are ordered
+    // before the copy on the same stream.
+    if (hipMemcpyAsync(b->d_split, img.data(), bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return bfail(b, DRV_ERR_HIP, "batch_light_caches: descriptor upload failed");
+    b->h_split.swap(img);
+  }
+  for (size_t v = 0; v < n; ++v) join_voxels(b->views[v]);
+  drv_ctx* v0 = b->views[0];
+  const int grid = persistent_grid(v0);
+  drvk::launch_batch_split(b->d_split, static_cast<uint32_t>(n), b->d_totals, b->stride, b->sh_bands, b->shadow,
+                           max_vpl, static_cast<uint32_t>(grid) * 4u, v0->shc, grid, b->stream);
+  for (size_t v = 0; v < n; ++v) {
+    drv_ctx* ctx = b->views[v];
+    ctx->last_n_blocks = hv[v].n_blocks;
+    ctx->counted_steps = false;
+    if (b->shadow) {
+      ctx->label_caches = ctx->caches;
+      ctx->label_counter = ctx->counter;
+      ctx->label_vis = ctx->visibility;
+      ctx->label_pitch = static_cast<uint32_t>(ctx->cfg.max_caches);
+    } else {
+      ctx->label_caches = nullptr;
+    }
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, DRV_ERR_HIP, std::string("batch_light_caches: ") + hipGetErrorString(e));
+  return DRV_OK;
 }
 
 }  // namespace
@@ -1459,6 +1556,8 @@ int drv_batch_destroy(drv_batch* b) {
   dev_free(b->partial);
   dev_free(b->ft_x);
   dev_free(b->ft_y);
+  if (b->d_split) (void)hipFree(b->d_split);
+  dev_free(b->d_totals);
   delete b;
   return DRV_OK;
 }
@@ -1467,15 +1566,16 @@ const char* drv_batch_last_error(const drv_batch* b) { return b ? b->err.c_str()
 
 int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t waves_per_cu) {
   if (!b) return DRV_ERR_INVALID_ARG;
-  if ((cpw != 64 && cpw != 32 && cpw != 16 && cpw != 0) || !is_pow2(vpl_tile) || vpl_tile < 16 || waves_per_cu < 4 ||
-      waves_per_cu > 64)
+  if ((cpw != 64 && cpw != 32 && cpw != 16 && cpw != 0 && cpw != DRV_BATCH_SPLIT_QUEUE) || !is_pow2(vpl_tile) ||
+      vpl_tile < 16 || waves_per_cu < 4 || waves_per_cu > 64)
     return bfail(b, DRV_ERR_INVALID_ARG,
-                 "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes), VPL tile a power of two >= 16");
+                 "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes, -1: split queue), VPL tile a power "
+                 "of two >= 16");
   if (b->specular && cpw != 0)
     return bfail(b, DRV_ERR_INVALID_ARG, "batch_set_tuning: views with indirect_specular need per-view passes (0)");
   if (any_surrogate(b) && cpw != 0)
     return bfail(b, DRV_ERR_STATE, "batch_set_tuning: views with the surrogate enabled need per-view passes (0)");
-  b->cpw = static_cast<uint32_t>(cpw);
+  b->cpw = cpw;
   b->vpl_tile = static_cast<uint32_t>(vpl_tile);
   b->waves_per_cu = waves_per_cu;
   return DRV_OK;
@@ -1483,7 +1583,7 @@ int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t wa
 
 int drv_batch_get_tuning(const drv_batch* b, int32_t* cpw, int32_t* vpl_tile, int32_t* waves_per_cu) {
   if (!b) return DRV_ERR_INVALID_ARG;
-  if (cpw) *cpw = static_cast<int32_t>(b->cpw);
+  if (cpw) *cpw = b->cpw;
   if (vpl_tile) *vpl_tile = static_cast<int32_t>(b->vpl_tile);
   if (waves_per_cu) *waves_per_cu = b->waves_per_cu;
   return DRV_OK;
@@ -1504,8 +1604,9 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   }
   if (any_surrogate(b))
     return bfail(b, DRV_ERR_STATE,
-                 "batch_light_caches: a view has the surrogate enabled; the fused queue only cone-traces "
+                 "batch_light_caches: a view has the surrogate enabled; the fused and split queues only cone-trace "
                  "(select caches per wave 0)");
+  if (b->cpw == DRV_BATCH_SPLIT_QUEUE) return batch_split(b, rsms, lights);
   const bool capturing = is_capturing(b->stream);
   size_t partial_need = 1;
   std::vector<drvk::BatchView> hv(n);
@@ -1531,8 +1632,9 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
     d.n_vpl = n_vpl;
     d.interval = interval;
     const size_t splits = n_vpl / std::min<uint32_t>(b->vpl_tile, n_vpl);
-    const size_t tiles = (static_cast<size_t>(ctx->cfg.max_caches) + b->cpw - 1) / b->cpw;
-    if (splits > 1) partial_need += tiles * splits * 27 * b->cpw;
+    const size_t cpw = static_cast<size_t>(b->cpw);
+    const size_t tiles = (static_cast<size_t>(ctx->cfg.max_caches) + cpw - 1) / cpw;
+    if (splits > 1) partial_need += tiles * splits * 27 * cpw;
   }
   // Split partials are bounded (the plan kernel lengthens the VPL tile
   // when the live caches would need more).
@@ -1562,7 +1664,8 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
     ctx->counted_steps = false;
   }
   const int grid = std::max(1, b->cu_count * b->waves_per_cu / 4);
-  drvk::launch_batch(b->d_views, static_cast<uint32_t>(n), b->d_queue, b->stride, b->sh_bands, b->shadow, b->cpw,
+  drvk::launch_batch(b->d_views, static_cast<uint32_t>(n), b->d_queue, b->stride, b->sh_bands, b->shadow,
+                     static_cast<uint32_t>(b->cpw),
                      b->vpl_tile, b->partial, b->partial_size, b->views[0]->shc, grid, b->stream);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, DRV_ERR_HIP, std::string("batch_light_caches: ") + hipGetErrorString(e));
@@ -1598,15 +1701,17 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const uint32_t keep_cpw = b->cpw, keep_tile = b->vpl_tile;
+  const int32_t keep_cpw = b->cpw;
+  const uint32_t keep_tile = b->vpl_tile;
   float best = 1e30f;
-  uint32_t best_cpw = keep_cpw, best_tile = keep_tile;
-  const uint32_t cpws[4] = {0, 64, 32, 16};
+  int32_t best_cpw = keep_cpw;
+  uint32_t best_tile = keep_tile;
+  const int32_t cpws[5] = {0, DRV_BATCH_SPLIT_QUEUE, 64, 32, 16};
   const bool per_view_only = b->specular || any_surrogate(b);
-  for (uint32_t cpw : cpws) {
+  for (int32_t cpw : cpws) {
     if (per_view_only && cpw != 0) continue;  // env-map or surrogate views: per-view passes only
-    // Per-view passes (cpw 0) have no VPL tile: one candidate.
-    for (uint32_t tile = cpw == 0 ? max_vpl : 256; tile <= max_vpl; tile *= 2) {
+    // Per-view passes and the split queue have no VPL tile: one candidate.
+    for (uint32_t tile = cpw <= 0 ? max_vpl : 256; tile <= max_vpl; tile *= 2) {
       b->cpw = cpw;
       b->vpl_tile = tile;
       std::vector<float> t;
@@ -1639,7 +1744,7 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   (void)hipEventDestroy(e1);
   cleanup();
   if (rc != DRV_OK) return rc;
-  if (cpw_out) *cpw_out = static_cast<int32_t>(b->cpw);
+  if (cpw_out) *cpw_out = b->cpw;
   if (tile_out) *tile_out = static_cast<int32_t>(b->vpl_tile);
   if (ms_out) *ms_out = best;
   return DRV_OK;

@@ -111,6 +111,32 @@ struct BatchQueue {
   uint32_t pad;
 };
 
+// One view of the batched split pipeline, as the host fills it: the view's
+// cache list and buffers, its light (for the VPL and shadow-block records)
+// and the most VPL splits its partial buffer holds.
+struct SplitViewHost {
+  float* caches;
+  Counter* counter;
+  Vpl* vpls;
+  ShadowBlock* blocks;
+  float* vis;
+  uint32_t vis_pitch;
+  float* partial;
+  uint32_t partial_pitch;
+  uint32_t max_splits;
+  VoxelView vox;
+  float vmin[3];
+  float voxel_size;
+  uint32_t n_vpl, interval, n_blocks;
+  drv_rsm rsm;
+  drv_spot_light light;
+};
+
+// Item totals of the three queued launches of a batched split pass.
+struct SplitTotals {
+  uint32_t vis, gather, reduce, pad;
+};
+
 // Launch shapes chosen per context (drv_set_tuning).
 // Defaults measured on boxes_1080p (6352 caches x 16384 VPLs).
 struct Tuning {
@@ -226,6 +252,17 @@ unsigned char* static_scratch(int device);
 void launch_light_static(unsigned char* base, const drv_rsm& rsm, const drv_spot_light& light, BatchView view,
                          int stride, int sh_bands, int shadow, const drv_sh_constants& shc, int grid,
                          hipStream_t s);
+
+// Batched split pass: the device descriptors of n views (split_desc_bytes()
+// each, built on the host by build_split_descs and uploaded by the caller),
+// then one launch each for the VPL records of every view, the plan (split
+// counts and item offsets from the live counts), the cone-traced visibility,
+// the SH gather and the split reduction, each over the items of every view.
+size_t split_desc_bytes();
+void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_out);
+void launch_batch_split(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands, int shadow,
+                        uint32_t max_vpl, uint32_t target_items, const drv_sh_constants& shc, int grid,
+                        hipStream_t s);
 
 void launch_spot_select(const Counter* counter, uint32_t k, const float* caches, int stride, float* spot,
                         uint32_t spot_cap, Counter* spot_ctr, hipStream_t s);

@@ -18,12 +18,23 @@
 //   with one split per view the sums go straight into the entries, otherwise
 //   into per-item partials;
 // * k_batch_reduce sums the split partials of the views that have them.
+//
+// The split queue (caches per wave DRV_BATCH_SPLIT_QUEUE) runs the
+// single-view kernels' work items of every view in one launch per stage
+// instead: VPL records of every view, a one-thread plan (split counts and
+// item offsets from the live counts), then the cone-traced visibility, the SH
+// gather and the reduction, each persistent over the concatenated items of
+// all views.  Small views no longer leave the GPU idle behind their own
+// launches, and no view waits for another's tail.
 #include <algorithm>
+#include <cstring>
 
 #include "drv_internal.hpp"
 #include "k_common.hpp"
+#include "k_light.hpp"
 #include "k_sh.hpp"
 #include "k_trace.hpp"
+#include "split_plan.hpp"
 
 namespace drvk {
 namespace {
@@ -186,7 +197,255 @@ __global__ __launch_bounds__(256) void k_batch_reduce(BatchArgs a, int sh2) {
   }
 }
 
+// ---- split queue ---------------------------------------------------------------
+
+struct SplitDev {
+  PrepArgs prep;  // VPL and shadow-block records of the view's light
+  float* caches;
+  const Counter* counter;
+  float* vis;
+  float* partial;
+  VoxelView vox;
+  float vmin[3];
+  float voxel_size;
+  uint32_t vis_pitch, partial_pitch, max_splits;
+  uint32_t n_vpl, interval, n_blocks;
+  // written by the plan
+  uint32_t live, tiles, splits;
+  uint32_t vis_begin, gather_begin, reduce_begin;
+};
+
+__global__ __launch_bounds__(256) void k_split_prepare(const SplitDev* views) {
+  const PrepArgs& a = views[blockIdx.y].prep;
+  const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  prepare_vpl(a, m, n);
+}
+
+// Split counts and item offsets of every view: the smallest power of two s
+// of splits with sum(tiles) x s >= target_items (a view takes fewer when its
+// VPLs or its partial buffer allow fewer), so the gather fills the grid over
+// all views together instead of per view.  One workgroup: the views' live
+// counts are loaded in parallel (a chain of dependent loads per view in one
+// thread took ~1 us per view), then one thread walks the LDS copy.
+constexpr uint32_t kPlanChunk = 1024;
+__global__ __launch_bounds__(256) void k_split_plan(SplitDev* views, uint32_t n_views, uint32_t target_items,
+                                                    int shadow, int sh2, SplitTotals* totals) {
+  __shared__ uint32_t live_s[kPlanChunk];
+  __shared__ uint32_t all_tiles_s;
+  __shared__ uint32_t s_s;
+  __shared__ uint32_t run[3];  // running vis, gather, reduce offsets
+  if (threadIdx.x == 0) {
+    all_tiles_s = 0u;
+    run[0] = run[1] = run[2] = 0u;
+  }
+  __syncthreads();
+  // pass 1: tiles of every view, summed
+  for (uint32_t base = 0; base < n_views; base += kPlanChunk) {
+    const uint32_t m = min(kPlanChunk, n_views - base);
+    uint32_t sum = 0;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+      SplitDev& d = views[base + i];
+      d.live = d.counter->live;
+      d.tiles = (d.live + 63u) / 64u;
+      sum += d.tiles;
+    }
+    atomicAdd(&all_tiles_s, sum);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 1u;
+    while (s < kMaxSplits && static_cast<uint64_t>(all_tiles_s) * s < target_items) s *= 2u;
+    s_s = s;
+  }
+  __syncthreads();
+  const uint32_t n_jc = sh2 ? 27u : 12u;
+  // pass 2: per-view splits in parallel, offsets by one thread over LDS
+  for (uint32_t base = 0; base < n_views; base += kPlanChunk) {
+    const uint32_t m = min(kPlanChunk, n_views - base);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+      SplitDev& d = views[base + i];
+      const uint32_t quantum = d.interval > 64u ? d.interval : 64u;
+      d.splits = plan_split_count(d.live, d.n_vpl, quantum, min(s_s, d.max_splits), 0xffffffffu);
+      live_s[i] = (d.tiles << 9) | ((d.splits + 3u) / 4u);  // tiles < 2^23, groups <= 64
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t i = 0; i < m; ++i) {
+        SplitDev& d = views[base + i];
+        const uint32_t tiles = live_s[i] >> 9, groups = live_s[i] & 511u;
+        d.vis_begin = run[0];
+        d.gather_begin = run[1];
+        d.reduce_begin = run[2];
+        if (shadow) run[0] += tiles * d.n_blocks;
+        run[1] += tiles * groups;
+        run[2] += tiles * n_jc;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    totals->vis = run[0];
+    totals->gather = run[1];
+    totals->reduce = run[2];
+    totals->pad = 0u;
+  }
+}
+
+// The view an item of a stage belongs to: the largest v with begin(v) <= item
+// (views in order, offsets non-decreasing; a view without items of the stage
+// shares its offset with the next and is never returned for a valid item).
+template <int kStage>
+__device__ __forceinline__ uint32_t split_view(const SplitDev* views, uint32_t n_views, uint32_t item) {
+  uint32_t lo = 0, hi = n_views;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t b = kStage == 0 ? views[mid].vis_begin : kStage == 1 ? views[mid].gather_begin
+                                                                          : views[mid].reduce_begin;
+    if (b <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Cone-traced visibility of every view: items (view, shadow block, 64-cache
+// tile), block-major within a view, dealt round robin to the persistent
+// waves.  Each lane traces its cache towards the item's block with the code
+// of k_visibility, so the values are those of the single-view kernel.
+__global__ __launch_bounds__(256) void k_split_visibility(const SplitDev* views, uint32_t n_views, int stride,
+                                                          const SplitTotals* totals) {
+  __shared__ LevelInfo lvs[4][16];  // per wave: the voxel level table of its current view
+  const uint32_t lane = threadIdx.x & 63u;
+  LevelInfo* lvi = lvs[threadIdx.x >> 6];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t items = totals->vis;
+  uint32_t staged = 0xffffffffu;
+  for (uint32_t item = wave; item < items; item += n_waves) {
+    const uint32_t v = split_view<0>(views, n_views, item);
+    const SplitDev& d = views[v];
+    if (v != staged) {  // wave-local table: store, then read after a wavefront fence
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 16u) lvi[lane] = level_info(d.vox, static_cast<int>(lane));
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      staged = v;
+    }
+    const uint32_t local = item - d.vis_begin;
+    const uint32_t b = local / d.tiles, tile = local - b * d.tiles;
+    const uint32_t cache = tile * 64u + lane;
+    if (cache >= d.live) continue;
+    const float* e = d.caches + static_cast<size_t>(cache) * stride;
+    const f3 wp = make3(e[0], e[1], e[2]);
+    const float vsz = d.voxel_size * static_cast<float>(d.vox.res);
+    // cacheLightingRSM.comp:103-104
+    const f3 vpos = make3((wp.x - d.vmin[0]) / vsz, (wp.y - d.vmin[1]) / vsz, (wp.z - d.vmin[2]) / vsz);
+    const __amdgpu_buffer_rsrc_t cr = corner_rsrc(d.vox);
+    Trace t = trace_setup(d.vox, d.voxel_size, wp, vpos, d.prep.blocks[b]);
+    while (!trace_batch<1>(d.vox, cr, lvi, t)) {
+    }
+    d.vis[static_cast<size_t>(b) * d.vis_pitch + cache] = trace_visibility(t);
+  }
+}
+
+// SH gather of every view: workgroup items (view, 64-cache tile, group of 4
+// VPL splits), the single-view gather's item body.
+template <bool kSh2, bool kShadow>
+__global__ __launch_bounds__(256) void k_split_gather(const SplitDev* views, uint32_t n_views, int stride,
+                                                      const SplitTotals* totals) {
+  __shared__ float part[2][27][64];
+  const uint32_t items = totals->gather;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    const uint32_t v = split_view<1>(views, n_views, item);
+    const SplitDev& d = views[v];
+    const uint32_t groups = (d.splits + 3u) / 4u;
+    const uint32_t local = item - d.gather_begin;
+    const uint32_t tile = local / groups, group = local - tile * groups;
+    gather_item<kSh2, kShadow>(d.caches, stride, d.live, d.prep.vpls, d.n_vpl, d.vis, d.vis_pitch, d.interval,
+                               d.partial, d.partial_pitch, d.splits, tile, group, part);
+    __syncthreads();  // part is rewritten by the next item
+  }
+}
+
+// Split reduction of every view: workgroup items (view, coefficient x
+// channel, 64-cache tile), the single-view reduction's item body.
+__global__ __launch_bounds__(256) void k_split_reduce(const SplitDev* views, uint32_t n_views, int stride,
+                                                      const SplitTotals* totals, drv_sh_constants shc, int sh2) {
+  __shared__ float part[2][4][64];
+  const uint32_t items = totals->reduce;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    const uint32_t v = split_view<2>(views, n_views, item);
+    const SplitDev& d = views[v];
+    const uint32_t local = item - d.reduce_begin;
+    const uint32_t jc = local / d.tiles, tile = local - jc * d.tiles;
+    reduce_item(d.caches, stride, d.live, d.partial, d.partial_pitch, (d.splits + 3u) / 4u, tile, jc, shc, part);
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+size_t split_desc_bytes() { return sizeof(SplitDev); }
+
+void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_out) {
+  SplitDev* out = static_cast<SplitDev*>(host_out);
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const SplitViewHost& h = views[v];
+    SplitDev d;
+    std::memset(&d, 0, sizeof(d));
+    PrepArgs& a = d.prep;
+    for (int i = 0; i < 16; ++i) a.inv_lvp[i] = h.light.inverse_light_view_projection[i];
+    for (int i = 0; i < 3; ++i) a.light_pos[i] = h.light.light_position[i];
+    a.val_area_factor = h.light.val_area_factor;
+    a.sampling_offset = h.light.indirect_shadow_sampling_offset;
+    a.super_val_width = h.light.indirect_shadow_super_val_width;
+    a.flux = h.rsm.flux;
+    a.normal = h.rsm.normal;
+    a.dls = h.rsm.depth_lin_sq;
+    a.dls_shadow = h.rsm.depth_lin_sq_shadow;
+    a.R = h.rsm.read_resolution;
+    a.R_shadow = h.rsm.read_resolution >> static_cast<int>(h.light.indirect_shadow_lod);
+    a.interval = static_cast<uint32_t>(h.light.indirect_shadow_sample_interval);
+    a.block = 1 << static_cast<int>(h.light.indirect_shadow_lod);
+    a.flux_cut = 0;
+    a.vpls = h.vpls;
+    a.vpls_lanes = nullptr;
+    a.plan = nullptr;
+    a.blocks = h.blocks;
+    d.caches = h.caches;
+    d.counter = h.counter;
+    d.vis = h.vis;
+    d.partial = h.partial;
+    d.vox = h.vox;
+    for (int i = 0; i < 3; ++i) d.vmin[i] = h.vmin[i];
+    d.voxel_size = h.voxel_size;
+    d.vis_pitch = h.vis_pitch;
+    d.partial_pitch = h.partial_pitch;
+    d.max_splits = h.max_splits;
+    d.n_vpl = h.n_vpl;
+    d.interval = h.interval;
+    d.n_blocks = h.n_blocks;
+    out[v] = d;
+  }
+}
+
+void launch_batch_split(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands, int shadow,
+                        uint32_t max_vpl, uint32_t target_items, const drv_sh_constants& shc, int grid,
+                        hipStream_t s) {
+  SplitDev* d = static_cast<SplitDev*>(descs);
+  hipLaunchKernelGGL(k_split_prepare, dim3((max_vpl + 255u) / 256u, n_views), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_split_plan, dim3(1), dim3(256), 0, s, d, n_views, target_items, shadow, sh_bands >= 2 ? 1 : 0,
+                     totals);
+  if (shadow) hipLaunchKernelGGL(k_split_visibility, dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
+  const bool sh2 = sh_bands >= 2;
+  if (sh2 && shadow) hipLaunchKernelGGL((k_split_gather<true, true>), dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
+  else if (sh2) hipLaunchKernelGGL((k_split_gather<true, false>), dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
+  else if (shadow) hipLaunchKernelGGL((k_split_gather<false, true>), dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
+  else hipLaunchKernelGGL((k_split_gather<false, false>), dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
+  hipLaunchKernelGGL(k_split_reduce, dim3(grid), dim3(256), 0, s, d, n_views, stride, totals, shc, sh2 ? 1 : 0);
+}
 
 void launch_batch(BatchView* views, uint32_t n_views, BatchQueue* q, int stride, int sh_bands, int shadow,
                   uint32_t cpw, uint32_t vpl_tile, float* partial, size_t partial_floats,

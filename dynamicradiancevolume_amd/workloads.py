@@ -114,15 +114,22 @@ BATCH_RESOLUTIONS = ((1280, 720), (1600, 900), (1920, 1080), (2560, 1440), (3200
 BATCH_RSM = ((128, 1), (256, 1))  # (render resolution, read LOD) -> 64^2 or 128^2 VPLs
 
 
-def batch_views(n: int, seed: int = 0, family: Workload = BOXES_1080P) -> list:
+# Many small views (thumbnails, probes): 320x180 with 4096 VPLs each, where
+# one launch per view and stage leaves most of the GPU idle.
+SMALL_RESOLUTIONS = ((320, 180),)
+SMALL_RSM = ((128, 1),)
+
+
+def batch_views(n: int, seed: int = 0, family: Workload = BOXES_1080P, resolutions=BATCH_RESOLUTIONS,
+                rsm=BATCH_RSM) -> list:
     """`n` views of seeded scenes of `family`, each with its own scene seed,
     camera, resolution and VPL count (drawn with numpy's PCG64 from `seed`).
     Capacities are sized for the resolution."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        w_, h_ = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
-        rsm_res, lod = BATCH_RSM[int(rng.integers(len(BATCH_RSM)))]
+        w_, h_ = resolutions[int(rng.integers(len(resolutions)))]
+        rsm_res, lod = rsm[int(rng.integers(len(rsm)))]
         ang = float(rng.uniform(-0.6, 0.6))
         eye = (6.0 * math.sin(ang), float(rng.uniform(4.0, 8.0)), 6.0 * math.cos(ang))
         target = (float(rng.uniform(-3, 3)), 0.0, float(rng.uniform(-8, -4)))

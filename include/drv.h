@@ -433,6 +433,13 @@ int drv_get_tuning(const drv_ctx* ctx, drv_tuning* out);
  * rendering/renderer.cpp:539-594). */
 typedef struct drv_batch drv_batch;
 
+/* Caches-per-wave value of drv_batch_set_tuning that selects the split queue:
+ * the single-view pass's stages (VPL records, cone-traced visibility, SH
+ * gather, split reduction) each run as one persistent launch over the work
+ * items of every view, and the gather's VPL splits are planned over all views
+ * together. */
+#define DRV_BATCH_SPLIT_QUEUE (-1)
+
 int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out);
 int drv_batch_destroy(drv_batch* batch);
 const char* drv_batch_last_error(const drv_batch* batch);
@@ -442,8 +449,8 @@ const char* drv_batch_last_error(const drv_batch* batch);
  * VPLs is split into partial sums), persistent grid size in waves per CU.
  * Caches per wavefront 0 selects per-view passes instead of the fused queue:
  * each view's own cone trace, SH gather and reduction (drv_light_caches)
- * back to back on the shared stream, the better choice for views that fill
- * the GPU on their own. */
+ * back to back on the shared stream; DRV_BATCH_SPLIT_QUEUE selects the split
+ * queue (the VPL tile is then unused). */
 int drv_batch_set_tuning(drv_batch* batch, int32_t caches_per_wave, int32_t vpl_tile, int32_t waves_per_cu);
 int drv_batch_get_tuning(const drv_batch* batch, int32_t* caches_per_wave, int32_t* vpl_tile, int32_t* waves_per_cu);
 /* rsms[i], lights[i] belong to view i. */

@@ -139,7 +139,7 @@ def test_batch_view_matches_oracle():
     assert rel_l2(s1, entries_sh(e[order], 1)) <= 1e-4
 
 
-@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536), (0, 256)])
+@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536), (0, 256), (-1, 1024)])
 def test_batch_decompositions_agree(cpw, tile):
     from dynamicradiancevolume_amd.pipeline import Batch
 
@@ -159,6 +159,43 @@ def test_batch_decompositions_agree(cpw, tile):
         assert rel_l2(s1, s0) <= SH_TOL
 
 
+def test_split_queue_visibility_and_sh_equal_single_views():
+    """The split queue runs the single-view kernels' item bodies over all
+    views: every view's per-pair visibility is bit-identical to its own
+    drv_light_caches, and its SH differ only by the VPL split count (planned
+    over all views together)."""
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce(f"split queue of {len(frames)} views vs single views (visibility bit-exact)")
+    ref = []
+    rs1, dfs1 = _group(frames, 2)
+    for r, df, fi in zip(rs1, dfs1, frames):
+        r.render_frame(df)
+        _, live = r.cache_count()
+        nb = fi.read_resolution ** 2 // int(fi.light.indirect_shadow_sample_interval)
+        ref.append((_by_position(r), r.read_visibility(live, nb), r.read_caches(live)[:, :3].copy()))
+        r.close()
+    rs, dfs = _group(frames, 2)
+    b = Batch(rs)
+    b.set_tuning(-1, 1024)
+    b.render_frame(dfs)
+    for ((p0, s0), v0, pos0), r, fi in zip(ref, rs, frames):
+        p1, s1 = _by_position(r)
+        np.testing.assert_array_equal(p0, p1)
+        assert rel_l2(s1, s0) <= SH_TOL, fi.workload.name
+        _, live = r.cache_count()
+        nb = v0.shape[1]
+        # cache order by position (phantom caches append in atomic order;
+        # caches at one position trace the same rays)
+        pos1 = r.read_caches(live)[:, :3].copy()
+        o0 = np.lexsort(pos0.view(np.uint32).T)
+        o1 = np.lexsort(pos1.view(np.uint32).T)
+        np.testing.assert_array_equal(pos1[o1], pos0[o0])
+        np.testing.assert_array_equal(r.read_visibility(live, nb)[o1], v0[o0])
+    b.close()
+
+
 def test_batch_autotune_restores_entries():
     from dynamicradiancevolume_amd.pipeline import Batch
 
@@ -169,20 +206,22 @@ def test_batch_autotune_restores_entries():
     b.render_frame(dfs)
     before = [r.read_caches(r.cache_count()[1]) for r in rs]
     cpw, tile, ms = b.autotune(dfs, reps=1)
-    assert cpw in (64, 32, 16, 0) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
+    assert cpw in (64, 32, 16, 0, -1) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
     assert b.tuning()[:2] == (cpw, tile)
     for r, e in zip(rs, before):
         np.testing.assert_array_equal(r.read_caches(len(e)), e)
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("cpw", [64, -1])
+def test_graph_replay_matches_eager(cpw):
     from dynamicradiancevolume_amd.pipeline import Batch, FrameGraph
 
     frames = _frames()
-    announce("graph capture of a batched frame, 3 replays")
+    announce(f"graph capture of a batched frame (caches per wave {cpw}), 3 replays")
     stream = torch.cuda.Stream()
     rs, dfs = _group(frames, 2, stream)
     b = Batch(rs)
+    b.set_tuning(cpw, 1024)
     with torch.cuda.stream(stream):
         b.render_frame(dfs)           # eager frame (also grows every buffer)
         stream.synchronize()

@@ -85,7 +85,10 @@ def test_batch_surrogate_views_equal_single_views():
         one.render_frame(DeviceFrame(fi))
         np.testing.assert_array_equal(_sorted_entries(r), _sorted_entries(one))
         sb, so = r.surrogate_stats(), one.surrogate_stats()
-        assert sb == so and not sb["fell_back"] and sb["spot_caches"] > 0
+        assert sb["spot_caches"] == so["spot_caches"] > 0 and not sb["fell_back"] and not so["fell_back"]
+        # the spot set takes every k-th cache of the list, whose phantom
+        # caches (corners outside the volume) arrive in atomic order
+        assert sb["spot_error"] == pytest.approx(so["spot_error"], rel=1e-3)
         one.close()
     b.close()
 
