@@ -250,12 +250,12 @@ def _surrogate(r, df, fi, stream, w, steps: int, warmup: int, traced_ms: float, 
     a = np.stack([sur[k] for k in keys])
     b = np.stack([ref[k] for k in keys])
     out["sh_rel_l2_vs_traced"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    out["fp64"] = _fp64_capture(r, fi, 512, 7)
+    out["fp64"] = _fp64_capture(r, fi, 1 << 30, 7)
     r.set_surrogate(False)
     return out
 
 
-def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, fp64_caches: int = 64,
+def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, fp64_caches: int = 256,
            with_surrogate: bool = True):
     """The batched-views workload: `n_views` seeded views of the boxes family
     (720p to 4K, 4096 or 16384 VPLs, own scene each) rendered (a) view after
@@ -573,7 +573,7 @@ def _frame_section(w, stream, steps: int, warmup: int, settle_s: float, n_thread
         "k_visibility_frac_executed": vis_exec / (vms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS if vms > 0 else None,
         "k_gather_frac": FLOP_SH[w.sh_bands] * live * n_vpl / (gms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS
         if gms > 0 else None,
-        "fp64": _fp64_capture(r, fi, 512, 11),
+        "fp64": _fp64_capture(r, fi, 4096, 11),
     }
     r.close()
     return out
@@ -675,7 +675,7 @@ def main() -> int:
     # timed).
     stages, traces, steps_, steps_exec = _stage_profile(r, df, args.steps)
     n_threads = min(16, os.cpu_count() or 1)
-    plain_fp64 = _fp64_capture(r, fi, 512, 7)
+    plain_fp64 = _fp64_capture(r, fi, 1 << 30, 7)
 
     vis_flop = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_
     vis_flop_exec = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_exec
