@@ -79,7 +79,7 @@ def _cpu_baseline(fi, r, n_threads: int, budget_s: float):
     t0 = time.perf_counter()
     orc.apply(cfg, vol, fi.frame(), shc, fi, cav, o_entries, min(count, cfg.max_caches), n_threads=n_threads)
     t_apply = time.perf_counter() - t0
-    n_vpl = fi.read_resolution ** 2
+    n_vpl = fi.n_vpl
     # Calibrate on a few caches, then size the sample to the time budget.
     probe = np.linspace(0, live - 1, min(live, n_threads * 2)).astype(np.uint32)
     t0 = time.perf_counter()
@@ -301,7 +301,7 @@ def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, f
     k = max(2, steps // 4)
     seq_ms = timed(lambda: [r.render_frame(df) for r, df in zip(rs, dfs)], k)
     lives = [r.cache_count()[1] for r in rs]
-    interactions = sum(lv * df.inputs.read_resolution ** 2 for lv, df in zip(lives, dfs))
+    interactions = sum(lv * df.inputs.n_vpl for lv, df in zip(lives, dfs))
     b = Batch(rs)
     b.render_frame(dfs)
     cpw, tile, tune_ms = b.autotune(dfs, reps=2)
@@ -320,7 +320,7 @@ def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, f
     out = {
         "views": n_views,
         "resolutions": sorted({f"{w.width}x{w.height}" for w in views}),
-        "vpls_per_view": sorted({df.inputs.read_resolution ** 2 for df in dfs}),
+        "vpls_per_view": sorted({df.inputs.n_vpl for df in dfs}),
         "live_caches": [min(lives), max(lives), sum(lives)],
         "interactions_per_batch": interactions,
         "sequential_ms": seq_ms,
@@ -463,7 +463,7 @@ def _inputs_and_reference(fi, df, stream, w):
         "rsm_ms": rsm_ms,
         "triangles": int(len(fi.positions)),
         "bruteforce_ms": bf_ms,
-        "bruteforce_pixel_val_interactions_per_s": w.width * w.height * fi.read_resolution ** 2 / (bf_ms * 1e-3),
+        "bruteforce_pixel_val_interactions_per_s": w.width * w.height * fi.n_vpl / (bf_ms * 1e-3),
         "unshadowed_cache_path_vs_bruteforce_rel_l2": err,
         "lit_pixels": n_px,
     }
@@ -551,7 +551,7 @@ def _frame_section(w, stream, steps: int, warmup: int, settle_s: float, n_thread
     torch.cuda.synchronize()
     ms, _ = _time_frames(r, df, stream, steps)
     _, live = r.cache_count()
-    n_vpl = fi.read_resolution ** 2
+    n_vpl = fi.n_vpl
     stages, traces, steps_, steps_exec = _stage_profile(r, df, steps)
     vis_flop = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_
     vis_exec = FLOP_CONE_SETUP * traces + FLOP_CONE_STEP * steps_exec
@@ -637,7 +637,7 @@ def main() -> int:
     stream = torch.cuda.Stream()
     r = Renderer(w.config(), stream)
     torch.cuda.synchronize()
-    n_vpl = fi.read_resolution ** 2
+    n_vpl = fi.n_vpl
 
     settled = _settle(lambda: r.render_frame(df), stream, args.settle_s)
     print(f"[bench] clock settle {settled} frames, warmup {args.warmup}", file=sys.stderr, flush=True)

@@ -90,6 +90,7 @@ class DrvSpotParams(C.Structure):
         ("indirect_shadow_lod", C.c_uint32),
         ("near_plane", C.c_float),
         ("far_plane", C.c_float),
+        ("rsm_wide", C.c_uint32),
     ]
 
 
@@ -109,6 +110,8 @@ class DrvSpotLight(C.Structure):
         ("indirect_shadow_sample_interval", C.c_int32),
         ("indirect_shadow_super_val_width", C.c_float),
         ("indirect_shadow_sampling_offset", C.c_float),
+        ("rsm_render_height", C.c_int32),
+        ("rsm_read_height", C.c_int32),
     ]
 
 
@@ -130,6 +133,7 @@ class DrvRsm(C.Structure):
         ("depth_lin_sq", C.c_void_p),
         ("depth_lin_sq_shadow", C.c_void_p),
         ("read_resolution", C.c_int32),
+        ("read_height", C.c_int32),
     ]
 
 
@@ -238,7 +242,7 @@ INPUT_FUNCTIONS = {
     "drvin_camera": (C.c_int, [P(C.c_float), P(C.c_float), C.c_float, C.c_float, C.c_float, C.c_float, vp, vp]),
     "drvin_raster_gbuffer": (C.c_int, [vp, vp, vp, C.c_int64, vp, C.c_int32, C.c_int32, vp, vp, vp, C.c_int32]),
     "drvin_raster_rsm": (C.c_int, [vp, vp, vp, C.c_int64, P(DrvSpotLight), vp, vp, vp, C.c_int32]),
-    "drvin_rsm_downsample": (C.c_int, [C.c_int32, vp, vp, vp, vp, vp, vp]),
+    "drvin_rsm_downsample": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
     "drvin_f32_to_f16": (C.c_uint16, [C.c_float]),
     "drvin_f16_to_f32": (C.c_float, [C.c_uint16]),
     "drvin_pack_normal16i": (None, [P(C.c_float), P(C.c_int16)]),

@@ -632,12 +632,20 @@ int drv_prepare_spot_light(const drv_spot_params* p, drv_spot_light* o) {
   const int read_res = static_cast<int>(res_pow2 / std::pow(2.0, static_cast<double>(p->rsm_read_lod)));
   if (read_res < 1) return DRV_ERR_INVALID_ARG;
   o->rsm_read_resolution = read_res;
+  // A 2:1 RSM keeps the square light frustum and halves the rows.
+  const bool wide = p->rsm_wide != 0;
+  o->rsm_render_height = wide ? res_pow2 / 2 : res_pow2;
+  const int read_h = wide ? read_res / 2 : read_res;
+  if (read_h < 1) return DRV_ERR_INVALID_ARG;
+  o->rsm_read_height = read_h;
+  // The solid angle of one read texel at unit distance (renderer.cpp:700-705
+  // for a square read: the clip width squared over the texel count).
   const float clip_w = std::sin(p->half_angle) * p->near_plane * 2.0f;
-  const float vaf = clip_w * clip_w / (p->near_plane * p->near_plane * read_res * read_res);
+  const float vaf = clip_w * clip_w / (p->near_plane * p->near_plane * read_res * read_h);
   o->val_area_factor = vaf;
   o->indirect_shadow_lod = static_cast<float>(p->indirect_shadow_lod);
   const float block = static_cast<float>(1 << p->indirect_shadow_lod);
-  if (block > static_cast<float>(read_res)) return DRV_ERR_INVALID_ARG;  // renderer.cpp:712
+  if (block > static_cast<float>(read_h)) return DRV_ERR_INVALID_ARG;  // renderer.cpp:712
   o->indirect_shadow_block_size = block;
   o->indirect_shadow_sample_interval = static_cast<int>(block * block);
   o->indirect_shadow_super_val_width = std::sqrt(vaf) * block;
@@ -720,15 +728,21 @@ int validate_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light
   if (!ctx || !rsm || !light || !rsm->flux || !rsm->normal || !rsm->depth_lin_sq) return DRV_ERR_INVALID_ARG;
   if (!ctx->allocated) return fail(ctx, DRV_ERR_STATE, "light_caches: no allocation yet");
   const int R = rsm->read_resolution;
+  const int H = rsm->read_height > 0 ? rsm->read_height : R;
+  const int LH = light->rsm_read_height > 0 ? light->rsm_read_height : light->rsm_read_resolution;
   if (!is_pow2(R) || R < 8 || R > 65536 || R != light->rsm_read_resolution)
     return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: read resolution must be a power of two >= 8 matching the light");
-  const uint32_t n_vpl = static_cast<uint32_t>(R) * static_cast<uint32_t>(R);
+  // Square reads (cacheLightingRSM.comp:142) or the 2:1 extension, whose
+  // Morton decode covers the 2h x h texels once.
+  if ((H != R && 2 * H != R) || H != LH || R * static_cast<int64_t>(H) < 64)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: read height must equal the width or half of it, matching the light");
+  const uint32_t n_vpl = static_cast<uint32_t>(R) * static_cast<uint32_t>(H);
   const bool shadow = ctx->cfg.indirect_shadow != 0;
   const uint32_t interval = static_cast<uint32_t>(light->indirect_shadow_sample_interval);
   const int lod = static_cast<int>(light->indirect_shadow_lod);
   if (shadow) {
     if (!rsm->depth_lin_sq_shadow) return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: shadow LOD level missing");
-    if (interval == 0 || interval != (1u << (2 * lod)) || (R >> lod) < 1)
+    if (interval == 0 || interval != (1u << (2 * lod)) || (H >> lod) < 1)
       return fail(ctx, DRV_ERR_INVALID_ARG, "light_caches: inconsistent indirect shadow LOD");
   }
   const uint32_t n_blocks = shadow ? n_vpl / interval : 0u;
@@ -1046,6 +1060,8 @@ int drv_render_rsm(drv_ctx* ctx, const drv_scene* scene, const drv_spot_light* l
   const int res = light->rsm_render_resolution;
   if (!is_pow2(res) || res > 8192 || light->rsm_read_resolution < 1 || res % light->rsm_read_resolution != 0)
     return fail(ctx, DRV_ERR_INVALID_ARG, "render_rsm: render resolution must be a power of two <= 8192");
+  if (light->rsm_render_height > 0 && light->rsm_render_height != res)
+    return fail(ctx, DRV_ERR_INVALID_ARG, "render_rsm: the device RSM producer renders square RSMs (rsm_wide 0)");
   const size_t chain = rsm_chain_texels(res);
   int rc;
   if ((rc = ensure_raster(ctx, static_cast<size_t>(res) * res, scene->n_tri)) != DRV_OK) return rc;
@@ -1066,6 +1082,7 @@ int drv_render_rsm(drv_ctx* ctx, const drv_scene* scene, const drv_spot_light* l
   out->depth_lin_sq = ctx->rsm_dls + 2 * o_read;
   out->depth_lin_sq_shadow = ctx->rsm_dls + 2 * o_shadow;
   out->read_resolution = light->rsm_read_resolution;
+  out->read_height = light->rsm_read_resolution;
   return check_hip(ctx, hipGetLastError(), "render_rsm");
 }
 
@@ -1161,9 +1178,11 @@ int drv_bruteforce_rsm(drv_ctx* ctx, const drv_gbuffer* gb, const uint8_t* rough
     return fail(ctx, DRV_ERR_INVALID_ARG, "bruteforce_rsm: G-buffer size differs from the configured backbuffer");
   if (!ctx->have_frame) return fail(ctx, DRV_ERR_STATE, "bruteforce_rsm: frame not set");
   const int R = rsm->read_resolution;
-  if (!is_pow2(R) || R > 65536 || R != light->rsm_read_resolution)
+  const int H = rsm->read_height > 0 ? rsm->read_height : R;
+  const int LH = light->rsm_read_height > 0 ? light->rsm_read_height : light->rsm_read_resolution;
+  if (!is_pow2(R) || R > 65536 || R != light->rsm_read_resolution || (H != R && 2 * H != R) || H != LH)
     return fail(ctx, DRV_ERR_INVALID_ARG, "bruteforce_rsm: read resolution must be a power of two matching the light");
-  const uint32_t n = static_cast<uint32_t>(R) * static_cast<uint32_t>(R);
+  const uint32_t n = static_cast<uint32_t>(R) * static_cast<uint32_t>(H);
   int rc;
   if (n > ctx->vpl_cap) {
     dev_free(ctx->vpls);
@@ -1696,7 +1715,8 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   uint32_t max_vpl = 16;
   for (size_t v = 0; v < n; ++v) {
     const uint32_t r = static_cast<uint32_t>(rsms[v].read_resolution);
-    max_vpl = std::max(max_vpl, r * r);
+    const uint32_t h = rsms[v].read_height > 0 ? static_cast<uint32_t>(rsms[v].read_height) : r;
+    max_vpl = std::max(max_vpl, r * h);
   }
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);

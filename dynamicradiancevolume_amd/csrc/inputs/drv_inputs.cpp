@@ -463,7 +463,10 @@ int drvin_raster_rsm(const float* positions, const float* albedo, const float* n
                      int32_t n_threads) {
   if (!positions || !albedo || !normals || !light) return -1;
   const int res = light->rsm_render_resolution;
-  if (res <= 0) return -1;
+  // rows: the width for a square RSM, half of it for a 2:1 RSM (the square
+  // light frustum rendered with texels twice as tall as wide)
+  const int res_h = light->rsm_render_height > 0 ? light->rsm_render_height : res;
+  if (res <= 0 || res_h <= 0) return -1;
   double vp[16], ivp[16];
   for (int i = 0; i < 16; ++i) {
     vp[i] = light->light_view_projection[i];
@@ -471,9 +474,9 @@ int drvin_raster_rsm(const float* positions, const float* albedo, const float* n
   }
   std::vector<float> d;
   std::vector<int64_t> id;
-  raster(positions, n_tri, vp, res, res, d, id, n_threads);
+  raster(positions, n_tri, vp, res, res_h, d, id, n_threads);
   const float cos_half = light->light_cos_half_angle;
-  for (int y = 0; y < res; ++y)
+  for (int y = 0; y < res_h; ++y)
     for (int x = 0; x < res; ++x) {
       const size_t i = static_cast<size_t>(y) * res + x;
       const int64_t t = id[i];
@@ -485,7 +488,7 @@ int drvin_raster_rsm(const float* positions, const float* albedo, const float* n
       }
       // World position of the texel centre from its depth (the raster pass
       // interpolates the vertex position; both give the surface point).
-      const double ndc[4] = {(x + 0.5) / res * 2.0 - 1.0, (y + 0.5) / res * 2.0 - 1.0, d[i], 1.0};
+      const double ndc[4] = {(x + 0.5) / res * 2.0 - 1.0, (y + 0.5) / res_h * 2.0 - 1.0, d[i], 1.0};
       double wp[4];
       for (int r = 0; r < 4; ++r)
         wp[r] = ivp[4 * r] * ndc[0] + ivp[4 * r + 1] * ndc[1] + ivp[4 * r + 2] * ndc[2] + ivp[4 * r + 3];
@@ -500,7 +503,7 @@ int drvin_raster_rsm(const float* positions, const float* albedo, const float* n
                              to_light[2] * light->light_direction[2]);
       cos_to_light = std::min(1.0f, std::max(0.0f, cos_to_light));
       const float total_sr = kPi2 * (1.0f - cos_half);
-      const float pixel_sr = total_sr * cos_to_light / res / res;
+      const float pixel_sr = total_sr * cos_to_light / res / res_h;
       const float falloff = std::min(1.0f, std::max(0.0f, cos_to_light - cos_half)) / (1.0f - cos_half);
       for (int k = 0; k < 3; ++k)
         flux[3 * i + k] = f32_to_f16(albedo[3 * t + k] * light->light_intensity[k] * (falloff * pixel_sr / kPi));
@@ -511,11 +514,12 @@ int drvin_raster_rsm(const float* positions, const float* albedo, const float* n
   return 0;
 }
 
-int drvin_rsm_downsample(int32_t res, const uint16_t* flux, const int16_t* normal, const uint16_t* depth_lin_sq,
-                         uint16_t* flux_out, int16_t* normal_out, uint16_t* depth_lin_sq_out) {
-  if (res < 2 || (res & 1)) return -1;
-  const int r2 = res / 2;
-  for (int y = 0; y < r2; ++y)
+int drvin_rsm_downsample(int32_t res, int32_t res_h, const uint16_t* flux, const int16_t* normal,
+                         const uint16_t* depth_lin_sq, uint16_t* flux_out, int16_t* normal_out,
+                         uint16_t* depth_lin_sq_out) {
+  if (res < 2 || (res & 1) || res_h < 2 || (res_h & 1)) return -1;
+  const int r2 = res / 2, rh2 = res_h / 2;
+  for (int y = 0; y < rh2; ++y)
     for (int x = 0; x < r2; ++x) {
       const size_t o = static_cast<size_t>(y) * r2 + x;
       const size_t s[4] = {static_cast<size_t>(2 * y) * res + 2 * x, static_cast<size_t>(2 * y) * res + 2 * x + 1,

@@ -44,16 +44,17 @@ int drvin_raster_gbuffer(const float* positions, const float* albedo, const floa
                          int16_t* normal, uint8_t* albedo_srgb, int32_t n_threads);
 
 /* Rasterizes the RSM of one spot light at its render resolution (as
- * shader/fillrsm.frag): flux RGB16F uint16[3*res*res], normal int16[2*res*res],
- * depth_lin_sq RG16F uint16[2*res*res]. */
+ * shader/fillrsm.frag), res x h texels (h = rsm_render_height: res for a
+ * square RSM, res / 2 for a 2:1 one): flux RGB16F uint16[3*res*h], normal
+ * int16[2*res*h], depth_lin_sq RG16F uint16[2*res*h]. */
 int drvin_raster_rsm(const float* positions, const float* albedo, const float* normals, int64_t n_tri,
                      const drv_spot_light* light, uint16_t* flux, int16_t* normal,
                      uint16_t* depth_lin_sq, int32_t n_threads);
 
-/* One RSM mip step res -> res/2 (as shader/downsamplersm.frag): flux is the
- * 2x2 sum, the normal the renormalized mean of the four unpacked normals,
- * (d, d^2) the 2x2 mean. */
-int drvin_rsm_downsample(int32_t res, const uint16_t* flux, const int16_t* normal,
+/* One RSM mip step res x res_h -> res/2 x res_h/2 (as
+ * shader/downsamplersm.frag): flux is the 2x2 sum, the normal the
+ * renormalized mean of the four unpacked normals, (d, d^2) the 2x2 mean. */
+int drvin_rsm_downsample(int32_t res, int32_t res_h, const uint16_t* flux, const int16_t* normal,
                          const uint16_t* depth_lin_sq, uint16_t* flux_out, int16_t* normal_out,
                          uint16_t* depth_lin_sq_out);
 

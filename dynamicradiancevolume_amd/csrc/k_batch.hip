@@ -217,7 +217,7 @@ struct SplitDev {
 
 __global__ __launch_bounds__(256) void k_split_prepare(const SplitDev* views) {
   const PrepArgs& a = views[blockIdx.y].prep;
-  const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
+  const uint32_t n = static_cast<uint32_t>(a.R) * static_cast<uint32_t>(a.H);
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   prepare_vpl(a, m, n);
@@ -407,6 +407,8 @@ void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_
     a.dls_shadow = h.rsm.depth_lin_sq_shadow;
     a.R = h.rsm.read_resolution;
     a.R_shadow = h.rsm.read_resolution >> static_cast<int>(h.light.indirect_shadow_lod);
+    a.H = h.rsm.read_height > 0 ? h.rsm.read_height : h.rsm.read_resolution;
+    a.H_shadow = a.H >> static_cast<int>(h.light.indirect_shadow_lod);
     a.interval = static_cast<uint32_t>(h.light.indirect_shadow_sample_interval);
     a.block = 1 << static_cast<int>(h.light.indirect_shadow_lod);
     a.flux_cut = 0;

@@ -49,7 +49,7 @@ __device__ __forceinline__ void plan_splits(Counter* ctr, uint32_t n_vpl, uint32
 }
 
 __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
-  const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
+  const uint32_t n = static_cast<uint32_t>(a.R) * static_cast<uint32_t>(a.H);
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m == 0u && a.plan) plan_splits(a.plan, n, a.quantum, a.max_splits, a.target_items);
   if (m >= n) return;
@@ -302,6 +302,8 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.dls_shadow = rsm.depth_lin_sq_shadow;
   a.R = rsm.read_resolution;
   a.R_shadow = rsm.read_resolution >> static_cast<int>(light.indirect_shadow_lod);
+  a.H = rsm.read_height > 0 ? rsm.read_height : rsm.read_resolution;
+  a.H_shadow = a.H >> static_cast<int>(light.indirect_shadow_lod);
   a.interval = static_cast<uint32_t>(light.indirect_shadow_sample_interval);
   a.block = 1 << static_cast<int>(light.indirect_shadow_lod);
   a.flux_cut = flux_cut;
@@ -312,7 +314,7 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.max_splits = plan ? plan->max_splits : 0u;
   a.target_items = plan ? plan->target_items : 0u;
   a.blocks = blocks;
-  const uint32_t n = static_cast<uint32_t>(a.R) * a.R;
+  const uint32_t n = static_cast<uint32_t>(a.R) * static_cast<uint32_t>(a.H);
   hipLaunchKernelGGL(k_prepare_vpls, dim3((n + 255) / 256), dim3(256), 0, s, a);
 }
 

@@ -21,8 +21,10 @@ struct PrepArgs {
   const int16_t* normal;
   const uint16_t* dls;
   const uint16_t* dls_shadow;
-  int R;
-  int R_shadow;
+  int R;         // read level width (the Morton order's x range)
+  int R_shadow;  // width of the shadow level
+  int H;         // read level height: R, or R / 2 for a 2:1 RSM
+  int H_shadow;
   uint32_t interval;
   int block;  // B = 2^lod texels per block side
   int flux_cut;
@@ -46,9 +48,9 @@ __device__ __forceinline__ f3 rsm_position(const PrepArgs& a, float s, float t, 
   return make3(a.light_pos[0] + px / l * d, a.light_pos[1] + py / l * d, a.light_pos[2] + pz / l * d);
 }
 
-__device__ __forceinline__ float rg16f_texel(const uint16_t* img, int w, int i, int j, int comp) {
+__device__ __forceinline__ float rg16f_texel(const uint16_t* img, int w, int h, int i, int j, int comp) {
   i = min(w - 1, max(0, i));
-  j = min(w - 1, max(0, j));
+  j = min(h - 1, max(0, j));
   return half_to_float(img[2 * (static_cast<size_t>(j) * w + i) + comp]);
 }
 
@@ -59,9 +61,9 @@ __device__ __forceinline__ void prepare_vpl(const PrepArgs& a, uint32_t m, uint3
   uint32_t x, y;
   morton_decode(m, x, y);
   // cacheLightingRSM.comp:141-158
-  const float fR = static_cast<float>(a.R);
+  const float fR = static_cast<float>(a.R), fH = static_cast<float>(a.H);
   const float s = (static_cast<float>(x) + 0.5f) / fR;
-  const float t = (static_cast<float>(y) + 0.5f) / fR;
+  const float t = (static_cast<float>(y) + 0.5f) / fH;
   const size_t i = static_cast<size_t>(y) * a.R + x;
   const float d = half_to_float(a.dls[2 * i]);
   const f3 p = rsm_position(a, s, t, d);
@@ -102,19 +104,19 @@ __device__ __forceinline__ void prepare_vpl(const PrepArgs& a, uint32_t m, uint3
   if (a.blocks && (m % a.interval) == 0u) {
     // cacheLightingRSM.comp:171-192: bilinear (d, d^2) at the shadow LOD
     const float ss = (static_cast<float>(x) + a.sampling_offset) / fR;
-    const float st = (static_cast<float>(y) + a.sampling_offset) / fR;
-    const int w = a.R_shadow;
-    const float u = ss * static_cast<float>(w) - 0.5f, vv = st * static_cast<float>(w) - 0.5f;
+    const float st = (static_cast<float>(y) + a.sampling_offset) / fH;
+    const int w = a.R_shadow, h = a.H_shadow;
+    const float u = ss * static_cast<float>(w) - 0.5f, vv = st * static_cast<float>(h) - 0.5f;
     const float fu = floorf(u), fv = floorf(vv);
     const int i0 = to_int(fu), j0 = to_int(fv);
     const float al = u - fu, be = vv - fv;
     float dd[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
-      dd[c] = (1.0f - al) * (1.0f - be) * rg16f_texel(a.dls_shadow, w, i0, j0, c) +
-              al * (1.0f - be) * rg16f_texel(a.dls_shadow, w, i0 + 1, j0, c) +
-              (1.0f - al) * be * rg16f_texel(a.dls_shadow, w, i0, j0 + 1, c) +
-              al * be * rg16f_texel(a.dls_shadow, w, i0 + 1, j0 + 1, c);
+      dd[c] = (1.0f - al) * (1.0f - be) * rg16f_texel(a.dls_shadow, w, h, i0, j0, c) +
+              al * (1.0f - be) * rg16f_texel(a.dls_shadow, w, h, i0 + 1, j0, c) +
+              (1.0f - al) * be * rg16f_texel(a.dls_shadow, w, h, i0, j0 + 1, c) +
+              al * be * rg16f_texel(a.dls_shadow, w, h, i0 + 1, j0 + 1, c);
     const f3 ap = rsm_position(a, ss, st, dd[0]);
     const float variance = dd[1] - dd[0] * dd[0];
     const float cand = sqrtf(variance) * 2.0f / dd[0];

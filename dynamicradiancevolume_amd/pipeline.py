@@ -73,7 +73,7 @@ class DeviceFrame:
 
     def rsm(self) -> _abi.DrvRsm:
         return _abi.DrvRsm(self.flux.data_ptr(), self.rnormal.data_ptr(), self.dls.data_ptr(),
-                           self.dls_shadow.data_ptr(), self.inputs.read_resolution)
+                           self.dls_shadow.data_ptr(), self.inputs.read_resolution, self.inputs.read_height)
 
 
 class Renderer:

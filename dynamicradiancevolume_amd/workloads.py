@@ -36,6 +36,7 @@ class SpotSpec:
     indirect_shadow_lod: int = 2
     near_plane: float = 0.1   # Light::nearPlane (scene/scene.cpp:6)
     far_plane: float = 10000.0  # Light::farPlane (scene/scene.cpp:7)
+    rsm_wide: int = 0         # 1: a 2:1 RSM (twice as wide as tall)
 
 
 @dataclass(frozen=True)
@@ -108,16 +109,17 @@ BOXES_1080P_DENSE = BOXES_1080P.with_(name="boxes_1080p_dense", cav_resolution=2
 WORKLOADS = {w.name: w for w in (CORNELL_256, BOXES_1080P, BOXES_1080P_DENSE)}
 
 # Batched views: resolutions of 16:9 frames from 720p to 4K, and RSM reads of
-# 4096 or 16384 VPLs (square power-of-two RSM reads, which the Morton order of
-# cacheLightingRSM.comp:142 requires).
+# 4096, 16384 or 32768 VPLs: square power-of-two reads (the Morton order of
+# cacheLightingRSM.comp:142) or the 2:1 extension (256 x 128).
 BATCH_RESOLUTIONS = ((1280, 720), (1600, 900), (1920, 1080), (2560, 1440), (3200, 1800), (3840, 2160))
-BATCH_RSM = ((128, 1), (256, 1))  # (render resolution, read LOD) -> 64^2 or 128^2 VPLs
+# (render width, read LOD, 2:1) -> 64^2, 128^2 or 256 x 128 VPLs
+BATCH_RSM = ((128, 1, 0), (256, 1, 0), (512, 1, 1))
 
 
 # Many small views (thumbnails, probes): 320x180 with 4096 VPLs each, where
 # one launch per view and stage leaves most of the GPU idle.
 SMALL_RESOLUTIONS = ((320, 180),)
-SMALL_RSM = ((128, 1),)
+SMALL_RSM = ((128, 1, 0),)
 
 
 def batch_views(n: int, seed: int = 0, family: Workload = BOXES_1080P, resolutions=BATCH_RESOLUTIONS,
@@ -129,11 +131,11 @@ def batch_views(n: int, seed: int = 0, family: Workload = BOXES_1080P, resolutio
     out = []
     for i in range(n):
         w_, h_ = resolutions[int(rng.integers(len(resolutions)))]
-        rsm_res, lod = rsm[int(rng.integers(len(rsm)))]
+        rsm_res, lod, wide = rsm[int(rng.integers(len(rsm)))]
         ang = float(rng.uniform(-0.6, 0.6))
         eye = (6.0 * math.sin(ang), float(rng.uniform(4.0, 8.0)), 6.0 * math.cos(ang))
         target = (float(rng.uniform(-3, 3)), 0.0, float(rng.uniform(-8, -4)))
-        light = replace(family.light, rsm_resolution=rsm_res, rsm_read_lod=lod)
+        light = replace(family.light, rsm_resolution=rsm_res, rsm_read_lod=lod, rsm_wide=wide)
         cap = 65536 if w_ * h_ <= 1920 * 1080 else 131072
         out.append(family.with_(name=f"{family.name}_view{i}", seed=int(rng.integers(1 << 30)), width=w_, height=h_,
                                 eye=eye, target=target, light=light, max_caches=cap))
@@ -168,7 +170,17 @@ class FrameInputs:
 
     @property
     def read_resolution(self) -> int:
+        """Width of the RSM read level."""
         return int(self.light.rsm_read_resolution)
+
+    @property
+    def read_height(self) -> int:
+        """Rows of the RSM read level (the width, or half of it for a 2:1 RSM)."""
+        return int(self.light.rsm_read_height)
+
+    @property
+    def n_vpl(self) -> int:
+        return self.read_resolution * self.read_height
 
     def frame(self) -> _abi.DrvFrame:
         f = _abi.DrvFrame()
@@ -186,6 +198,7 @@ def spot_light(spec: SpotSpec) -> _abi.DrvSpotLight:
         direction=_abi.f3(*spec.direction), half_angle=spec.half_angle,
         rsm_resolution=spec.rsm_resolution, rsm_read_lod=spec.rsm_read_lod,
         indirect_shadow_lod=spec.indirect_shadow_lod, near_plane=spec.near_plane, far_plane=spec.far_plane,
+        rsm_wide=spec.rsm_wide,
     )
     out = _abi.DrvSpotLight()
     rc = lib.drv_prepare_spot_light(C.byref(p), C.byref(out))
@@ -225,22 +238,23 @@ def make_inputs(w: Workload, n_threads: int = 0) -> FrameInputs:
                              depth.ctypes.data, gn.ctypes.data, ga.ctypes.data, n_threads)
     light = spot_light(w.light)
     r = int(light.rsm_render_resolution)
-    flux = np.empty((r, r, 3), np.uint16)
-    rn = np.empty((r, r, 2), np.int16)
-    dls = np.empty((r, r, 2), np.uint16)
+    rh = int(light.rsm_render_height)
+    flux = np.empty((rh, r, 3), np.uint16)
+    rn = np.empty((rh, r, 2), np.int16)
+    dls = np.empty((rh, r, 2), np.uint16)
     lib.drvin_raster_rsm(pos.ctypes.data, alb.ctypes.data, nrm.ctypes.data, len(pos), C.byref(light),
                          flux.ctypes.data, rn.ctypes.data, dls.ctypes.data, n_threads)
     levels = [(flux, rn, dls)]
-    while r > 1:
-        r2 = r // 2
-        f2 = np.empty((r2, r2, 3), np.uint16)
-        n2 = np.empty((r2, r2, 2), np.int16)
-        d2 = np.empty((r2, r2, 2), np.uint16)
+    while rh > 1:
+        r2, rh2 = r // 2, rh // 2
+        f2 = np.empty((rh2, r2, 3), np.uint16)
+        n2 = np.empty((rh2, r2, 2), np.int16)
+        d2 = np.empty((rh2, r2, 2), np.uint16)
         pf, pn, pd = levels[-1]
-        lib.drvin_rsm_downsample(r, pf.ctypes.data, pn.ctypes.data, pd.ctypes.data, f2.ctypes.data,
+        lib.drvin_rsm_downsample(r, rh, pf.ctypes.data, pn.ctypes.data, pd.ctypes.data, f2.ctypes.data,
                                  n2.ctypes.data, d2.ctypes.data)
         levels.append((f2, n2, d2))
-        r = r2
+        r, rh = r2, rh2
     return FrameInputs(workload=w, positions=pos, albedo=alb, normals=nrm, bbox=bbox, view_proj=vp,
                        inv_view_proj=ivp, depth=depth, gnormal=gn, galbedo=ga, light=light, rsm_levels=levels,
                        groughmetal=roughness_metallic(w))

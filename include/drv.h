@@ -118,6 +118,13 @@ typedef struct drv_spot_params {
   uint32_t indirect_shadow_lod;
   float near_plane;            /* Light::nearPlane = 0.1 (scene/scene.cpp:6) */
   float far_plane;             /* Light::farPlane = 10000 (scene/scene.cpp:7) */
+  /* 1: a 2:1 RSM (render and read levels twice as wide as tall, texels of
+   * the square light frustum twice as tall as wide), e.g. 256 x 128 = 32768
+   * VPLs at the read level; 0: the reference's square RSM.  No counterpart
+   * in the reference, whose Morton order (cacheLightingRSM.comp:142) is
+   * defined for square reads: the decode of m < 2 h^2 covers the 2h x h
+   * texels exactly once (x takes the extra bit). */
+  uint32_t rsm_wide;
 } drv_spot_params;
 
 /* UBO SpotLight (shader/globalubos.glsl:88-112), as filled by
@@ -137,6 +144,8 @@ typedef struct drv_spot_light {
   int32_t indirect_shadow_sample_interval;
   float indirect_shadow_super_val_width;
   float indirect_shadow_sampling_offset;
+  int32_t rsm_render_height;  /* = rsm_render_resolution (square) or half of it (rsm_wide) */
+  int32_t rsm_read_height;    /* = rsm_read_resolution (square) or half of it (rsm_wide) */
 } drv_spot_light;
 
 /* G-buffer (renderer.cpp:466-471, texture units 0-3 of shader/gbuffer.glsl). */
@@ -160,7 +169,9 @@ typedef struct drv_rsm {
   const int16_t* normal;             /* device, RG16I */
   const uint16_t* depth_lin_sq;      /* device, RG16F (d, d^2) at the read level */
   const uint16_t* depth_lin_sq_shadow; /* device, RG16F (d, d^2) at read level + indirect_shadow_lod */
-  int32_t read_resolution;           /* power of two; RSMReadResolution */
+  int32_t read_resolution;           /* power of two; RSMReadResolution (the width) */
+  int32_t read_height;               /* rows of the read level: read_resolution (square), read_resolution / 2
+                                        (a 2:1 RSM), or 0 for square */
 } drv_rsm;
 
 /* Per-stage timings of the last frame when profiling is on (the scopes of

@@ -225,16 +225,17 @@ T sample_voxels(int res, const uint8_t* mips, const T* p, T lod) {
   return (T(1) - f) * t1 + f * t2;
 }
 
-// Bilinear fetch of an RG16F level (clamp to edge), component `comp`.
+// Bilinear fetch of an RG16F level of w x h texels (clamp to edge),
+// component `comp`.
 template <typename T>
-T bilinear_rg16f(const uint16_t* img, int w, T s, T t, int comp) {
-  const T u = s * T(w) - T(0.5), v = t * T(w) - T(0.5);
+T bilinear_rg16f(const uint16_t* img, int w, int h, T s, T t, int comp) {
+  const T u = s * T(w) - T(0.5), v = t * T(h) - T(0.5);
   const T fu = std::floor(u), fv = std::floor(v);
   const int i0 = to_int(fu), j0 = to_int(fv);
   const T a = u - fu, b = v - fv;
   auto tx = [&](int i, int j) {
     i = std::min(w - 1, std::max(0, i));
-    j = std::min(w - 1, std::max(0, j));
+    j = std::min(h - 1, std::max(0, j));
     return T(half_to_float(img[2 * (static_cast<size_t>(j) * w + i) + comp]));
   };
   return (T(1) - a) * (T(1) - b) * tx(i0, j0) + a * (T(1) - b) * tx(i0 + 1, j0) + (T(1) - a) * b * tx(i0, j0 + 1) +
@@ -268,17 +269,21 @@ V3<T> rsm_position(const drv_spot_light* light, T s, T t, T d) {
   return lp + normalize(p - lp) * d;
 }
 
+// Rows of the read level: the width for the reference's square RSM, half
+// of it for the 2:1 extension (whose Morton decode covers 2h x h).
+int rsm_height(const orc_rsm* rsm) { return rsm->read_height > 0 ? rsm->read_height : rsm->read_resolution; }
+
 template <typename T>
 void prepare_vpls(const drv_spot_light* light, const orc_rsm* rsm, std::vector<Vpl<T>>& vpls) {
   // cacheLightingRSM.comp:138-158 (one LightInfo per RSM texel, Morton order)
-  const int R = rsm->read_resolution;
-  const uint32_t n = static_cast<uint32_t>(R) * R;
+  const int R = rsm->read_resolution, H = rsm_height(rsm);
+  const uint32_t n = static_cast<uint32_t>(R) * static_cast<uint32_t>(H);
   vpls.resize(n);
   for (uint32_t m = 0; m < n; ++m) {
     uint32_t x, y;
     morton_decode(m, &x, &y);
     const T s = (T(x) + T(0.5)) / T(R);
-    const T t = (T(y) + T(0.5)) / T(R);
+    const T t = (T(y) + T(0.5)) / T(H);
     const size_t i = static_cast<size_t>(y) * R + x;
     Vpl<T>& v = vpls[m];
     v.flux = mk<T>(half_to_float(rsm->flux[3 * i]), half_to_float(rsm->flux[3 * i + 1]),
@@ -295,20 +300,20 @@ template <typename T>
 void prepare_blocks(const drv_spot_light* light, const orc_rsm* rsm, std::vector<Block<T>>& blocks) {
   // cacheLightingRSM.comp:169-192: per shadow block (every SampleInterval-th
   // Morton index) the averaged VAL position and cone aperture.
-  const int R = rsm->read_resolution;
+  const int R = rsm->read_resolution, H = rsm_height(rsm);
   const int interval = light->indirect_shadow_sample_interval;
   const int lod = static_cast<int>(light->indirect_shadow_lod);
-  const int Rl = R >> lod;
-  const uint32_t n = static_cast<uint32_t>(R) * R / static_cast<uint32_t>(interval);
+  const int Rl = R >> lod, Hl = H >> lod;
+  const uint32_t n = static_cast<uint32_t>(R) * static_cast<uint32_t>(H) / static_cast<uint32_t>(interval);
   blocks.resize(n);
   for (uint32_t b = 0; b < n; ++b) {
     uint32_t x, y;
     morton_decode(b * static_cast<uint32_t>(interval), &x, &y);
     const T off = T(light->indirect_shadow_sampling_offset);
     const T s = (T(x) + off) / T(R);
-    const T t = (T(y) + off) / T(R);
-    const T d = bilinear_rg16f<T>(rsm->depth_lin_sq_shadow, Rl, s, t, 0);
-    const T dsq = bilinear_rg16f<T>(rsm->depth_lin_sq_shadow, Rl, s, t, 1);
+    const T t = (T(y) + off) / T(H);
+    const T d = bilinear_rg16f<T>(rsm->depth_lin_sq_shadow, Rl, Hl, s, t, 0);
+    const T dsq = bilinear_rg16f<T>(rsm->depth_lin_sq_shadow, Rl, Hl, s, t, 1);
     blocks[b].avg_pos = rsm_position<T>(light, s, t, d);
     const T variance = dsq - d * d;
     blocks[b].rho = glsl_max<T>(T(light->indirect_shadow_super_val_width), std::sqrt(variance) * T(2.0) / d);
@@ -562,8 +567,12 @@ void orc_prepare_spot_scalars(const drv_spot_params* p, drv_spot_light* o) {
   o->rsm_render_resolution = res_pow2;
   const int read_res = static_cast<int>(res_pow2 / std::pow(2.0, static_cast<double>(p->rsm_read_lod)));
   o->rsm_read_resolution = read_res;
+  // the 2:1 extension halves the rows of the render and read levels
+  o->rsm_render_height = p->rsm_wide ? res_pow2 / 2 : res_pow2;
+  const int read_h = p->rsm_wide ? read_res / 2 : read_res;
+  o->rsm_read_height = read_h;
   const float clip_w = std::sin(p->half_angle) * p->near_plane * 2.0f;
-  const float vaf = clip_w * clip_w / (p->near_plane * p->near_plane * read_res * read_res);
+  const float vaf = clip_w * clip_w / (p->near_plane * p->near_plane * read_res * read_h);
   o->val_area_factor = vaf;
   o->indirect_shadow_lod = static_cast<float>(p->indirect_shadow_lod);
   const float block = static_cast<float>(1 << p->indirect_shadow_lod);
@@ -1024,16 +1033,16 @@ void orc_bruteforce_rsm(const drv_config* cfg, const drv_frame* frame, const drv
                         const orc_rsm* rsm, const float* depth, const int16_t* normal, const uint8_t* albedo,
                         const uint8_t* rm, int32_t diffuse_only, const uint32_t* pixels, int64_t n, float* out,
                         int32_t n_threads) {
-  const int W = cfg->width, H = cfg->height, R = rsm->read_resolution;
+  const int W = cfg->width, H = cfg->height, R = rsm->read_resolution, RH = rsm_height(rsm);
   if (!pixels) n = static_cast<int64_t>(W) * H;
   // The VALs of the read level (bruteforcersm.frag:49-53), shader order.
-  std::vector<Vpl<float>> vals(static_cast<size_t>(R) * R);
+  std::vector<Vpl<float>> vals(static_cast<size_t>(R) * RH);
   for (int x = 0; x < R; ++x)
-    for (int y = 0; y < R; ++y) {
+    for (int y = 0; y < RH; ++y) {
       const size_t i = static_cast<size_t>(y) * R + x;
-      Vpl<float>& v = vals[static_cast<size_t>(x) * R + y];
+      Vpl<float>& v = vals[static_cast<size_t>(x) * RH + y];
       const float s = (static_cast<float>(x) + 0.5f) / static_cast<float>(R);
-      const float t = (static_cast<float>(y) + 0.5f) / static_cast<float>(R);
+      const float t = (static_cast<float>(y) + 0.5f) / static_cast<float>(RH);
       const float d = half_to_float(rsm->depth_lin_sq[2 * i]);
       v.pos = rsm_position<float>(light, s, t, d);
       v.flux = mk<float>(half_to_float(rsm->flux[3 * i]), half_to_float(rsm->flux[3 * i + 1]),

@@ -88,7 +88,8 @@ typedef struct orc_rsm {
   const int16_t* normal;               /* RG16I */
   const uint16_t* depth_lin_sq;        /* RG16F at the read level */
   const uint16_t* depth_lin_sq_shadow; /* RG16F at read level + lod */
-  int32_t read_resolution;
+  int32_t read_resolution;             /* width of the read level */
+  int32_t read_height;                 /* rows: 0 or read_resolution (square), read_resolution / 2 (2:1) */
 } orc_rsm;
 
 /* cacheLightingRSM.comp:80-391 (SH1/SH2, INDIRECT_SHADOW) for the caches

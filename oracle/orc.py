@@ -26,6 +26,7 @@ class OrcRsm(C.Structure):
         ("depth_lin_sq", vp),
         ("depth_lin_sq_shadow", vp),
         ("read_resolution", C.c_int32),
+        ("read_height", C.c_int32),
     ]
 
 
@@ -166,7 +167,7 @@ def orc_rsm(fi) -> tuple[OrcRsm, list]:
     flux, rn, dls = fi.read_level
     keep = [np.ascontiguousarray(flux), np.ascontiguousarray(rn), np.ascontiguousarray(dls),
             np.ascontiguousarray(fi.shadow_dls)]
-    r = OrcRsm(_ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]), _ptr(keep[3]), fi.read_resolution)
+    r = OrcRsm(_ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]), _ptr(keep[3]), fi.read_resolution, fi.read_height)
     return r, keep
 
 
@@ -177,7 +178,7 @@ def gather_f32(cfg, vol, shc, light, fi, mips: np.ndarray, entries: np.ndarray, 
     if ids is not None:
         ids = np.ascontiguousarray(ids, np.uint32)
         n = len(ids)
-    nb = (fi.read_resolution ** 2) // int(light.indirect_shadow_sample_interval) if cfg.indirect_shadow else 0
+    nb = fi.n_vpl // int(light.indirect_shadow_sample_interval) if cfg.indirect_shadow else 0
     vis = np.zeros((n, nb), np.float32) if want_visibility and nb else None
     steps = C.c_int64(0)
     lib().orc_gather_f32(C.byref(cfg), C.byref(vol), C.byref(shc), C.byref(light), C.byref(rsm),
@@ -193,7 +194,7 @@ def gather_f64(cfg, vol, shc, light, fi, mips: np.ndarray, positions: np.ndarray
     pos = np.ascontiguousarray(positions, np.float32)
     n = len(pos)
     sh = np.zeros((n, 27), np.float64)
-    nb = (fi.read_resolution ** 2) // int(light.indirect_shadow_sample_interval) if cfg.indirect_shadow else 0
+    nb = fi.n_vpl // int(light.indirect_shadow_sample_interval) if cfg.indirect_shadow else 0
     vis = np.zeros((n, nb), np.float64) if want_visibility and nb else None
     lib().orc_gather_f64(C.byref(cfg), C.byref(vol), C.byref(shc), C.byref(light), C.byref(rsm),
                          _ptr(np.ascontiguousarray(mips)) if mips is not None else None, _ptr(pos), n, _ptr(sh),
@@ -204,7 +205,7 @@ def gather_f64(cfg, vol, shc, light, fi, mips: np.ndarray, positions: np.ndarray
 
 def prepare_vpls(light, rsm: OrcRsm) -> np.ndarray:
     """[n_vpl][12] LightInfo records in Morton order: position, DiscArea, flux, 0, normal, 0."""
-    n = int(rsm.read_resolution) ** 2
+    n = int(rsm.read_resolution) * int(rsm.read_height or rsm.read_resolution)
     out = np.zeros((n, 12), np.float32)
     lib().orc_prepare_vpls(C.byref(light), C.byref(rsm), _ptr(out))
     return out
@@ -213,7 +214,7 @@ def prepare_vpls(light, rsm: OrcRsm) -> np.ndarray:
 def shadow_blocks(light, fi) -> np.ndarray:
     """[n_blocks][8] = VAL position, aperture, mean VPL normal, 0."""
     rsm, keep = orc_rsm(fi)
-    nb = (fi.read_resolution ** 2) // int(light.indirect_shadow_sample_interval)
+    nb = fi.n_vpl // int(light.indirect_shadow_sample_interval)
     out = np.zeros((nb, 8), np.float32)
     lib().orc_shadow_blocks(C.byref(light), C.byref(rsm), _ptr(out))
     del keep

@@ -105,6 +105,23 @@ def test_prepare_spot_light_matches_oracle_and_is_consistent():
         c = vp @ np.append(ahead, 1.0)
         assert abs(c[0] / c[3]) < 1e-4 and abs(c[1] / c[3]) < 1e-4 and 0 < c[2] / c[3] < 1
         # the cone edge maps to NDC +-tan(half)/tan(half) = 1 along the frustum
+    # the 2:1 extension: half the rows, the texel solid angle over W x H texels
+    for res, lod in ((512, 1), (64, 0)):
+        sq = _abi.DrvSpotParams(intensity=_abi.f3(1, 1, 1), position=_abi.f3(0, 3, 0), direction=_abi.f3(0, -1, 0),
+                                half_angle=0.4, rsm_resolution=res, rsm_read_lod=lod, indirect_shadow_lod=2,
+                                near_plane=0.1, far_plane=100.0)
+        wd = _abi.DrvSpotParams.from_buffer_copy(sq)
+        wd.rsm_wide = 1
+        a, b = _abi.DrvSpotLight(), _abi.DrvSpotLight()
+        assert lib.drv_prepare_spot_light(C.byref(sq), C.byref(a)) == _abi.DRV_OK
+        assert lib.drv_prepare_spot_light(C.byref(wd), C.byref(b)) == _abi.DRV_OK
+        assert (a.rsm_render_height, a.rsm_read_height) == (a.rsm_render_resolution, a.rsm_read_resolution)
+        assert (b.rsm_render_height, b.rsm_read_height) == (res // 2, (res >> lod) // 2)
+        assert b.val_area_factor == pytest.approx(2.0 * a.val_area_factor, rel=1e-6)
+        assert list(b.light_view_projection) == list(a.light_view_projection)   # the same square frustum
+        ref = _abi.DrvSpotLight()
+        orc.lib().orc_prepare_spot_scalars(C.byref(wd), C.byref(ref))
+        assert (ref.rsm_read_height, ref.val_area_factor) == (b.rsm_read_height, b.val_area_factor)
     bad = _abi.DrvSpotParams(rsm_resolution=64, rsm_read_lod=0, indirect_shadow_lod=7, half_angle=0.5,
                              direction=_abi.f3(0, -1, 0), near_plane=0.1, far_plane=100.0)
     assert lib.drv_prepare_spot_light(C.byref(bad), C.byref(_abi.DrvSpotLight())) == _abi.DRV_ERR_INVALID_ARG

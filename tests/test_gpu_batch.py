@@ -173,7 +173,7 @@ def test_split_queue_visibility_and_sh_equal_single_views():
     for r, df, fi in zip(rs1, dfs1, frames):
         r.render_frame(df)
         _, live = r.cache_count()
-        nb = fi.read_resolution ** 2 // int(fi.light.indirect_shadow_sample_interval)
+        nb = fi.n_vpl // int(fi.light.indirect_shadow_sample_interval)
         ref.append((_by_position(r), r.read_visibility(live, nb), r.read_caches(live)[:, :3].copy()))
         r.close()
     rs, dfs = _group(frames, 2)

@@ -295,6 +295,29 @@ def test_gather_boxes_subset():
     _gather_case("boxes_1080p", n_sub=384)
 
 
+def test_gather_cornell_wide_rsm():
+    """The 2:1 RSM extension: a 64 x 32 read level (2048 VPLs, 16 x 8 shadow
+    blocks) against the oracle."""
+    from dynamicradiancevolume_amd import workloads as W
+
+    announce("gather cornell 2:1 RSM (64 x 32)")
+    fi = frame_inputs("cornell_256", light=W.replace(W.CORNELL_256.light, rsm_wide=1))
+    assert (fi.read_resolution, fi.read_height) == (64, 32)
+    _gather_case("cornell_256", light=W.replace(W.CORNELL_256.light, rsm_wide=1))
+
+
+def test_gather_boxes_32768_vpls_subset():
+    """The 2:1 RSM extension at 32768 VPLs (512 x 256 rendered, 256 x 128
+    read) on the 1080p frame: a spread subset of the caches against the oracle."""
+    from dynamicradiancevolume_amd import workloads as W
+
+    light = W.replace(W.BOXES_1080P.light, rsm_resolution=512, rsm_wide=1)
+    announce("gather boxes_1080p 32768 VPLs (2:1 RSM) subset")
+    fi = frame_inputs("boxes_1080p", light=light)
+    assert fi.n_vpl == 32768 and fi.read_height == 128
+    _gather_case("boxes_1080p", n_sub=256, light=light)
+
+
 def test_gather_boxes_dense_subset():
     """The ~60k-cache frame: a spread subset of its caches against the oracle."""
     announce("gather boxes_1080p_dense subset")
@@ -386,7 +409,7 @@ def test_tuning_variants_agree(tuning):
         r.render_frame(df)
         _, live = r.cache_count()
         e = r.read_caches(live)
-        nb = (fi.read_resolution ** 2) // int(fi.light.indirect_shadow_sample_interval)
+        nb = fi.n_vpl // int(fi.light.indirect_shadow_sample_interval)
         order = np.lexsort(e[:, :3].view(np.uint32).T)
         vis = r.read_visibility(live, nb)
         res.append((e[order], vis[order]))

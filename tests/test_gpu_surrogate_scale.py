@@ -64,7 +64,7 @@ def test_surrogate_visibility_is_network_of_features_1080p(boxes):
     r.mlp_set_params(S.load_weights())
     r.set_surrogate(True, spot_fraction=0.0, fallback_threshold=1e9)
     r.light_caches_rsm(df.rsm(), boxes.light)
-    n_blocks = boxes.read_resolution ** 2 // int(boxes.light.indirect_shadow_sample_interval)
+    n_blocks = boxes.n_vpl // int(boxes.light.indirect_shadow_sample_interval)
     vis = r.read_visibility(live, n_blocks)
     rng = np.random.default_rng(12)
     n = 20000

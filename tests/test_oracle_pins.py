@@ -6,10 +6,12 @@ that a plausible mistake (a wrong texel or sampling offset, a wrong LOD,
 a transposed corner order, a dropped transition claim, a wrong mix weight)
 fails it:
 
-* VPL records (cacheLightingRSM.comp:138-158) on an RSM of constant depth:
-  every VAL lies on the sphere of that radius around the light, projects
-  back through the light's forward matrix to its own Morton texel centre,
-  and carries that texel's flux and normal; DiscArea = d^2 ValAreaFactor;
+* VPL records (cacheLightingRSM.comp:138-158) on an RSM of constant depth,
+  square and 2:1: every VAL lies on the sphere of that radius around the
+  light, projects back through the light's forward matrix to its own Morton
+  texel centre, and carries that texel's flux and normal; DiscArea =
+  d^2 ValAreaFactor; the Morton decode visits every texel of a square or 2:1
+  read exactly once;
 * shadow blocks (cacheLightingRSM.comp:169-192) on a depth ramp at the
   shadow level: the averaged VAL projects to the block's sampling point and
   lies at the ramp's value there (bilinear of a linear ramp is linear); the
@@ -61,41 +63,68 @@ def _project(light, p):
     return c[:2] / c[3]
 
 
-def _synthetic_rsm(fi, read_depth, shadow_d, shadow_dsq, normals=None):
-    R = fi.read_resolution
-    ys, xs = np.mgrid[0:R, 0:R]
-    flux = np.stack([xs, ys, np.ones_like(xs)], -1).astype(np.float32)   # exact in FP16
+def _synthetic_rsm(R, H, read_depth, shadow_d, shadow_dsq, normals=None):
+    """Read level of R x H texels (flux = (x, y, 1), exact in FP16; constant
+    depth) and the given shadow level."""
+    ys, xs = np.mgrid[0:H, 0:R]
+    flux = np.stack([xs, ys, np.ones_like(xs)], -1).astype(np.float32)
     if normals is None:
         normals = np.stack([xs * 97 - 3000, ys * 211 - 6000], -1).astype(np.int16)
-    dls = np.stack([np.full((R, R), read_depth), np.full((R, R), read_depth * read_depth)], -1)
+    dls = np.stack([np.full((H, R), read_depth), np.full((H, R), read_depth * read_depth)], -1)
     sh = np.stack([shadow_d, shadow_dsq], -1)
     keep = [np.ascontiguousarray(_f16(flux)), np.ascontiguousarray(normals), np.ascontiguousarray(_f16(dls)),
             np.ascontiguousarray(_f16(sh))]
-    rsm = orc.OrcRsm(keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, keep[3].ctypes.data, R)
+    rsm = orc.OrcRsm(keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, keep[3].ctypes.data, R, H)
     return rsm, keep
+
+
+def _light(wide: int):
+    """The Cornell workload's spot light (64 x 64 read level, shadow LOD 2),
+    or its 2:1 variant (64 x 32)."""
+    from dataclasses import replace
+
+    from dynamicradiancevolume_amd import workloads as W
+
+    light = W.spot_light(replace(W.CORNELL_256.light, rsm_wide=wide))
+    return light, int(light.rsm_read_resolution), int(light.rsm_read_height)
 
 
 def _shadow_level_shape(fi):
     return fi.read_resolution >> int(fi.light.indirect_shadow_lod)
 
 
+def test_morton_decode_covers_square_and_wide_reads_once():
+    for R, H in ((64, 64), (64, 32), (256, 128), (8, 4)):
+        seen = np.zeros((H, R), np.int32)
+        xo, yo = (C.c_uint32(), C.c_uint32())
+        for m in range(R * H):
+            L().orc_morton_decode(m, C.byref(xo), C.byref(yo))
+            assert (xo.value, yo.value) == _morton_xy(m)
+            seen[yo.value, xo.value] += 1
+        assert (seen == 1).all(), (R, H)
+
+
 # ---- VPL records -------------------------------------------------------------
 
-def test_vpl_records_on_a_constant_depth_sphere():
-    fi = frame_inputs("cornell_256")
-    light = fi.light
-    R = fi.read_resolution
-    Rl = _shadow_level_shape(fi)
-    rsm, keep = _synthetic_rsm(fi, 2.0, np.full((Rl, Rl), 2.0), np.full((Rl, Rl), 4.0))
+@pytest.mark.parametrize("wide", [0, 1])
+def test_vpl_records_on_a_constant_depth_sphere(wide):
+    light, R, H = _light(wide)
+    Rl, Hl = R >> int(light.indirect_shadow_lod), H >> int(light.indirect_shadow_lod)
+    rsm, keep = _synthetic_rsm(R, H, 2.0, np.full((Hl, Rl), 2.0), np.full((Hl, Rl), 4.0))
     v = orc.prepare_vpls(light, rsm)
+    assert len(v) == R * H
     lp = np.array(light.light_position[:], np.float64)
     nn = (C.c_float * 3)()
     vaf = np.float32(light.val_area_factor)
-    for m in range(R * R):
+    # the solid angle of one read texel at unit distance: the clip plane's
+    # square (2 sin(half) near)^2 over the texel count, over near^2
+    clip = 2.0 * math.sin(0.45) * 0.1
+    assert float(vaf) == pytest.approx(clip * clip / (0.01 * R * H), rel=1e-5)
+    for m in range(R * H):
         x, y = _morton_xy(m)
         pos = v[m, :3].astype(np.float64)
         assert abs(np.linalg.norm(pos - lp) - 2.0) <= 2e-6
-        np.testing.assert_allclose(_project(light, pos), [(x + 0.5) / R * 2 - 1, (y + 0.5) / R * 2 - 1], atol=2e-5)
+        np.testing.assert_allclose(_project(light, pos), [(x + 0.5) / R * 2 - 1, (y + 0.5) / H * 2 - 1], atol=2e-5)
         assert v[m, 3] == np.float32(4.0) * vaf
         assert tuple(v[m, 4:7]) == (x, y, 1.0)
         L().orc_unpack_normal16i(int(keep[1][y, x, 0]), int(keep[1][y, x, 1]), nn)
@@ -104,46 +133,49 @@ def test_vpl_records_on_a_constant_depth_sphere():
 
 # ---- shadow blocks -------------------------------------------------------------
 
-def _blocks(fi, shadow_d, shadow_dsq, normals=None):
-    rsm, keep = _synthetic_rsm(fi, 2.0, shadow_d, shadow_dsq, normals)
-    nb = fi.read_resolution ** 2 // int(fi.light.indirect_shadow_sample_interval)
+def _blocks(light, R, H, shadow_d, shadow_dsq, normals=None):
+    rsm, keep = _synthetic_rsm(R, H, 2.0, shadow_d, shadow_dsq, normals)
+    nb = R * H // int(light.indirect_shadow_sample_interval)
     out = np.zeros((nb, 8), np.float32)
-    L().orc_shadow_blocks(C.byref(fi.light), C.byref(rsm), out.ctypes.data)
+    L().orc_shadow_blocks(C.byref(light), C.byref(rsm), out.ctypes.data)
     return out, keep
 
 
-def test_shadow_block_positions_follow_the_shadow_level_ramp():
-    fi = frame_inputs("cornell_256")
-    light = fi.light
-    R, Rl = fi.read_resolution, _shadow_level_shape(fi)
+@pytest.mark.parametrize("wide", [0, 1])
+def test_shadow_block_positions_follow_the_shadow_level_ramp(wide):
+    light, R, H = _light(wide)
+    lod = int(light.indirect_shadow_lod)
+    Rl, Hl = R >> lod, H >> lod
     interval = int(light.indirect_shadow_sample_interval)
     off = float(light.indirect_shadow_sampling_offset)
     assert off == pytest.approx(0.5 + math.sqrt(2.0) * 4 / 2)   # renderer.cpp:718, B = 4
-    ramp = np.tile(2.0 + np.arange(Rl) / 16.0, (Rl, 1))          # d = 2 + column / 16, exact in FP16
-    blk, _ = _blocks(fi, ramp, ramp * ramp)
+    ramp = np.tile(2.0 + np.arange(Rl) / 16.0, (Hl, 1))          # d = 2 + column / 16, exact in FP16
+    blk, _ = _blocks(light, R, H, ramp, ramp * ramp)
+    assert len(blk) == R * H // interval
     lp = np.array(light.light_position[:], np.float64)
     for b in range(len(blk)):
         x, y = _morton_xy(b * interval)           # upper-left texel of the block
-        s, t = (x + off) / R, (y + off) / R
+        s, t = (x + off) / R, (y + off) / H
         u = min(max(s * Rl - 0.5, 0.0), Rl - 1.0)   # clamp-to-edge bilinear of a linear ramp
         pos = blk[b, :3].astype(np.float64)
         np.testing.assert_allclose(_project(light, pos), [2 * s - 1, 2 * t - 1], atol=2e-5)
         assert np.linalg.norm(pos - lp) == pytest.approx(2.0 + u / 16.0, rel=2e-6)
 
 
-def test_shadow_block_aperture_zero_variance_and_depth_step():
-    fi = frame_inputs("cornell_256")
-    light = fi.light
-    R, Rl = fi.read_resolution, _shadow_level_shape(fi)
+@pytest.mark.parametrize("wide", [0, 1])
+def test_shadow_block_aperture_zero_variance_and_depth_step(wide):
+    light, R, H = _light(wide)
+    lod = int(light.indirect_shadow_lod)
+    Rl, Hl = R >> lod, H >> lod
     svw = np.float32(light.indirect_shadow_super_val_width)
     interval = int(light.indirect_shadow_sample_interval)
     off = float(light.indirect_shadow_sampling_offset)
     # constant depth: variance exactly 0, the aperture is the super-VAL width
-    blk, _ = _blocks(fi, np.full((Rl, Rl), 2.0), np.full((Rl, Rl), 4.0))
+    blk, _ = _blocks(light, R, H, np.full((Hl, Rl), 2.0), np.full((Hl, Rl), 4.0))
     assert (blk[:, 3] == svw).all()
     # a step between shadow columns 7 and 8: d 2 -> 3, d^2 4 -> 9
-    d = np.where(np.arange(Rl)[None, :] < 8, 2.0, 3.0) * np.ones((Rl, 1))
-    blk, _ = _blocks(fi, d, d * d)
+    d = np.where(np.arange(Rl)[None, :] < 8, 2.0, 3.0) * np.ones((Hl, 1))
+    blk, _ = _blocks(light, R, H, d, d * d)
     stepped = 0
     for b in range(len(blk)):
         x, _y = _morton_xy(b * interval)
@@ -156,14 +188,14 @@ def test_shadow_block_aperture_zero_variance_and_depth_step():
             stepped += 1
         else:
             assert blk[b, 3] == svw
-    assert stepped == R // int(2 ** light.indirect_shadow_lod)   # one block column
+    assert stepped == Hl   # one block column
 
 
 def test_shadow_block_mean_normal():
-    fi = frame_inputs("cornell_256")
-    R, Rl = fi.read_resolution, _shadow_level_shape(fi)
-    B = 1 << int(fi.light.indirect_shadow_lod)
-    interval = int(fi.light.indirect_shadow_sample_interval)
+    light, R, H = _light(0)
+    B = 1 << int(light.indirect_shadow_lod)
+    Rl = R // B
+    interval = int(light.indirect_shadow_sample_interval)
     rng = np.random.default_rng(4)
     n = rng.normal(size=(R, R, 3))
     n /= np.linalg.norm(n, axis=-1, keepdims=True)
@@ -173,7 +205,7 @@ def test_shadow_block_mean_normal():
         for x in range(R):
             L().orc_pack_normal16i(_abi.f3(*n[y, x]), out2)
             packed[y, x] = out2[:]
-    blk, _ = _blocks(fi, np.full((Rl, Rl), 2.0), np.full((Rl, Rl), 4.0), packed)
+    blk, _ = _blocks(light, R, R, np.full((Rl, Rl), 2.0), np.full((Rl, Rl), 4.0), packed)
     nn = (C.c_float * 3)()
     unpacked = np.zeros((R, R, 3))
     for y in range(R):

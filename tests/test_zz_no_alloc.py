@@ -60,7 +60,7 @@ def _inputs():
                                                       max_caches=65536)))
     big = _big[0]
     small = frame_inputs("cornell_256")
-    assert big.read_resolution ** 2 == 16384 and small.read_resolution ** 2 == 4096
+    assert big.n_vpl == 16384 and small.n_vpl == 4096
     assert W.BOXES_1080P.sh_bands == 2
     return big, small
 
@@ -141,9 +141,9 @@ def test_noalloc_refuses_what_it_cannot_do():
     # 512^2 VPLs exceed the scratch; the host refuses before any launch (the
     # pointers are never read)
     big = _abi.DrvSpotLight.from_buffer_copy(fi.light)
-    big.rsm_read_resolution = 512
+    big.rsm_read_resolution = big.rsm_read_height = 512
     rsm = df.rsm()
-    rsm.read_resolution = 512
+    rsm.read_resolution = rsm.read_height = 512
     rc = r.lib.drv_light_caches_noalloc(r.ctx, C.byref(rsm), C.byref(big))
     assert rc == _abi.DRV_ERR_INVALID_ARG
     assert b"65536" in r.lib.drv_last_error(r.ctx)
