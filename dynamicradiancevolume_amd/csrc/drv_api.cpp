@@ -48,7 +48,7 @@ struct drv_ctx {
   size_t cav_cells = 0;
   uint8_t* vox_target = nullptr;
   uint8_t* vox_mips = nullptr;
-  uint64_t* vox_corners = nullptr;
+  drvk::VoxelCorner* vox_corners = nullptr;
   drvk::VoxelLayout vox;
   float* srgb_lut = nullptr;
 
@@ -219,7 +219,7 @@ float elapsed(drv_ctx* ctx, int a, int b) {
 drvk::VoxelView voxel_view(const drv_ctx* ctx) {
   drvk::VoxelView v{};
   v.corners = ctx->vox_corners;
-  v.corner_bytes = static_cast<uint32_t>(ctx->vox.corner_count * sizeof(uint64_t));
+  v.corner_bytes = static_cast<uint32_t>(ctx->vox.corner_count * sizeof(drvk::VoxelCorner));
   v.res = ctx->vox.res;
   v.levels = ctx->vox.levels;
   for (int l = 0; l < ctx->vox.levels && l < 16; ++l) v.corner_offset[l] = static_cast<uint32_t>(ctx->vox.corner_offset[l]);
@@ -452,7 +452,7 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   drvk::launch_fill_u32(ctx->alloc_counts, drvk::kAllocChunks + 1, 0u, nullptr);
   drvk::launch_fill_u8(ctx->vox_target, res3, 0, nullptr);
   drvk::launch_fill_u8(ctx->vox_mips, L.mip_bytes, 0, nullptr);
-  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->vox_corners), L.corner_count * 2, 0u, nullptr);
+  drvk::launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->vox_corners), L.corner_count * 4, 0u, nullptr);
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(DRV_ERR_HIP);
   if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||

@@ -45,24 +45,32 @@ struct ShadowBlock {
   float nx, ny, nz, k;
 };
 
+// One record of the corner-expanded voxel copy: the 2x2x2 footprint of a
+// lower filter corner as four rows along x, row r = dy + 2 dz holding
+// (t(0, dy, dz), t(1, dy, dz) - t(0, dy, dz)) as two IEEE halves (exact:
+// texels are integers 0..255), so each x lerp is one mixed-precision FMA.
+struct alignas(16) VoxelCorner {
+  uint32_t row[4];
+};
+
 // Voxel volume layout: per mip level l (w = res >> l) the R8 texels, x
 // fastest, plus the "corner-expanded" copy: for every lower filter corner
-// (i, j, k) in [-1, w-1]^3 the 8 clamped texels of its 2x2x2 footprint packed
-// into one uint64 (byte dx + 2 dy + 4 dz), so a trilinear fetch is one load.
+// (i, j, k) in [-1, w-1]^3 the 8 clamped texels of its 2x2x2 footprint in
+// one 16-byte VoxelCorner, so a trilinear fetch is one load.
 struct VoxelLayout {
   int res = 0;
   int levels = 0;                  // log2(res) + 1
   std::vector<size_t> mip_offset;  // byte offsets into mips
-  std::vector<size_t> corner_offset;  // uint64 offsets into corners
+  std::vector<size_t> corner_offset;  // record offsets into corners
   size_t mip_bytes = 0;
   size_t corner_count = 0;
 };
 
 // Device view of the voxel corner layout passed to kernels.
 struct VoxelView {
-  const uint64_t* corners;
+  const VoxelCorner* corners;
   uint32_t corner_offset[16];
-  uint32_t corner_bytes;  // < 2^32 (voxel resolution <= 512)
+  uint32_t corner_bytes;  // < 2^32 (voxel resolution <= 512: 2.5e9 bytes)
   int res;
   int levels;
 };
@@ -186,7 +194,7 @@ void launch_voxelize(const drv_volume& vol, int res, const float* positions, con
 // launch, its first workgroup building the upper levels).
 void launch_voxel_blend_mips(const VoxelLayout& layout, const uint8_t* target, uint8_t* mips, float adaption,
                              hipStream_t s);
-void launch_voxel_corners(const VoxelLayout& L, uint8_t* mips, uint64_t* corners, hipStream_t s);
+void launch_voxel_corners(const VoxelLayout& L, uint8_t* mips, VoxelCorner* corners, hipStream_t s);
 
 // Split plan of the gather for a cache counter: enough (tile, split) items
 // for the persistent grid (4 waves per workgroup), each split a whole number

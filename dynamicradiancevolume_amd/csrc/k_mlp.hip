@@ -120,19 +120,27 @@ __device__ __forceinline__ void features_pair(const FeatureCtx& fc, const LevelI
     }
     fp[k] = footprint(lvi, lvl, p0x + tx * d, p0y + ty * d, p0z + tz * d);
   }
-  uint64_t c[12];
+  // Two rounds of six 16-byte corner loads in flight (twelve would take 48
+  // registers at once).
+  float occ[12];
   const __amdgpu_buffer_rsrc_t cr = corner_rsrc(fc.vox);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) c[k] = corner_load(cr, fp[k].index);
+  for (int h = 0; h < 2; ++h) {
+    corner_t c[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = corner_load(cr, fp[6 * h + k].index);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const Footprint& f = fp[6 * h + k];
+      occ[6 * h + k] = blend_corner(c[k], f.ax, f.ay, f.az) * (1.0f / 255.0f);
+    }
+  }
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
 #pragma unroll
     for (int e = 0; e < 5; ++e) out[8 * g + e] = G[5 * g + e];
 #pragma unroll
-    for (int e = 0; e < 3; ++e) {
-      const int k = 3 * g + e;
-      out[8 * g + 5 + e] = blend8(c[k], fp[k].ax, fp[k].ay, fp[k].az) * (1.0f / 255.0f);
-    }
+    for (int e = 0; e < 3; ++e) out[8 * g + 5 + e] = occ[3 * g + e];
   }
 }
 

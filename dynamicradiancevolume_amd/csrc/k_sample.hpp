@@ -42,13 +42,14 @@ __device__ __forceinline__ void stage_levels(const VoxelView& vx, LevelInfo* lv)
 // The corner-expanded copy as a buffer resource: fetches take a 32-bit byte
 // offset (no 64-bit address arithmetic per tap) and are bounds-checked.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t corner_rsrc(const VoxelView& vx) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(vx.corners), static_cast<short>(0),
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<VoxelCorner*>(vx.corners), static_cast<short>(0),
                                            static_cast<int>(vx.corner_bytes), 0x00020000);
 }
 
-__device__ __forceinline__ uint64_t corner_load(__amdgpu_buffer_rsrc_t r, uint32_t index) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(index * 8u), 0, 0);
-  return (static_cast<uint64_t>(static_cast<uint32_t>(v[1])) << 32) | static_cast<uint32_t>(v[0]);
+typedef uint32_t corner_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ corner_t corner_load(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(index * 16u), 0, 0);
 }
 
 // Corner-expanded fetch index of point p at mip level l and its trilinear
@@ -82,18 +83,22 @@ __device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float
   return f;
 }
 
-// Trilinear blend of the 8 packed unorm8 texels (byte dx + 2 dy + 4 dz),
-// in units of 1/255.
-__device__ __forceinline__ float blend8(uint64_t c, float ax, float ay, float az) {
-  const uint32_t lo = static_cast<uint32_t>(c), hi = static_cast<uint32_t>(c >> 32);
-  const float t000 = static_cast<float>(lo & 0xffu), t100 = static_cast<float>((lo >> 8) & 0xffu);
-  const float t010 = static_cast<float>((lo >> 16) & 0xffu), t110 = static_cast<float>(lo >> 24);
-  const float t001 = static_cast<float>(hi & 0xffu), t101 = static_cast<float>((hi >> 8) & 0xffu);
-  const float t011 = static_cast<float>((hi >> 16) & 0xffu), t111 = static_cast<float>(hi >> 24);
-  const float x00 = __builtin_fmaf(ax, t100 - t000, t000);
-  const float x10 = __builtin_fmaf(ax, t110 - t010, t010);
-  const float x01 = __builtin_fmaf(ax, t101 - t001, t001);
-  const float x11 = __builtin_fmaf(ax, t111 - t011, t011);
+// The two halves of a corner row as floats (exact).
+__device__ __forceinline__ float half_lo(uint32_t w) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu)));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
+}
+
+// Trilinear blend of the 8 texels of a corner record, in units of 1/255:
+// x lerps t0 + ax (t1 - t0), each one mixed-precision FMA of the record's
+// halves (the same FP32 value as converting both texels first), then y and z.
+__device__ __forceinline__ float blend_corner(corner_t c, float ax, float ay, float az) {
+  const float x00 = __builtin_fmaf(ax, half_hi(c[0]), half_lo(c[0]));
+  const float x10 = __builtin_fmaf(ax, half_hi(c[1]), half_lo(c[1]));
+  const float x01 = __builtin_fmaf(ax, half_hi(c[2]), half_lo(c[2]));
+  const float x11 = __builtin_fmaf(ax, half_hi(c[3]), half_lo(c[3]));
   const float y0 = __builtin_fmaf(ay, x10 - x00, x00);
   const float y1 = __builtin_fmaf(ay, x11 - x01, x01);
   return __builtin_fmaf(az, y1 - y0, y0);
@@ -103,7 +108,7 @@ __device__ __forceinline__ float blend8(uint64_t c, float ax, float ay, float az
 __device__ __forceinline__ float sample_level(__amdgpu_buffer_rsrc_t r, const LevelInfo* lv, int l, float px,
                                               float py, float pz) {
   const Footprint f = footprint(lv, l, px, py, pz);
-  return blend8(corner_load(r, f.index), f.ax, f.ay, f.az) * (1.0f / 255.0f);
+  return blend_corner(corner_load(r, f.index), f.ax, f.ay, f.az) * (1.0f / 255.0f);
 }
 
 }  // namespace drvk

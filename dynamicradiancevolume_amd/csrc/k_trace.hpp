@@ -98,9 +98,9 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, __amdgpu_buffer
       else t.step = fmaxf(1.0f, r * t.k);
     }
   }
-  uint64_t c1[kBatch], c2[kBatch];
+  corner_t c1[kBatch], c2[kBatch];
 #pragma unroll
-  for (int i = 0; i < kBatch; ++i) c1[i] = take[i] ? corner_load(cr, f1[i].index) : 0ull;
+  for (int i = 0; i < kBatch; ++i) c1[i] = take[i] ? corner_load(cr, f1[i].index) : corner_t{0u, 0u, 0u, 0u};
   float o[kBatch];
   // The second level is only planned when some lane of the wave blends two.
   if (__any(need2)) {
@@ -108,17 +108,17 @@ __device__ __forceinline__ bool trace_batch(const VoxelView& vx, __amdgpu_buffer
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
       f2[i] = footprint(lvi, min(lv[i] + 1, q), sx[i], sy[i], sz[i]);
-      c2[i] = frac[i] > 0.0f ? corner_load(cr, f2[i].index) : 0ull;
+      c2[i] = frac[i] > 0.0f ? corner_load(cr, f2[i].index) : corner_t{0u, 0u, 0u, 0u};
     }
 #pragma unroll
     for (int i = 0; i < kBatch; ++i) {
-      const float o1 = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
-      const float o2 = blend8(c2[i], f2[i].ax, f2[i].ay, f2[i].az);
+      const float o1 = blend_corner(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+      const float o2 = blend_corner(c2[i], f2[i].ax, f2[i].ay, f2[i].az);
       o[i] = __builtin_fmaf(frac[i], o2 - o1, o1);
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < kBatch; ++i) o[i] = blend8(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
+    for (int i = 0; i < kBatch; ++i) o[i] = blend_corner(c1[i], f1[i].ax, f1[i].ay, f1[i].az);
   }
 #pragma unroll
   for (int i = 0; i < kBatch; ++i) {

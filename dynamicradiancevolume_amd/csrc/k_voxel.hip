@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_voxel_blend_mips(MipArgs a) {
 
 struct CornerArgs {
   uint8_t* mips;
-  uint64_t* corners;
+  VoxelCorner* corners;
   uint32_t mip_offset[16];
   uint32_t corner_offset[17];
   uint32_t quad_offset[17];  // threads (four corners of a row each) before level l
@@ -283,18 +283,21 @@ __device__ __forceinline__ void corner_quad(const CornerArgs& a, const uint8_t* 
 #pragma unroll
     for (int c = 0; c < 5; ++c) t[r][c] = src[min(w - 1, max(0, cx0 + c))];
   }
-  uint64_t* dst = a.corners + a.corner_offset[l] + static_cast<size_t>(row) * wp;
+  VoxelCorner* dst = a.corners + a.corner_offset[l] + static_cast<size_t>(row) * wp;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int cx = cx0 + k;
     if (cx > w - 1) break;
-    // byte b: x offset b & 1, y offset (b >> 1) & 1, z offset b >> 2
-    uint32_t lo = 0, hi = 0;
+    // row r = dy + 2 dz: (t at x, t at x + 1 minus t at x) as halves
+    VoxelCorner c;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) lo |= t[b >> 1][k + (b & 1)] << (8 * b);
-#pragma unroll
-    for (int b = 4; b < 8; ++b) hi |= t[b >> 1][k + (b & 1)] << (8 * (b - 4));
-    dst[cx + 1] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+    for (int r = 0; r < 4; ++r) {
+      const int t0 = static_cast<int>(t[r][k]), d = static_cast<int>(t[r][k + 1]) - t0;
+      const uint32_t h0 = __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(t0)));
+      const uint32_t h1 = __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(d)));
+      c.row[r] = h0 | (h1 << 16);
+    }
+    dst[cx + 1] = c;
   }
 }
 
@@ -380,7 +383,7 @@ void launch_voxel_blend_mips(const VoxelLayout& L, const uint8_t* target, uint8_
   hipLaunchKernelGGL(k_voxel_blend_mips, dim3(static_cast<unsigned>(nb * nb * nb)), dim3(256), lds, s, a);
 }
 
-void launch_voxel_corners(const VoxelLayout& L, uint8_t* mips, uint64_t* corners, hipStream_t s) {
+void launch_voxel_corners(const VoxelLayout& L, uint8_t* mips, VoxelCorner* corners, hipStream_t s) {
   CornerArgs a;
   a.mips = mips;
   a.corners = corners;
