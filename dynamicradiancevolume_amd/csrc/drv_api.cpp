@@ -55,6 +55,7 @@ struct drv_ctx {
   drvk::Vpl* vpls = nullptr;
   size_t vpl_cap = 0;
   drvk::ShadowBlock* blocks = nullptr;
+  drvk::MarchStep* march = nullptr;  // kMarchSteps per block
   size_t block_cap = 0;
   float* visibility = nullptr;
   size_t vis_blocks_cap = 0;  // blocks the visibility buffer holds (pitch = max_caches)
@@ -290,7 +291,7 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
     DRV_HIP(ctx, hipStreamWaitEvent(ss, ctx->spot_fork_ev, 0));
     drvk::launch_spot_select(ctx->counter, k, ctx->caches, stride, spot_list, sc, ctx->spot_ctr, ss);
     drvk::launch_visibility(ctx->vol, vox, spot_list, stride, &ctx->spot_ctr[0], ctx->blocks, n_blocks, ctx->spot_vis,
-                            pitch2, nullptr, ctx->tune, grid, ss);
+                            pitch2, nullptr, ctx->tune, grid, ss, ctx->march);
     drvk::launch_visibility_mlp(fh, ctx->mlp_pack, ctx->mlp_params, spot_list, stride, &ctx->spot_ctr[1],
                                 &ctx->spot_ctr[0], ctx->blocks, n_blocks, ctx->spot_vis, pitch2, network_grid(ctx), ss);
     // A small list fits more splits into the partial buffer than the main
@@ -318,7 +319,8 @@ int surrogate_visibility(drv_ctx* ctx, uint32_t n_blocks, uint32_t n_vpl, uint32
   if (spot) {  // gate counter: live = 0 unless the spot check failed
     DRV_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->spot_done_ev, 0));
     drvk::launch_visibility(ctx->vol, vox, ctx->caches, stride, &ctx->spot_ctr[2], ctx->blocks, n_blocks,
-                            ctx->visibility, static_cast<uint32_t>(cap), nullptr, ctx->tune, grid, ctx->stream);
+                            ctx->visibility, static_cast<uint32_t>(cap), nullptr, ctx->tune, grid, ctx->stream,
+                            ctx->march);
   }
   return DRV_OK;
 }
@@ -494,6 +496,7 @@ int drv_destroy(drv_ctx* ctx) {
   dev_free(ctx->srgb_lut);
   dev_free(ctx->vpls);
   dev_free(ctx->blocks);
+  dev_free(ctx->march);
   dev_free(ctx->visibility);
   dev_free(ctx->partial);
   dev_free(ctx->step_counter);
@@ -791,7 +794,10 @@ int prepare_light(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* light,
   }
   if (shadow && n_blocks > ctx->block_cap) {
     dev_free(ctx->blocks);
+    dev_free(ctx->march);
+    ctx->block_cap = 0;
     if ((rc = dev_alloc(ctx, &ctx->blocks, n_blocks)) != DRV_OK) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->march, static_cast<size_t>(n_blocks) * drvk::kMarchSteps)) != DRV_OK) return rc;
     ctx->block_cap = n_blocks;
   }
   return DRV_OK;
@@ -833,16 +839,19 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   // The gather's split plan rides on the VPL preparation (one launch less).
   const drvk::GatherPlan plan = drvk::gather_plan(ctx->counter, interval,
                                                   static_cast<uint32_t>(ctx->tune.gather_max_splits), grid);
+  const drvk::VoxelView vview = voxel_view(ctx);
   drvk::launch_prepare_vpls(*rsm, *light, ctx->vpls, shadow ? ctx->blocks : nullptr, specular,
-                            specular ? ctx->spec.vpls_lanes : nullptr, &plan, ctx->stream);
+                            specular ? ctx->spec.vpls_lanes : nullptr, &plan, ctx->stream,
+                            shadow ? ctx->march : nullptr, &vview);
   record(ctx, 6);
   join_voxels(ctx);
   if (use_net) {
     if ((rc = surrogate_visibility(ctx, n_blocks, n_vpl, interval, grid)) != DRV_OK) return rc;
     steps = nullptr;
   } else if (shadow) {
-    drvk::launch_visibility(ctx->vol, voxel_view(ctx), ctx->caches, stride, ctx->counter, ctx->blocks, n_blocks,
-                            ctx->visibility, static_cast<uint32_t>(cap), steps, ctx->tune, grid, ctx->stream);
+    drvk::launch_visibility(ctx->vol, vview, ctx->caches, stride, ctx->counter, ctx->blocks, n_blocks,
+                            ctx->visibility, static_cast<uint32_t>(cap), steps, ctx->tune, grid, ctx->stream,
+                            ctx->march);
     ctx->label_caches = ctx->caches;
     ctx->label_counter = ctx->counter;
     ctx->label_vis = ctx->visibility;
@@ -1454,7 +1463,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
   const size_t n = b->views.size();
   const bool capturing = is_capturing(b->stream);
   std::vector<drvk::SplitViewHost> hv(n);
-  uint32_t max_vpl = 64;
+  uint32_t max_vpl = 64, max_march = 0;
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
     uint32_t n_vpl = 0, interval = 0, n_blocks = 0;
@@ -1473,6 +1482,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
     d.counter = ctx->counter;
     d.vpls = ctx->vpls;
     d.blocks = b->shadow ? ctx->blocks : nullptr;
+    d.march = b->shadow ? ctx->march : nullptr;
     d.vis = b->shadow ? ctx->visibility : nullptr;
     d.vis_pitch = cap;
     d.partial = ctx->partial;
@@ -1488,6 +1498,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
     d.rsm = rsms[v];
     d.light = lights[v];
     max_vpl = std::max(max_vpl, n_vpl);
+    if (b->shadow) max_march = std::max(max_march, n_blocks * static_cast<uint32_t>(drvk::kMarchSteps));
   }
   const size_t bytes = drvk::split_desc_bytes() * n;
   std::vector<uint8_t> img(bytes);
@@ -1510,7 +1521,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
   drv_ctx* v0 = b->views[0];
   const int grid = persistent_grid(v0);
   drvk::launch_batch_split(b->d_split, static_cast<uint32_t>(n), b->d_totals, b->stride, b->sh_bands, b->shadow,
-                           max_vpl, static_cast<uint32_t>(grid) * 4u, v0->shc, grid, b->stream);
+                           max_vpl, max_march, static_cast<uint32_t>(grid) * 4u, v0->shc, grid, b->stream);
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
     ctx->last_n_blocks = hv[v].n_blocks;

@@ -46,12 +46,37 @@ struct ShadowBlock {
 };
 
 // One record of the corner-expanded voxel copy: the 2x2x2 footprint of a
-// lower filter corner as four rows along x, row r = dy + 2 dz holding
-// (t(0, dy, dz), t(1, dy, dz) - t(0, dy, dz)) as two IEEE halves (exact:
-// texels are integers 0..255), so each x lerp is one mixed-precision FMA.
+// lower filter corner as the eight coefficients of its trilinear polynomial
+// a + b x + y (c + d x) + z (e + f x + y (g + h x)) in four words (a, b),
+// (c, d), (e, f), (g, h), each two IEEE halves, low half first (exact: the
+// coefficients are integer differences of texels 0..255, in [-1020, 1020]),
+// so each word's x term is one mixed-precision FMA.
 struct alignas(16) VoxelCorner {
   uint32_t row[4];
 };
+
+// One sample of a shadow block's cone march (cacheLightingRSM.comp:211-228).
+// The march distance, the step and the mip level pair of every sample depend
+// only on the block's aperture (rho and k), not on the ray, so the VPL
+// preparation plans the 32 samples once per block and every ray towards the
+// block reads them (wave-uniform, scalar loads).  lv[0] is level floor(lod),
+// lv[1] level floor(lod) + 1 when two levels are blended (two = 1, frac > 0),
+// each as the constants of the corner-expanded copy: byte offset of the
+// level's first record, corners per row and per slice, w and 2^-l.
+struct MarchLevel {
+  uint32_t off16, wp, wp2;
+  float w, scale;
+};
+struct alignas(64) MarchStep {
+  float step;  // advance to this sample
+  float dist;  // march distance after the advance
+  float frac;  // weight of lv[1]
+  uint32_t two;
+  MarchLevel lv[2];
+  uint32_t pad[2];
+};
+static_assert(sizeof(MarchStep) == 64, "MarchStep is one 64-byte scalar load");
+constexpr int kMarchSteps = 32;
 
 // Voxel volume layout: per mip level l (w = res >> l) the R8 texels, x
 // fastest, plus the "corner-expanded" copy: for every lower filter corner
@@ -127,6 +152,7 @@ struct SplitViewHost {
   Counter* counter;
   Vpl* vpls;
   ShadowBlock* blocks;
+  MarchStep* march;  // kMarchSteps per block (with blocks)
   float* vis;
   uint32_t vis_pitch;
   float* partial;
@@ -212,11 +238,18 @@ inline GatherPlan gather_plan(Counter* counter, uint32_t interval, uint32_t max_
 // vpls_lanes (optional): the same records lane-interleaved for the specular
 // gather, record m at (m % per_lane) * 64 + m / per_lane, per_lane = n / 64.
 // plan (optional): the gather's split plan, written by the first thread.
+// march (optional, with blocks): kMarchSteps MarchStep records per shadow
+// block, planned for the voxel levels of `vox`.
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         int flux_cut, Vpl* vpls_lanes, const GatherPlan* plan, hipStream_t s);
+                         int flux_cut, Vpl* vpls_lanes, const GatherPlan* plan, hipStream_t s,
+                         MarchStep* march = nullptr, const VoxelView* vox = nullptr);
+// march (optional): the blocks' march plans; without them (or with a tuning
+// other than one block walk at one step per batch) the rays plan their own
+// samples, with the same values.
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
-                       uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid, hipStream_t s);
+                       uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid, hipStream_t s,
+                       const MarchStep* march = nullptr);
 void launch_gather(const drv_sh_constants& shc, int sh_bands, const float* caches, int stride,
                    const Counter* counter, const Vpl* vpls, uint32_t n_vpl, const float* visibility,
                    uint32_t vis_pitch, uint32_t sample_interval, int indirect_shadow, float* partial,
@@ -268,9 +301,11 @@ void launch_light_static(unsigned char* base, const drv_rsm& rsm, const drv_spot
 // the SH gather and the split reduction, each over the items of every view.
 size_t split_desc_bytes();
 void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_out);
+// max_march: the most march records of any view (n_blocks x kMarchSteps,
+// 0 without indirect shadows).
 void launch_batch_split(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands, int shadow,
-                        uint32_t max_vpl, uint32_t target_items, const drv_sh_constants& shc, int grid,
-                        hipStream_t s);
+                        uint32_t max_vpl, uint32_t max_march, uint32_t target_items, const drv_sh_constants& shc,
+                        int grid, hipStream_t s);
 
 void launch_spot_select(const Counter* counter, uint32_t k, const float* caches, int stride, float* spot,
                         uint32_t spot_cap, Counter* spot_ctr, hipStream_t s);

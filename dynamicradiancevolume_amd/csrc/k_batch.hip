@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_batch_gather(BatchArgs a) {
           t.done = true;
           t.s = 32;
         }
-        bool finished = false;
+        bool finished = !valid;
         const __amdgpu_buffer_rsrc_t cr = corner_rsrc(bv.vox);
         while (!finished) finished = trace_batch(bv.vox, cr, lvi, t);
         shadowing = trace_visibility(t);
@@ -215,9 +215,16 @@ struct SplitDev {
   uint32_t vis_begin, gather_begin, reduce_begin;
 };
 
-__global__ __launch_bounds__(256) void k_split_prepare(const SplitDev* views) {
+// Workgroups [0, vpl_groups) of row y prepare view y's VPL and block
+// records, the ones after them its march records (one per thread).
+__global__ __launch_bounds__(256) void k_split_prepare(const SplitDev* views, uint32_t vpl_groups) {
   const PrepArgs& a = views[blockIdx.y].prep;
   const uint32_t n = static_cast<uint32_t>(a.R) * static_cast<uint32_t>(a.H);
+  if (blockIdx.x >= vpl_groups) {
+    const uint32_t idx = (blockIdx.x - vpl_groups) * blockDim.x + threadIdx.x;
+    if (a.march && idx < n / a.interval * kMarchSteps) prepare_march(a, idx);
+    return;
+  }
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   prepare_vpl(a, m, n);
@@ -312,41 +319,30 @@ __device__ __forceinline__ uint32_t split_view(const SplitDev* views, uint32_t n
 // Cone-traced visibility of every view: items (view, shadow block, 64-cache
 // tile), block-major within a view, dealt round robin to the persistent
 // waves.  Each lane traces its cache towards the item's block with the code
-// of k_visibility, so the values are those of the single-view kernel.
+// of k_visibility_planned (the view's march plan), so the values are those
+// of the single-view kernels.
 __global__ __launch_bounds__(256) void k_split_visibility(const SplitDev* views, uint32_t n_views, int stride,
                                                           const SplitTotals* totals) {
-  __shared__ LevelInfo lvs[4][16];  // per wave: the voxel level table of its current view
   const uint32_t lane = threadIdx.x & 63u;
-  LevelInfo* lvi = lvs[threadIdx.x >> 6];
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t items = totals->vis;
-  uint32_t staged = 0xffffffffu;
   for (uint32_t item = wave; item < items; item += n_waves) {
     const uint32_t v = split_view<0>(views, n_views, item);
     const SplitDev& d = views[v];
-    if (v != staged) {  // wave-local table: store, then read after a wavefront fence
-      __builtin_amdgcn_wave_barrier();
-      if (lane < 16u) lvi[lane] = level_info(d.vox, static_cast<int>(lane));
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      staged = v;
-    }
     const uint32_t local = item - d.vis_begin;
     const uint32_t b = local / d.tiles, tile = local - b * d.tiles;
     const uint32_t cache = tile * 64u + lane;
-    if (cache >= d.live) continue;
-    const float* e = d.caches + static_cast<size_t>(cache) * stride;
+    const bool valid = cache < d.live;
+    const float* e = d.caches + static_cast<size_t>(valid ? cache : d.live - 1u) * stride;
     const f3 wp = make3(e[0], e[1], e[2]);
     const float vsz = d.voxel_size * static_cast<float>(d.vox.res);
     // cacheLightingRSM.comp:103-104
     const f3 vpos = make3((wp.x - d.vmin[0]) / vsz, (wp.y - d.vmin[1]) / vsz, (wp.z - d.vmin[2]) / vsz);
     const __amdgpu_buffer_rsrc_t cr = corner_rsrc(d.vox);
-    Trace t = trace_setup(d.vox, d.voxel_size, wp, vpos, d.prep.blocks[b]);
-    while (!trace_batch<1>(d.vox, cr, lvi, t)) {
-    }
-    d.vis[static_cast<size_t>(b) * d.vis_pitch + cache] = trace_visibility(t);
+    Trace t = trace_setup(d.vox, d.voxel_size, wp, vpos, block_record(d.prep.blocks, b));
+    trace_planned(march_steps(d.prep.march, b), cr, t, valid);
+    if (valid) d.vis[static_cast<size_t>(b) * d.vis_pitch + cache] = trace_visibility(t);
   }
 }
 
@@ -416,6 +412,8 @@ void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_
     a.vpls_lanes = nullptr;
     a.plan = nullptr;
     a.blocks = h.blocks;
+    a.march = h.blocks ? h.march : nullptr;
+    a.vox = h.vox;
     d.caches = h.caches;
     d.counter = h.counter;
     d.vis = h.vis;
@@ -434,10 +432,12 @@ void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_
 }
 
 void launch_batch_split(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands, int shadow,
-                        uint32_t max_vpl, uint32_t target_items, const drv_sh_constants& shc, int grid,
-                        hipStream_t s) {
+                        uint32_t max_vpl, uint32_t max_march, uint32_t target_items, const drv_sh_constants& shc,
+                        int grid, hipStream_t s) {
   SplitDev* d = static_cast<SplitDev*>(descs);
-  hipLaunchKernelGGL(k_split_prepare, dim3((max_vpl + 255u) / 256u, n_views), dim3(256), 0, s, d);
+  const uint32_t vpl_groups = (max_vpl + 255u) / 256u;
+  hipLaunchKernelGGL(k_split_prepare, dim3(vpl_groups + (max_march + 255u) / 256u, n_views), dim3(256), 0, s, d,
+                     vpl_groups);
   hipLaunchKernelGGL(k_split_plan, dim3(1), dim3(256), 0, s, d, n_views, target_items, shadow, sh_bands >= 2 ? 1 : 0,
                      totals);
   if (shadow) hipLaunchKernelGGL(k_split_visibility, dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);

@@ -12,9 +12,9 @@
 //                         record (averaged VAL position, cone aperture) is
 //                         built once per light into HBM;
 //   2. k_visibility     - one lane per (cache, shadow block) cone trace over
-//                         the corner-expanded voxel mips (one 8-byte load per
-//                         level per step), written block-major so both writes
-//                         and the gather's reads coalesce;
+//                         the corner-expanded voxel mips (one 16-byte load
+//                         per level per step), written block-major so both
+//                         writes and the gather's reads coalesce;
 //   3. k_gather         - persistent workgroups pull (64-cache tile, group
 //                         of 4 VPL splits) items, one split per wave; VPL
 //                         records are wave-uniform, so they stream through
@@ -48,8 +48,16 @@ __device__ __forceinline__ void plan_splits(Counter* ctr, uint32_t n_vpl, uint32
   ctr->splits = plan_split_count(ctr->live, n_vpl, quantum, max_splits, target_items);
 }
 
+// Workgroups [0, ceil(n / 256)) prepare the VPL and block records; with
+// march plans, the workgroups after them write one march record per thread.
 __global__ __launch_bounds__(256) void k_prepare_vpls(PrepArgs a) {
   const uint32_t n = static_cast<uint32_t>(a.R) * static_cast<uint32_t>(a.H);
+  const uint32_t vpl_groups = (n + 255u) / 256u;
+  if (blockIdx.x >= vpl_groups) {
+    const uint32_t idx = (blockIdx.x - vpl_groups) * blockDim.x + threadIdx.x;
+    if (idx < n / a.interval * kMarchSteps) prepare_march(a, idx);
+    return;
+  }
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m == 0u && a.plan) plan_splits(a.plan, n, a.quantum, a.max_splits, a.target_items);
   if (m >= n) return;
@@ -72,6 +80,7 @@ struct VisArgs {
   uint64_t* steps;  // optional {traces, reference steps, executed steps}
   uint32_t blocks_per_item;
   uint32_t refill_min;  // lane pool: idle lanes that trigger a refill
+  const MarchStep* march;  // the blocks' march plans (k_visibility_planned)
 };
 
 // Shadow blocks per work item: the requested count, or one when there are
@@ -156,6 +165,50 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
   if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
 }
 
+// The traces of k_visibility with the blocks' march plans: a wave's 64 rays
+// share their block, so the march distance, the step and the level pair of
+// every sample (and the block record) are wave-uniform scalar loads, and
+// each lane only advances, fetches and blends.  The wave traces its item's
+// blocks one after the other, all lanes on the same block.
+__global__ __launch_bounds__(256) void k_visibility_planned(VisArgs a) {
+  const __amdgpu_buffer_rsrc_t cr = corner_rsrc(a.vox);
+  const uint32_t live = a.counter->live;
+  const uint32_t tiles = (live + 63u) / 64u;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t bpi = items_blocks(tiles, a.n_blocks, a.blocks_per_item, n_waves);
+  const uint32_t groups = (a.n_blocks + bpi - 1) / bpi;
+  const uint32_t items = tiles * groups;
+  const float fres = static_cast<float>(a.vox.res);
+  const float vsz = a.voxel_size * fres;
+  uint32_t my_traces = 0, my_steps = 0, my_exec = 0;
+  for (uint32_t item = wave; item < items; item += n_waves) {
+    const uint32_t group = item / tiles, tile = item - group * tiles;
+    const uint32_t cache = tile * 64u + lane;
+    const bool valid = cache < live;
+    const float* e = a.caches + static_cast<size_t>(valid ? cache : live - 1u) * a.stride;
+    const f3 wp = make3(e[0], e[1], e[2]);
+    // cacheLightingRSM.comp:103-104
+    const f3 vpos = make3((wp.x - a.vmin[0]) / vsz, (wp.y - a.vmin[1]) / vsz, (wp.z - a.vmin[2]) / vsz);
+    const uint32_t b0 = group * bpi;
+    const uint32_t b1 = min(a.n_blocks, b0 + bpi);
+    for (uint32_t b = b0; b < b1; ++b) {
+      Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, block_record(a.blocks, b));
+      trace_planned(march_steps(a.march, b), cr, t, valid);
+      if (valid) {
+        a.vis[static_cast<size_t>(b) * a.pitch + cache] = trace_visibility(t);
+        my_traces += 1u;
+        my_exec += static_cast<uint32_t>(t.s);
+        if (a.steps)
+          my_steps += static_cast<uint32_t>(
+              reference_steps_planned(a.march + static_cast<size_t>(b) * kMarchSteps, t));
+      }
+    }
+  }
+  if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
+}
+
 // The same traces with lane refilling: a work item's n_valid x n_blocks
 // traces form a pool (block-major, so lanes that refill together mostly
 // share a block), every lane holds one trace, and after each batch of steps
@@ -222,8 +275,8 @@ __global__ __launch_bounds__(256) void k_visibility_pool(VisArgs a) {
         next += n_idle;
       }
       if (!__any(has)) break;
-      const bool ended = trace_batch<kBatch>(a.vox, cr, lvi, t);
-      if (has && ended) {
+      const bool ended = has && trace_batch<kBatch>(a.vox, cr, lvi, t);
+      if (ended) {
         a.vis[static_cast<size_t>(blk) * a.pitch + tile * 64u + cl] = trace_visibility(t);
         my_traces += 1u;
         my_exec += static_cast<uint32_t>(t.s);
@@ -289,7 +342,8 @@ __global__ __launch_bounds__(256) void k_reduce(float* caches, int stride, const
 }  // namespace
 
 void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* vpls, ShadowBlock* blocks,
-                         int flux_cut, Vpl* vpls_lanes, const GatherPlan* plan, hipStream_t s) {
+                         int flux_cut, Vpl* vpls_lanes, const GatherPlan* plan, hipStream_t s, MarchStep* march,
+                         const VoxelView* vox) {
   PrepArgs a;
   for (int i = 0; i < 16; ++i) a.inv_lvp[i] = light.inverse_light_view_projection[i];
   for (int i = 0; i < 3; ++i) a.light_pos[i] = light.light_position[i];
@@ -314,14 +368,17 @@ void launch_prepare_vpls(const drv_rsm& rsm, const drv_spot_light& light, Vpl* v
   a.max_splits = plan ? plan->max_splits : 0u;
   a.target_items = plan ? plan->target_items : 0u;
   a.blocks = blocks;
+  a.march = blocks && vox ? march : nullptr;
+  a.vox = vox ? *vox : VoxelView{};
   const uint32_t n = static_cast<uint32_t>(a.R) * static_cast<uint32_t>(a.H);
-  hipLaunchKernelGGL(k_prepare_vpls, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  const uint32_t march_groups = a.march ? (n / a.interval * kMarchSteps + 255u) / 256u : 0u;
+  hipLaunchKernelGGL(k_prepare_vpls, dim3((n + 255) / 256 + march_groups), dim3(256), 0, s, a);
 }
 
 void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float* caches, int stride,
                        const Counter* counter, const ShadowBlock* blocks, uint32_t n_blocks, float* visibility,
                        uint32_t vis_pitch, uint64_t* step_counter, const Tuning& tune, int grid,
-                       hipStream_t s) {
+                       hipStream_t s, const MarchStep* march) {
   VisArgs a;
   a.vox = vox;
   for (int i = 0; i < 3; ++i) a.vmin[i] = vol.volume_world_min[i];
@@ -336,8 +393,11 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
   a.steps = step_counter;
   a.blocks_per_item = static_cast<uint32_t>(tune.blocks_per_item);
   a.refill_min = static_cast<uint32_t>(tune.refill_min_lanes);
+  a.march = march;
   const dim3 g(grid), b(256);
-  if (tune.visibility_kernel == 1) {
+  if (march && tune.visibility_kernel == 0 && tune.steps_per_batch == 1) {
+    hipLaunchKernelGGL(k_visibility_planned, g, b, 0, s, a);
+  } else if (tune.visibility_kernel == 1) {
     switch (tune.steps_per_batch) {
       case 1: hipLaunchKernelGGL(k_visibility_pool<1>, g, b, 0, s, a); break;
       case 3: hipLaunchKernelGGL(k_visibility_pool<3>, g, b, 0, s, a); break;

@@ -7,6 +7,7 @@
 
 #include "drv_internal.hpp"
 #include "k_common.hpp"
+#include "k_sample.hpp"
 #include "k_sh.hpp"
 
 namespace drvk {
@@ -34,6 +35,10 @@ struct PrepArgs {
   Counter* plan;
   uint32_t quantum, max_splits, target_items;
   ShadowBlock* blocks;
+  // Optional march plans of the blocks (kMarchSteps records each) for the
+  // voxel levels of vox.
+  MarchStep* march;
+  VoxelView vox;
 };
 
 // World position of an RSM sample at distance d from the light
@@ -52,6 +57,34 @@ __device__ __forceinline__ float rg16f_texel(const uint16_t* img, int w, int h, 
   i = min(w - 1, max(0, i));
   j = min(h - 1, max(0, j));
   return half_to_float(img[2 * (static_cast<size_t>(j) * w + i) + comp]);
+}
+
+// Bilinear (d, d^2) of the shadow level at a block's sampling point, the
+// block's first texel (x, y) plus the sampling offset
+// (cacheLightingRSM.comp:171-174).
+__device__ __forceinline__ void block_depth_moments(const PrepArgs& a, uint32_t x, uint32_t y, float dd[2]) {
+  const float fR = static_cast<float>(a.R), fH = static_cast<float>(a.H);
+  const float ss = (static_cast<float>(x) + a.sampling_offset) / fR;
+  const float st = (static_cast<float>(y) + a.sampling_offset) / fH;
+  const int w = a.R_shadow, h = a.H_shadow;
+  const float u = ss * static_cast<float>(w) - 0.5f, vv = st * static_cast<float>(h) - 0.5f;
+  const float fu = floorf(u), fv = floorf(vv);
+  const int i0 = to_int(fu), j0 = to_int(fv);
+  const float al = u - fu, be = vv - fv;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+    dd[c] = (1.0f - al) * (1.0f - be) * rg16f_texel(a.dls_shadow, w, h, i0, j0, c) +
+            al * (1.0f - be) * rg16f_texel(a.dls_shadow, w, h, i0 + 1, j0, c) +
+            (1.0f - al) * be * rg16f_texel(a.dls_shadow, w, h, i0, j0 + 1, c) +
+            al * be * rg16f_texel(a.dls_shadow, w, h, i0 + 1, j0 + 1, c);
+}
+
+// Cone aperture of a block from its depth moments (cacheLightingRSM.comp:180-193).
+__device__ __forceinline__ float block_aperture(const PrepArgs& a, const float dd[2]) {
+  const float variance = dd[1] - dd[0] * dd[0];
+  const float cand = sqrtf(variance) * 2.0f / dd[0];
+  // GLSL max(x, y) = (x < y) ? y : x, so a NaN candidate keeps SuperValWidth.
+  return (a.super_val_width < cand) ? cand : a.super_val_width;
 }
 
 // The VPL record m (and the shadow block record when m starts a block) of
@@ -102,26 +135,11 @@ __device__ __forceinline__ void prepare_vpl(const PrepArgs& a, uint32_t m, uint3
     }
   }
   if (a.blocks && (m % a.interval) == 0u) {
-    // cacheLightingRSM.comp:171-192: bilinear (d, d^2) at the shadow LOD
-    const float ss = (static_cast<float>(x) + a.sampling_offset) / fR;
-    const float st = (static_cast<float>(y) + a.sampling_offset) / fH;
-    const int w = a.R_shadow, h = a.H_shadow;
-    const float u = ss * static_cast<float>(w) - 0.5f, vv = st * static_cast<float>(h) - 0.5f;
-    const float fu = floorf(u), fv = floorf(vv);
-    const int i0 = to_int(fu), j0 = to_int(fv);
-    const float al = u - fu, be = vv - fv;
     float dd[2];
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-      dd[c] = (1.0f - al) * (1.0f - be) * rg16f_texel(a.dls_shadow, w, h, i0, j0, c) +
-              al * (1.0f - be) * rg16f_texel(a.dls_shadow, w, h, i0 + 1, j0, c) +
-              (1.0f - al) * be * rg16f_texel(a.dls_shadow, w, h, i0, j0 + 1, c) +
-              al * be * rg16f_texel(a.dls_shadow, w, h, i0 + 1, j0 + 1, c);
-    const f3 ap = rsm_position(a, ss, st, dd[0]);
-    const float variance = dd[1] - dd[0] * dd[0];
-    const float cand = sqrtf(variance) * 2.0f / dd[0];
-    // GLSL max(x, y) = (x < y) ? y : x, so a NaN candidate keeps SuperValWidth.
-    const float rho = (a.super_val_width < cand) ? cand : a.super_val_width;
+    block_depth_moments(a, x, y, dd);
+    const f3 ap = rsm_position(a, (static_cast<float>(x) + a.sampling_offset) / fR,
+                               (static_cast<float>(y) + a.sampling_offset) / fH, dd[0]);
+    const float rho = block_aperture(a, dd);
     float sx = bsx, sy = bsy, sz = bsz;
     if (a.interval > 64u) {
       sx = sy = sz = 0.0f;
@@ -139,6 +157,19 @@ __device__ __forceinline__ void prepare_vpl(const PrepArgs& a, uint32_t m, uint3
     const float inv = nl > 0.0f ? 1.0f / nl : 0.0f;
     a.blocks[m / a.interval] = ShadowBlock{ap.x, ap.y, ap.z, rho, sx * inv, sy * inv, sz * inv, 2.0f / (1.0f - rho)};
   }
+}
+
+// Record n of the march plan of shadow block b (one thread per record,
+// n_blocks x kMarchSteps of them).  The aperture is recomputed from the
+// shadow level as the block's own thread computes it (the same value).
+__device__ __forceinline__ void prepare_march(const PrepArgs& a, uint32_t idx) {
+  const uint32_t b = idx / kMarchSteps, n = idx % kMarchSteps;
+  uint32_t x, y;
+  morton_decode(b * a.interval, x, y);
+  float dd[2];
+  block_depth_moments(a, x, y, dd);
+  const float rho = block_aperture(a, dd);
+  a.march[idx] = plan_march_step(a.vox, rho, 2.0f / (1.0f - rho), static_cast<int>(n));
 }
 
 // One (64-cache tile, group of 4 VPL splits) item of the SH gather, one

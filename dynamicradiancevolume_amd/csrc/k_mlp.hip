@@ -128,7 +128,7 @@ __device__ __forceinline__ void features_pair(const FeatureCtx& fc, const LevelI
   for (int h = 0; h < 2; ++h) {
     corner_t c[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) c[k] = corner_load(cr, fp[6 * h + k].index);
+    for (int k = 0; k < 6; ++k) c[k] = corner_load(cr, fp[6 * h + k].offset);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const Footprint& f = fp[6 * h + k];

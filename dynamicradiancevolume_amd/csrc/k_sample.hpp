@@ -9,10 +9,13 @@ namespace drvk {
 
 // Per-level constants of the corner-expanded copy, staged in LDS by every
 // sampling kernel (a per-lane level index into a kernel-argument table would
-// be a memory load on the address path of every tap): corner offset, corners
-// per row (w + 1) and per slice ((w + 1)^2), w = res >> l as float and 2^-l.
+// be a memory load on the address path of every tap): byte offset of the
+// level's first corner record, corners per row (w + 1) and per slice
+// ((w + 1)^2), w = res >> l as float and 2^-l.  Entries past the last level
+// repeat the last level, so entry l + 1 can be read beside entry l for any
+// level l (the cone trace fetches both with one address).
 struct __attribute__((aligned(32))) LevelInfo {  // 32 B: the table index is a shift
-  uint32_t off;
+  uint32_t off16;
   uint32_t wp;
   uint32_t wp2;
   float w;
@@ -22,12 +25,13 @@ struct __attribute__((aligned(32))) LevelInfo {  // 32 B: the table index is a s
 
 __device__ __forceinline__ LevelInfo level_info(const VoxelView& vx, int l) {
   LevelInfo li;
-  const int w = vx.res >> l;
-  li.off = vx.corner_offset[l];
+  const int lc = l < vx.levels ? l : vx.levels - 1;
+  const int w = vx.res >> lc;
+  li.off16 = vx.corner_offset[lc] * 16u;
   li.wp = static_cast<uint32_t>(w + 1);
   li.wp2 = li.wp * li.wp;
   li.w = static_cast<float>(w);
-  li.scale = __builtin_amdgcn_ldexpf(1.0f, -l);
+  li.scale = __builtin_amdgcn_ldexpf(1.0f, -lc);
   li.pad[0] = li.pad[1] = li.pad[2] = 0u;
   return li;
 }
@@ -48,22 +52,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t corner_rsrc(const VoxelView& v
 
 typedef uint32_t corner_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ corner_t corner_load(__amdgpu_buffer_rsrc_t r, uint32_t index) {
-  return __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(index * 16u), 0, 0);
+__device__ __forceinline__ corner_t corner_load(__amdgpu_buffer_rsrc_t r, uint32_t offset) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(offset), 0, 0);
 }
 
-// Corner-expanded fetch index of point p at mip level l and its trilinear
+// Corner-expanded fetch offset of point p at a mip level and its trilinear
 // weights (GL 4.5 section 8.14.2, clamp to edge).  p is in level-0 texel
 // units; the lower filter corner floor(p * 2^-l - 0.5) clamped to [-1, w - 1]
 // selects the same clamped texels, and is computed one up, as
 // floor(p * 2^-l + 0.5) clamped to [0, w] (the corner table's row index).
 struct Footprint {
-  uint32_t index;
+  uint32_t offset;  // byte offset of the corner record
   float ax, ay, az;
 };
 
-__device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float px, float py, float pz) {
-  const LevelInfo li = lv[l];
+// L: LevelInfo or MarchLevel (off16, wp, wp2, w, scale).
+template <class L>
+__device__ __forceinline__ Footprint footprint(const L& li, float px, float py, float pz) {
   const float u = __builtin_fmaf(px, li.scale, 0.5f);
   const float v = __builtin_fmaf(py, li.scale, 0.5f);
   const float q = __builtin_fmaf(pz, li.scale, 0.5f);
@@ -73,7 +78,8 @@ __device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float
   const uint32_t j = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(v, 0.0f, li.w));
   const uint32_t k = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(q, 0.0f, li.w));
   Footprint f;
-  f.index = li.off + i + __umul24(j, li.wp) + __umul24(k, li.wp2);  // (w + 1)^2 < 2^24 for res <= 512
+  // (w + 1)^2 < 2^24 for res <= 512: two 24-bit multiply-adds
+  f.offset = ((__umul24(k, li.wp2) + (__umul24(j, li.wp) + i)) << 4) + li.off16;
   // u - floor(u) (v_fract_f32 caps it below 1, which only happens for
   // u in (-2^-24, 0), where both texels of the footprint are the clamped
   // edge texel and the weight has no effect).
@@ -83,7 +89,11 @@ __device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float
   return f;
 }
 
-// The two halves of a corner row as floats (exact).
+__device__ __forceinline__ Footprint footprint(const LevelInfo* lv, int l, float px, float py, float pz) {
+  return footprint(lv[l], px, py, pz);
+}
+
+// The two halves of a corner record word as floats (exact).
 __device__ __forceinline__ float half_lo(uint32_t w) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu)));
 }
@@ -91,24 +101,59 @@ __device__ __forceinline__ float half_hi(uint32_t w) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
 }
 
-// Trilinear blend of the 8 texels of a corner record, in units of 1/255:
-// x lerps t0 + ax (t1 - t0), each one mixed-precision FMA of the record's
-// halves (the same FP32 value as converting both texels first), then y and z.
+// Trilinear blend of a corner record, in units of 1/255.  The record holds
+// the footprint's texels t(x, y, z), x, y, z in {0, 1}, as the coefficients
+// of the trilinear polynomial
+//   a + b x + y (c + d x) + z (e + f x + y (g + h x)),
+// words (a, b), (c, d), (e, f), (g, h), low half first: a = t000,
+// b = t100 - t000, c and e the y and z differences of t at x = 0, d, f the
+// mixed second differences and g, h the y-z differences of those.  All are
+// integers in [-1020, 1020], exact in FP16.  Each word's x term is one
+// mixed-precision FMA (FP16 operands, FP32 arithmetic), then one FMA each in
+// y and z: 7 operations against 10 for lerps of lerps.
 __device__ __forceinline__ float blend_corner(corner_t c, float ax, float ay, float az) {
-  const float x00 = __builtin_fmaf(ax, half_hi(c[0]), half_lo(c[0]));
-  const float x10 = __builtin_fmaf(ax, half_hi(c[1]), half_lo(c[1]));
-  const float x01 = __builtin_fmaf(ax, half_hi(c[2]), half_lo(c[2]));
-  const float x11 = __builtin_fmaf(ax, half_hi(c[3]), half_lo(c[3]));
-  const float y0 = __builtin_fmaf(ay, x10 - x00, x00);
-  const float y1 = __builtin_fmaf(ay, x11 - x01, x01);
-  return __builtin_fmaf(az, y1 - y0, y0);
+  const float s0 = __builtin_fmaf(ax, half_hi(c[0]), half_lo(c[0]));
+  const float s1 = __builtin_fmaf(ax, half_hi(c[1]), half_lo(c[1]));
+  const float s2 = __builtin_fmaf(ax, half_hi(c[2]), half_lo(c[2]));
+  const float s3 = __builtin_fmaf(ax, half_hi(c[3]), half_lo(c[3]));
+  const float y0 = __builtin_fmaf(ay, s1, s0);
+  const float y1 = __builtin_fmaf(ay, s3, s2);
+  return __builtin_fmaf(az, y1, y0);
+}
+
+// Record n of a shadow block's march plan (see MarchStep): the per-ray
+// recurrence of the cone trace with the ray-independent quantities only,
+// the same FP32 operations, so the values are those each ray would compute.
+__device__ __forceinline__ MarchStep plan_march_step(const VoxelView& vx, float rho, float k, int n) {
+  const int q = vx.levels - 1;
+  float dist = 0.0f, step = 1.0f, r;
+  for (int i = 0;; ++i) {
+    dist = dist + step;
+    r = dist * rho;
+    if (i == n) break;
+    step = fmaxf(1.0f, r * k);
+  }
+  const float lod = __builtin_amdgcn_fmed3f(__builtin_amdgcn_logf(r), 0.0f, static_cast<float>(q));
+  const int l = static_cast<int>(lod);
+  MarchStep e;
+  e.step = step;
+  e.dist = dist;
+  e.frac = __builtin_amdgcn_fractf(lod);
+  e.two = e.frac > 0.0f ? 1u : 0u;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const LevelInfo li = level_info(vx, min(l + h, q));
+    e.lv[h] = MarchLevel{li.off16, li.wp, li.wp2, li.w, li.scale};
+  }
+  e.pad[0] = e.pad[1] = 0u;
+  return e;
 }
 
 // Single-level sample (0..1) of the voxel mips at level l, p in level-0 texel units.
 __device__ __forceinline__ float sample_level(__amdgpu_buffer_rsrc_t r, const LevelInfo* lv, int l, float px,
                                               float py, float pz) {
   const Footprint f = footprint(lv, l, px, py, pz);
-  return blend_corner(corner_load(r, f.index), f.ax, f.ay, f.az) * (1.0f / 255.0f);
+  return blend_corner(corner_load(r, f.offset), f.ax, f.ay, f.az) * (1.0f / 255.0f);
 }
 
 }  // namespace drvk

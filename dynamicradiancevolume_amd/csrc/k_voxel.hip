@@ -288,13 +288,22 @@ __device__ __forceinline__ void corner_quad(const CornerArgs& a, const uint8_t* 
   for (int k = 0; k < 4; ++k) {
     const int cx = cx0 + k;
     if (cx > w - 1) break;
-    // row r = dy + 2 dz: (t at x, t at x + 1 minus t at x) as halves
+    // Per row r = dy + 2 dz: A_r = t at x, B_r = t at x + 1 minus t at x.
+    // The record's words are the trilinear polynomial's coefficient pairs
+    // (A, B) of row 0, then their differences in y, in z and in y and z.
+    int A[4], B[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      A[r] = static_cast<int>(t[r][k]);
+      B[r] = static_cast<int>(t[r][k + 1]) - A[r];
+    }
+    const int lo[4] = {A[0], A[1] - A[0], A[2] - A[0], A[3] - A[2] - A[1] + A[0]};
+    const int hi[4] = {B[0], B[1] - B[0], B[2] - B[0], B[3] - B[2] - B[1] + B[0]};
     VoxelCorner c;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int t0 = static_cast<int>(t[r][k]), d = static_cast<int>(t[r][k + 1]) - t0;
-      const uint32_t h0 = __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(t0)));
-      const uint32_t h1 = __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(d)));
+      const uint32_t h0 = __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(lo[r])));
+      const uint32_t h1 = __builtin_bit_cast(uint16_t, static_cast<_Float16>(static_cast<float>(hi[r])));
       c.row[r] = h0 | (h1 << 16);
     }
     dst[cx + 1] = c;
