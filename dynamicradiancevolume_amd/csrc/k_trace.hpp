@@ -175,8 +175,10 @@ __device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsr
   // The lanes that march stay in lockstep (all start at sample 0 and take
   // one sample per iteration until they stop), so the sample index is
   // wave-uniform and the record load is scalar.
+  MarchStep next = march_step(pl, 0);
   for (int n = 0; n < kMarchSteps; ++n) {
-    const MarchStep cur = march_step(pl, __builtin_amdgcn_readfirstlane(n));
+    const MarchStep cur = next;
+    next = march_step(pl, __builtin_amdgcn_readfirstlane(min(n + 1, kMarchSteps - 1)));  // a step ahead
     t.px = __builtin_fmaf(t.dx, cur.step, t.px);
     t.py = __builtin_fmaf(t.dy, cur.step, t.py);
     t.pz = __builtin_fmaf(t.dz, cur.step, t.pz);
