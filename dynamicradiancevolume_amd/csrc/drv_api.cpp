@@ -1412,7 +1412,8 @@ struct drv_batch {
   int sh_bands = 1;
   int shadow = 0;
   int cu_count = 256;
-  int32_t cpw = 64;  // caches per wave of the fused queue; 0 per-view passes; DRV_BATCH_SPLIT_QUEUE
+  int32_t cpw = 64;  // caches per wave of the fused queue; 0 per-view passes; DRV_BATCH_SPLIT_QUEUE or
+                     // DRV_BATCH_SPLIT_GATHER
   uint32_t vpl_tile = 4096;
   int waves_per_cu = 64;
   bool specular = false;  // some view keeps env-maps: per-view passes only
@@ -1520,8 +1521,22 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
   for (size_t v = 0; v < n; ++v) join_voxels(b->views[v]);
   drv_ctx* v0 = b->views[0];
   const int grid = persistent_grid(v0);
-  drvk::launch_batch_split(b->d_split, static_cast<uint32_t>(n), b->d_totals, b->stride, b->sh_bands, b->shadow,
-                           max_vpl, max_march, static_cast<uint32_t>(grid) * 4u, v0->shc, grid, b->stream);
+  const uint32_t nv = static_cast<uint32_t>(n);
+  drvk::launch_batch_split_prepare(b->d_split, nv, b->d_totals, b->sh_bands, b->shadow, max_vpl, max_march,
+                                   static_cast<uint32_t>(grid) * 4u, b->stream);
+  if (b->shadow && b->cpw == DRV_BATCH_SPLIT_GATHER) {
+    // Each view's visibility with the single-view kernel and launch shape.
+    for (size_t v = 0; v < n; ++v) {
+      drv_ctx* ctx = b->views[v];
+      drvk::launch_visibility(ctx->vol, hv[v].vox, ctx->caches, b->stride, ctx->counter, ctx->blocks,
+                              hv[v].n_blocks, ctx->visibility, hv[v].vis_pitch, nullptr, ctx->tune,
+                              persistent_grid(ctx), b->stream, ctx->march);
+    }
+  } else if (b->shadow) {
+    drvk::launch_batch_split_visibility(b->d_split, nv, b->d_totals, b->stride, grid, b->stream);
+  }
+  drvk::launch_batch_split_gather(b->d_split, nv, b->d_totals, b->stride, b->sh_bands, b->shadow, v0->shc, grid,
+                                  b->stream);
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
     ctx->last_n_blocks = hv[v].n_blocks;
@@ -1596,11 +1611,12 @@ const char* drv_batch_last_error(const drv_batch* b) { return b ? b->err.c_str()
 
 int drv_batch_set_tuning(drv_batch* b, int32_t cpw, int32_t vpl_tile, int32_t waves_per_cu) {
   if (!b) return DRV_ERR_INVALID_ARG;
-  if ((cpw != 64 && cpw != 32 && cpw != 16 && cpw != 0 && cpw != DRV_BATCH_SPLIT_QUEUE) || !is_pow2(vpl_tile) ||
-      vpl_tile < 16 || waves_per_cu < 4 || waves_per_cu > 64)
+  if ((cpw != 64 && cpw != 32 && cpw != 16 && cpw != 0 && cpw != DRV_BATCH_SPLIT_QUEUE &&
+       cpw != DRV_BATCH_SPLIT_GATHER) ||
+      !is_pow2(vpl_tile) || vpl_tile < 16 || waves_per_cu < 4 || waves_per_cu > 64)
     return bfail(b, DRV_ERR_INVALID_ARG,
-                 "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes, -1: split queue), VPL tile a power "
-                 "of two >= 16");
+                 "batch_set_tuning: caches per wave 64/32/16 (0: per-view passes, -1: split queue, -2: split queue "
+                 "with per-view visibility), VPL tile a power of two >= 16");
   if (b->specular && cpw != 0)
     return bfail(b, DRV_ERR_INVALID_ARG, "batch_set_tuning: views with indirect_specular need per-view passes (0)");
   if (any_surrogate(b) && cpw != 0)
@@ -1636,7 +1652,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
     return bfail(b, DRV_ERR_STATE,
                  "batch_light_caches: a view has the surrogate enabled; the fused and split queues only cone-trace "
                  "(select caches per wave 0)");
-  if (b->cpw == DRV_BATCH_SPLIT_QUEUE) return batch_split(b, rsms, lights);
+  if (b->cpw == DRV_BATCH_SPLIT_QUEUE || b->cpw == DRV_BATCH_SPLIT_GATHER) return batch_split(b, rsms, lights);
   const bool capturing = is_capturing(b->stream);
   size_t partial_need = 1;
   std::vector<drvk::BatchView> hv(n);
@@ -1737,7 +1753,7 @@ int drv_batch_autotune(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* 
   float best = 1e30f;
   int32_t best_cpw = keep_cpw;
   uint32_t best_tile = keep_tile;
-  const int32_t cpws[5] = {0, DRV_BATCH_SPLIT_QUEUE, 64, 32, 16};
+  const int32_t cpws[6] = {0, DRV_BATCH_SPLIT_QUEUE, DRV_BATCH_SPLIT_GATHER, 64, 32, 16};
   const bool per_view_only = b->specular || any_surrogate(b);
   for (int32_t cpw : cpws) {
     if (per_view_only && cpw != 0) continue;  // env-map or surrogate views: per-view passes only

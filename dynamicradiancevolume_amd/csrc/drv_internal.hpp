@@ -168,7 +168,8 @@ struct SplitViewHost {
 
 // Item totals of the three queued launches of a batched split pass.
 struct SplitTotals {
-  uint32_t vis, gather, reduce, pad;
+  uint32_t vis, gather, reduce;
+  uint32_t vis_head;  // next visibility item to hand out (reset by the plan)
 };
 
 // Launch shapes chosen per context (drv_set_tuning).
@@ -301,11 +302,16 @@ void launch_light_static(unsigned char* base, const drv_rsm& rsm, const drv_spot
 // the SH gather and the split reduction, each over the items of every view.
 size_t split_desc_bytes();
 void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_out);
+// The split queue's launches: VPL/block/march records and the plan of every
+// view; the queued visibility; the queued SH gather and reduction.
 // max_march: the most march records of any view (n_blocks x kMarchSteps,
 // 0 without indirect shadows).
-void launch_batch_split(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands, int shadow,
-                        uint32_t max_vpl, uint32_t max_march, uint32_t target_items, const drv_sh_constants& shc,
-                        int grid, hipStream_t s);
+void launch_batch_split_prepare(void* descs, uint32_t n_views, SplitTotals* totals, int sh_bands, int shadow,
+                                uint32_t max_vpl, uint32_t max_march, uint32_t target_items, hipStream_t s);
+void launch_batch_split_visibility(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int grid,
+                                   hipStream_t s);
+void launch_batch_split_gather(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands,
+                               int shadow, const drv_sh_constants& shc, int grid, hipStream_t s);
 
 void launch_spot_select(const Counter* counter, uint32_t k, const float* caches, int stride, float* spot,
                         uint32_t spot_cap, Counter* spot_ctr, hipStream_t s);

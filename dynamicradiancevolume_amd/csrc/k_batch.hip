@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_split_plan(SplitDev* views, uint32_t n_
     totals->vis = run[0];
     totals->gather = run[1];
     totals->reduce = run[2];
-    totals->pad = 0u;
+    totals->vis_head = 0u;
   }
 }
 
@@ -316,33 +316,81 @@ __device__ __forceinline__ uint32_t split_view(const SplitDev* views, uint32_t n
   return lo;
 }
 
+// What the visibility stage reads of a view, loaded when a wave's items
+// move into it (wave-uniform scalar loads).
+struct VisView {
+  uint32_t begin, end, tiles, live, pitch;
+  const float* caches;
+  float* vis;
+  const ShadowBlock* blocks;
+  const MarchStep* march;
+  VoxelView vox;
+  float vmin[3];
+  float voxel_size;
+};
+
+__device__ __forceinline__ VisView vis_view(const SplitDev* views, uint32_t v) {
+  typedef const __attribute__((address_space(4))) SplitDev* Desc;
+  const Desc d = (Desc)(views + v);  // NOLINT(old-style-cast)
+  VisView w;
+  w.begin = d->vis_begin;
+  w.tiles = d->tiles;
+  w.end = w.begin + w.tiles * d->n_blocks;
+  w.live = d->live;
+  w.pitch = d->vis_pitch;
+  w.caches = d->caches;
+  w.vis = d->vis;
+  w.blocks = d->prep.blocks;
+  w.march = d->prep.march;
+  w.vox.corners = d->vox.corners;
+  w.vox.corner_bytes = d->vox.corner_bytes;
+  w.vox.res = d->vox.res;
+  w.vox.levels = d->vox.levels;
+  for (int i = 0; i < 3; ++i) w.vmin[i] = d->vmin[i];
+  w.voxel_size = d->voxel_size;
+  return w;
+}
+
+// Items a wave takes from the visibility queue at a time (4 and 128 were
+// slower on the 64-view workload: queue-head contention and the tail).
+constexpr uint32_t kVisChunk = 32;
+
 // Cone-traced visibility of every view: items (view, shadow block, 64-cache
-// tile), block-major within a view, dealt round robin to the persistent
-// waves.  Each lane traces its cache towards the item's block with the code
-// of k_visibility_planned (the view's march plan), so the values are those
-// of the single-view kernels.
+// tile), block-major within a view, handed out in chunks of kVisChunk
+// consecutive items by an atomic queue head, so a wave's items mostly share
+// a view and a block (its descriptor, block record and march records stay
+// in SGPRs and the scalar cache) and the waves that finish first take more.
+// Each lane traces its cache towards the item's block with the code of
+// k_visibility_planned (the view's march plan), so the values are those of
+// the single-view kernels.
 __global__ __launch_bounds__(256) void k_split_visibility(const SplitDev* views, uint32_t n_views, int stride,
-                                                          const SplitTotals* totals) {
+                                                          SplitTotals* totals) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t items = totals->vis;
-  for (uint32_t item = wave; item < items; item += n_waves) {
-    const uint32_t v = split_view<0>(views, n_views, item);
-    const SplitDev& d = views[v];
-    const uint32_t local = item - d.vis_begin;
-    const uint32_t b = local / d.tiles, tile = local - b * d.tiles;
-    const uint32_t cache = tile * 64u + lane;
-    const bool valid = cache < d.live;
-    const float* e = d.caches + static_cast<size_t>(valid ? cache : d.live - 1u) * stride;
-    const f3 wp = make3(e[0], e[1], e[2]);
-    const float vsz = d.voxel_size * static_cast<float>(d.vox.res);
-    // cacheLightingRSM.comp:103-104
-    const f3 vpos = make3((wp.x - d.vmin[0]) / vsz, (wp.y - d.vmin[1]) / vsz, (wp.z - d.vmin[2]) / vsz);
-    const __amdgpu_buffer_rsrc_t cr = corner_rsrc(d.vox);
-    Trace t = trace_setup(d.vox, d.voxel_size, wp, vpos, block_record(d.prep.blocks, b));
-    trace_planned(march_steps(d.prep.march, b), cr, t, valid);
-    if (valid) d.vis[static_cast<size_t>(b) * d.vis_pitch + cache] = trace_visibility(t);
+  VisView w;
+  w.begin = w.end = 0u;
+  while (true) {
+    uint32_t base = 0u;
+    if (lane == 0u) base = atomicAdd(&totals->vis_head, kVisChunk);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0));
+    if (base >= items) break;
+    const uint32_t stop = min(items, base + kVisChunk);
+    for (uint32_t item = base; item < stop; ++item) {
+      if (item < w.begin || item >= w.end) w = vis_view(views, split_view<0>(views, n_views, item));
+      const uint32_t local = item - w.begin;
+      const uint32_t b = local / w.tiles, tile = local - b * w.tiles;
+      const uint32_t cache = tile * 64u + lane;
+      const bool valid = cache < w.live;
+      const float* e = w.caches + static_cast<size_t>(valid ? cache : w.live - 1u) * stride;
+      const f3 wp = make3(e[0], e[1], e[2]);
+      const float vsz = w.voxel_size * static_cast<float>(w.vox.res);
+      // cacheLightingRSM.comp:103-104
+      const f3 vpos = make3((wp.x - w.vmin[0]) / vsz, (wp.y - w.vmin[1]) / vsz, (wp.z - w.vmin[2]) / vsz);
+      const __amdgpu_buffer_rsrc_t cr = corner_rsrc(w.vox);
+      Trace t = trace_setup(w.vox, w.voxel_size, wp, vpos, block_record(w.blocks, b));
+      trace_planned(march_steps(w.march, b), cr, t, valid);
+      if (valid) w.vis[static_cast<size_t>(b) * w.pitch + cache] = trace_visibility(t);
+    }
   }
 }
 
@@ -431,16 +479,25 @@ void build_split_descs(const SplitViewHost* views, uint32_t n_views, void* host_
   }
 }
 
-void launch_batch_split(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands, int shadow,
-                        uint32_t max_vpl, uint32_t max_march, uint32_t target_items, const drv_sh_constants& shc,
-                        int grid, hipStream_t s) {
+void launch_batch_split_prepare(void* descs, uint32_t n_views, SplitTotals* totals, int sh_bands, int shadow,
+                                uint32_t max_vpl, uint32_t max_march, uint32_t target_items, hipStream_t s) {
   SplitDev* d = static_cast<SplitDev*>(descs);
   const uint32_t vpl_groups = (max_vpl + 255u) / 256u;
   hipLaunchKernelGGL(k_split_prepare, dim3(vpl_groups + (max_march + 255u) / 256u, n_views), dim3(256), 0, s, d,
                      vpl_groups);
   hipLaunchKernelGGL(k_split_plan, dim3(1), dim3(256), 0, s, d, n_views, target_items, shadow, sh_bands >= 2 ? 1 : 0,
                      totals);
-  if (shadow) hipLaunchKernelGGL(k_split_visibility, dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
+}
+
+void launch_batch_split_visibility(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int grid,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL(k_split_visibility, dim3(grid), dim3(256), 0, s, static_cast<SplitDev*>(descs), n_views, stride,
+                     totals);
+}
+
+void launch_batch_split_gather(void* descs, uint32_t n_views, SplitTotals* totals, int stride, int sh_bands,
+                               int shadow, const drv_sh_constants& shc, int grid, hipStream_t s) {
+  SplitDev* d = static_cast<SplitDev*>(descs);
   const bool sh2 = sh_bands >= 2;
   if (sh2 && shadow) hipLaunchKernelGGL((k_split_gather<true, true>), dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);
   else if (sh2) hipLaunchKernelGGL((k_split_gather<true, false>), dim3(grid), dim3(256), 0, s, d, n_views, stride, totals);

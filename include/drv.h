@@ -450,6 +450,10 @@ typedef struct drv_batch drv_batch;
  * items of every view, and the gather's VPL splits are planned over all views
  * together. */
 #define DRV_BATCH_SPLIT_QUEUE (-1)
+/* The split queue with the cone-traced visibility of each view in a launch of
+ * its own (the single-view kernel) and the other stages queued over all
+ * views. */
+#define DRV_BATCH_SPLIT_GATHER (-2)
 
 int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out);
 int drv_batch_destroy(drv_batch* batch);
@@ -461,7 +465,8 @@ const char* drv_batch_last_error(const drv_batch* batch);
  * Caches per wavefront 0 selects per-view passes instead of the fused queue:
  * each view's own cone trace, SH gather and reduction (drv_light_caches)
  * back to back on the shared stream; DRV_BATCH_SPLIT_QUEUE selects the split
- * queue (the VPL tile is then unused). */
+ * queue and DRV_BATCH_SPLIT_GATHER the split queue with per-view visibility
+ * launches (the VPL tile is then unused). */
 int drv_batch_set_tuning(drv_batch* batch, int32_t caches_per_wave, int32_t vpl_tile, int32_t waves_per_cu);
 int drv_batch_get_tuning(const drv_batch* batch, int32_t* caches_per_wave, int32_t* vpl_tile, int32_t* waves_per_cu);
 /* rsms[i], lights[i] belong to view i. */

@@ -139,7 +139,8 @@ def test_batch_view_matches_oracle():
     assert rel_l2(s1, entries_sh(e[order], 1)) <= 1e-4
 
 
-@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536), (0, 256), (-1, 1024)])
+@pytest.mark.parametrize("cpw,tile", [(64, 256), (32, 1024), (16, 16384), (64, 65536), (0, 256), (-1, 1024),
+                                      (-2, 1024)])
 def test_batch_decompositions_agree(cpw, tile):
     from dynamicradiancevolume_amd.pipeline import Batch
 
@@ -159,15 +160,17 @@ def test_batch_decompositions_agree(cpw, tile):
         assert rel_l2(s1, s0) <= SH_TOL
 
 
-def test_split_queue_visibility_and_sh_equal_single_views():
+@pytest.mark.parametrize("mode", [-1, -2])
+def test_split_queue_visibility_and_sh_equal_single_views(mode):
     """The split queue runs the single-view kernels' item bodies over all
-    views: every view's per-pair visibility is bit-identical to its own
+    views (mode -2: the visibility of each view in a single-view launch):
+    every view's per-pair visibility is bit-identical to its own
     drv_light_caches, and its SH differ only by the VPL split count (planned
     over all views together)."""
     from dynamicradiancevolume_amd.pipeline import Batch
 
     frames = _frames()
-    announce(f"split queue of {len(frames)} views vs single views (visibility bit-exact)")
+    announce(f"split queue (mode {mode}) of {len(frames)} views vs single views (visibility bit-exact)")
     ref = []
     rs1, dfs1 = _group(frames, 2)
     for r, df, fi in zip(rs1, dfs1, frames):
@@ -178,7 +181,7 @@ def test_split_queue_visibility_and_sh_equal_single_views():
         r.close()
     rs, dfs = _group(frames, 2)
     b = Batch(rs)
-    b.set_tuning(-1, 1024)
+    b.set_tuning(mode, 1024)
     b.render_frame(dfs)
     for ((p0, s0), v0, pos0), r, fi in zip(ref, rs, frames):
         p1, s1 = _by_position(r)
@@ -206,13 +209,13 @@ def test_batch_autotune_restores_entries():
     b.render_frame(dfs)
     before = [r.read_caches(r.cache_count()[1]) for r in rs]
     cpw, tile, ms = b.autotune(dfs, reps=1)
-    assert cpw in (64, 32, 16, 0, -1) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
+    assert cpw in (64, 32, 16, 0, -1, -2) and tile >= 256 and (tile & (tile - 1)) == 0 and ms > 0
     assert b.tuning()[:2] == (cpw, tile)
     for r, e in zip(rs, before):
         np.testing.assert_array_equal(r.read_caches(len(e)), e)
 
 
-@pytest.mark.parametrize("cpw", [64, -1])
+@pytest.mark.parametrize("cpw", [64, -1, -2])
 def test_graph_replay_matches_eager(cpw):
     from dynamicradiancevolume_amd.pipeline import Batch, FrameGraph
 

@@ -3,8 +3,8 @@ family (1280x720 to 3840x2160, 4096 or 16384 VPLs, own scene each; the
 views the benchmark renders).
 
 * the fused work-queue kernel (64 caches per wave, 1024-VPL tiles) and the
-  split queue give every view the SH its own drv_light_caches gives, up to
-  the order of the VPL sums;
+  split queue (with queued or per-view visibility) give every view the SH
+  its own drv_light_caches gives, up to the order of the VPL sums;
 * per-view passes give them bit for bit;
 * a spread subset of the caches of the largest view matches the FP32 oracle.
 """
@@ -46,7 +46,7 @@ def test_batch_of_64_views_matches_single_views():
     dfs = [DeviceFrame(fi) for fi in frames]
     b = Batch(rs)
     results = {}
-    for cpw, tile in ((64, 1024), (0, 1024), (-1, 1024)):
+    for cpw, tile in ((64, 1024), (0, 1024), (-1, 1024), (-2, 1024)):
         announce(f"batch of {N_VIEWS}: caches per wave {cpw}")
         b.set_tuning(cpw, tile)
         with torch.cuda.stream(stream):
@@ -63,11 +63,12 @@ def test_batch_of_64_views_matches_single_views():
         df = DeviceFrame(fi)
         one.render_frame(df)
         pos, sh, entries = _sorted_sh(one)
-        for cpw in (64, 0, -1):
+        for cpw in (64, 0, -1, -2):
             gp, gs, _ = results[cpw][i]
             np.testing.assert_array_equal(gp.view(np.uint32), pos.view(np.uint32))
         assert rel_l2(results[64][i][1], sh) <= 1e-5, (i, w.name)
         assert rel_l2(results[-1][i][1], sh) <= 1e-5, (i, w.name)
+        assert rel_l2(results[-2][i][1], sh) <= 1e-5, (i, w.name)
         np.testing.assert_array_equal(results[0][i][1], sh)
         if i == big:
             from oracle import orc
