@@ -25,6 +25,12 @@ struct drv_ctx {
   // back into `stream`, so it overlaps stage A (allocation).
   hipStream_t aux = nullptr;
   hipEvent_t fork_ev = nullptr, vox_done_ev = nullptr;
+  // Recorded on `stream` after the last enqueued work that reads the voxel
+  // volume (valid only outside graph capture): the next voxelization waits
+  // for it instead of for everything on `stream`, so it overlaps the rest of
+  // the frame (SH gather, reduction, apply).
+  hipEvent_t vox_free_ev = nullptr;
+  bool vox_free_valid = false;
   // The surrogate's spot check also runs on `aux`, beside the network pass.
   hipEvent_t spot_fork_ev = nullptr, spot_done_ev = nullptr;
   bool vox_pending = false;
@@ -201,6 +207,16 @@ void join_voxels(drv_ctx* ctx) {
   if (!ctx->vox_pending) return;
   (void)hipStreamWaitEvent(ctx->stream, ctx->vox_done_ev, 0);
   ctx->vox_pending = false;
+}
+
+bool is_capturing(hipStream_t s);
+
+// Every call that enqueues reads of the voxel volume on `stream` calls this
+// after them: the next voxelization may start once they have run.  Under
+// graph capture the event would belong to the graph, so the next
+// voxelization then waits for the whole stream instead.
+void voxels_read(drv_ctx* ctx) {
+  ctx->vox_free_valid = !is_capturing(ctx->stream) && hipEventRecord(ctx->vox_free_ev, ctx->stream) == hipSuccess;
 }
 
 // Time between two recorded events; 0 for a pair that was never recorded.
@@ -460,6 +476,7 @@ int drv_create(const drv_config* config, drv_ctx** out_ctx) {
   if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->vox_done_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->vox_free_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->spot_fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->spot_done_ev, hipEventDisableTiming) != hipSuccess)
     return bail(DRV_ERR_HIP);
@@ -478,6 +495,7 @@ int drv_destroy(drv_ctx* ctx) {
   }
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->vox_done_ev) (void)hipEventDestroy(ctx->vox_done_ev);
+  if (ctx->vox_free_ev) (void)hipEventDestroy(ctx->vox_free_ev);
   if (ctx->spot_fork_ev) (void)hipEventDestroy(ctx->spot_fork_ev);
   if (ctx->spot_done_ev) (void)hipEventDestroy(ctx->spot_done_ev);
   dev_free(ctx->caches);
@@ -660,10 +678,16 @@ int drv_prepare_spot_light(const drv_spot_params* p, drv_spot_light* o) {
 
 int drv_voxelize_begin(drv_ctx* ctx) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
-  // Fork: the side stream starts after everything already on `stream` (the
-  // previous frame still reads the voxels).
-  DRV_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
-  DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->fork_ev, 0));
+  // Fork: the side stream starts after the last enqueued work that reads the
+  // voxels (the previous frame's cone trace), or after everything already on
+  // `stream` when that point is not known (a capture, no reader since).
+  if (ctx->vox_free_valid && !is_capturing(ctx->stream)) {
+    DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->vox_free_ev, 0));
+  } else {
+    DRV_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
+    DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->fork_ev, 0));
+  }
+  ctx->vox_free_valid = false;
   record(ctx, 0, ctx->aux);
   const size_t res3 = static_cast<size_t>(ctx->vox.res) * ctx->vox.res * ctx->vox.res;
   drvk::launch_fill_u8(ctx->vox_target, res3, 0, ctx->aux);  // voxelization.cpp:105
@@ -857,6 +881,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     ctx->label_vis = ctx->visibility;
     ctx->label_pitch = static_cast<uint32_t>(cap);
   }
+  voxels_read(ctx);
   record(ctx, 7);
   drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
                       ctx->visibility, static_cast<uint32_t>(cap), interval, shadow ? 1 : 0,
@@ -913,6 +938,7 @@ int drv_light_caches_noalloc(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_li
   join_voxels(ctx);
   drvk::launch_light_static(base, *rsm, *light, v, drvk::entry_stride(ctx->cfg.sh_bands), ctx->cfg.sh_bands,
                             shadow ? 1 : 0, ctx->shc, std::max(1, ctx->cu_count * 64 / 4), ctx->stream);
+  voxels_read(ctx);
   // No per-pair visibility or training labels are left behind.
   ctx->last_n_blocks = 0;
   ctx->label_caches = nullptr;
@@ -1329,6 +1355,7 @@ int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float
   drvk::launch_mlp_features(fh, ctx->label_caches, drvk::entry_stride(ctx->cfg.sh_bands), c.live, ctx->blocks,
                             ctx->last_n_blocks, ctx->mlp_pairs, n, x_dev, ctx->label_vis, ctx->label_pitch, y_dev,
                             ctx->stream);
+  voxels_read(ctx);
   return check_hip(ctx, hipGetLastError(), "mlp_collect");
 }
 
@@ -1344,6 +1371,7 @@ int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float*
   if ((rc = drv_get_cache_count(ctx, &total, &live)) != DRV_OK) return rc;
   drvk::launch_mlp_features(fh, ctx->caches, drvk::entry_stride(ctx->cfg.sh_bands), live, ctx->blocks,
                             ctx->last_n_blocks, pairs_dev, n, x_dev, nullptr, 0, nullptr, ctx->stream);
+  voxels_read(ctx);
   return check_hip(ctx, hipGetLastError(), "mlp_features");
 }
 
@@ -1535,6 +1563,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
   } else if (b->shadow) {
     drvk::launch_batch_split_visibility(b->d_split, nv, b->d_totals, b->stride, grid, b->stream);
   }
+  for (size_t v = 0; v < n; ++v) voxels_read(b->views[v]);
   drvk::launch_batch_split_gather(b->d_split, nv, b->d_totals, b->stride, b->sh_bands, b->shadow, v0->shc, grid,
                                   b->stream);
   for (size_t v = 0; v < n; ++v) {
@@ -1713,6 +1742,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   drvk::launch_batch(b->d_views, static_cast<uint32_t>(n), b->d_queue, b->stride, b->sh_bands, b->shadow,
                      static_cast<uint32_t>(b->cpw),
                      b->vpl_tile, b->partial, b->partial_size, b->views[0]->shc, grid, b->stream);
+  for (size_t v = 0; v < n; ++v) voxels_read(b->views[v]);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, DRV_ERR_HIP, std::string("batch_light_caches: ") + hipGetErrorString(e));
   return DRV_OK;
