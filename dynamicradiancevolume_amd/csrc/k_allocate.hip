@@ -260,28 +260,72 @@ __global__ __launch_bounds__(256) void k_allocate_clear(uint32_t* cav, const uin
   if (blockIdx.x == 0 && threadIdx.x < sizeof(Counter) / 4) reinterpret_cast<uint32_t*>(counter)[threadIdx.x] = 0u;
 }
 
-// Steps of 256 cells a thread loads ahead in the count / assign scans.
+// Steps (256 cells each, 1024 with bricks) a workgroup loads ahead in the
+// count / assign scans.
 constexpr int kScanAhead = 4;
+
+// With bricks (R % 4 == 0) the 4 consecutive cells L .. L + 3 (L % 4 == 0)
+// are one brick row, 4 consecutive x at a 16-byte aligned CAV address: a
+// thread takes them with one load.  Chunks are multiples of 256 cells.
+__device__ __forceinline__ uint4 cav_row(const AssignArgs& a, uint32_t L, uint32_t& flat, CellCoord& k) {
+  k = cell_coord(a, L);
+  flat = cell_flat(a, k);
+  return *reinterpret_cast<const uint4*>(a.cav + flat);
+}
+
+__device__ __forceinline__ uint32_t claims4(const uint4& v) {
+  return static_cast<uint32_t>(is_claim(v.x)) + is_claim(v.y) + is_claim(v.z) + is_claim(v.w);
+}
 
 __global__ __launch_bounds__(256) void k_allocate_count(AssignArgs a) {
   __shared__ uint32_t part[4];
   const uint32_t begin = blockIdx.x * a.chunk;
   const uint32_t end = min(a.n_cells, begin + a.chunk);
   uint32_t n = 0;
-  for (uint32_t L0 = begin + threadIdx.x; L0 < end; L0 += 256u * kScanAhead) {
-    uint32_t v[kScanAhead];
+  if ((a.R & 3) == 0) {
+    for (uint32_t L0 = begin + 4u * threadIdx.x; L0 < end; L0 += 1024u * kScanAhead) {
+      uint4 v[kScanAhead];
 #pragma unroll
-    for (int u = 0; u < kScanAhead; ++u) {
-      const uint32_t L = L0 + 256u * u;
-      v[u] = L < end ? a.cav[cell_flat(a, cell_coord(a, L))] : 0u;
+      for (int u = 0; u < kScanAhead; ++u) {
+        const uint32_t L = L0 + 1024u * u;
+        uint32_t flat;
+        CellCoord k;
+        v[u] = L < end ? cav_row(a, L, flat, k) : make_uint4(0u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < kScanAhead; ++u) n += claims4(v[u]);
     }
+  } else {
+    for (uint32_t L0 = begin + threadIdx.x; L0 < end; L0 += 256u * kScanAhead) {
+      uint32_t v[kScanAhead];
 #pragma unroll
-    for (int u = 0; u < kScanAhead; ++u) n += is_claim(v[u]);
+      for (int u = 0; u < kScanAhead; ++u) {
+        const uint32_t L = L0 + 256u * u;
+        v[u] = L < end ? a.cav[cell_flat(a, cell_coord(a, L))] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < kScanAhead; ++u) n += is_claim(v[u]);
+    }
   }
   n = wave_sum(n);
   if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = n;
   __syncthreads();
   if (threadIdx.x == 0) a.counts[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// Cache idx for the claimed cell k (CAV value v, flat index flat): its entry
+// (Entry.Position from the claiming cascade, cacheGather.comp:75-77, and
+// zero SH) and its place in the list of claimed cells.
+__device__ __forceinline__ void assign_cell(const AssignArgs& a, const CellCoord& k, uint32_t v, uint32_t flat,
+                                            uint32_t idx) {
+  if (idx < a.capacity) {
+    const bool left = v == kClaimedLeft;
+    const float4 p = corner_position(a.vol, left ? k.c - 1 : k.c, left ? a.R : k.x, k.y, k.z);
+    float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
+    e[0] = p;
+    for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  a.claimed[idx] = flat;
 }
 
 __global__ __launch_bounds__(256) void k_allocate_assign(AssignArgs a) {
@@ -324,43 +368,84 @@ __global__ __launch_bounds__(256) void k_allocate_assign(AssignArgs a) {
   const uint32_t begin = blockIdx.x * a.chunk;
   const uint32_t end = min(a.n_cells, begin + a.chunk);
   uint32_t step = 0;
-  for (uint32_t L0 = begin; L0 < end; L0 += 256u * kScanAhead) {
-    CellCoord k[kScanAhead];
-    uint32_t flat[kScanAhead], v[kScanAhead];
+  if ((a.R & 3) == 0) {
+    // Bricks: a thread takes a brick row of 4 cells (one 16-byte load and,
+    // when it holds a claim, one 16-byte store of the new CAV values); cells
+    // are numbered in thread order, 1024 per step of the workgroup.
+    for (uint32_t L0 = begin; L0 < end; L0 += 1024u * kScanAhead) {
+      CellCoord k[kScanAhead];
+      uint32_t flat[kScanAhead];
+      uint4 v[kScanAhead];
 #pragma unroll
-    for (int u = 0; u < kScanAhead; ++u) {
-      const uint32_t L = L0 + 256u * u + t;
-      k[u] = CellCoord{};
-      flat[u] = 0u;
-      v[u] = 0u;
-      if (L < end) {
-        k[u] = cell_coord(a, L);
-        flat[u] = cell_flat(a, k[u]);
-        v[u] = a.cav[flat[u]];
+      for (int u = 0; u < kScanAhead; ++u) {
+        const uint32_t L = L0 + 1024u * u + 4u * t;
+        k[u] = CellCoord{};
+        flat[u] = 0u;
+        v[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (L < end) v[u] = cav_row(a, L, flat[u], k[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kScanAhead; ++u, step ^= 1u) {
+        const uint32_t cv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+        uint64_t m[4];
+        uint32_t wave_n = 0, before_lane = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          m[i] = __ballot(is_claim(cv[i]));
+          wave_n += static_cast<uint32_t>(__popcll(m[i]));
+          before_lane += static_cast<uint32_t>(__popcll(m[i] & lt_mask));
+        }
+        if (lane == 0) wcnt[step][w] = wave_n;
+        __syncthreads();  // wcnt[step] is rewritten two steps on, after the next barrier
+        uint32_t off = base + before_lane;
+        for (uint32_t j = 0; j < w; ++j) off += wcnt[step][j];
+        uint32_t nv[4] = {cv[0], cv[1], cv[2], cv[3]};
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (!is_claim(cv[i])) continue;
+          any = true;
+          CellCoord ki = k[u];
+          ki.x += i;
+          assign_cell(a, ki, cv[i], flat[u] + static_cast<uint32_t>(i), off);
+          nv[i] = off + 1u;  // cacheGather.comp:79
+          ++off;
+        }
+        if (any) *reinterpret_cast<uint4*>(a.cav + flat[u]) = make_uint4(nv[0], nv[1], nv[2], nv[3]);
+        base += (wcnt[step][0] + wcnt[step][1]) + (wcnt[step][2] + wcnt[step][3]);
       }
     }
+  } else {
+    for (uint32_t L0 = begin; L0 < end; L0 += 256u * kScanAhead) {
+      CellCoord k[kScanAhead];
+      uint32_t flat[kScanAhead], v[kScanAhead];
 #pragma unroll
-    for (int u = 0; u < kScanAhead; ++u, step ^= 1u) {
-      const bool claim = is_claim(v[u]);  // 0 past the end
-      const uint64_t m = __ballot(claim);
-      if (lane == 0) wcnt[step][w] = static_cast<uint32_t>(__popcll(m));
-      __syncthreads();  // wcnt[step] is rewritten two steps on, after the next barrier
-      uint32_t off = base;
-      for (uint32_t j = 0; j < w; ++j) off += wcnt[step][j];
-      if (claim) {
-        const uint32_t idx = off + static_cast<uint32_t>(__popcll(m & lt_mask));
-        if (idx < a.capacity) {
-          // Entry.Position from the claiming cascade, cacheGather.comp:75-77.
-          const bool left = v[u] == kClaimedLeft;
-          const float4 p = corner_position(a.vol, left ? k[u].c - 1 : k[u].c, left ? a.R : k[u].x, k[u].y, k[u].z);
-          float4* e = reinterpret_cast<float4*>(a.caches + static_cast<size_t>(idx) * a.stride);
-          e[0] = p;
-          for (int j = 1; j < a.stride / 4; ++j) e[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int u = 0; u < kScanAhead; ++u) {
+        const uint32_t L = L0 + 256u * u + t;
+        k[u] = CellCoord{};
+        flat[u] = 0u;
+        v[u] = 0u;
+        if (L < end) {
+          k[u] = cell_coord(a, L);
+          flat[u] = cell_flat(a, k[u]);
+          v[u] = a.cav[flat[u]];
         }
-        a.cav[flat[u]] = idx + 1u;  // cacheGather.comp:79
-        a.claimed[idx] = flat[u];
       }
-      base += (wcnt[step][0] + wcnt[step][1]) + (wcnt[step][2] + wcnt[step][3]);
+#pragma unroll
+      for (int u = 0; u < kScanAhead; ++u, step ^= 1u) {
+        const bool claim = is_claim(v[u]);  // 0 past the end
+        const uint64_t m = __ballot(claim);
+        if (lane == 0) wcnt[step][w] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();  // wcnt[step] is rewritten two steps on, after the next barrier
+        uint32_t off = base;
+        for (uint32_t j = 0; j < w; ++j) off += wcnt[step][j];
+        if (claim) {
+          const uint32_t idx = off + static_cast<uint32_t>(__popcll(m & lt_mask));
+          assign_cell(a, k[u], v[u], flat[u], idx);
+          a.cav[flat[u]] = idx + 1u;  // cacheGather.comp:79
+        }
+        base += (wcnt[step][0] + wcnt[step][1]) + (wcnt[step][2] + wcnt[step][3]);
+      }
     }
   }
   // The phantoms, after the claimed cells.

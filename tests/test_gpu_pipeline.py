@@ -89,6 +89,8 @@ def test_allocation_bit_exact(name):
         dict(transition_size=0.0),                 # no transition list
         dict(cav_cascades=1),                      # single cascade: clamped cells + phantoms
         dict(cav_cascades=2, cav_resolution=16),
+        dict(cav_cascades=2, cav_resolution=20),   # bricks, 5^3 of them (not a power of two)
+        dict(cav_cascades=3, cav_resolution=18),   # R % 4 != 0: rows of x, one cell per lane
     ],
 )
 def test_allocation_variants(overrides):
