@@ -42,6 +42,16 @@ FLOP_CONE_STEP = 60               # per march step
 FINETUNE_SAMPLES, FINETUNE_BATCH, FINETUNE_STEPS, FINETUNE_LR = 8192, 2048, 4, 1e-4
 
 
+def _renderer(cfg, stream):
+    """A renderer of the bench's static scenes: their geometry is never
+    rewritten on the stream, so the next frame's voxelization may start right
+    after the previous frame's cone trace (drv_set_voxel_overlap)."""
+    from dynamicradiancevolume_amd.pipeline import Renderer
+    r = Renderer(cfg, stream)
+    r.set_voxel_overlap(True)
+    return r
+
+
 def _host_cpu() -> dict:
     model = "unknown"
     try:
@@ -270,7 +280,7 @@ def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, f
 
     from dynamicradiancevolume_amd import surrogate as S
     from dynamicradiancevolume_amd import workloads as W
-    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, FrameGraph, Renderer
+    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, FrameGraph
 
     views = W.batch_views(n_views, seed=0)
     frames = []
@@ -278,7 +288,7 @@ def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, f
         frames.append(W.make_inputs(w, n_threads=n_threads))
         if (i + 1) % 8 == 0:
             print(f"[bench] batch inputs {i + 1}/{n_views}", file=sys.stderr, flush=True)
-    rs = [Renderer(w.config(), stream) for w in views]
+    rs = [_renderer(w.config(), stream) for w in views]
     for r in rs:
         r.set_tuning(gather_max_splits=32)
     dfs = [DeviceFrame(fi) for fi in frames]
@@ -386,11 +396,11 @@ def _batch_small(n_views: int, steps: int, stream, n_threads: int):
     import torch
 
     from dynamicradiancevolume_amd import workloads as W
-    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, Renderer
+    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame
 
     views = W.batch_views(n_views, seed=1, resolutions=W.SMALL_RESOLUTIONS, rsm=W.SMALL_RSM)
     frames = [W.make_inputs(w.with_(max_caches=16384), n_threads=n_threads) for w in views]
-    rs = [Renderer(fi.workload.config(), stream) for fi in frames]
+    rs = [_renderer(fi.workload.config(), stream) for fi in frames]
     for r in rs:
         r.set_tuning(gather_max_splits=32)
     dfs = [DeviceFrame(fi) for fi in frames]
@@ -437,9 +447,8 @@ def _inputs_and_reference(fi, df, stream, w):
     import numpy as np
     import torch
 
-    from dynamicradiancevolume_amd.pipeline import Renderer
 
-    r = Renderer(w.with_(indirect_shadow=0).config(), stream)
+    r = _renderer(w.with_(indirect_shadow=0).config(), stream)
     scene = df.scene()
 
     def timed(fn, k=5):
@@ -487,10 +496,9 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int, settle_s: 
     import numpy as np
     import torch
 
-    from dynamicradiancevolume_amd.pipeline import Renderer
 
     ws = w.with_(specular=1)
-    r = Renderer(ws.config(), stream)
+    r = _renderer(ws.config(), stream)
     _settle(lambda: r.render_frame(df), stream, settle_s)
     for _ in range(warmup):
         r.render_frame(df)
@@ -510,7 +518,7 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int, settle_s: 
     width, per_row, levels = r.specular_atlas_geometry()
     r.close()
 
-    ru = Renderer(ws.with_(indirect_shadow=0).config(), stream)
+    ru = _renderer(ws.with_(indirect_shadow=0).config(), stream)
     ru.render_frame(df)
     stream.synchronize()
     cache_rgb = df.out.clone()
@@ -523,7 +531,7 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int, settle_s: 
     err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
     ru.close()
     # the same unshadowed frame without the specular term, against the same reference
-    rd = Renderer(w.with_(indirect_shadow=0).config(), stream)
+    rd = _renderer(w.with_(indirect_shadow=0).config(), stream)
     rd.render_frame(df)
     stream.synchronize()
     d = df.out[lit].double()
@@ -550,11 +558,11 @@ def _frame_section(w, stream, steps: int, warmup: int, settle_s: float, n_thread
     import torch
 
     from dynamicradiancevolume_amd import workloads as W
-    from dynamicradiancevolume_amd.pipeline import DeviceFrame, Renderer
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame
 
     fi = W.make_inputs(w, n_threads=n_threads)
     df = DeviceFrame(fi)
-    r = Renderer(w.config(), stream)
+    r = _renderer(w.config(), stream)
     _settle(lambda: r.render_frame(df), stream, settle_s)
     for _ in range(warmup):
         r.render_frame(df)
@@ -636,7 +644,7 @@ def main() -> int:
     import torch
 
     from dynamicradiancevolume_amd import workloads as W
-    from dynamicradiancevolume_amd.pipeline import DeviceFrame, Renderer
+    from dynamicradiancevolume_amd.pipeline import DeviceFrame
 
     w = W.WORKLOADS[args.workload]
     print(f"[bench] building inputs for {w.name}", file=sys.stderr, flush=True)
@@ -645,7 +653,7 @@ def main() -> int:
     host_inputs_s = time.perf_counter() - t_in
     df = DeviceFrame(fi)
     stream = torch.cuda.Stream()
-    r = Renderer(w.config(), stream)
+    r = _renderer(w.config(), stream)
     torch.cuda.synchronize()
     n_vpl = fi.n_vpl
 

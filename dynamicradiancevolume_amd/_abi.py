@@ -184,6 +184,7 @@ DRV_FUNCTIONS = {
     "drv_get_volume": (C.c_int, [vp, P(DrvVolume)]),
     "drv_prepare_spot_light": (C.c_int, [P(DrvSpotParams), P(DrvSpotLight)]),
     "drv_voxelize_begin": (C.c_int, [vp]),
+    "drv_set_voxel_overlap": (C.c_int, [vp, C.c_int32]),
     "drv_voxelize_mesh": (C.c_int, [vp, vp, vp, C.c_uint32, P(C.c_float)]),
     "drv_voxelize_end": (C.c_int, [vp, C.c_float]),
     "drv_allocate_caches": (C.c_int, [vp, P(DrvGBuffer)]),

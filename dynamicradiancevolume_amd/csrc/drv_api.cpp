@@ -25,12 +25,14 @@ struct drv_ctx {
   // back into `stream`, so it overlaps stage A (allocation).
   hipStream_t aux = nullptr;
   hipEvent_t fork_ev = nullptr, vox_done_ev = nullptr;
-  // Recorded on `stream` after the last enqueued work that reads the voxel
-  // volume (valid only outside graph capture): the next voxelization waits
-  // for it instead of for everything on `stream`, so it overlaps the rest of
-  // the frame (SH gather, reduction, apply).
+  // With the voxel overlap on (drv_set_voxel_overlap), recorded on `stream`
+  // after the last enqueued work that reads the voxel volume (valid only
+  // outside graph capture): the next voxelization waits for it instead of
+  // for everything on `stream`, so it overlaps the rest of the frame (SH
+  // gather, reduction, apply).
   hipEvent_t vox_free_ev = nullptr;
   bool vox_free_valid = false;
+  bool vox_overlap = false;
   // The surrogate's spot check also runs on `aux`, beside the network pass.
   hipEvent_t spot_fork_ev = nullptr, spot_done_ev = nullptr;
   bool vox_pending = false;
@@ -212,11 +214,12 @@ void join_voxels(drv_ctx* ctx) {
 bool is_capturing(hipStream_t s);
 
 // Every call that enqueues reads of the voxel volume on `stream` calls this
-// after them: the next voxelization may start once they have run.  Under
-// graph capture the event would belong to the graph, so the next
-// voxelization then waits for the whole stream instead.
+// after them: with the voxel overlap on, the next voxelization may start once
+// they have run.  Under graph capture the event would belong to the graph, so
+// the next voxelization then waits for the whole stream instead.
 void voxels_read(drv_ctx* ctx) {
-  ctx->vox_free_valid = !is_capturing(ctx->stream) && hipEventRecord(ctx->vox_free_ev, ctx->stream) == hipSuccess;
+  ctx->vox_free_valid = ctx->vox_overlap && !is_capturing(ctx->stream) &&
+                        hipEventRecord(ctx->vox_free_ev, ctx->stream) == hipSuccess;
 }
 
 // Time between two recorded events; 0 for a pair that was never recorded.
@@ -676,11 +679,18 @@ int drv_prepare_spot_light(const drv_spot_params* p, drv_spot_light* o) {
 
 // ---- stage B ----------------------------------------------------------
 
+int drv_set_voxel_overlap(drv_ctx* ctx, int32_t on) {
+  if (!ctx) return DRV_ERR_INVALID_ARG;
+  ctx->vox_overlap = on != 0;
+  return DRV_OK;
+}
+
 int drv_voxelize_begin(drv_ctx* ctx) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
-  // Fork: the side stream starts after the last enqueued work that reads the
-  // voxels (the previous frame's cone trace), or after everything already on
-  // `stream` when that point is not known (a capture, no reader since).
+  // Fork: the side stream starts after everything already on `stream`, or,
+  // with the voxel overlap on, after the last enqueued work that reads the
+  // voxels (the previous frame's cone trace) when that point is known (not
+  // under capture, a reader since the last voxelization).
   if (ctx->vox_free_valid && !is_capturing(ctx->stream)) {
     DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->vox_free_ev, 0));
   } else {

@@ -141,6 +141,12 @@ class Renderer:
                                                _abi.fvec(world, 16)), "drv_voxelize_mesh")
         self._check(self.lib.drv_voxelize_end(self.ctx, adaption_factor), "drv_voxelize_end")
 
+    def set_voxel_overlap(self, on: bool):
+        """drv_set_voxel_overlap: the next voxelization may start right after
+        the previous light pass (the scene's geometry must not be written on
+        the stream after it)."""
+        self._check(self.lib.drv_set_voxel_overlap(self.ctx, int(on)), "drv_set_voxel_overlap")
+
     def allocate_caches(self, gbuffer: _abi.DrvGBuffer):
         self._check(self.lib.drv_allocate_caches(self.ctx, C.byref(gbuffer)), "drv_allocate_caches")
 

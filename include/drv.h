@@ -229,6 +229,15 @@ int drv_prepare_spot_light(const drv_spot_params* params, drv_spot_light* out);
  * end blends the target into the volume with `adaption_factor`
  * (voxelblend.comp) and rebuilds the mip chain (voxelmipmap.comp). */
 int drv_voxelize_begin(drv_ctx* ctx);
+/* Off by default: a voxelization starts after all work already enqueued on
+ * the stream.  On: drv_voxelize_begin starts it right after the last call
+ * that read the previous voxels (the previous light pass), so it runs beside
+ * the rest of that frame (SH gather, reduction, apply).  The caller then must
+ * not write, on the stream after that light pass, the positions / indices the
+ * following drv_voxelize_mesh calls read (static geometry, or geometry written
+ * before the light pass, qualifies).  Graph capture always waits for the
+ * whole stream.  No counterpart in the reference (GL orders the passes). */
+int drv_set_voxel_overlap(drv_ctx* ctx, int32_t on);
 int drv_voxelize_mesh(drv_ctx* ctx, const float* positions, const uint32_t* indices,
                       uint32_t n_tri, const float world[16]);
 int drv_voxelize_end(drv_ctx* ctx, float adaption_factor);

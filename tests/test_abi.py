@@ -71,6 +71,7 @@ def test_invalid_arguments_are_rejected():
     bad.specular_hole_fill_level = -1
     assert lib.drv_create(C.byref(bad), C.byref(ctx)) == _abi.DRV_ERR_INVALID_ARG
     assert lib.drv_destroy(None) == _abi.DRV_OK
+    assert lib.drv_set_voxel_overlap(None, 1) == _abi.DRV_ERR_INVALID_ARG
     # the atlas entry points refuse a null context
     assert lib.drv_get_specular_atlas(None, None, None, None) == _abi.DRV_ERR_INVALID_ARG
     assert lib.drv_read_specular_atlas(None, 0, None) == _abi.DRV_ERR_INVALID_ARG

@@ -237,6 +237,53 @@ def test_voxel_blend_partial_adaption():
         np.testing.assert_array_equal(r.read_voxels(level), orc.mip_level(mips2, w.voxel_resolution, level))
 
 
+@pytest.mark.parametrize("overlap", [False, True])
+def test_voxelization_order_on_the_stream(overlap):
+    """A whole frame, then the scene moved and voxelized again.  Without the
+    voxel overlap the move is written on the stream after the frame and the
+    voxelization sees it; with the overlap on (drv_set_voxel_overlap) the
+    moved geometry is written before the frame's light pass, as the contract
+    asks.  Either way the second voxelization equals the oracle's for the
+    moved scene, and the frame after it lights the caches as a renderer
+    without the overlap does."""
+    orc = _orc()
+    fi = frame_inputs("cornell_256")
+    w = fi.workload
+    df = _device(fi)
+    moved = fi.positions.copy()
+    moved[:, 0::3] += 0.7
+    announce(f"voxelization order, overlap {overlap}")
+    r = _renderer(fi)
+    r.set_voxel_overlap(overlap)
+    geometry = df.positions.clone()
+    moved_dev = torch.from_numpy(moved).cuda()
+    if overlap:
+        # written before the frame's light pass
+        geometry2 = moved_dev
+        r.render_frame(df)
+    else:
+        geometry2 = geometry
+        r.render_frame(df)
+        geometry.copy_(moved_dev)  # on the renderer's stream, after the frame
+    r.update_volume(w.eye, fi.bbox[:3], fi.bbox[3:])
+    r.voxelize_scene(geometry2, len(moved), adaption_factor=1.0)
+    vol = orc.update_volume(w.config(), w.eye, fi.bbox[:3], fi.bbox[3:])
+    target, mips = orc.voxelize(w.voxel_resolution, vol, moved, adaption_factor=1.0)
+    np.testing.assert_array_equal(r.read_voxels(-1), target)
+    for level in range(3):
+        np.testing.assert_array_equal(r.read_voxels(level), orc.mip_level(mips, w.voxel_resolution, level))
+    # the next frame (its voxelization overlapping this one's tail when on)
+    # against a renderer that never overlaps
+    ref = _renderer(fi)
+    for rr in (r, ref):
+        rr.render_frame(df)
+        rr.render_frame(df)
+    torch.cuda.synchronize()
+    live = r.cache_count()[1]
+    assert live == ref.cache_count()[1]
+    np.testing.assert_array_equal(r.read_caches(live), ref.read_caches(live))
+
+
 def _gather_case(name, n_sub=None, **overrides):
     orc = _orc()
     fi = frame_inputs(name)
