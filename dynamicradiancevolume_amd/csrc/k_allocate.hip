@@ -15,6 +15,7 @@
 // leaves to its atomics, becomes deterministic for the claimed cells and
 // groups spatial neighbours into the same gather tile.
 #include <algorithm>
+#include <cmath>
 
 #include "drv_internal.hpp"
 #include "k_common.hpp"
@@ -50,6 +51,7 @@ constexpr uint32_t kClaimedLeft = 0xFFFFFFFEu;  // claimed by the x = R corner o
 struct AllocArgs {
   drv_volume vol;
   float inv_vp[16];
+  float inv_size[DRV_MAX_CASCADES];  // 1 / world_voxel_size where that is a power of two (exact)
   const float* depth;
   uint32_t* cav;
   Counter* counter;
@@ -59,16 +61,36 @@ struct AllocArgs {
   int transitions;
 };
 
-// GetCache1DCoord, cacheGather.comp:20-30.
-__device__ __forceinline__ int cache_1d_coord(const drv_volume& vol, int R, f3 wp, int c) {
-  const drv_cascade& k = vol.cascades[c];
-  int gx = to_int((wp.x - k.min[0]) / k.world_voxel_size);
-  int gy = to_int((wp.y - k.min[1]) / k.world_voxel_size);
-  int gz = to_int((wp.z - k.min[2]) / k.world_voxel_size);
-  gx = min(R - 1, max(0, gx));
-  gy = min(R - 1, max(0, gy));
-  gz = min(R - 1, max(0, gz));
-  return gx + gy * R + gz * R * R + c * R * R * R;
+// A cell of cascade c: local coordinates and the flat index of
+// GetCache1DCoord (cacheGather.comp:20-30), which the leader rule compares.
+struct Cell {
+  int x, y, z, flat;
+};
+
+// GetCache1DCoord, cacheGather.comp:20-30.  With power-of-two voxel sizes
+// (kPow2) the divisions are multiplications by the exact reciprocal: the
+// same correctly rounded quotient.
+template <bool kPow2>
+__device__ __forceinline__ Cell cell_of(const AllocArgs& a, f3 wp, int c) {
+  const drv_cascade& k = a.vol.cascades[c];
+  float qx, qy, qz;
+  if (kPow2) {
+    const float s = a.inv_size[c];
+    qx = (wp.x - k.min[0]) * s;
+    qy = (wp.y - k.min[1]) * s;
+    qz = (wp.z - k.min[2]) * s;
+  } else {
+    qx = (wp.x - k.min[0]) / k.world_voxel_size;
+    qy = (wp.y - k.min[1]) / k.world_voxel_size;
+    qz = (wp.z - k.min[2]) / k.world_voxel_size;
+  }
+  const int R = a.R;
+  Cell e;
+  e.x = min(R - 1, max(0, to_int(qx)));
+  e.y = min(R - 1, max(0, to_int(qy)));
+  e.z = min(R - 1, max(0, to_int(qz)));
+  e.flat = e.x + e.y * R + e.z * R * R + c * R * R * R;
+  return e;
 }
 
 // Entry.Position of corner (lx, ly, lz) of cascade c, cacheGather.comp:75-77.
@@ -87,16 +109,10 @@ __device__ __forceinline__ float4 corner_position(const drv_volume& vol, int c, 
 // outside the image "succeeds" (image atomics outside the image return 0)
 // and appends a cache no cell points to: those appends are made here, one
 // atomicAdd per wave, into the phantom list.  Called by every lane of the
-// wave (inactive lanes pass active = false).
-__device__ void mark_corners(const AllocArgs& a, bool active, int coord, int c) {
-  const int R = a.R, R2 = R * R, R3 = R2 * R;
-  int bx = 0, by = 0, bz = 0;
-  if (active) {
-    coord -= R3 * c;
-    bz = coord / R2;
-    by = (coord - bz * R2) / R;
-    bx = coord % R;
-  }
+// wave (inactive lanes pass active = false) with the cell (bx, by, bz) of
+// cascade c.
+__device__ void mark_corners(const AllocArgs& a, bool active, int bx, int by, int bz, int c) {
+  const int R = a.R;
   uint32_t outside = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -130,6 +146,7 @@ __device__ void mark_corners(const AllocArgs& a, bool active, int coord, int c) 
   }
 }
 
+template <bool kPow2>
 __global__ __launch_bounds__(256) void k_allocate(AllocArgs a) {
   __shared__ int list[256];
   __shared__ int list2[256];
@@ -137,15 +154,20 @@ __global__ __launch_bounds__(256) void k_allocate(AllocArgs a) {
   const int lx = t & 15, ly = t >> 4;
   const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
   int own = -1, own2 = -1, c = -1;
+  Cell e{0, 0, 0, -1}, e2{0, 0, 0, -1};
   if (x < a.width && y < a.height) {
     const float d = a.depth[static_cast<size_t>(y) * a.width + x];
     if (d > 0.0001f) {
       const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d);
       c = cascade_of(a.vol, a.C, wp);
-      own = cache_1d_coord(a.vol, a.R, wp, c);
+      e = cell_of<kPow2>(a, wp, c);
+      own = e.flat;
       if (a.transitions) {
         const float tr = cascade_transition(a.vol, c, wp);
-        if (tr > 0.0f && c < a.C - 1) own2 = cache_1d_coord(a.vol, a.R, wp, c + 1);
+        if (tr > 0.0f && c < a.C - 1) {
+          e2 = cell_of<kPow2>(a, wp, c + 1);
+          own2 = e2.flat;
+        }
       }
     }
   }
@@ -167,8 +189,8 @@ __global__ __launch_bounds__(256) void k_allocate(AllocArgs a) {
                        (list2[vy * 16 + vx] != own2)) ||
                       (vx == 15 && vy == 15)) &&
                      own2 != -1;
-  mark_corners(a, lead, own, c);
-  if (a.transitions) mark_corners(a, lead2, own2, c + 1);
+  mark_corners(a, lead, e.x, e.y, e.z, c);
+  if (a.transitions) mark_corners(a, lead2, e2.x, e2.y, e2.z, c + 1);
 }
 
 // ---- pass 2: cache indices in cell order -------------------------------
@@ -489,8 +511,24 @@ void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_fra
   a.C = cfg.cav_cascades;
   a.capacity = static_cast<uint32_t>(cfg.max_caches);
   a.transitions = cfg.cav_transition_size > 0.0f ? 1 : 0;
+  bool pow2 = true;
+  for (int c = 0; c < DRV_MAX_CASCADES; ++c) {
+    a.inv_size[c] = 0.0f;
+    if (c >= a.C) continue;
+    int e = 0;
+    const float m = std::frexp(vol.cascades[c].world_voxel_size, &e);  // size = m 2^e, m in [0.5, 1)
+    if (m == 0.5f && e > -120 && e < 120) {
+      a.inv_size[c] = std::ldexp(1.0f, 1 - e);
+    } else {
+      pow2 = false;
+    }
+  }
   const dim3 grid((cfg.width + 15) / 16, (cfg.height + 15) / 16);
-  hipLaunchKernelGGL(k_allocate, grid, dim3(256), 0, s, a);
+  if (pow2) {
+    hipLaunchKernelGGL(k_allocate<true>, grid, dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_allocate<false>, grid, dim3(256), 0, s, a);
+  }
 
   AssignArgs b;
   b.vol = vol;
