@@ -170,6 +170,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
 // every sample (and the block record) are wave-uniform scalar loads, and
 // each lane only advances, fetches and blends.  The wave traces its item's
 // blocks one after the other, all lanes on the same block.
+template <bool kCount>
 __global__ __launch_bounds__(256) void k_visibility_planned(VisArgs a) {
   const __amdgpu_buffer_rsrc_t cr = corner_rsrc(a.vox);
   const uint32_t live = a.counter->live;
@@ -195,18 +196,19 @@ __global__ __launch_bounds__(256) void k_visibility_planned(VisArgs a) {
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
       Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, block_record(a.blocks, b));
-      trace_planned(march_steps(a.march, b), cr, t, valid);
+      trace_planned<kCount>(march_steps(a.march, b), cr, t, valid);
       if (valid) {
         a.vis[static_cast<size_t>(b) * a.pitch + cache] = trace_visibility(t);
-        my_traces += 1u;
-        my_exec += static_cast<uint32_t>(t.s);
-        if (a.steps)
+        if (kCount) {
+          my_traces += 1u;
+          my_exec += static_cast<uint32_t>(t.s);
           my_steps += static_cast<uint32_t>(
               reference_steps_planned(a.march + static_cast<size_t>(b) * kMarchSteps, t));
+        }
       }
     }
   }
-  if (a.steps) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
+  if (kCount) count_steps(a.steps, lane, my_traces, my_steps, my_exec);
 }
 
 // The same traces with lane refilling: a work item's n_valid x n_blocks
@@ -396,7 +398,11 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
   a.march = march;
   const dim3 g(grid), b(256);
   if (march && tune.visibility_kernel == 0 && tune.steps_per_batch == 1) {
-    hipLaunchKernelGGL(k_visibility_planned, g, b, 0, s, a);
+    if (step_counter) {
+      hipLaunchKernelGGL(k_visibility_planned<true>, g, b, 0, s, a);
+    } else {
+      hipLaunchKernelGGL(k_visibility_planned<false>, g, b, 0, s, a);
+    }
   } else if (tune.visibility_kernel == 1) {
     switch (tune.steps_per_batch) {
       case 1: hipLaunchKernelGGL(k_visibility_pool<1>, g, b, 0, s, a); break;

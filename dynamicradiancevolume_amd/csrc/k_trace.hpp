@@ -169,7 +169,10 @@ __device__ __forceinline__ MarchStep march_step(MarchSteps pl, int n) {
 // same block (pl is wave-uniform); lanes with live = false take no samples.
 // Each sample is the one trace_batch takes (same positions, levels, weights
 // and transmittance updates, so the same values), without the ray computing
-// the plan itself: per step the lanes only advance, fetch and blend.
+// the plan itself: per step the lanes only advance, fetch and blend.  t.s
+// (samples taken) is kept only with kCount: it costs a VALU instruction per
+// sample and only the step counters read it.
+template <bool kCount = false>
 __device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsrc_t cr, Trace& t, bool live) {
   if (!live) return;
   // The lanes that march stay in lockstep (all start at sample 0 and take
@@ -192,7 +195,7 @@ __device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsr
     float o = o1;
     if (cur.two) o = __builtin_fmaf(cur.frac, blend_corner(c2, f2.ax, f2.ay, f2.az) - o1, o1);
     t.trans = __builtin_fmaf(-(t.trans * (1.0f / 255.0f)), o, t.trans);
-    t.s = n + 1;
+    if (kCount) t.s = n + 1;
     if (cur.dist >= t.goal || t.trans <= kOpaque) break;
   }
 }
