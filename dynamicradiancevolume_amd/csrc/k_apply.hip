@@ -189,6 +189,11 @@ __device__ __forceinline__ void cache_irradiance(const ApplyArgs& a, const float
 // entry.  Per lane the arithmetic and its order are those of one fetch per
 // pixel.  Every lane of the wave must call this (the staging is
 // cooperative); `active` = false lanes add nothing.
+// Two neighbouring CAV cells (dword alignment suffices for the 8-byte load).
+struct __attribute__((aligned(4))) CellPair {
+  uint32_t a, b;
+};
+
 template <bool kSpec>
 __device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp, f3 n, int c, const float* diffuse,
                                          uint32_t live, const SpecPixel& sp, float* stage, float* out) {
@@ -203,15 +208,29 @@ __device__ __forceinline__ void lighting(const ApplyArgs& a, bool active, f3 wp,
                       jx * jy * iz, ix * jy * iz, jx * iy * iz, ix * iy * iz};
   const int R = a.R, CR = a.C * a.R;
   uint32_t addr[8];
+  // Offsets x fastest: (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),(1,0,1),
+  // (0,1,1),(1,1,1).  The two cells of a pair (x, x + 1) are neighbours in
+  // the CAV row, so each pair is one 8-byte load (4-byte aligned) when both
+  // lie in the row; texelFetch outside the CAV reads 0 -> address 0xFFFFFFFF.
+  const int x0 = ax + R * c;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    // offsets x fastest: (0,0,0),(1,0,0),(0,1,0),(1,1,0),(0,0,1),(1,0,1),(0,1,1),(1,1,1)
-    const int x = ax + (i & 1) + R * c, y = ay + ((i >> 1) & 1), z = az + (i >> 2);
-    // texelFetch outside the CAV reads 0 -> address 0xFFFFFFFF
-    uint32_t v = 0u;
-    if (active && x >= 0 && y >= 0 && z >= 0 && x < CR && y < R && z < R)
-      v = a.cav[(static_cast<uint32_t>(z) * R + y) * CR + x];
-    addr[i] = v - 1u;
+  for (int p = 0; p < 4; ++p) {
+    const int y = ay + (p & 1), z = az + (p >> 1);
+    const bool row = active && y >= 0 && z >= 0 && y < R && z < R;
+    uint32_t v0 = 0u, v1 = 0u;
+    if (row) {
+      const uint32_t* r = a.cav + (static_cast<uint32_t>(z) * R + y) * CR;
+      if (x0 >= 0 && x0 + 1 < CR) {
+        const CellPair q = *reinterpret_cast<const CellPair*>(r + x0);
+        v0 = q.a;
+        v1 = q.b;
+      } else {
+        if (x0 >= 0 && x0 < CR) v0 = r[x0];
+        if (x0 + 1 >= 0 && x0 + 1 < CR) v1 = r[x0 + 1];
+      }
+    }
+    addr[2 * p] = v0 - 1u;
+    addr[2 * p + 1] = v1 - 1u;
   }
   const int nq = a.sh2 ? 8 : 4;
   float acc[3] = {0.0f, 0.0f, 0.0f};
