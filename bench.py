@@ -508,7 +508,8 @@ def _specular(fi, df, stream, w, steps: int, warmup: int, n_vpl: int, settle_s: 
     acc = {k: 0.0 for k in keys}
     n_prof = max(3, min(10, steps))
     for _ in range(n_prof):
-        r.render_frame(df)
+        for _ in range(8):  # back to back, as in the timed region; the last frame's stage times
+            r.render_frame(df)
         t = r.stage_times()
         for k in keys:
             acc[k] += t[k]
@@ -597,16 +598,20 @@ def _frame_section(w, stream, steps: int, warmup: int, settle_s: float, n_thread
     return out
 
 
-def _stage_profile(r, df, steps: int):
-    """Per-stage ms (HIP events inside the library, averaged over a few
-    frames) and the march-step counts of one more frame with counting on."""
+def _stage_profile(r, df, steps: int, frames_per_round: int = 8):
+    """Per-stage ms (HIP events inside the library) and the march-step
+    counts of one more frame with counting on.  Each round renders frames
+    back to back, as the timed region does, and reads the stage times of the
+    last one (the library keeps the latest frame's events): a frame read
+    right after a host synchronization would follow an idle GPU."""
     r.set_profiling(1)
     keys = ("voxelize_ms", "voxel_mips_ms", "allocate_ms", "light_caches_ms", "prepare_ms", "visibility_ms",
             "gather_ms", "reduce_ms", "apply_ms")
     acc = {k: 0.0 for k in keys}
     n_prof = max(3, min(10, steps))
     for _ in range(n_prof):
-        r.render_frame(df)
+        for _ in range(frames_per_round):
+            r.render_frame(df)
         t = r.stage_times()
         for k in keys:
             acc[k] += t[k]

@@ -891,8 +891,8 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
     ctx->label_vis = ctx->visibility;
     ctx->label_pitch = static_cast<uint32_t>(cap);
   }
-  voxels_read(ctx);
   record(ctx, 7);
+  voxels_read(ctx);  // after the stage's end event, outside the visibility interval
   drvk::launch_gather(ctx->shc, ctx->cfg.sh_bands, ctx->caches, stride, ctx->counter, ctx->vpls, n_vpl,
                       ctx->visibility, static_cast<uint32_t>(cap), interval, shadow ? 1 : 0,
                       ctx->partial, static_cast<uint32_t>(cap), static_cast<uint32_t>(ctx->tune.gather_max_splits),
