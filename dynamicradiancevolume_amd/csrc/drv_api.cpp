@@ -682,6 +682,9 @@ int drv_prepare_spot_light(const drv_spot_params* p, drv_spot_light* o) {
 int drv_set_voxel_overlap(drv_ctx* ctx, int32_t on) {
   if (!ctx) return DRV_ERR_INVALID_ARG;
   ctx->vox_overlap = on != 0;
+  // An event recorded while the overlap was on must not start a
+  // voxelization after it is turned off.
+  if (!ctx->vox_overlap) ctx->vox_free_valid = false;
   return DRV_OK;
 }
 

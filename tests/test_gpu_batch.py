@@ -199,6 +199,36 @@ def test_split_queue_visibility_and_sh_equal_single_views(mode):
     b.close()
 
 
+@pytest.mark.parametrize("mode", [0, -2, 64])
+def test_batch_with_voxel_overlap_equals_without(mode):
+    """With the voxel overlap on, each view's next voxelization starts after
+    the batch's cone trace and runs beside the rest of the batch frame; over
+    three batch frames the caches equal those of a batch without it, bit for
+    bit, in every decomposition (compared by position: phantom caches are
+    appended in atomic order)."""
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce(f"batch mode {mode}: voxel overlap on vs off")
+    out = []
+    for overlap in (False, True):
+        rs, dfs = _group(frames, 2)
+        for r in rs:
+            r.set_voxel_overlap(overlap)
+        b = Batch(rs)
+        b.set_tuning(mode, 1024)
+        for _ in range(3):
+            b.render_frame(dfs)
+        torch.cuda.synchronize()
+        out.append([_by_position(r) for r in rs])
+        b.close()
+        for r in rs:
+            r.close()
+    for (p0, s0), (p1, s1), fi in zip(out[0], out[1], frames):
+        np.testing.assert_array_equal(p0, p1, err_msg=fi.workload.name)
+        np.testing.assert_array_equal(s0, s1, err_msg=fi.workload.name)
+
+
 def test_batch_autotune_restores_entries():
     from dynamicradiancevolume_amd.pipeline import Batch
 
