@@ -387,6 +387,69 @@ def _batch(n_views: int, steps: int, stream, n_threads: int, threshold: float, f
     return out, sur
 
 
+def _batch_dense(n_views: int, steps: int, stream, n_threads: int):
+    """Batched views at the cache scale of the batched configuration: seeded
+    boxes views at 1080p and 1440p with the 4 x 240^3 address volume (tens of
+    thousands of caches per view), rendered view after view and as one batch
+    (the autotuned decomposition), with each decomposition's light pass."""
+    import torch
+
+    from dynamicradiancevolume_amd import workloads as W
+    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame
+
+    views = [w.with_(max_caches=262144) for w in
+             W.batch_views(n_views, seed=2, family=W.BOXES_1080P_DENSE, resolutions=W.DENSE_BATCH_RESOLUTIONS)]
+    frames = [W.make_inputs(w, n_threads=n_threads) for w in views]
+    rs = [_renderer(w.config(), stream) for w in views]
+    dfs = [DeviceFrame(fi) for fi in frames]
+    torch.cuda.synchronize()
+
+    def timed(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    k = max(2, steps // 4)
+    seq_ms = timed(lambda: [r.render_frame(df) for r, df in zip(rs, dfs)], k)
+    lives = [r.cache_count()[1] for r in rs]
+    interactions = sum(lv * df.inputs.n_vpl for lv, df in zip(lives, dfs))
+    b = Batch(rs)
+    b.render_frame(dfs)
+    by_mode = {}
+    for label, mode in (("per_view_passes", 0), ("split_queue", -1), ("split_gather", -2), ("fused_queue", 64)):
+        b.set_tuning(mode, 1024)
+        by_mode[label] = timed(lambda: b.light_caches(dfs), k)
+    cpw, tile, tune_ms = b.autotune(dfs, reps=2)
+    batch_ms = timed(lambda: b.render_frame(dfs), k)
+    print(f"[bench] dense batch of {n_views}: sequential {seq_ms:.2f} ms, batched {batch_ms:.2f} ms "
+          f"({sum(lives)} caches)", file=sys.stderr, flush=True)
+    out = {
+        "views": n_views,
+        "cav": "4x240^3",
+        "resolutions": sorted({f"{w.width}x{w.height}" for w in views}),
+        "vpls_per_view": sorted({df.inputs.n_vpl for df in dfs}),
+        "live_caches": [min(lives), max(lives), sum(lives)],
+        "interactions_per_batch": interactions,
+        "sequential_ms": seq_ms,
+        "batched_ms": batch_ms,
+        "speedup_batched_vs_sequential": seq_ms / batch_ms,
+        "interactions_per_s_batched": interactions / (batch_ms * 1e-3),
+        "autotuned": {"caches_per_wave": cpw, "vpl_tile": tile, "light_pass_ms": tune_ms},
+        "light_pass_ms_by_decomposition": by_mode,
+        "frames_timed": k,
+    }
+    b.close()
+    for r in rs:
+        r.close()
+    return out
+
+
 def _batch_small(n_views: int, steps: int, stream, n_threads: int):
     """Many small views (320x180, 4096 VPLs each, own scene each): the light
     pass of the whole batch view by view (per-view passes) against the split
@@ -637,6 +700,8 @@ def main() -> int:
     ap.add_argument("--no-specular", action="store_true")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched workload (0 = skip)")
     ap.add_argument("--small-views", type=int, default=64, help="views of the small-view batch (0 = skip)")
+    ap.add_argument("--dense-views", type=int, default=8,
+                    help="views of the batch with 4 x 240^3 address volumes (0 = skip)")
     ap.add_argument("--dense", default="boxes_1080p_dense",
                     help="second single-view workload at the ~60k-cache scale ('' = skip)")
     ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
@@ -760,6 +825,12 @@ def main() -> int:
         print(f"[bench] batch of {args.small_views} small views", file=sys.stderr, flush=True)
         batch_small = _batch_small(args.small_views, args.steps, stream, n_threads)
 
+    batch_dense = None
+    if args.dense_views > 0:
+        print(f"[bench] batch of {args.dense_views} views with 4 x 240^3 address volumes", file=sys.stderr,
+              flush=True)
+        batch_dense = _batch_dense(args.dense_views, args.steps, stream, n_threads)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline and FP64 error statements (oracle on the host)", file=sys.stderr, flush=True)
@@ -816,6 +887,7 @@ def main() -> int:
         "batch": batch,
         "batch_surrogate": batch_sur,
         "batch_small": batch_small,
+        "batch_dense": batch_dense,
         "inputs_and_reference": ref,
     }
     print(json.dumps(line))

@@ -119,6 +119,8 @@ BATCH_RSM = ((128, 1, 0), (256, 1, 0), (512, 1, 1))
 # Many small views (thumbnails, probes): 320x180 with 4096 VPLs each, where
 # one launch per view and stage leaves most of the GPU idle.
 SMALL_RESOLUTIONS = ((320, 180),)
+# Views of the batch at the tens-of-thousands-of-caches scale (4 x 240^3 CAV).
+DENSE_BATCH_RESOLUTIONS = ((1920, 1080), (2560, 1440))
 SMALL_RSM = ((128, 1, 0),)
 
 
