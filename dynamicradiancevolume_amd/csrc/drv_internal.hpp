@@ -62,10 +62,11 @@ struct alignas(16) VoxelCorner {
 // block reads them (wave-uniform, scalar loads).  lv[0] is level floor(lod),
 // lv[1] level floor(lod) + 1 when two levels are blended (two = 1, frac > 0),
 // each as the constants of the corner-expanded copy: byte offset of the
-// level's first record, corners per row and per slice, w and 2^-l.
+// level's first record, corners per row and per slice (as floats: exact
+// integers below 2^24), w and 2^-l.
 struct MarchLevel {
-  uint32_t off16, wp, wp2;
-  float w, scale;
+  uint32_t off16;
+  float wp, wp2, w, scale;
 };
 struct alignas(64) MarchStep {
   float step;  // advance to this sample

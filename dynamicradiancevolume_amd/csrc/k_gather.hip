@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_visibility(VisArgs a) {
 // every sample (and the block record) are wave-uniform scalar loads, and
 // each lane only advances, fetches and blends.  The wave traces its item's
 // blocks one after the other, all lanes on the same block.
-template <bool kCount>
+template <bool kCount, bool kFloatIndex>
 __global__ __launch_bounds__(256) void k_visibility_planned(VisArgs a) {
   const __amdgpu_buffer_rsrc_t cr = corner_rsrc(a.vox);
   const uint32_t live = a.counter->live;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_visibility_planned(VisArgs a) {
     const uint32_t b1 = min(a.n_blocks, b0 + bpi);
     for (uint32_t b = b0; b < b1; ++b) {
       Trace t = trace_setup(a.vox, a.voxel_size, wp, vpos, block_record(a.blocks, b));
-      trace_planned<kCount>(march_steps(a.march, b), cr, t, valid);
+      trace_planned<kCount, kFloatIndex>(march_steps(a.march, b), cr, t, valid);
       if (valid) {
         a.vis[static_cast<size_t>(b) * a.pitch + cache] = trace_visibility(t);
         if (kCount) {
@@ -398,10 +398,20 @@ void launch_visibility(const drv_volume& vol, const VoxelView& vox, const float*
   a.march = march;
   const dim3 g(grid), b(256);
   if (march && tune.visibility_kernel == 0 && tune.steps_per_batch == 1) {
+    // Record indices in FP32 while every level's corner copy has fewer than
+    // 2^24 records ((res + 1)^3 < 2^24, res <= 128).
+    const uint64_t r1 = static_cast<uint64_t>(vox.res) + 1u;
+    const bool fidx = r1 * r1 * r1 < (1ull << 24);
     if (step_counter) {
-      hipLaunchKernelGGL(k_visibility_planned<true>, g, b, 0, s, a);
+      if (fidx) {
+        hipLaunchKernelGGL((k_visibility_planned<true, true>), g, b, 0, s, a);
+      } else {
+        hipLaunchKernelGGL((k_visibility_planned<true, false>), g, b, 0, s, a);
+      }
+    } else if (fidx) {
+      hipLaunchKernelGGL((k_visibility_planned<false, true>), g, b, 0, s, a);
     } else {
-      hipLaunchKernelGGL(k_visibility_planned<false>, g, b, 0, s, a);
+      hipLaunchKernelGGL((k_visibility_planned<false, false>), g, b, 0, s, a);
     }
   } else if (tune.visibility_kernel == 1) {
     switch (tune.steps_per_batch) {

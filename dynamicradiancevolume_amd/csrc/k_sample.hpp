@@ -66,7 +66,7 @@ struct Footprint {
   float ax, ay, az;
 };
 
-// L: LevelInfo or MarchLevel (off16, wp, wp2, w, scale).
+// L: LevelInfo (off16, wp, wp2, w, scale).
 template <class L>
 __device__ __forceinline__ Footprint footprint(const L& li, float px, float py, float pz) {
   const float u = __builtin_fmaf(px, li.scale, 0.5f);
@@ -83,6 +83,36 @@ __device__ __forceinline__ Footprint footprint(const L& li, float px, float py, 
   // u - floor(u) (v_fract_f32 caps it below 1, which only happens for
   // u in (-2^-24, 0), where both texels of the footprint are the clamped
   // edge texel and the weight has no effect).
+  f.ax = __builtin_amdgcn_fractf(u);
+  f.ay = __builtin_amdgcn_fractf(v);
+  f.az = __builtin_amdgcn_fractf(q);
+  return f;
+}
+
+// The footprint with a march record's level.  The record index i + j wp +
+// k wp^2 is formed in FP32 from the truncated coordinates (exact while it is
+// below 2^24, kFloatIndex: a corner copy of at most 2^24 records per level,
+// i.e. res <= 128) with one conversion, instead of three float-to-integer
+// conversions and two 24-bit multiplies (a conversion or a 24-bit multiply
+// costs about two FP32 adds); without kFloatIndex only i + j wp is (res <=
+// 512).  The offset equals the integer form's bit for bit.
+template <bool kFloatIndex>
+__device__ __forceinline__ Footprint footprint(const MarchLevel& li, float px, float py, float pz) {
+  const float u = __builtin_fmaf(px, li.scale, 0.5f);
+  const float v = __builtin_fmaf(py, li.scale, 0.5f);
+  const float q = __builtin_fmaf(pz, li.scale, 0.5f);
+  // Clamped first, so the truncation is floor(u) clamped to [0, w].
+  const float fi = __builtin_truncf(__builtin_amdgcn_fmed3f(u, 0.0f, li.w));
+  const float fj = __builtin_truncf(__builtin_amdgcn_fmed3f(v, 0.0f, li.w));
+  const float fk = __builtin_truncf(__builtin_amdgcn_fmed3f(q, 0.0f, li.w));
+  const float ij = __builtin_fmaf(fj, li.wp, fi);
+  Footprint f;
+  if (kFloatIndex) {
+    f.offset = (static_cast<uint32_t>(__builtin_fmaf(fk, li.wp2, ij)) << 4) + li.off16;
+  } else {
+    const uint32_t k = static_cast<uint32_t>(fk), wp2 = static_cast<uint32_t>(li.wp2);
+    f.offset = ((__umul24(k, wp2) + static_cast<uint32_t>(ij)) << 4) + li.off16;
+  }
   f.ax = __builtin_amdgcn_fractf(u);
   f.ay = __builtin_amdgcn_fractf(v);
   f.az = __builtin_amdgcn_fractf(q);
@@ -143,7 +173,7 @@ __device__ __forceinline__ MarchStep plan_march_step(const VoxelView& vx, float 
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const LevelInfo li = level_info(vx, min(l + h, q));
-    e.lv[h] = MarchLevel{li.off16, li.wp, li.wp2, li.w, li.scale};
+    e.lv[h] = MarchLevel{li.off16, static_cast<float>(li.wp), static_cast<float>(li.wp2), li.w, li.scale};
   }
   e.pad[0] = e.pad[1] = 0u;
   return e;

@@ -158,8 +158,8 @@ __device__ __forceinline__ MarchStep march_step(MarchSteps pl, int n) {
   e.two = w[3];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
-    e.lv[h] = MarchLevel{w[4 + 5 * h], w[5 + 5 * h], w[6 + 5 * h], __uint_as_float(w[7 + 5 * h]),
-                         __uint_as_float(w[8 + 5 * h])};
+    e.lv[h] = MarchLevel{w[4 + 5 * h], __uint_as_float(w[5 + 5 * h]), __uint_as_float(w[6 + 5 * h]),
+                         __uint_as_float(w[7 + 5 * h]), __uint_as_float(w[8 + 5 * h])};
   e.pad[0] = e.pad[1] = 0u;
   return e;
 }
@@ -172,7 +172,7 @@ __device__ __forceinline__ MarchStep march_step(MarchSteps pl, int n) {
 // the plan itself: per step the lanes only advance, fetch and blend.  t.s
 // (samples taken) is kept only with kCount: it costs a VALU instruction per
 // sample and only the step counters read it.
-template <bool kCount = false>
+template <bool kCount = false, bool kFloatIndex = false>
 __device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsrc_t cr, Trace& t, bool live) {
   if (!live) return;
   // The lanes that march stay in lockstep (all start at sample 0 and take
@@ -186,8 +186,8 @@ __device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsr
     t.py = __builtin_fmaf(t.dy, cur.step, t.py);
     t.pz = __builtin_fmaf(t.dz, cur.step, t.pz);
     // Both loads are issued before either record is blended.
-    const Footprint f1 = footprint(cur.lv[0], t.px, t.py, t.pz);
-    const Footprint f2 = footprint(cur.lv[1], t.px, t.py, t.pz);
+    const Footprint f1 = footprint<kFloatIndex>(cur.lv[0], t.px, t.py, t.pz);
+    const Footprint f2 = footprint<kFloatIndex>(cur.lv[1], t.px, t.py, t.pz);
     const corner_t c1 = corner_load(cr, f1.offset);
     corner_t c2;  // read only when loaded (cur.two is wave-uniform)
     if (cur.two) c2 = corner_load(cr, f2.offset);
