@@ -779,7 +779,7 @@ def main() -> int:
     achieved_exec = dflop_exec / (dms * 1e-3) / 1e12 if dms > 0 else 0.0
     roofline = {
         "bound": "valu",
-        "pipe": "fp32 vector ALU (VALU-issue-bound; the FP32 vector peak is the roofline)",
+        "pipe": "fp32 vector ALU (VALU work and per-step load latency; the FP32 vector peak is the roofline)",
         "frac_executed_steps": round(achieved_exec / PEAK_FP32_TFLOPS, 4),
         "frac_note": "frac counts 60 FLOP for every march step the reference takes; frac_executed_steps counts "
                      "only the steps sampled before a ray turns opaque",
