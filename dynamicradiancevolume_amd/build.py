@@ -2,6 +2,7 @@
 
 - ``libdrv.so``        the HIP pipeline (gfx950) behind ``include/drv.h``;
 - ``libdrv_inputs.so`` the procedural scene / G-buffer / RSM producer;
+- ``libdrv_calib.so``  the VALU instruction-rate probe (``calibrate --valu``);
 - ``oracle/liborc.so`` the CPU oracle (test infrastructure, built by its own
   Makefile).
 
@@ -73,6 +74,15 @@ def build_hip() -> Path:
     return out
 
 
+def build_calib() -> Path:
+    """libdrv_calib.so: the VALU instruction-rate probe of ``calibrate --valu``."""
+    out = PKG / "libdrv_calib.so"
+    tmp = PKG / "libdrv_calib.so.tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), str(CSRC / "calib" / "valu_rate.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_inputs() -> Path:
     out = PKG / "libdrv_inputs.so"
     tmp = PKG / "libdrv_inputs.so.tmp"
@@ -94,6 +104,7 @@ def build_all() -> None:
     build_inputs()
     build_oracle()
     build_hip()
+    build_calib()
 
 
 if __name__ == "__main__":

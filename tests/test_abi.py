@@ -126,3 +126,16 @@ def test_prepare_spot_light_matches_oracle_and_is_consistent():
     bad = _abi.DrvSpotParams(rsm_resolution=64, rsm_read_lod=0, indirect_shadow_lod=7, half_angle=0.5,
                              direction=_abi.f3(0, -1, 0), near_plane=0.1, far_plane=100.0)
     assert lib.drv_prepare_spot_light(C.byref(bad), C.byref(_abi.DrvSpotLight())) == _abi.DRV_ERR_INVALID_ARG
+
+
+def test_calibration_library_exports_its_probe():
+    """libdrv_calib.so (the VALU instruction-rate probe behind `calibrate
+    --valu`) loads and exports drv_calib_valu; an unknown op is refused
+    before any device call."""
+    from dynamicradiancevolume_amd import calibrate
+
+    lib = C.CDLL(str(calibrate.CALIB))
+    assert hasattr(lib, "drv_calib_valu")
+    ms, n = C.c_float(), C.c_double()
+    assert lib.drv_calib_valu(7, 10, 1, C.byref(ms), C.byref(n)) == -1
+    assert lib.drv_calib_valu(0, 0, 1, C.byref(ms), C.byref(n)) == -1
