@@ -707,8 +707,8 @@ def main() -> int:
     ap.add_argument("--sh-threshold", type=float, default=2e-2, help="surrogate spot-check fallback threshold")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described "
-              "in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the path is not sharded across "
+              "GPUs here", file=sys.stderr)
         return 2
 
     import torch
