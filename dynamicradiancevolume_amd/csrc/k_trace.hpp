@@ -164,22 +164,24 @@ __device__ __forceinline__ MarchStep march_step(MarchSteps pl, int n) {
   return e;
 }
 
-// The whole trace of a ray set up by trace_setup, with the samples of its
-// block's march plan `pl`: every lane of the calling wave traces towards the
-// same block (pl is wave-uniform); lanes with live = false take no samples.
-// Each sample is the one trace_batch takes (same positions, levels, weights
-// and transmittance updates, so the same values), without the ray computing
-// the plan itself: per step the lanes only advance, fetch and blend.  t.s
+// Samples n0 .. n1 - 1 of a ray set up by trace_setup (the ray has taken
+// samples 0 .. n0 - 1), with the samples of its block's march plan `pl`:
+// every lane of the calling wave traces towards the same block (pl is
+// wave-uniform) and starts at n0.  Each sample is the one trace_batch takes
+// (same positions, levels, weights and transmittance updates, so the same
+// values), without the ray computing the plan itself: per step the lanes
+// only advance, fetch and blend.  Returns true when the ray stopped at one
+// of the samples (its march reached the goal or it turned opaque).  t.s
 // (samples taken) is kept only with kCount: it costs a VALU instruction per
 // sample and only the step counters read it.
 template <bool kCount = false, bool kFloatIndex = false>
-__device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsrc_t cr, Trace& t, bool live) {
-  if (!live) return;
-  // The lanes that march stay in lockstep (all start at sample 0 and take
+__device__ __forceinline__ bool march_planned(MarchSteps pl, __amdgpu_buffer_rsrc_t cr, Trace& t, int n0, int n1) {
+  // The lanes that march stay in lockstep (all start at sample n0 and take
   // one sample per iteration until they stop), so the sample index is
   // wave-uniform and the record load is scalar.
-  MarchStep next = march_step(pl, 0);
-  for (int n = 0; n < kMarchSteps; ++n) {
+  bool stopped = false;
+  MarchStep next = march_step(pl, __builtin_amdgcn_readfirstlane(n0));
+  for (int n = n0; n < n1; ++n) {
     const MarchStep cur = next;
     next = march_step(pl, __builtin_amdgcn_readfirstlane(min(n + 1, kMarchSteps - 1)));  // a step ahead
     t.px = __builtin_fmaf(t.dx, cur.step, t.px);
@@ -196,8 +198,19 @@ __device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsr
     if (cur.two) o = __builtin_fmaf(cur.frac, blend_corner(c2, f2.ax, f2.ay, f2.az) - o1, o1);
     t.trans = __builtin_fmaf(-(t.trans * (1.0f / 255.0f)), o, t.trans);
     if (kCount) t.s = n + 1;
-    if (cur.dist >= t.goal || t.trans <= kOpaque) break;
+    if (cur.dist >= t.goal || t.trans <= kOpaque) {
+      stopped = true;
+      break;
+    }
   }
+  return stopped;
+}
+
+// The whole trace (samples 0 .. 31) of a ray; lanes with live = false take
+// no samples.
+template <bool kCount = false, bool kFloatIndex = false>
+__device__ __forceinline__ void trace_planned(MarchSteps pl, __amdgpu_buffer_rsrc_t cr, Trace& t, bool live) {
+  if (live) march_planned<kCount, kFloatIndex>(pl, cr, t, 0, kMarchSteps);
 }
 
 // Steps the reference takes for a ray traced by trace_planned (geometry
