@@ -71,95 +71,108 @@ __device__ __forceinline__ int texel_index(float p, int S) {
   return static_cast<int>(fminf(fmaxf(__builtin_fmaf(p, fs, 0.5f), 0.0f), fs));
 }
 
-// Dynamic LDS: 4 maps of (S+1)^2 RGB floats.
+// Dynamic LDS: 4 maps of (S+1)^2 RGB floats.  Two waves share a cache:
+// lane l of wave half h walks the second index range h of the lane's VALs
+// (steps h * (per_lane / 2) ... of the lane-interleaved copy) into its own map,
+// and the pair writes map 0 + map 1 out (a fixed order), so twice as many
+// waves, each half as long, balance the SIMDs.
 __global__ __launch_bounds__(256) void k_specular_gather(SpecGatherArgs a) {
   extern __shared__ float lds[];
   const int S = a.S, S1 = S + 1;
   const uint32_t map_floats = static_cast<uint32_t>(S1 * S1 * 3);
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t pair = w >> 1, half = w & 1u;
   float* map = lds + w * map_floats;
+  const float* map0 = lds + (2u * pair) * map_floats;
+  const float* map1 = map0 + map_floats;
   const uint32_t live = a.counter->live;
   const uint32_t per_lane = a.n_vpl / 64u;  // n_vpl = R^2 with R >= 8
-  for (uint32_t base = blockIdx.x * 4u; base < live; base += gridDim.x * 4u) {
-    const uint32_t c = base + w;
-    if (c >= live) continue;
+  const uint32_t mid = per_lane / 2u;
+  const uint32_t i0 = half ? mid : 0u, i1 = half ? per_lane : mid;
+  for (uint32_t base = blockIdx.x * 2u; base < live; base += gridDim.x * 2u) {  // workgroup-uniform
+    const uint32_t c = base + pair;
+    const bool active = c < live;
     for (uint32_t i = lane; i < map_floats; i += 64u) map[i] = 0.0f;
     wave_lds_sync();
-    const float* e = a.caches + static_cast<size_t>(c) * a.stride;
-    const f3 wp = make3(e[0], e[1], e[2]);
-    const LocalFrame fr = local_view_space(a.cam, wp);
-    const uint32_t m0 = lane * per_lane;
-    float shadowing = 1.0f;
-    int run_tex = -1;
-    float run_r = 0.0f, run_g = 0.0f, run_b = 0.0f;
-    for (uint32_t i = 0; i < per_lane; ++i) {
-      const uint32_t m = m0 + i;
-      const Vpl v = a.vpls[static_cast<size_t>(i) * 64u + lane];
-      // The block's shadowing changes every `interval` VPLs.
-      if (a.vis && (i == 0u || (m & (a.interval - 1u)) == 0u))
-        shadowing = a.vis[static_cast<size_t>(m >> a.interval_log2) * a.vis_pitch + c];
-      // radianceIn, cacheLightingRSM.comp:246-264 (the SH gather's expressions)
-      const float tx0 = v.px - wp.x, ty0 = v.py - wp.y, tz0 = v.pz - wp.z;
-      const float d2 = __builtin_fmaf(tz0, tz0, __builtin_fmaf(ty0, ty0, tx0 * tx0));
-      const float inv = __builtin_amdgcn_rsqf(d2);
-      const float tx = tx0 * inv, ty = ty0 * inv, tz = tz0 * inv;
-      const float nd = -__builtin_fmaf(v.nz, tz, __builtin_fmaf(v.ny, ty, v.nx * tx));
-      const float f = fminf(1.0f, fmaxf(0.0f, nd)) * shadowing * __builtin_amdgcn_rcpf(d2 + v.disc_area);
-      // cacheLightingRSM.comp:281-335
-      const f3 l = to_local(fr, tx, ty, tz);
-      const float hz0 = l.z + 1.0f;
-      const float hi = __builtin_amdgcn_rsqf(__builtin_fmaf(hz0, hz0, __builtin_fmaf(l.y, l.y, l.x * l.x)));
-      const float hx = l.x * hi, hy = l.y * hi, hz = hz0 * hi;
-      float proj[2];
-      hemi_projection(hx, hy, hz, proj);
-      const int ix = texel_index(proj[0], S), iy = texel_index(proj[1], S);
-      const float k = f * fminf(1.0f, fmaxf(0.0f, __builtin_fmaf(hz, l.z, __builtin_fmaf(hy, l.y, hx * l.x))));
-      // Run accumulation: consecutive VALs of a lane mostly hit the same
-      // texel, so the lane sums in registers and adds the run into the map
-      // when the texel changes (zero contributions change nothing).
-      const float cr = v.fr * k, cg = v.fg * k, cb = v.fb * k;
-      if (!(cr == 0.0f && cg == 0.0f && cb == 0.0f)) {
-        const int tex = iy * S1 + ix;
-        if (tex != run_tex) {
-          if (run_tex >= 0) {
-            float* t = map + run_tex * 3;
-            atomicAdd(t, run_r);
-            atomicAdd(t + 1, run_g);
-            atomicAdd(t + 2, run_b);
+    if (active) {
+      const float* e = a.caches + static_cast<size_t>(c) * a.stride;
+      const f3 wp = make3(e[0], e[1], e[2]);
+      const LocalFrame fr = local_view_space(a.cam, wp);
+      const uint32_t m0 = lane * per_lane;
+      float shadowing = 1.0f;
+      int run_tex = -1;
+      float run_r = 0.0f, run_g = 0.0f, run_b = 0.0f;
+      for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t m = m0 + i;
+        const Vpl v = a.vpls[static_cast<size_t>(i) * 64u + lane];
+        // The block's shadowing changes every `interval` VPLs.
+        if (a.vis && (i == i0 || (m & (a.interval - 1u)) == 0u))
+          shadowing = a.vis[static_cast<size_t>(m >> a.interval_log2) * a.vis_pitch + c];
+        // radianceIn, cacheLightingRSM.comp:246-264 (the SH gather's expressions)
+        const float tx0 = v.px - wp.x, ty0 = v.py - wp.y, tz0 = v.pz - wp.z;
+        const float d2 = __builtin_fmaf(tz0, tz0, __builtin_fmaf(ty0, ty0, tx0 * tx0));
+        const float inv = __builtin_amdgcn_rsqf(d2);
+        const float tx = tx0 * inv, ty = ty0 * inv, tz = tz0 * inv;
+        const float nd = -__builtin_fmaf(v.nz, tz, __builtin_fmaf(v.ny, ty, v.nx * tx));
+        const float f = fminf(1.0f, fmaxf(0.0f, nd)) * shadowing * __builtin_amdgcn_rcpf(d2 + v.disc_area);
+        // cacheLightingRSM.comp:281-335
+        const f3 l = to_local(fr, tx, ty, tz);
+        const float hz0 = l.z + 1.0f;
+        const float hi = __builtin_amdgcn_rsqf(__builtin_fmaf(hz0, hz0, __builtin_fmaf(l.y, l.y, l.x * l.x)));
+        const float hx = l.x * hi, hy = l.y * hi, hz = hz0 * hi;
+        float proj[2];
+        hemi_projection(hx, hy, hz, proj);
+        const int ix = texel_index(proj[0], S), iy = texel_index(proj[1], S);
+        const float k = f * fminf(1.0f, fmaxf(0.0f, __builtin_fmaf(hz, l.z, __builtin_fmaf(hy, l.y, hx * l.x))));
+        // Run accumulation: consecutive VALs of a lane mostly hit the same
+        // texel, so the lane sums in registers and adds the run into the map
+        // when the texel changes (zero contributions change nothing).
+        const float cr = v.fr * k, cg = v.fg * k, cb = v.fb * k;
+        if (!(cr == 0.0f && cg == 0.0f && cb == 0.0f)) {
+          const int tex = iy * S1 + ix;
+          if (tex != run_tex) {
+            if (run_tex >= 0) {
+              float* t = map + run_tex * 3;
+              atomicAdd(t, run_r);
+              atomicAdd(t + 1, run_g);
+              atomicAdd(t + 2, run_b);
+            }
+            run_tex = tex;
+            run_r = cr;
+            run_g = cg;
+            run_b = cb;
+          } else {
+            run_r += cr;
+            run_g += cg;
+            run_b += cb;
           }
-          run_tex = tex;
-          run_r = cr;
-          run_g = cg;
-          run_b = cb;
+        }
+      }
+      if (run_tex >= 0) {
+        float* t = map + run_tex * 3;
+        atomicAdd(t, run_r);
+        atomicAdd(t + 1, run_g);
+        atomicAdd(t + 2, run_b);
+      }
+    }
+    __syncthreads();  // both maps of every pair are complete
+    if (active) {
+      float* wk = a.work + static_cast<size_t>(c) * S * S * 3;
+      float* sp = a.spill + static_cast<size_t>(c) * (2 * S + 1) * 3;
+      for (uint32_t i = half * 64u + lane; i < map_floats; i += 128u) {
+        const int tex = static_cast<int>(i / 3u), ch = static_cast<int>(i % 3u);
+        const int x = tex % S1, y = tex / S1;
+        const float val = map0[i] + map1[i];
+        if (x < S && y < S) {
+          float* o = wk + (y * S + x) * 3 + ch;
+          *o = a.first ? val : *o + val;
         } else {
-          run_r += cr;
-          run_g += cg;
-          run_b += cb;
+          const int k = (x == S && y < S) ? y : (y == S && x < S) ? S + x : 2 * S;
+          sp[k * 3 + ch] = val;
         }
       }
     }
-    if (run_tex >= 0) {
-      float* t = map + run_tex * 3;
-      atomicAdd(t, run_r);
-      atomicAdd(t + 1, run_g);
-      atomicAdd(t + 2, run_b);
-    }
-    wave_lds_sync();
-    float* wk = a.work + static_cast<size_t>(c) * S * S * 3;
-    float* sp = a.spill + static_cast<size_t>(c) * (2 * S + 1) * 3;
-    for (uint32_t i = lane; i < map_floats; i += 64u) {
-      const int tex = static_cast<int>(i / 3u), ch = static_cast<int>(i % 3u);
-      const int x = tex % S1, y = tex / S1;
-      const float val = map[i];
-      if (x < S && y < S) {
-        float* o = wk + (y * S + x) * 3 + ch;
-        *o = a.first ? val : *o + val;
-      } else {
-        const int k = (x == S && y < S) ? y : (y == S && x < S) ? S + x : 2 * S;
-        sp[k * 3 + ch] = val;
-      }
-    }
-    wave_lds_sync();  // the map is zeroed again for the next cache
+    __syncthreads();  // the maps are zeroed again for the next caches
   }
 }
 
@@ -306,7 +319,9 @@ void launch_specular(const float* cam, const float* caches, int stride, const Co
   g.spill = at.spill;
   g.first = first;
   const size_t glds = static_cast<size_t>(4) * (at.S + 1) * (at.S + 1) * 3 * sizeof(float);
-  hipLaunchKernelGGL(k_specular_gather, dim3(grid), dim3(256), glds, s, g);
+  // More workgroups than fit at once (each takes two caches a round): the
+  // hardware deals the last ones out as others finish.
+  hipLaunchKernelGGL(k_specular_gather, dim3(4 * grid), dim3(256), glds, s, g);
 
   SpecFinalArgs f;
   f.counter = counter;
