@@ -146,51 +146,76 @@ __device__ void mark_corners(const AllocArgs& a, bool active, int bx, int by, in
   }
 }
 
+// 16x16 pixel tiles per pass-1 workgroup.
+constexpr int kAllocTiles = 2;
+
 template <bool kPow2>
 __global__ __launch_bounds__(256) void k_allocate(AllocArgs a) {
-  __shared__ int list[256];
-  __shared__ int list2[256];
+  // Two horizontally neighbouring 16x16 tiles per workgroup: both tiles'
+  // depths are loaded before either is unprojected, and both tiles' cell
+  // lists are built before one barrier.  The leader rule stays per tile.
+  __shared__ int list[kAllocTiles][256];
+  __shared__ int list2[kAllocTiles][256];
   const int t = threadIdx.x;
   const int lx = t & 15, ly = t >> 4;
-  const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
-  int own = -1, own2 = -1, c = -1;
-  Cell e{0, 0, 0, -1}, e2{0, 0, 0, -1};
-  if (x < a.width && y < a.height) {
-    const float d = a.depth[static_cast<size_t>(y) * a.width + x];
-    if (d > 0.0001f) {
-      const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d);
-      c = cascade_of(a.vol, a.C, wp);
-      e = cell_of<kPow2>(a, wp, c);
-      own = e.flat;
+  const int y = blockIdx.y * 16 + ly;
+  float d[kAllocTiles];
+#pragma unroll
+  for (int h = 0; h < kAllocTiles; ++h) {
+    const int x = (blockIdx.x * kAllocTiles + h) * 16 + lx;
+    d[h] = (x < a.width && y < a.height) ? a.depth[static_cast<size_t>(y) * a.width + x] : 0.0f;
+  }
+  int own[kAllocTiles], own2[kAllocTiles], c[kAllocTiles];
+  Cell e[kAllocTiles], e2[kAllocTiles];
+#pragma unroll
+  for (int h = 0; h < kAllocTiles; ++h) {
+    const int x = (blockIdx.x * kAllocTiles + h) * 16 + lx;
+    own[h] = -1;
+    own2[h] = -1;
+    c[h] = -1;
+    e[h] = Cell{0, 0, 0, -1};
+    e2[h] = Cell{0, 0, 0, -1};
+    // Pixels outside the image read depth 0: no geometry, as in the reference.
+    if (d[h] > 0.0001f) {
+      const f3 wp = unproject_pixel(a.inv_vp, a.width, a.height, x, y, d[h]);
+      c[h] = cascade_of(a.vol, a.C, wp);
+      e[h] = cell_of<kPow2>(a, wp, c[h]);
+      own[h] = e[h].flat;
       if (a.transitions) {
-        const float tr = cascade_transition(a.vol, c, wp);
-        if (tr > 0.0f && c < a.C - 1) {
-          e2 = cell_of<kPow2>(a, wp, c + 1);
-          own2 = e2.flat;
+        const float tr = cascade_transition(a.vol, c[h], wp);
+        if (tr > 0.0f && c[h] < a.C - 1) {
+          e2[h] = cell_of<kPow2>(a, wp, c[h] + 1);
+          own2[h] = e2[h].flat;
         }
       }
     }
+    list[h][t] = own[h];
+    list2[h][t] = own2[h];
   }
-  list[t] = own;
-  list2[t] = own2;
   __syncthreads();
-  // Leader rule, cacheGather.comp:142-150: no left / up / up-left neighbour
-  // of the tile holds the same cell, or this is thread (0,0).  Threads on the
-  // left column and top row compare against themselves and never lead.
-  const int ux = max(0, lx - 1), uy = max(0, ly - 1);
-  const bool lead = (((list[uy * 16 + lx] != own) && (list[ly * 16 + ux] != own) && (list[uy * 16 + ux] != own)) ||
-                     (ux == lx && uy == ly)) &&
-                    own != -1;
-  // Transition list, cacheGather.comp:154-162: mirrored towards right/down;
-  // min(15, l + 1) == (15, 15) holds for the four threads with lx, ly >= 14.
-  const int vx = min(15, lx + 1), vy = min(15, ly + 1);
-  const bool lead2 = a.transitions &&
-                     (((list2[vy * 16 + lx] != own2) && (list2[ly * 16 + vx] != own2) &&
-                       (list2[vy * 16 + vx] != own2)) ||
-                      (vx == 15 && vy == 15)) &&
-                     own2 != -1;
-  mark_corners(a, lead, e.x, e.y, e.z, c);
-  if (a.transitions) mark_corners(a, lead2, e2.x, e2.y, e2.z, c + 1);
+#pragma unroll
+  for (int h = 0; h < kAllocTiles; ++h) {
+    // Leader rule, cacheGather.comp:142-150: no left / up / up-left neighbour
+    // of the tile holds the same cell, or this is thread (0,0).  Threads on
+    // the left column and top row compare against themselves and never lead.
+    const int ux = max(0, lx - 1), uy = max(0, ly - 1);
+    const int* l1 = list[h];
+    const bool lead = (((l1[uy * 16 + lx] != own[h]) && (l1[ly * 16 + ux] != own[h]) &&
+                        (l1[uy * 16 + ux] != own[h])) ||
+                       (ux == lx && uy == ly)) &&
+                      own[h] != -1;
+    // Transition list, cacheGather.comp:154-162: mirrored towards right/down;
+    // min(15, l + 1) == (15, 15) holds for the four threads with lx, ly >= 14.
+    const int vx = min(15, lx + 1), vy = min(15, ly + 1);
+    const int* l2 = list2[h];
+    const bool lead2 = a.transitions &&
+                       (((l2[vy * 16 + lx] != own2[h]) && (l2[ly * 16 + vx] != own2[h]) &&
+                         (l2[vy * 16 + vx] != own2[h])) ||
+                        (vx == 15 && vy == 15)) &&
+                       own2[h] != -1;
+    mark_corners(a, lead, e[h].x, e[h].y, e[h].z, c[h]);
+    if (a.transitions) mark_corners(a, lead2, e2[h].x, e2[h].y, e2[h].z, c[h] + 1);
+  }
 }
 
 // ---- pass 2: cache indices in cell order -------------------------------
@@ -523,7 +548,8 @@ void launch_allocate(const drv_config& cfg, const drv_volume& vol, const drv_fra
       pow2 = false;
     }
   }
-  const dim3 grid((cfg.width + 15) / 16, (cfg.height + 15) / 16);
+  const int tiles_x = (cfg.width + 15) / 16;
+  const dim3 grid((tiles_x + kAllocTiles - 1) / kAllocTiles, (cfg.height + 15) / 16);
   if (pow2) {
     hipLaunchKernelGGL(k_allocate<true>, grid, dim3(256), 0, s, a);
   } else {
