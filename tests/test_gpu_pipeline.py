@@ -475,6 +475,29 @@ def test_tuning_variants_agree(tuning):
     assert rel_l2(e1[:, 4:], e0[:, 4:]) <= 1e-5
 
 
+@pytest.mark.parametrize("res", [128, 256])
+def test_planned_trace_index_paths_agree_with_per_lane_tracer(res):
+    """The planned cone trace forms corner-record indices in FP32 while every
+    level has fewer than 2^24 records (voxel grids up to 128^3) and in
+    integers beyond; either way its visibility equals the per-lane tracer's
+    (the lane pool at one step per batch) bit for bit."""
+    fi = frame_inputs("cornell_256")
+    df = _device(fi)
+    announce(f"planned trace index path at {res}^3 voxels vs per-lane tracer")
+    vis = []
+    for tuning in ({}, dict(visibility_kernel=1, steps_per_batch=1)):
+        r = _renderer(fi, voxel_resolution=res)
+        if tuning:
+            r.set_tuning(**tuning)
+        r.render_frame(df)
+        _, live = r.cache_count()
+        nb = fi.n_vpl // int(fi.light.indirect_shadow_sample_interval)
+        vis.append(r.read_visibility(live, nb))
+        r.close()
+    assert (vis[0] > 0).any() and (vis[0] < 1).any()
+    np.testing.assert_array_equal(vis[0], vis[1])
+
+
 def test_stage_times_after_late_profiling():
     """Turning profiling on after unprofiled frames: querying stages that were
     never timed reports 0 and leaves no error for the next call."""
