@@ -339,9 +339,10 @@ def test_gather_cornell_unshadowed():
     _gather_case("cornell_256", indirect_shadow=0, sh_bands=2)
 
 
-def test_gather_boxes_subset():
-    announce("gather boxes_1080p subset")
-    _gather_case("boxes_1080p", n_sub=384)
+def test_gather_boxes_all_caches():
+    """Every cache of the 1080p frame (6.5M cone traces) against the oracle."""
+    announce("gather boxes_1080p, all caches")
+    _gather_case("boxes_1080p")
 
 
 def test_gather_cornell_wide_rsm():
@@ -370,7 +371,7 @@ def test_gather_boxes_32768_vpls_subset():
 def test_gather_boxes_dense_subset():
     """The ~60k-cache frame: a spread subset of its caches against the oracle."""
     announce("gather boxes_1080p_dense subset")
-    r, df, fi, cfg, vol, live = _gather_case("boxes_1080p_dense", n_sub=512)
+    r, df, fi, cfg, vol, live = _gather_case("boxes_1080p_dense", n_sub=4096)
     assert live > 50000
 
 
