@@ -459,7 +459,7 @@ def _batch_small(n_views: int, steps: int, stream, n_threads: int):
     import torch
 
     from dynamicradiancevolume_amd import workloads as W
-    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame
+    from dynamicradiancevolume_amd.pipeline import Batch, DeviceFrame, FrameGraph
 
     views = W.batch_views(n_views, seed=1, resolutions=W.SMALL_RESOLUTIONS, rsm=W.SMALL_RSM)
     frames = [W.make_inputs(w.with_(max_caches=16384), n_threads=n_threads) for w in views]
@@ -494,9 +494,22 @@ def _batch_small(n_views: int, steps: int, stream, n_threads: int):
                                                                            out["split_gather_light_ms"])
     cpw, tile, ms = b.autotune(dfs, reps=2)
     out["autotuned"] = {"caches_per_wave": cpw, "vpl_tile": tile, "light_pass_ms": ms}
+    # The eager frames above are issued view by view from Python (about ten
+    # library calls per view); a replayed graph of the autotuned batch frame
+    # issues the same work without that host time.
+    g = FrameGraph(rs[0])
+    with torch.cuda.stream(stream):
+        b.render_frame(dfs)
+        stream.synchronize()
+        g.begin()
+        b.render_frame(dfs)
+        g.end()
+    out["graph_frame_ms"] = timed(g.launch, k)
+    out["interactions_per_s_graph"] = sum(lives) * 4096 / (out["graph_frame_ms"] * 1e-3)
     print(f"[bench] {n_views} small views: light pass per-view {out['per_view_passes_light_ms']:.2f} ms, "
           f"split queue {out['split_queue_light_ms']:.2f} ms, split gather {out['split_gather_light_ms']:.2f} ms, "
-          f"fused {out['fused_queue_light_ms']:.2f} ms", file=sys.stderr, flush=True)
+          f"fused {out['fused_queue_light_ms']:.2f} ms; graph frame {out['graph_frame_ms']:.2f} ms",
+          file=sys.stderr, flush=True)
     b.close()
     for r in rs:
         r.close()

@@ -237,6 +237,9 @@ __global__ __launch_bounds__(256) void k_split_prepare(const SplitDev* views, ui
 // counts are loaded in parallel (a chain of dependent loads per view in one
 // thread took ~1 us per view), then one thread walks the LDS copy.
 constexpr uint32_t kPlanChunk = 1024;
+// Shadow blocks of one visibility item of the split queue (a lane traces its
+// cache towards them one after the other).
+constexpr uint32_t kSplitBpi = 4;
 __global__ __launch_bounds__(256) void k_split_plan(SplitDev* views, uint32_t n_views, uint32_t target_items,
                                                     int shadow, int sh2, SplitTotals* totals) {
   __shared__ uint32_t live_s[kPlanChunk];
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(256) void k_split_plan(SplitDev* views, uint32_t n_
         d.vis_begin = run[0];
         d.gather_begin = run[1];
         d.reduce_begin = run[2];
-        if (shadow) run[0] += tiles * d.n_blocks;
+        if (shadow) run[0] += tiles * ((d.n_blocks + kSplitBpi - 1u) / kSplitBpi);
         run[1] += tiles * groups;
         run[2] += tiles * n_jc;
       }
@@ -319,7 +322,8 @@ __device__ __forceinline__ uint32_t split_view(const SplitDev* views, uint32_t n
 // What the visibility stage reads of a view, loaded when a wave's items
 // move into it (wave-uniform scalar loads).
 struct VisView {
-  uint32_t begin, end, tiles, live, pitch;
+  uint32_t begin, end, tiles, live, pitch, n_blocks;
+  bool fidx;  // record indices in FP32 (every level below 2^24 records)
   const float* caches;
   float* vis;
   const ShadowBlock* blocks;
@@ -335,7 +339,8 @@ __device__ __forceinline__ VisView vis_view(const SplitDev* views, uint32_t v) {
   VisView w;
   w.begin = d->vis_begin;
   w.tiles = d->tiles;
-  w.end = w.begin + w.tiles * d->n_blocks;
+  w.n_blocks = d->n_blocks;
+  w.end = w.begin + w.tiles * ((w.n_blocks + kSplitBpi - 1u) / kSplitBpi);
   w.live = d->live;
   w.pitch = d->vis_pitch;
   w.caches = d->caches;
@@ -348,21 +353,24 @@ __device__ __forceinline__ VisView vis_view(const SplitDev* views, uint32_t v) {
   w.vox.levels = d->vox.levels;
   for (int i = 0; i < 3; ++i) w.vmin[i] = d->vmin[i];
   w.voxel_size = d->voxel_size;
+  const uint64_t r1 = static_cast<uint64_t>(w.vox.res) + 1u;
+  w.fidx = r1 * r1 * r1 < (1ull << 24);
   return w;
 }
 
-// Items a wave takes from the visibility queue at a time (4 and 128 were
-// slower on the 64-view workload: queue-head contention and the tail).
-constexpr uint32_t kVisChunk = 32;
+// Items a wave takes from the visibility queue at a time: 1 or 2 contend
+// for the queue head, 8 and more put the items of more views in flight at
+// once (each view traces its own voxel copy).
+constexpr uint32_t kVisChunk = 4;
 
-// Cone-traced visibility of every view: items (view, shadow block, 64-cache
-// tile), block-major within a view, handed out in chunks of kVisChunk
-// consecutive items by an atomic queue head, so a wave's items mostly share
-// a view and a block (its descriptor, block record and march records stay
-// in SGPRs and the scalar cache) and the waves that finish first take more.
-// Each lane traces its cache towards the item's block with the code of
-// k_visibility_planned (the view's march plan), so the values are those of
-// the single-view kernels.
+// Cone-traced visibility of every view: items (view, group of kSplitBpi
+// shadow blocks, 64-cache tile), group-major within a view, handed out in
+// chunks of kVisChunk consecutive items by an atomic queue head, so a wave's
+// items mostly share a view and a block group (its descriptor, block records
+// and march records stay in SGPRs and the scalar cache) and the waves that
+// finish first take more.  Each lane loads its cache once per item and traces
+// it towards the group's blocks with the code of k_visibility_planned (the
+// view's march plan), so the values are those of the single-view kernels.
 __global__ __launch_bounds__(256) void k_split_visibility(const SplitDev* views, uint32_t n_views, int stride,
                                                           SplitTotals* totals) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -378,7 +386,7 @@ __global__ __launch_bounds__(256) void k_split_visibility(const SplitDev* views,
     for (uint32_t item = base; item < stop; ++item) {
       if (item < w.begin || item >= w.end) w = vis_view(views, split_view<0>(views, n_views, item));
       const uint32_t local = item - w.begin;
-      const uint32_t b = local / w.tiles, tile = local - b * w.tiles;
+      const uint32_t group = local / w.tiles, tile = local - group * w.tiles;
       const uint32_t cache = tile * 64u + lane;
       const bool valid = cache < w.live;
       const float* e = w.caches + static_cast<size_t>(valid ? cache : w.live - 1u) * stride;
@@ -387,9 +395,13 @@ __global__ __launch_bounds__(256) void k_split_visibility(const SplitDev* views,
       // cacheLightingRSM.comp:103-104
       const f3 vpos = make3((wp.x - w.vmin[0]) / vsz, (wp.y - w.vmin[1]) / vsz, (wp.z - w.vmin[2]) / vsz);
       const __amdgpu_buffer_rsrc_t cr = corner_rsrc(w.vox);
-      Trace t = trace_setup(w.vox, w.voxel_size, wp, vpos, block_record(w.blocks, b));
-      trace_planned(march_steps(w.march, b), cr, t, valid);
-      if (valid) w.vis[static_cast<size_t>(b) * w.pitch + cache] = trace_visibility(t);
+      const uint32_t b0 = group * kSplitBpi, b1 = min(w.n_blocks, b0 + kSplitBpi);
+      for (uint32_t b = b0; b < b1; ++b) {
+        Trace t = trace_setup(w.vox, w.voxel_size, wp, vpos, block_record(w.blocks, b));
+        if (w.fidx) trace_planned<false, true>(march_steps(w.march, b), cr, t, valid);
+        else trace_planned<false, false>(march_steps(w.march, b), cr, t, valid);
+        if (valid) w.vis[static_cast<size_t>(b) * w.pitch + cache] = trace_visibility(t);
+      }
     }
   }
 }
