@@ -33,6 +33,12 @@ struct drv_ctx {
   hipEvent_t vox_free_ev = nullptr;
   bool vox_free_valid = false;
   bool vox_overlap = false;
+  // The event the next voxelization waits for when vox_free_valid: this
+  // context's vox_free_ev, or the one a batch records once for all its views.
+  hipEvent_t vox_free_wait = nullptr;
+  // Set by a batch around its views' own light passes: their voxel-free
+  // point is recorded once by the batch instead.
+  bool defer_vox_read = false;
   // The surrogate's spot check also runs on `aux`, beside the network pass.
   hipEvent_t spot_fork_ev = nullptr, spot_done_ev = nullptr;
   bool vox_pending = false;
@@ -218,8 +224,10 @@ bool is_capturing(hipStream_t s);
 // they have run.  Under graph capture the event would belong to the graph, so
 // the next voxelization then waits for the whole stream instead.
 void voxels_read(drv_ctx* ctx) {
+  if (ctx->defer_vox_read) return;
   ctx->vox_free_valid = ctx->vox_overlap && !is_capturing(ctx->stream) &&
                         hipEventRecord(ctx->vox_free_ev, ctx->stream) == hipSuccess;
+  ctx->vox_free_wait = ctx->vox_free_ev;
 }
 
 // Time between two recorded events; 0 for a pair that was never recorded.
@@ -695,7 +703,7 @@ int drv_voxelize_begin(drv_ctx* ctx) {
   // voxels (the previous frame's cone trace) when that point is known (not
   // under capture, a reader since the last voxelization).
   if (ctx->vox_free_valid && !is_capturing(ctx->stream)) {
-    DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->vox_free_ev, 0));
+    DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->vox_free_wait, 0));
   } else {
     DRV_HIP(ctx, hipEventRecord(ctx->fork_ev, ctx->stream));
     DRV_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->fork_ev, 0));
@@ -1449,6 +1457,9 @@ int drv_mlp_finetune(drv_ctx* ctx, uint32_t n_samples, uint32_t batch, uint32_t 
 struct drv_batch {
   std::vector<drv_ctx*> views;
   hipStream_t stream = nullptr;
+  // Recorded once after a batched light pass's last voxel read, for the
+  // views with the voxel overlap on (one event instead of one per view).
+  hipEvent_t vox_free_ev = nullptr;
   int stride = 16;
   int sh_bands = 1;
   int shadow = 0;
@@ -1495,6 +1506,18 @@ bool any_surrogate(const drv_batch* b) {
 bool is_capturing(hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+// The voxel-free point of every view after a batched light pass: one event
+// on the shared stream after the last cone trace of all views.
+void batch_voxels_read(drv_batch* b) {
+  bool any = false;
+  for (const drv_ctx* c : b->views) any = any || c->vox_overlap;
+  const bool ok = any && !is_capturing(b->stream) && hipEventRecord(b->vox_free_ev, b->stream) == hipSuccess;
+  for (drv_ctx* c : b->views) {
+    c->vox_free_valid = ok && c->vox_overlap;
+    if (c->vox_free_valid) c->vox_free_wait = b->vox_free_ev;
+  }
 }
 
 // The split queue: every view's buffers as its own light pass sizes them,
@@ -1576,7 +1599,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
   } else if (b->shadow) {
     drvk::launch_batch_split_visibility(b->d_split, nv, b->d_totals, b->stride, grid, b->stream);
   }
-  for (size_t v = 0; v < n; ++v) voxels_read(b->views[v]);
+  batch_voxels_read(b);
   drvk::launch_batch_split_gather(b->d_split, nv, b->d_totals, b->stride, b->sh_bands, b->shadow, v0->shc, grid,
                                   b->stream);
   for (size_t v = 0; v < n; ++v) {
@@ -1630,6 +1653,11 @@ int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out) {
     drv_batch_destroy(b);
     return DRV_ERR_NO_MEMORY;
   }
+  if (hipEventCreateWithFlags(&b->vox_free_ev, hipEventDisableTiming) != hipSuccess) {
+    b->vox_free_ev = nullptr;
+    drv_batch_destroy(b);
+    return DRV_ERR_HIP;
+  }
   *out = b;
   return DRV_OK;
 }
@@ -1638,6 +1666,15 @@ int drv_batch_destroy(drv_batch* b) {
   if (!b) return DRV_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   else (void)hipDeviceSynchronize();
+  // Views that would wait for the batch's voxel-free event fork from their
+  // stream at their next voxelization instead.
+  for (drv_ctx* c : b->views) {
+    if (c->vox_free_wait == b->vox_free_ev) {
+      c->vox_free_valid = false;
+      c->vox_free_wait = c->vox_free_ev;
+    }
+  }
+  if (b->vox_free_ev) (void)hipEventDestroy(b->vox_free_ev);
   dev_free(b->d_views);
   dev_free(b->d_queue);
   dev_free(b->partial);
@@ -1684,10 +1721,23 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
     // Per-view passes: each view's own cone trace, SH gather and reduction,
     // back to back on the shared stream (views large enough to fill the GPU
     // on their own gain nothing from the fused queue).
-    for (size_t v = 0; v < n; ++v) {
-      const int rc = drv_light_caches(b->views[v], &rsms[v], &lights[v]);
-      if (rc != DRV_OK) return bfail(b, rc, "batch view " + std::to_string(v) + ": " + b->views[v]->err);
+    struct Defer {  // the views' own voxel-free records, for the duration of the loop
+      drv_batch* b;
+      explicit Defer(drv_batch* bb) : b(bb) {
+        for (drv_ctx* c : b->views) c->defer_vox_read = true;
+      }
+      ~Defer() {
+        for (drv_ctx* c : b->views) c->defer_vox_read = false;
+      }
+    };
+    {
+      Defer defer(b);
+      for (size_t v = 0; v < n; ++v) {
+        const int rc = drv_light_caches(b->views[v], &rsms[v], &lights[v]);
+        if (rc != DRV_OK) return bfail(b, rc, "batch view " + std::to_string(v) + ": " + b->views[v]->err);
+      }
     }
+    batch_voxels_read(b);
     return DRV_OK;
   }
   if (any_surrogate(b))
@@ -1755,7 +1805,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   drvk::launch_batch(b->d_views, static_cast<uint32_t>(n), b->d_queue, b->stride, b->sh_bands, b->shadow,
                      static_cast<uint32_t>(b->cpw),
                      b->vpl_tile, b->partial, b->partial_size, b->views[0]->shc, grid, b->stream);
-  for (size_t v = 0; v < n; ++v) voxels_read(b->views[v]);
+  batch_voxels_read(b);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(b, DRV_ERR_HIP, std::string("batch_light_caches: ") + hipGetErrorString(e));
   return DRV_OK;

@@ -229,6 +229,39 @@ def test_batch_with_voxel_overlap_equals_without(mode):
         np.testing.assert_array_equal(s0, s1, err_msg=fi.workload.name)
 
 
+def test_views_after_their_batch_is_destroyed():
+    """A batch records one voxel-free event for all its views; once it is
+    destroyed, the views voxelize and light on their own again (two frames
+    each, voxel overlap on) with the caches of views that never joined a
+    batch."""
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    frames = _frames()
+    announce("views rendered on their own after their batch is destroyed")
+    out = []
+    for batched in (False, True):
+        rs, dfs = _group(frames, 2)
+        for r in rs:
+            r.set_voxel_overlap(True)
+        if batched:
+            b = Batch(rs)
+            b.set_tuning(0, 1024)
+            b.render_frame(dfs)
+            b.set_tuning(-2, 1024)
+            b.render_frame(dfs)
+            b.close()
+        for _ in range(2):
+            for r, df in zip(rs, dfs):
+                r.render_frame(df)
+        torch.cuda.synchronize()
+        out.append([_by_position(r) for r in rs])
+        for r in rs:
+            r.close()
+    for (p0, s0), (p1, s1), fi in zip(out[0], out[1], frames):
+        np.testing.assert_array_equal(p0, p1, err_msg=fi.workload.name)
+        np.testing.assert_array_equal(s0, s1, err_msg=fi.workload.name)
+
+
 def test_batch_autotune_restores_entries():
     from dynamicradiancevolume_amd.pipeline import Batch
 
