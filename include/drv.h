@@ -464,6 +464,8 @@ typedef struct drv_batch drv_batch;
  * views. */
 #define DRV_BATCH_SPLIT_GATHER (-2)
 
+/* A batch keeps pointers to its views (all on one stream): destroy it before
+ * any of them. */
 int drv_batch_create(drv_ctx* const* views, int32_t n_views, drv_batch** out);
 int drv_batch_destroy(drv_batch* batch);
 const char* drv_batch_last_error(const drv_batch* batch);
