@@ -199,6 +199,56 @@ def test_split_queue_visibility_and_sh_equal_single_views(mode):
     b.close()
 
 
+def test_split_queue_views_with_fewer_blocks_than_an_item():
+    """The split queue's visibility items hold 4 shadow blocks; views with 1
+    (an 8x8 read at shadow LOD 3) or 2 (a 16x8 read of the 2:1 extension)
+    blocks end their only item early.  Their visibility and that of a
+    4096-VPL view in the same batch equal each view's own light pass."""
+    from dynamicradiancevolume_amd import workloads as W
+    from dynamicradiancevolume_amd.pipeline import Batch
+
+    c = W.CORNELL_256
+    views = [
+        c.with_(name="cornell_8x8_lod3", light=W.replace(c.light, rsm_resolution=64, rsm_read_lod=3,
+                                                         indirect_shadow_lod=3)),
+        c.with_(name="cornell_16x8_lod3", seed=4, light=W.replace(c.light, rsm_resolution=32, rsm_read_lod=1,
+                                                                  rsm_wide=1, indirect_shadow_lod=3)),
+        c,
+    ]
+    frames = []
+    for w in views:
+        if w.name not in _cache:
+            _cache[w.name] = W.make_inputs(w)
+        frames.append(_cache[w.name])
+    blocks = [fi.n_vpl // int(fi.light.indirect_shadow_sample_interval) for fi in frames]
+    assert blocks[:2] == [1, 2] and blocks[2] % 4 == 0, blocks
+    announce(f"split queue over views of {blocks} shadow blocks vs single views")
+    ref = []
+    rs1, dfs1 = _group(frames, 2)
+    for r, df, nb in zip(rs1, dfs1, blocks):
+        r.render_frame(df)
+        _, live = r.cache_count()
+        ref.append((_by_position(r), r.read_visibility(live, nb), r.read_caches(live)[:, :3].copy()))
+        r.close()
+    rs, dfs = _group(frames, 2)
+    b = Batch(rs)
+    b.set_tuning(-1, 1024)
+    b.render_frame(dfs)
+    for ((p0, s0), v0, pos0), r, fi, nb in zip(ref, rs, frames, blocks):
+        p1, s1 = _by_position(r)
+        np.testing.assert_array_equal(p0, p1)
+        assert rel_l2(s1, s0) <= SH_TOL, fi.workload.name
+        _, live = r.cache_count()
+        pos1 = r.read_caches(live)[:, :3].copy()
+        o0 = np.lexsort(pos0.view(np.uint32).T)
+        o1 = np.lexsort(pos1.view(np.uint32).T)
+        np.testing.assert_array_equal(pos1[o1], pos0[o0])
+        np.testing.assert_array_equal(r.read_visibility(live, nb)[o1], v0[o0], err_msg=fi.workload.name)
+    b.close()
+    for r in rs:
+        r.close()
+
+
 @pytest.mark.parametrize("mode", [0, -2, 64])
 def test_batch_with_voxel_overlap_equals_without(mode):
     """With the voxel overlap on, each view's next voxelization starts after
