@@ -1734,7 +1734,12 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
       Defer defer(b);
       for (size_t v = 0; v < n; ++v) {
         const int rc = drv_light_caches(b->views[v], &rsms[v], &lights[v]);
-        if (rc != DRV_OK) return bfail(b, rc, "batch view " + std::to_string(v) + ": " + b->views[v]->err);
+        if (rc != DRV_OK) {
+          // The views lit so far read voxels after their last voxel-free
+          // point: their next voxelization waits for the whole stream.
+          for (drv_ctx* c : b->views) c->vox_free_valid = false;
+          return bfail(b, rc, "batch view " + std::to_string(v) + ": " + b->views[v]->err);
+        }
       }
     }
     batch_voxels_read(b);
