@@ -217,6 +217,15 @@ void join_voxels(drv_ctx* ctx) {
   ctx->vox_pending = false;
 }
 
+// The same before work that reads the voxels: until that work's voxels_read
+// records a new voxel-free point, the last one precedes reads already
+// enqueued, so a call that fails in between leaves none (the next
+// voxelization then waits for everything on the stream).
+void join_voxels_for_read(drv_ctx* ctx) {
+  join_voxels(ctx);
+  ctx->vox_free_valid = false;
+}
+
 bool is_capturing(hipStream_t s);
 
 // Every call that enqueues reads of the voxel volume on `stream` calls this
@@ -859,7 +868,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
   // The VPL preparation reads no voxels, so it runs while a voxelization on
   // the side stream finishes; the cone trace joins it.  With stage timing on
   // the join comes first, so no stage's time includes the wait.
-  if (ctx->profiling) join_voxels(ctx);
+  if (ctx->profiling) join_voxels_for_read(ctx);
   const bool shadow = ctx->cfg.indirect_shadow != 0;
   const size_t cap = static_cast<size_t>(ctx->cfg.max_caches);
   const bool use_net = shadow && ctx->sur.enabled && ctx->mlp_ready;
@@ -889,7 +898,7 @@ int drv_light_caches(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_light* lig
                             specular ? ctx->spec.vpls_lanes : nullptr, &plan, ctx->stream,
                             shadow ? ctx->march : nullptr, &vview);
   record(ctx, 6);
-  join_voxels(ctx);
+  join_voxels_for_read(ctx);
   if (use_net) {
     if ((rc = surrogate_visibility(ctx, n_blocks, n_vpl, interval, grid)) != DRV_OK) return rc;
     steps = nullptr;
@@ -956,7 +965,7 @@ int drv_light_caches_noalloc(drv_ctx* ctx, const drv_rsm* rsm, const drv_spot_li
   v.voxel_size = ctx->vol.voxel_size_in_world;
   v.n_vpl = n_vpl;
   v.interval = interval;
-  join_voxels(ctx);
+  join_voxels_for_read(ctx);
   drvk::launch_light_static(base, *rsm, *light, v, drvk::entry_stride(ctx->cfg.sh_bands), ctx->cfg.sh_bands,
                             shadow ? 1 : 0, ctx->shc, std::max(1, ctx->cu_count * 64 / 4), ctx->stream);
   voxels_read(ctx);
@@ -1036,7 +1045,7 @@ int drv_read_voxels(drv_ctx* ctx, int32_t level, uint8_t* host_out) {
   if (!ctx || !host_out) return DRV_ERR_INVALID_ARG;
   if (level < -1 || level >= ctx->vox.levels) return fail(ctx, DRV_ERR_INVALID_ARG, "read_voxels: bad level");
   // level -1 reads the voxelization target (before the blend).
-  join_voxels(ctx);
+  join_voxels_for_read(ctx);
   const uint8_t* src = level < 0 ? ctx->vox_target : ctx->vox_mips + ctx->vox.mip_offset[level];
   const size_t w = static_cast<size_t>(ctx->vox.res >> (level < 0 ? 0 : level));
   DRV_HIP(ctx, hipMemcpyAsync(host_out, src, w * w * w, hipMemcpyDeviceToHost, ctx->stream));
@@ -1361,7 +1370,7 @@ int drv_mlp_collect(drv_ctx* ctx, uint32_t n, uint32_t seed, float* x_dev, float
   int rc;
   if ((rc = require_features(ctx)) != DRV_OK) return rc;
   if (!ctx->label_caches) return fail(ctx, DRV_ERR_STATE, "mlp_collect: the last light pass has no cone-traced labels");
-  join_voxels(ctx);
+  join_voxels_for_read(ctx);
   if (n == 0) return DRV_OK;
   // Pairs are drawn over the label set's live caches (read on the device).
   Counter c{};
@@ -1386,7 +1395,7 @@ int drv_mlp_features(drv_ctx* ctx, const uint32_t* pairs_dev, uint32_t n, float*
   int rc;
   if ((rc = require_features(ctx)) != DRV_OK) return rc;
   const drvk::FeatureArgsHost fh = drvk::make_feature_args(ctx->vol, ctx->frame.camera_position, voxel_view(ctx));
-  join_voxels(ctx);
+  join_voxels_for_read(ctx);
   // Pairs name caches of the current list (live count read on the host).
   uint32_t total = 0, live = 0;
   if ((rc = drv_get_cache_count(ctx, &total, &live)) != DRV_OK) return rc;
@@ -1582,7 +1591,7 @@ int batch_split(drv_batch* b, const drv_rsm* rsms, const drv_spot_light* lights)
       return bfail(b, DRV_ERR_HIP, "batch_light_caches: descriptor upload failed");
     b->h_split.swap(img);
   }
-  for (size_t v = 0; v < n; ++v) join_voxels(b->views[v]);
+  for (size_t v = 0; v < n; ++v) join_voxels_for_read(b->views[v]);
   drv_ctx* v0 = b->views[0];
   const int grid = persistent_grid(v0);
   const uint32_t nv = static_cast<uint32_t>(n);
@@ -1798,7 +1807,7 @@ int drv_batch_light_caches(drv_batch* b, const drv_rsm* rsms, const drv_spot_lig
   }
   for (size_t v = 0; v < n; ++v) {
     drv_ctx* ctx = b->views[v];
-    join_voxels(ctx);  // the batch runs on the views' shared stream
+    join_voxels_for_read(ctx);  // the batch runs on the views' shared stream
     drvk::launch_prepare_vpls(rsms[v], lights[v], ctx->vpls, b->shadow ? ctx->blocks : nullptr, 0, nullptr,
                               nullptr, b->stream);
     // The batch leaves no per-pair visibility behind.
